@@ -1,0 +1,193 @@
+"""Procedural workloads: the Cornell box (256x256, 16 spp, depth 8), the
+displaced-icosphere traversal microbench (1,310,720 triangles, 4,194,304
+coherent and incoherent rays), the architectural interior, the motion-blur
+sphere field and the guided-interior case, built directly as reference
+`scene` objects (there are no scene files).  Everything is seeded.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import native as N
+from .scene import (SceneBuilder, area_light, image_texture, make_material, perlin_texture, spot_light)
+
+# ---------------------------------------------------------------------------
+# geometry helpers
+
+
+def icosphere(subdivisions):
+    """Unit icosphere: 20*4^s faces, 10*4^s+2 vertices (float64)."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    v = np.array([(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+                  (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)], dtype=np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array([(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+                  (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+                  (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)], dtype=np.int64)
+    for _ in range(subdivisions):
+        nv = v.shape[0]
+        a, b, c = f[:, 0], f[:, 1], f[:, 2]
+        e = np.stack([np.stack([a, b], 1), np.stack([b, c], 1), np.stack([c, a], 1)], 0).reshape(-1, 2)
+        e.sort(axis=1)
+        key = e[:, 0] * nv + e[:, 1]
+        uniq, inv = np.unique(key, return_inverse=True)
+        i0, i1 = uniq // nv, uniq % nv
+        mid = 0.5 * (v[i0] + v[i1])
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        v = np.concatenate([v, mid], 0)
+        m = (inv + nv).reshape(3, -1)
+        mab, mbc, mca = m[0], m[1], m[2]
+        f = np.concatenate([np.stack([a, mab, mca], 1), np.stack([b, mbc, mab], 1), np.stack([c, mca, mbc], 1),
+                            np.stack([mab, mbc, mca], 1)], 0)
+    return v, f
+
+
+_GRAD3 = np.array([(1, 1, 0), (-1, 1, 0), (1, -1, 0), (-1, -1, 0), (1, 0, 1), (-1, 0, 1), (1, 0, -1), (-1, 0, -1),
+                   (0, 1, 1), (0, -1, 1), (0, 1, -1), (0, -1, -1)], dtype=np.float64)
+
+
+def perlin3(p, perm):
+    """Gradient noise (Perlin 2002 improved noise) at points p (n,3)."""
+    pi = np.floor(p).astype(np.int64)
+    f = p - pi
+    u = f * f * f * (f * (f * 6 - 15) + 10)
+    P = np.concatenate([perm, perm])
+    res = 0.0
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                h = P[P[P[(pi[:, 0] + dx) & 255] + ((pi[:, 1] + dy) & 255)] + ((pi[:, 2] + dz) & 255)] % 12
+                g = _GRAD3[h]
+                d = f - np.array([dx, dy, dz], dtype=np.float64)
+                w = (u[:, 0] if dx else 1 - u[:, 0]) * (u[:, 1] if dy else 1 - u[:, 1]) * (u[:, 2] if dz else 1 - u[:, 2])
+                res = res + w * np.einsum("ij,ij->i", g, d)
+    return res
+
+
+def fbm3(p, seed, octaves=3, base_freq=2.0):
+    perm = np.random.default_rng(seed).permutation(256).astype(np.int64)
+    s, amp, freq = 0.0, 1.0, base_freq
+    for _ in range(octaves):
+        s = s + amp * perlin3(p * freq, perm)
+        amp *= 0.5
+        freq *= 2.0
+    return s
+
+
+def displaced_icosphere(subdivisions, seed, amplitude=0.05):
+    v, f = icosphere(subdivisions)
+    n = fbm3(v, seed)
+    v = v * (1.0 + amplitude * n)[:, None]
+    return v.astype(np.float32), f.astype(np.uint32)
+
+
+def look_at_camera(position, target, up, width, height, fovy_deg, spp=1, aperture=0.0, focus=0.0):
+    cam = N.camera()
+    cam.Type = N.CameraType_Simple
+    cam.NearPlane = N.vec4(float(fovy_deg), 0.0, 0.0, 0.0)   # FovY aliases NearPlane.x
+    cam.Position = N.vec3(*position)
+    cam.Gaze = N.vec3(*target)
+    cam.GazeIsPoint = 1
+    cam.Up = N.vec3(*up)
+    cam.Resolution = N.vec2u(int(width), int(height))
+    cam.NearDistance = 1.0
+    cam.FocusDistance = float(focus)
+    cam.ApertureSize = float(aperture)
+    cam.Gamma = cam.InvGamma = 1.0
+    cam.SampleCount = int(spp)
+    return cam
+
+
+# ---------------------------------------------------------------------------
+# traversal microbench (displaced icosphere)
+
+
+def icosphere_scene(subdivisions=8, seed=1, amplitude=0.05):
+    v, f = displaced_icosphere(subdivisions, seed, amplitude)
+    b = SceneBuilder(max_depth=0)
+    b.add_material(make_material())
+    b.add_mesh(v, f, material=0, smooth=True)
+    sc = b.build(meta={"vertices": v, "faces": f, "bbox": (v.min(0), v.max(0))})
+    return sc
+
+
+def coherent_rays(width=2048, height=2048, eye=(0.0, 0.0, 3.0), fovy_deg=45.0):
+    """Pinhole camera rays through pixel centres, unit directions, looking at
+    the origin along -z."""
+    ys, xs = np.meshgrid(np.arange(height, dtype=np.float64), np.arange(width, dtype=np.float64), indexing="ij")
+    h = math.tan(math.radians(fovy_deg) / 2.0)
+    w = h * width / height
+    px = ((xs + 0.5) / width * 2.0 - 1.0) * w
+    py = (1.0 - (ys + 0.5) / height * 2.0) * h
+    d = np.stack([px, py, -np.ones_like(px)], -1).reshape(-1, 3)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.zeros(d.shape[0], dtype=N.RAY_DTYPE)
+    r["Origin"] = np.asarray(eye, dtype=np.float32)
+    r["Direction"] = d.astype(np.float32)
+    r["TMax"] = np.float32(np.inf)
+    return r
+
+
+def incoherent_rays(n, bbox, seed=2):
+    """Origins uniform in the box, directions uniform on the sphere."""
+    g = np.random.default_rng(seed)
+    lo, hi = np.asarray(bbox[0], np.float64), np.asarray(bbox[1], np.float64)
+    o = lo + (hi - lo) * g.random((n, 3))
+    z = 1.0 - 2.0 * g.random(n)
+    phi = 2.0 * math.pi * g.random(n)
+    s = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    d = np.stack([s * np.cos(phi), s * np.sin(phi), z], -1)
+    r = np.zeros(n, dtype=N.RAY_DTYPE)
+    r["Origin"] = o.astype(np.float32)
+    r["Direction"] = d.astype(np.float32)
+    r["TMax"] = np.float32(np.inf)
+    return r
+
+
+# ---------------------------------------------------------------------------
+# Cornell box
+
+
+def cornell_scene(width=256, height=256, spp=16, max_depth=8, light_mode="mesh", with_spheres=True):
+    """Unit-cube Cornell box: five walls as 10 triangles, a 0.25x0.25 ceiling
+    light of radiance 15 as a 2-triangle emitting mesh, a dielectric sphere
+    (r=0.18, IOR 1.5) and a Phong metal sphere (r=0.2).  Walls are Lambertian
+    through NormalizedModifiedBlinnPhong with Ks=0 (diffuse Kd/pi)."""
+    b = SceneBuilder(max_depth=max_depth, path_trace=True, importance_sampling=True)
+    grey = b.add_material(make_material(kd=(0.73, 0.73, 0.73)))
+    red = b.add_material(make_material(kd=(0.65, 0.05, 0.05)))
+    green = b.add_material(make_material(kd=(0.12, 0.45, 0.15)))
+    glass = b.add_material(make_material(kd=(0, 0, 0), transparency=(1, 1, 1), ior=1.5))
+    metal = b.add_material(make_material(kd=(0.05, 0.05, 0.05), ks=(0.6, 0.6, 0.6), mirror=(0.4, 0.4, 0.4),
+                                         exponent=50.0, brdf_type=N.BRDF_Phong))
+    lightm = b.add_material(make_material(kd=(0, 0, 0)))
+
+    def quad(p0, p1, p2, p3):
+        return np.array([p0, p1, p2, p3], np.float32), np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+
+    # floor (y=0, normal +y), ceiling (y=1, normal -y), back (z=0, normal +z)
+    v, f = quad((0, 0, 0), (0, 0, 1), (1, 0, 1), (1, 0, 0))
+    b.add_mesh(v, f, material=grey)
+    v, f = quad((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1))
+    b.add_mesh(v, f, material=grey)
+    v, f = quad((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0))
+    b.add_mesh(v, f, material=grey)
+    v, f = quad((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1))   # left, normal +x
+    b.add_mesh(v, f, material=red)
+    v, f = quad((1, 0, 0), (1, 0, 1), (1, 1, 1), (1, 1, 0))   # right, normal -x
+    b.add_mesh(v, f, material=green)
+    y = 1.0 - 1e-3
+    v, f = quad((0.375, y, 0.375), (0.625, y, 0.375), (0.625, y, 0.625), (0.375, y, 0.625))
+    if light_mode == "mesh":
+        b.add_mesh(v, f, material=lightm, emitter=True, radiance=(15, 15, 15))
+    else:
+        b.add_mesh(v, f, material=lightm, emitter=True, radiance=(15, 15, 15))
+        b.add_light(area_light((0.375, y, 0.375), (15, 15, 15), (0.25, 0, 0), (0, 0, 0.25), (0, -1, 0)))
+    if with_spheres:
+        b.add_sphere((0.30, 0.18, 0.45), 0.18, material=glass)
+        b.add_sphere((0.70, 0.20, 0.65), 0.20, material=metal)
+    sc = b.build(meta={"name": "cornell"})
+    cam = look_at_camera((0.5, 0.5, 2.0), (0.5, 0.5, 0.0), (0, 1, 0), width, height, 50.0, spp=spp)
+    return sc, cam

@@ -1,0 +1,309 @@
+// bvh_build.cpp -- binned-SAH binary BVH collapsed into 4-wide nodes.
+//
+// The device BVH is this project's own structure and is not the reference's
+// entity tree: the reference BVH (bvh.cpp:1-328) splits at the box midpoint
+// and is walked without pruning (main.cpp:282-319).  Any tree whose boxes
+// contain every point the reference triangle and sphere tests can accept gives
+// the same closest hit when walked front to back with distance pruning; the
+// padding in PadBox keeps that containment under float rounding.
+#include "bvh_build.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+namespace az {
+
+namespace {
+
+struct Aabb {
+    float lo[3], hi[3];
+    void reset() {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = FLT_MAX;
+            hi[k] = -FLT_MAX;
+        }
+    }
+    void grow(const Aabb &b) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], b.lo[k]);
+            hi[k] = std::max(hi[k], b.hi[k]);
+        }
+    }
+    void growPoint(const float *p) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], p[k]);
+            hi[k] = std::max(hi[k], p[k]);
+        }
+    }
+    float area() const {
+        float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (dx < 0.0f || dy < 0.0f || dz < 0.0f) return 0.0f;
+        return 2.0f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+struct Node2 {
+    Aabb box;
+    uint32_t left = 0, right = 0;
+    uint32_t first = 0, count = 0;
+    bool leaf = false;
+};
+
+struct Bin {
+    Aabb box;
+    uint32_t count;
+};
+
+}  // namespace
+
+void PadBox(BuildBox &b) {
+    float ext = 0.0f, mag = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        ext = std::max(ext, b.hi[k] - b.lo[k]);
+        mag = std::max(mag, std::max(std::fabs(b.lo[k]), std::fabs(b.hi[k])));
+    }
+    float pad = 4e-6f * ext + 2e-6f * mag + 1e-30f;
+    for (int k = 0; k < 3; ++k) {
+        b.lo[k] -= pad;
+        b.hi[k] += pad;
+    }
+}
+
+uint32_t BuildWideBvh(const std::vector<BuildBox> &boxes, const BuildParams &P, uint32_t itemBase,
+                      std::vector<Node4> &nodes, std::vector<uint32_t> &order, BuildStats *stats) {
+    const uint32_t n = (uint32_t)boxes.size();
+    const uint32_t maxLeaf = std::max(1u, std::min(P.maxLeafItems, kMaxLeafItems));
+    order.resize(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+    std::vector<Aabb> ib(n);
+    std::vector<float> cen(3 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            ib[i].lo[k] = boxes[i].lo[k];
+            ib[i].hi[k] = boxes[i].hi[k];
+            cen[3 * (size_t)i + k] = 0.5f * (boxes[i].lo[k] + boxes[i].hi[k]);
+        }
+    }
+
+    // ---- binary build -------------------------------------------------------
+    std::vector<Node2> b2;
+    b2.reserve(n > 0 ? 2 * (size_t)n : 1);
+    struct Work { uint32_t node, begin, end; };
+    std::vector<Work> stack;
+    b2.emplace_back();
+    if (n > 0) stack.push_back({0, 0, n});
+    const uint32_t nb = std::max(4u, std::min(P.bins, 64u));
+    std::vector<Bin> bins(nb);
+    std::vector<Aabb> rightBox(nb);
+    std::vector<uint32_t> rightCount(nb);
+    while (!stack.empty()) {
+        Work w = stack.back();
+        stack.pop_back();
+        Node2 &nd = b2[w.node];
+        Aabb box, cb;
+        box.reset();
+        cb.reset();
+        for (uint32_t i = w.begin; i < w.end; ++i) {
+            box.grow(ib[order[i]]);
+            cb.growPoint(&cen[3 * (size_t)order[i]]);
+        }
+        nd.box = box;
+        const uint32_t cnt = w.end - w.begin;
+        float bestCost = FLT_MAX;
+        int bestAxis = -1;
+        uint32_t bestSplit = 0;
+        if (cnt > 1) {
+            for (int ax = 0; ax < 3; ++ax) {
+                float ext = cb.hi[ax] - cb.lo[ax];
+                if (!(ext > 0.0f)) continue;
+                float scale = (float)nb / ext;
+                for (uint32_t b = 0; b < nb; ++b) {
+                    bins[b].box.reset();
+                    bins[b].count = 0;
+                }
+                for (uint32_t i = w.begin; i < w.end; ++i) {
+                    uint32_t it = order[i];
+                    int b = (int)((cen[3 * (size_t)it + ax] - cb.lo[ax]) * scale);
+                    b = std::min(std::max(b, 0), (int)nb - 1);
+                    bins[b].box.grow(ib[it]);
+                    bins[b].count++;
+                }
+                Aabb acc;
+                acc.reset();
+                uint32_t c = 0;
+                for (int b = (int)nb - 1; b > 0; --b) {
+                    acc.grow(bins[b].box);
+                    c += bins[b].count;
+                    rightBox[b] = acc;
+                    rightCount[b] = c;
+                }
+                acc.reset();
+                c = 0;
+                for (uint32_t b = 0; b + 1 < nb; ++b) {
+                    acc.grow(bins[b].box);
+                    c += bins[b].count;
+                    uint32_t rc = rightCount[b + 1];
+                    if (c == 0 || rc == 0) continue;
+                    float cost = acc.area() * (float)c + rightBox[b + 1].area() * (float)rc;
+                    if (cost < bestCost) {
+                        bestCost = cost;
+                        bestAxis = ax;
+                        bestSplit = b + 1;
+                    }
+                }
+            }
+        }
+        float area = box.area();
+        float splitCost = bestAxis >= 0 && area > 0.0f
+                              ? 0.5f * P.nodeCost + P.itemCost * bestCost / area
+                              : FLT_MAX;
+        float leafCost = P.itemCost * (float)cnt;
+        bool makeLeaf = cnt <= 1 || (cnt <= maxLeaf && leafCost <= splitCost);
+        if (makeLeaf) {
+            nd.leaf = true;
+            nd.first = w.begin;
+            nd.count = cnt;
+            continue;
+        }
+        uint32_t mid;
+        if (bestAxis >= 0) {
+            float ext = cb.hi[bestAxis] - cb.lo[bestAxis];
+            float scale = (float)nb / ext;
+            uint32_t *b0 = order.data() + w.begin, *b1 = order.data() + w.end;
+            uint32_t *m = std::partition(b0, b1, [&](uint32_t it) {
+                int b = (int)((cen[3 * (size_t)it + bestAxis] - cb.lo[bestAxis]) * scale);
+                b = std::min(std::max(b, 0), (int)nb - 1);
+                return (uint32_t)b < bestSplit;
+            });
+            mid = (uint32_t)(m - order.data());
+            if (mid == w.begin || mid == w.end) mid = w.begin + cnt / 2;
+        } else {
+            mid = w.begin + cnt / 2;   // all centroids coincide
+        }
+        uint32_t l = (uint32_t)b2.size();
+        b2.emplace_back();
+        b2.emplace_back();
+        b2[w.node].left = l;
+        b2[w.node].right = l + 1;
+        stack.push_back({l + 1, mid, w.end});
+        stack.push_back({l, w.begin, mid});
+    }
+
+    // ---- collapse to 4-wide -------------------------------------------------
+    BuildStats st;
+    const uint32_t rootIndex = (uint32_t)nodes.size();
+    struct Frame { uint32_t b2node, slotNode, slot, depth; };
+    // Each 4-wide node is created from one binary node; children are gathered
+    // by repeatedly opening the internal child with the largest area.
+    std::vector<uint32_t> maxStackOf;   // per created node, filled bottom-up
+    std::vector<uint32_t> parentOf, childCountOf;
+    struct Pend { uint32_t b2node, parent, slot, depth; };
+    std::vector<Pend> pend;
+    auto emptyNode = []() {
+        Node4 z;
+        for (int k = 0; k < 4; ++k) {
+            z.lox[k] = z.loy[k] = z.loz[k] = INFINITY;
+            z.hix[k] = z.hiy[k] = z.hiz[k] = -INFINITY;
+            z.child[k] = kEmpty;
+            z.pad[k] = 0;
+        }
+        return z;
+    };
+    nodes.push_back(emptyNode());
+    parentOf.push_back(kEmpty);
+    childCountOf.push_back(0);
+    if (n == 0) {
+        if (stats) *stats = st;
+        return rootIndex;
+    }
+    // The root: when the whole tree is one leaf, the root node holds it as
+    // its single child.
+    pend.push_back({0, kEmpty, 0, 1});
+    std::vector<uint32_t> depthOf;
+    depthOf.push_back(1);
+    bool first = true;
+    while (!pend.empty()) {
+        Pend p = pend.back();
+        pend.pop_back();
+        uint32_t me;
+        if (first) {
+            me = rootIndex;
+            first = false;
+        } else {
+            me = (uint32_t)nodes.size();
+            nodes.push_back(emptyNode());
+            parentOf.push_back(p.parent);
+            childCountOf.push_back(0);
+            depthOf.push_back(p.depth);
+            nodes[p.parent].child[p.slot] = me;
+        }
+        uint32_t local = me - rootIndex;
+        st.maxDepth = std::max(st.maxDepth, p.depth);
+        uint32_t kids[4];
+        uint32_t k = 0;
+        const Node2 &src = b2[p.b2node];
+        if (src.leaf) {
+            kids[k++] = p.b2node;
+        } else {
+            kids[k++] = src.left;
+            kids[k++] = src.right;
+            while (k < 4) {
+                int best = -1;
+                float bestArea = -1.0f;
+                for (uint32_t c = 0; c < k; ++c) {
+                    const Node2 &cn = b2[kids[c]];
+                    if (cn.leaf) continue;
+                    float a = cn.box.area();
+                    if (a > bestArea) {
+                        bestArea = a;
+                        best = (int)c;
+                    }
+                }
+                if (best < 0) break;
+                uint32_t open = kids[best];
+                kids[best] = b2[open].left;
+                kids[k++] = b2[open].right;
+            }
+        }
+        childCountOf[local] = k;
+        for (uint32_t c = 0; c < k; ++c) {
+            const Node2 &cn = b2[kids[c]];
+            Node4 &dst = nodes[me];
+            dst.lox[c] = cn.box.lo[0];
+            dst.loy[c] = cn.box.lo[1];
+            dst.loz[c] = cn.box.lo[2];
+            dst.hix[c] = cn.box.hi[0];
+            dst.hiy[c] = cn.box.hi[1];
+            dst.hiz[c] = cn.box.hi[2];
+            if (cn.leaf) {
+                dst.child[c] = make_leaf(itemBase + cn.first, cn.count);
+                st.leaves++;
+            } else {
+                pend.push_back({kids[c], me, c, p.depth + 1});
+            }
+        }
+    }
+    st.nodes = (uint32_t)(nodes.size() - rootIndex);
+    // Stack bound: at a node with k children the walk keeps one and pushes
+    // k-1, then descends into any one child.
+    maxStackOf.assign(st.nodes, 0);
+    for (uint32_t i = st.nodes; i-- > 0;) {
+        uint32_t best = 0;
+        const Node4 &nd = nodes[rootIndex + i];
+        for (int c = 0; c < 4; ++c) {
+            uint32_t r = nd.child[c];
+            if (r == kEmpty || ref_is_leaf(r)) continue;
+            best = std::max(best, maxStackOf[r - rootIndex]);
+        }
+        uint32_t kc = childCountOf[i];
+        maxStackOf[i] = (kc > 0 ? kc - 1 : 0) + best;
+    }
+    st.maxStack = maxStackOf[0];
+    (void)parentOf;
+    if (stats) *stats = st;
+    return rootIndex;
+}
+
+}  // namespace az

@@ -1,0 +1,43 @@
+// bvh_build.h -- host builder of the device BVH: binned-SAH binary tree,
+// collapsed into 4-wide 128-byte nodes.
+#pragma once
+
+#include <stdint.h>
+#include <vector>
+
+#include "device_scene.h"
+
+namespace az {
+
+struct BuildBox {
+    float lo[3], hi[3];
+};
+
+struct BuildParams {
+    uint32_t maxLeafItems = 4;     // leaves never hold more items than this
+    uint32_t bins = 32;            // SAH bins per axis
+    float nodeCost = 1.0f;         // SAH cost of visiting one 4-wide node
+    float itemCost = 1.0f;         // SAH cost of testing one item
+};
+
+struct BuildStats {
+    uint32_t nodes = 0;
+    uint32_t leaves = 0;
+    uint32_t maxDepth = 0;         // 4-wide levels on the deepest path
+    uint32_t maxStack = 0;         // stack entries a front-to-back walk can need
+};
+
+// Builds a 4-wide BVH over `boxes` and appends its nodes to `nodes`.  `order`
+// receives the item permutation: leaf references index into it, so item
+// order[i] must be stored at position itemBase + i.  Returns the root node
+// index.  The root is always an internal node (one leaf child when there are
+// few items).  Boxes are used as given; callers pad them.
+uint32_t BuildWideBvh(const std::vector<BuildBox> &boxes, const BuildParams &params,
+                      uint32_t itemBase, std::vector<Node4> &nodes, std::vector<uint32_t> &order,
+                      BuildStats *stats);
+
+// Pads a box outward so that a point the reference triangle test accepts
+// (barycentrics down to -1e-6) and the rounding of the slab test stay inside.
+void PadBox(BuildBox &b);
+
+}  // namespace az

@@ -1,0 +1,363 @@
+// device_trace.h -- ray traversal and primitive tests on gfx950.
+//
+// Primitive arithmetic is the reference's, operation for operation, so that
+// hit ids, t and barycentrics match it bit for bit when this file is compiled
+// with -ffp-contract=off and IEEE division/sqrt:
+//   triangle : Cramer's rule, main.cpp:147-243
+//   sphere   : local quadratic + world distance, main.cpp:64-146
+//   instance : motion shift, GetLocalRay (ray.h:24-66) with the renormalized
+//              direction, world t as a length ratio, main.cpp:244-281
+// Traversal is this project's own: a 4-wide BVH walked front to back with
+// distance pruning and an LDS stack (the reference visits every box a ray
+// crosses, main.cpp:282-319).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_scene.h"
+
+namespace az {
+
+constexpr float kFltMax = 3.402823466e+38f;   // common.h:39 f32_Max
+constexpr float kBaryLower = -0.000001f;      // main.cpp:176
+
+struct Ray3 {
+    float ox, oy, oz;
+    float dx, dy, dz;
+    float time;
+};
+
+// Result of a closest-hit walk.
+struct HitRec {
+    uint32_t prim;      // primitive id or kEmpty
+    uint32_t item;      // top-level item index (TLAS leaf order)
+    float t;            // world t
+    float tl;           // t along the local (renormalized) ray of the item
+    float beta, gamma;
+};
+
+__device__ inline float len3(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
+
+// GetLocalRay (ray.h:24-66) on a ray whose origin is already motion-shifted.
+__device__ inline void local_ray(const Affine &m, float ox, float oy, float oz, float dx, float dy,
+                                 float dz, float &lox, float &loy, float &loz, float &ldx, float &ldy,
+                                 float &ldz, float &dirScale) {
+    lox = m.m[0][0] * ox + m.m[0][1] * oy + m.m[0][2] * oz + m.m[0][3];
+    loy = m.m[1][0] * ox + m.m[1][1] * oy + m.m[1][2] * oz + m.m[1][3];
+    loz = m.m[2][0] * ox + m.m[2][1] * oy + m.m[2][2] * oz + m.m[2][3];
+    float x = m.m[0][0] * dx + m.m[0][1] * dy + m.m[0][2] * dz;
+    float y = m.m[1][0] * dx + m.m[1][1] * dy + m.m[1][2] * dz;
+    float z = m.m[2][0] * dx + m.m[2][1] * dy + m.m[2][2] * dz;
+    float l = len3(x, y, z);
+    dirScale = l;
+    if (l != 0.0f) {
+        ldx = x / l;
+        ldy = y / l;
+        ldz = z / l;
+    } else {
+        ldx = ldy = ldz = 0.0f;
+    }
+}
+
+// PointToWorld (math.cpp:1774-1800).
+__device__ inline void point_to_world(const Affine &m, float x, float y, float z, float &wx, float &wy,
+                                      float &wz) {
+    wx = m.m[0][0] * x + m.m[0][1] * y + m.m[0][2] * z + m.m[0][3];
+    wy = m.m[1][0] * x + m.m[1][1] * y + m.m[1][2] * z + m.m[1][3];
+    wz = m.m[2][0] * x + m.m[2][1] * y + m.m[2][2] * z + m.m[2][3];
+}
+
+// World t of a local hit point, main.cpp:258-262 / 100-103:
+// Length(WorldPosition - Origin - Motion) / Length(Direction) with
+// WorldPosition = PointToWorld(P) + Motion and Origin already shifted.
+__device__ inline float world_t(const Affine &fwd, float px, float py, float pz, float mx, float my,
+                                float mz, float sox, float soy, float soz, float dirLen) {
+    float wx, wy, wz;
+    point_to_world(fwd, px, py, pz, wx, wy, wz);
+    wx = wx + mx;
+    wy = wy + my;
+    wz = wz + mz;
+    float ex = (wx - sox) - mx, ey = (wy - soy) - my, ez = (wz - soz) - mz;
+    return len3(ex, ey, ez) / dirLen;
+}
+
+// Triangle test of main.cpp:154-189 against the ray (o, d).  Accepts when the
+// barycentrics pass the reference bounds and eps < t < tBest.
+__device__ inline bool tri_test(const TriPacked &tp, float ox, float oy, float oz, float dx, float dy,
+                                float dz, float eps, float tBest, float &t, float &beta, float &gamma) {
+    const float A = tp.a, B = tp.b, C = dx;
+    const float D = tp.d, E = tp.e, F = dy;
+    const float G = tp.g, H = tp.h, I = dz;
+    float detM = (A * E * I) + (D * H * C) + (G * B * F) - (C * E * G) - (F * H * A) - (I * B * D);
+    float K = (tp.p0x - ox), M = (tp.p0y - oy), N = (tp.p0z - oz);
+    float detB = (K * E * I) + (M * H * C) + (N * B * F) - (C * E * N) - (F * H * K) - (I * B * M);
+    beta = detB / detM;
+    if (!(beta > kBaryLower && beta <= 1.0f)) return false;
+    float detG = (A * M * I) + (D * N * C) + (G * K * F) - (C * M * G) - (F * N * A) - (I * K * D);
+    gamma = detG / detM;
+    if (!(gamma > kBaryLower && gamma <= 1.0f)) return false;
+    float detT = (A * E * N) + (D * H * K) + (G * B * M) - (K * E * G) - (M * H * A) - (N * B * D);
+    float alpha = 1.0f - beta - gamma;
+    if (!(alpha > kBaryLower)) return false;
+    t = detT / detM;
+    return t > eps && t < tBest;
+}
+
+// Sphere test of main.cpp:64-103 on an item; returns the world t (and the
+// local t) when the reference would report a hit.
+__device__ inline bool sphere_test(const TopItem &it, const Ray3 &r, float eps, float dirLen, float &tw,
+                                   float &tl) {
+    float mx = r.time * it.motion[0], my = r.time * it.motion[1], mz = r.time * it.motion[2];
+    float sox = r.ox - mx, soy = r.oy - my, soz = r.oz - mz;
+    float lox, loy, loz, ldx, ldy, ldz, sc;
+    local_ray(it.inv, sox, soy, soz, r.dx, r.dy, r.dz, lox, loy, loz, ldx, ldy, ldz, sc);
+    float distx = lox - it.center[0], disty = loy - it.center[1], distz = loz - it.center[2];
+    float A = ldx * ldx + ldy * ldy + ldz * ldz;
+    float B = distx * ldx + disty * ldy + distz * ldz;
+    float C = (distx * distx + disty * disty + distz * distz) - it.radius * it.radius;
+    float disc = (B * B) - (A * C);
+    if (!(disc > 0.0f)) return false;
+    float sd = sqrtf(disc);
+    float T1 = (-B + sd) / (A), T2 = (-B - sd) / (A);
+    float t = kFltMax;
+    if (T1 > eps && T2 > eps) t = T1 < T2 ? T1 : T2;
+    else if (T1 > eps) t = T1;
+    else if (T2 > eps) t = T2;
+    if (!(t > eps && t < kFltMax)) return false;
+    float hx = lox + (ldx * t), hy = loy + (ldy * t), hz = loz + (ldz * t);
+    float w = world_t(it.fwd, hx, hy, hz, mx, my, mz, sox, soy, soz, dirLen);
+    if (!(w > eps && w < kFltMax)) return false;
+    tw = w;
+    tl = t;
+    return true;
+}
+
+// Slab test of one node against (o, 1/d), children hit within [0, tclip].
+// Near/far planes are chosen by the sign of 1/d as in RayAABBIntersection
+// (ray.h:78-102); empty children (lo=+inf, hi=-inf) never hit.
+struct NodeHits {
+    float tn[4];
+    uint32_t ref[4];
+};
+
+__device__ inline int node_test(const Node4 *__restrict__ nodes, uint32_t idx, float ox, float oy,
+                                float oz, float ix, float iy, float iz, float tclip, NodeHits &h) {
+    const float4 *p = reinterpret_cast<const float4 *>(nodes + idx);
+    float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
+    uint4 ch = reinterpret_cast<const uint4 *>(p)[6];
+    const bool sx = ix >= 0.0f, sy = iy >= 0.0f, sz = iz >= 0.0f;
+    float4 nx = sx ? lx : hx, fx = sx ? hx : lx;
+    float4 ny = sy ? ly : hy, fy = sy ? hy : ly;
+    float4 nz = sz ? lz : hz, fz = sz ? hz : lz;
+    float tn[4], tf[4];
+#define AZ_SLAB(k, c)                                                                   \
+    {                                                                                   \
+        float a0 = (nx.c - ox) * ix, a1 = (fx.c - ox) * ix;                             \
+        float b0 = (ny.c - oy) * iy, b1 = (fy.c - oy) * iy;                             \
+        float c0 = (nz.c - oz) * iz, c1 = (fz.c - oz) * iz;                             \
+        tn[k] = fmaxf(fmaxf(a0, b0), fmaxf(c0, 0.0f));                                  \
+        tf[k] = fminf(fminf(a1, b1), fminf(c1, tclip));                                 \
+    }
+    AZ_SLAB(0, x)
+    AZ_SLAB(1, y)
+    AZ_SLAB(2, z)
+    AZ_SLAB(3, w)
+#undef AZ_SLAB
+    uint32_t cr[4] = {ch.x, ch.y, ch.z, ch.w};
+    int n = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        bool hit = tn[k] <= tf[k] && cr[k] != kEmpty;
+        h.tn[k] = hit ? tn[k] : INFINITY;
+        h.ref[k] = cr[k];
+        n += hit ? 1 : 0;
+    }
+    // Sort the four (distance, ref) pairs ascending; misses sort last.
+#define AZ_CSWAP(i, j)                                                                  \
+    if (h.tn[j] < h.tn[i]) {                                                            \
+        float tt = h.tn[i];                                                             \
+        h.tn[i] = h.tn[j];                                                              \
+        h.tn[j] = tt;                                                                   \
+        uint32_t rr = h.ref[i];                                                         \
+        h.ref[i] = h.ref[j];                                                            \
+        h.ref[j] = rr;                                                                  \
+    }
+    AZ_CSWAP(0, 1)
+    AZ_CSWAP(2, 3)
+    AZ_CSWAP(0, 2)
+    AZ_CSWAP(1, 3)
+    AZ_CSWAP(1, 2)
+#undef AZ_CSWAP
+    return n;
+}
+
+// Per-lane traversal stack in LDS, laid out [depth][lane] so that lanes at
+// the same depth touch consecutive words.
+template <int DEPTH, int BLOCK>
+struct LdsStack {
+    uint32_t *base;   // &smem[0][lane]
+    __device__ inline void push(int &sp, uint32_t v) {
+        base[sp * BLOCK] = v;
+        ++sp;
+    }
+    __device__ inline uint32_t pop(int &sp) {
+        --sp;
+        return base[sp * BLOCK];
+    }
+};
+
+struct TraceCounters {
+    uint32_t nodes;
+    uint32_t tris;
+    uint32_t prims;
+};
+
+// Two-level walk.  ANY: stop at the first primitive with eps < t < tMax
+// (world t); otherwise the closest hit with eps < t (tMax ignored).
+template <bool ANY, bool COUNT, int DEPTH, int BLOCK>
+__device__ inline bool trace_ray(const DeviceGeometry &G, const Ray3 &r, float tMax, float eps, bool cull,
+                                 LdsStack<DEPTH, BLOCK> stack, HitRec &out, TraceCounters &cnt) {
+    const float dirLen = len3(r.dx, r.dy, r.dz);
+    float bestW = ANY ? tMax : kFltMax;
+    bool found = false;
+    out.prim = kEmpty;
+    out.item = kEmpty;
+    out.t = 0.0f;
+    out.tl = 0.0f;
+    out.beta = 0.0f;
+    out.gamma = 0.0f;
+
+    // Current walk ray: world ray in the TLAS, instance-local ray in a BLAS.
+    float ox = r.ox, oy = r.oy, oz = r.oz;
+    float dx = r.dx, dy = r.dy, dz = r.dz;
+    float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+    // Instance state while inside a BLAS.
+    bool inBlas = false;
+    uint32_t curItem = 0;
+    float sox = 0.f, soy = 0.f, soz = 0.f, mx = 0.f, my = 0.f, mz = 0.f, dscale = 1.0f;
+    float localBest = kFltMax;      // closest local t inside the current BLAS
+    uint32_t localPrim = kEmpty;
+    float localBeta = 0.f, localGamma = 0.f;
+
+    int sp = 0;
+    uint32_t cur = G.tlasRoot;
+    for (;;) {
+        if (cur == kEmpty) {
+            // Leaving a BLAS: convert its closest local hit (main.cpp:256-276).
+            if (!ANY && localPrim != kEmpty) {
+                const TopItem &it = G.items[curItem];
+                float px = ox + (dx * localBest), py = oy + (dy * localBest), pz = oz + (dz * localBest);
+                float w = world_t(it.fwd, px, py, pz, mx, my, mz, sox, soy, soz, dirLen);
+                if (w > eps && w < bestW) {
+                    bestW = w;
+                    found = true;
+                    out.prim = localPrim;
+                    out.item = curItem;
+                    out.t = w;
+                    out.tl = localBest;
+                    out.beta = localBeta;
+                    out.gamma = localGamma;
+                }
+            }
+            inBlas = false;
+            ox = r.ox;
+            oy = r.oy;
+            oz = r.oz;
+            dx = r.dx;
+            dy = r.dy;
+            dz = r.dz;
+            ix = 1.0f / dx;
+            iy = 1.0f / dy;
+            iz = 1.0f / dz;
+        } else if (!ref_is_leaf(cur)) {
+            if (COUNT) cnt.nodes++;
+            float tclip = inBlas ? localBest : bestW;
+            NodeHits h;
+            int n = node_test(G.nodes, cur, ox, oy, oz, ix, iy, iz, tclip, h);
+            if (n > 0) {
+                cur = h.ref[0];
+                if (n > 3) stack.push(sp, h.ref[3]);
+                if (n > 2) stack.push(sp, h.ref[2]);
+                if (n > 1) stack.push(sp, h.ref[1]);
+                continue;
+            }
+        } else if (!inBlas) {
+            // TLAS leaves hold exactly one item.
+            uint32_t itemIdx = leaf_first(cur);
+            const TopItem &it = G.items[itemIdx];
+            if (COUNT) cnt.prims++;
+            if (it.kind == kItemSphere) {
+                float tw, tl;
+                if (sphere_test(it, r, eps, dirLen, tw, tl) && tw < bestW) {
+                    bestW = tw;
+                    found = true;
+                    out.prim = G.triTotal + it.entityId;
+                    out.item = itemIdx;
+                    out.t = tw;
+                    out.tl = tl;
+                    out.beta = 0.0f;
+                    out.gamma = 0.0f;
+                    if (ANY) return true;
+                }
+            } else {
+                mx = r.time * it.motion[0];
+                my = r.time * it.motion[1];
+                mz = r.time * it.motion[2];
+                sox = r.ox - mx;
+                soy = r.oy - my;
+                soz = r.oz - mz;
+                local_ray(it.inv, sox, soy, soz, r.dx, r.dy, r.dz, ox, oy, oz, dx, dy, dz, dscale);
+                ix = 1.0f / dx;
+                iy = 1.0f / dy;
+                iz = 1.0f / dz;
+                inBlas = true;
+                curItem = itemIdx;
+                // Local distances scale with |M^-1 d|; the bound is widened so
+                // that rounding never prunes a hit that could still win.
+                localBest = bestW >= kFltMax ? kFltMax : bestW * dscale * 1.00001f + 1e-30f;
+                localPrim = kEmpty;
+                stack.push(sp, kEmpty);
+                cur = it.blasRoot;
+                continue;
+            }
+        } else {
+            const uint32_t first = leaf_first(cur), count = leaf_count(cur);
+            for (uint32_t k = 0; k < count; ++k) {
+                const TriPacked tp = G.tris[first + k];
+                if (COUNT) cnt.tris++;
+                if (cull) {
+                    const float *fn = G.faceNormals + 3 * (size_t)(first + k);
+                    if (!(dx * fn[0] + dy * fn[1] + dz * fn[2] < 0.0f)) continue;
+                }
+                float t, b, g;
+                if (tri_test(tp, ox, oy, oz, dx, dy, dz, eps, localBest, t, b, g)) {
+                    if (ANY) {
+                        const TopItem &it = G.items[curItem];
+                        float px = ox + (dx * t), py = oy + (dy * t), pz = oz + (dz * t);
+                        float w = world_t(it.fwd, px, py, pz, mx, my, mz, sox, soy, soz, dirLen);
+                        if (w > eps && w < bestW) {
+                            out.prim = tp.prim;
+                            out.item = curItem;
+                            out.t = w;
+                            out.tl = t;
+                            out.beta = b;
+                            out.gamma = g;
+                            return true;
+                        }
+                    } else {
+                        localBest = t;
+                        localPrim = tp.prim;
+                        localBeta = b;
+                        localGamma = g;
+                    }
+                }
+            }
+        }
+        if (sp == 0) break;
+        cur = stack.pop(sp);
+    }
+    return found;
+}
+
+}  // namespace az

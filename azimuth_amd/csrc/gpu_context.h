@@ -1,0 +1,91 @@
+// gpu_context.h -- state behind an az_gpu_ctx.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "azimuth_gpu.h"
+#include "device_render.h"
+#include "device_scene.h"
+#include "kernels.h"
+
+namespace az {
+
+// Owning device allocation.
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t alloc(size_t n) {
+        if (n <= bytes && ptr) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc(&ptr, n ? n : 1);
+        if (e == hipSuccess) bytes = n;
+        else ptr = nullptr;
+        return e;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    ~DeviceBuffer() { release(); }
+};
+
+struct GeometryBuffers {
+    DeviceBuffer nodes, tris, faceNormals, items;
+    void free() {
+        nodes.release();
+        tris.release();
+        faceNormals.release();
+        items.release();
+    }
+};
+
+// What the geometry upload learned, for the shading upload.
+struct UploadInfo {
+    std::vector<uint32_t> meshPrimBase, meshBlasRoot, meshTriBase;
+    std::vector<std::vector<const entity *>> meshTriEntities;   // per mesh, in primitive-id order
+    std::vector<uint32_t> leafOrder;       // per device triangle slot: index within its mesh
+    std::vector<const entity *> topEntities;   // per top-level item (device order)
+    uint32_t triTotal = 0;
+    uint32_t maxBlasStack = 0, maxTlasStack = 0, stackNeeded = 0;
+    uint32_t leafCount = 0;
+};
+
+}  // namespace az
+
+struct az_gpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, tev0 = nullptr, tev1 = nullptr;
+    hipEvent_t lastStart = nullptr, lastStop = nullptr;   // not owned
+    std::string lastError;
+    bool hasScene = false;
+    bool counting = false;
+    bool pendingTiming = false;
+    uint32_t maxLeafItems = 4;
+    epsilon_values eps{0.0f, 0.000275f};
+    az::GeometryBuffers geom;
+    az::DeviceGeometry dg{};
+    std::vector<az::TopItem> hostItems;
+    az::TraceLaunch launch;
+    az::DeviceBuffer counters;
+    az::RenderState render;
+    az_stats stats{};
+    bool recordLaunches = false;
+    std::vector<hipEvent_t> launchEvents;   // pairs: start, stop
+    size_t launchUsed = 0;                  // pairs in use since the last collect
+    ~az_gpu_ctx() {
+        for (hipEvent_t e : launchEvents) (void)hipEventDestroy(e);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (tev0) (void)hipEventDestroy(tev0);
+        if (tev1) (void)hipEventDestroy(tev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, az::UploadInfo &info);
+az_status az_upload_shading(az_gpu_ctx *C, const scene *S, const az::UploadInfo &info);

@@ -1,0 +1,343 @@
+"""Host-side objects over the C ABI: a built `scene` and a GPU context.
+
+`SceneBuilder` drives az_sb_* (the importer post-process and the reference
+BVH build, include/azimuth_scene.h); `GpuContext` drives az_gpu_* (upload,
+batch closest/any hit, render).  Errors raise `native.AzError`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import native as N
+
+
+def _vec3(v):
+    return N.vec3(*[float(a) for a in v])
+
+
+def make_material(kd=(0.5, 0.5, 0.5), ks=(0, 0, 0), ka=(0, 0, 0), mirror=(0, 0, 0), transparency=(0, 0, 0),
+                  ior=1.0, exponent=1.0, brdf_type=N.BRDF_NormalizedModifiedBlinnPhong, brdf_ior=0.0,
+                  glossiness=0.0):
+    m = N.material()
+    m.BRDF.Type = brdf_type
+    m.BRDF.Exponent = float(exponent)
+    m.BRDF.RefractiveIndex = float(brdf_ior)
+    m.DiffuseReflectance = _vec3(kd)
+    m.SpecularReflectance = _vec3(ks)
+    m.AmbientReflectance = _vec3(ka)
+    m.MirrorReflectance = _vec3(mirror)
+    m.Transparency = _vec3(transparency)
+    m.RefractionIndex = float(ior)
+    m.Glossiness = float(glossiness)
+    return m
+
+
+def point_light(position, intensity):
+    L = N.light()
+    L.Type = N.LightType_PointLight
+    L.u.area.Position = _vec3(position)
+    L.u.area.Intensity = _vec3(intensity)
+    return L
+
+
+def area_light(position, intensity, edge0, edge1, normal):
+    L = N.light()
+    L.Type = N.LightType_AreaLight
+    L.u.area.Position = _vec3(position)
+    L.u.area.Intensity = _vec3(intensity)
+    L.u.area.EdgeVectors[0] = _vec3(edge0)
+    L.u.area.EdgeVectors[1] = _vec3(edge1)
+    L.u.area.Normal = _vec3(normal)
+    return L
+
+
+def directional_light(direction, intensity):
+    L = N.light()
+    L.Type = N.LightType_DirectionalLight
+    L.u.area.Position = _vec3(direction)       # Direction aliases Position (light.h:22-25)
+    L.u.area.Intensity = _vec3(intensity)
+    return L
+
+
+def spot_light(position, intensity, direction, coverage_deg, falloff_deg):
+    """Spot light with the cosines the importer derives (half angles)."""
+    L = N.light()
+    L.Type = N.LightType_SpotLight
+    s = L.u.spot
+    s.Position = _vec3(position)
+    s.Intensity = _vec3(intensity)
+    s.SpotDirection = _vec3(direction)
+    s.CoverageAngle = float(coverage_deg)
+    s.FallOffAngle = float(falloff_deg)
+    s.CosHalfCoverage = float(np.float32(np.cos(np.float32(coverage_deg / 2.0 / 180.0) * np.float32(np.pi))))
+    s.CosHalfFallOff = float(np.float32(np.cos(np.float32(falloff_deg / 2.0 / 180.0) * np.float32(np.pi))))
+    return L
+
+
+def image_texture(pixels_rgb, normalizer=255.0, wrap=N.TextureWrapMode_Repeat, filt=N.TextureFilter_Nearest,
+                  blend=N.TextureBlendMode_ReplaceDiffuse, bump=False, bump_multiplier=1.0):
+    """RGB texture from an (H, W, 3) float32 or uint8 array.  The array must
+    stay alive until the builder has copied it (az_sb_add_texture)."""
+    arr = np.ascontiguousarray(pixels_rgb)
+    t = N.texture()
+    t.Type = N.TextureType_2D
+    t.u.img.Pixels = arr.ctypes.data
+    t.u.img.PixelFormat = N.PixelFormat_RGB_F32 if arr.dtype == np.float32 else N.PixelFormat_RGB_U8
+    t.u.img.Height, t.u.img.Width = int(arr.shape[0]), int(arr.shape[1])
+    t.u.img.InvWidth = float(np.float32(1.0) / np.float32(arr.shape[1]))
+    t.u.img.InvHeight = float(np.float32(1.0) / np.float32(arr.shape[0]))
+    t.u.img.Normalizer = float(normalizer)
+    t.u.img.Loaded = 1
+    t.WrapMode, t.Filter, t.BlendMode = wrap, filt, blend
+    t.BumpMap = 1 if bump else 0
+    t.BumpMapMultiplier = float(bump_multiplier)
+    return t, arr
+
+
+def perlin_texture(seed, scaling=1.0, appearance=N.TexturePerlinAppearance_Patch, bump=False,
+                   bump_multiplier=1.0, blend=N.TextureBlendMode_BlendDiffuse):
+    """Perlin texture whose permutation is a seeded Fisher-Yates shuffle."""
+    t = N.texture()
+    t.Type = N.TextureType_PerlinNoise
+    perm = np.random.default_rng(seed).permutation(256).astype(np.uint32)
+    for i in range(256):
+        t.u.perlin.Perm[i] = int(perm[i])
+    t.u.perlin.ScalingFactor = float(scaling)
+    t.u.perlin.PerlinAppearance = appearance
+    t.BumpMap = 1 if bump else 0
+    t.BumpMapMultiplier = float(bump_multiplier)
+    t.BlendMode = blend
+    return t
+
+
+class Scene:
+    """An owned `scene*` produced by az_sb_build."""
+
+    def __init__(self, ptr, meta=None):
+        self.ptr = ptr
+        self.meta = meta or {}
+
+    @property
+    def triangle_count(self):
+        return N.load().az_scene_triangle_count(self.ptr)
+
+    def close(self):
+        if self.ptr:
+            N.load().az_scene_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SceneBuilder:
+    def __init__(self, max_depth=1, path_trace=False, importance_sampling=False, ambient=(0, 0, 0),
+                 background=(0, 0, 0)):
+        self.lib = N.load()
+        self.h = C.c_void_p()
+        N.check(self.lib.az_sb_create(C.byref(self.h)), "az_sb_create")
+        p = N.az_scene_params()
+        p.AmbientLight = _vec3(ambient)
+        p.BackgroundColor = _vec3(background)
+        p.MaxRecursionDepth = int(max_depth)
+        p.PathTrace = 1 if path_trace else 0
+        p.ImportanceSampling = 1 if importance_sampling else 0
+        N.check(self.lib.az_sb_set_params(self.h, C.byref(p)), "az_sb_set_params")
+        self._keep = []
+
+    def add_material(self, m):
+        idx = C.c_uint32()
+        N.check(self.lib.az_sb_add_material(self.h, C.byref(m), C.byref(idx)), "az_sb_add_material")
+        return idx.value
+
+    def add_texture(self, t):
+        idx = C.c_uint32()
+        N.check(self.lib.az_sb_add_texture(self.h, C.byref(t), C.byref(idx)), "az_sb_add_texture")
+        return idx.value
+
+    def set_environment_map(self, t):
+        N.check(self.lib.az_sb_set_environment_map(self.h, C.byref(t)), "az_sb_set_environment_map")
+
+    def add_light(self, L):
+        N.check(self.lib.az_sb_add_light(self.h, C.byref(L)), "az_sb_add_light")
+
+    def add_mesh(self, vertices, faces, material=0, transform=None, motion=(0, 0, 0), emitter=False,
+                 radiance=(0, 0, 0), smooth=False, uvs=None, texture=0):
+        v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+        f = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1, 3)
+        d = N.az_mesh_desc()
+        d.Vertices = v.ctypes.data
+        d.VertexCount = v.shape[0]
+        d.Faces = f.ctypes.data
+        d.FaceCount = f.shape[0]
+        if uvs is not None:
+            uv = np.ascontiguousarray(uvs, dtype=np.float32).reshape(-1, 2)
+            d.UVs = uv.ctypes.data
+            d.UVCount = uv.shape[0]
+            self._keep.append(uv)
+        d.Transformation = N.mat4.from_numpy(np.eye(4) if transform is None else transform)
+        d.MotionVector = _vec3(motion)
+        d.Radiance = _vec3(radiance)
+        d.IsEmitter = 1 if emitter else 0
+        d.TextureIndex = int(texture)
+        d.MaterialIndex = int(material)
+        d.ShadingMode = N.ShadingMode_Smooth if smooth else N.ShadingMode_Flat
+        N.check(self.lib.az_sb_add_mesh(self.h, C.byref(d)), "az_sb_add_mesh")
+
+    def add_sphere(self, center, radius, material=0, transform=None, motion=(0, 0, 0), emitter=False,
+                   radiance=(0, 0, 0), texture=0):
+        d = N.az_sphere_desc()
+        d.Center = _vec3(center)
+        d.Radius = float(radius)
+        d.Transformation = N.mat4.from_numpy(np.eye(4) if transform is None else transform)
+        d.MotionVector = _vec3(motion)
+        d.Radiance = _vec3(radiance)
+        d.IsEmitter = 1 if emitter else 0
+        d.TextureIndex = int(texture)
+        d.MaterialIndex = int(material)
+        N.check(self.lib.az_sb_add_sphere(self.h, C.byref(d)), "az_sb_add_sphere")
+
+    def build(self, meta=None):
+        out = C.c_void_p()
+        N.check(self.lib.az_sb_build(self.h, C.byref(out)), "az_sb_build")
+        return Scene(out, meta)
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.lib.az_sb_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """Device allocation owned through a GpuContext."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        self.ptr = C.c_void_p()
+        N.check(ctx.lib.az_gpu_malloc(ctx.h, C.byref(self.ptr), self.nbytes), "az_gpu_malloc", ctx.h)
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        N.check(self.ctx.lib.az_gpu_upload(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes), "az_gpu_upload", self.ctx.h)
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        N.check(self.ctx.lib.az_gpu_download(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes), "az_gpu_download",
+                self.ctx.h)
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx.lib.az_gpu_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class GpuContext:
+    def __init__(self, device=0):
+        self.lib = N.load()
+        self.h = C.c_void_p()
+        N.check(self.lib.az_gpu_create(C.byref(self.h), int(device)), "az_gpu_create")
+
+    def upload_scene(self, scene, eps_intersection=0.0, eps_shadow=0.000275):
+        e = N.epsilon_values(float(eps_intersection), float(eps_shadow))
+        N.check(self.lib.az_gpu_upload_scene(self.h, scene.ptr, C.byref(e)), "az_gpu_upload_scene", self.h)
+
+    def alloc(self, nbytes):
+        return DeviceArray(self, nbytes)
+
+    def trace_closest_device(self, d_rays, n, d_hits, cull=False):
+        N.check(self.lib.az_gpu_trace_closest(self.h, d_rays.ptr, int(n), d_hits.ptr, 1 if cull else 0),
+                "az_gpu_trace_closest", self.h)
+
+    def trace_any_device(self, d_rays, n, d_hits):
+        N.check(self.lib.az_gpu_trace_any(self.h, d_rays.ptr, int(n), d_hits.ptr), "az_gpu_trace_any", self.h)
+
+    def trace(self, rays, any_hit=False, cull=False):
+        """Host convenience: uploads rays, traces, downloads hits."""
+        rays = np.ascontiguousarray(rays, dtype=N.RAY_DTYPE)
+        n = rays.shape[0]
+        dr = self.alloc(max(1, rays.nbytes))
+        dh = self.alloc(max(1, n * 16))
+        dr.upload(rays)
+        if any_hit:
+            self.trace_any_device(dr, n, dh)
+        else:
+            self.trace_closest_device(dr, n, dh, cull)
+        out = dh.download(N.HIT_DTYPE, n)
+        dr.free()
+        dh.free()
+        return out
+
+    def stats(self):
+        s = N.az_stats()
+        N.check(self.lib.az_gpu_get_stats(self.h, C.byref(s)), "az_gpu_get_stats", self.h)
+        return s
+
+    def set_counting(self, on):
+        N.check(self.lib.az_gpu_set_counting(self.h, 1 if on else 0), "az_gpu_set_counting", self.h)
+
+    def set_block(self, block):
+        N.check(self.lib.az_gpu_set_launch(self.h, int(block), 0), "az_gpu_set_launch", self.h)
+
+    def synchronize(self):
+        N.check(self.lib.az_gpu_synchronize(self.h), "az_gpu_synchronize", self.h)
+
+    def record_launch_times(self, on=True):
+        N.check(self.lib.az_gpu_record_launch_times(self.h, 1 if on else 0), "az_gpu_record_launch_times", self.h)
+
+    def collect_launch_times(self, capacity=65536):
+        buf = (C.c_double * capacity)()
+        n = C.c_uint32()
+        N.check(self.lib.az_gpu_collect_launch_times(self.h, buf, capacity, C.byref(n)),
+                "az_gpu_collect_launch_times", self.h)
+        return np.array(buf[:min(n.value, capacity)])
+
+    def timer_start(self):
+        N.check(self.lib.az_gpu_timer_start(self.h), "az_gpu_timer_start", self.h)
+
+    def timer_stop(self):
+        ms = C.c_double()
+        N.check(self.lib.az_gpu_timer_stop(self.h, C.byref(ms)), "az_gpu_timer_stop", self.h)
+        return ms.value
+
+    def render(self, camera, filt=N.FilterType_Box, cull=False, seed=0, spp=0, row_begin=0, row_end=0,
+               guide=False, guide_fraction=0.5):
+        o = N.az_render_opts()
+        o.Seed = int(seed)
+        o.SampleCountOverride = int(spp)
+        o.RowBegin, o.RowEnd = int(row_begin), int(row_end)
+        o.GuideEnable = 1 if guide else 0
+        o.GuideFraction = float(guide_fraction)
+        W, H = camera.Resolution.x, camera.Resolution.y
+        out = np.zeros(W * H, dtype=N.PIXEL_DTYPE)
+        N.check(self.lib.az_gpu_render(self.h, C.byref(camera), int(filt), 1 if cull else 0, C.byref(o),
+                                       out.ctypes.data), "az_gpu_render", self.h)
+        return out
+
+    def close(self):
+        if self.h:
+            self.lib.az_gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

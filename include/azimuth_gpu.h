@@ -1,0 +1,159 @@
+/*
+ * azimuth_gpu.h -- C ABI of the MI355X (gfx950) render hot path.
+ *
+ * The reference has no plugin or FFI layer: its render path is entered once
+ * per camera through
+ *     static b32 RenderScene(scene *Scene, camera *Camera,
+ *                            epsilon_values *Epsilons, u32 ThreadCount,
+ *                            b32 BackfaceCulling)        (main.cpp:1799-1901)
+ * which spawns ThreadCount workers running ProduceImageInLinearSpace over
+ * scanline bands (main.cpp:1256-1397) and then writes the image.  The entry
+ * points below replace the body of that call: the caller keeps its scene,
+ * camera and epsilon structures (azimuth_types.h), uploads the scene once and
+ * receives the same `pixel {Color, TotalWeight}` buffer (main.cpp:45-48) that
+ * the reference hands to OutputPixelsToFile.
+ *
+ * Conventions: every call returns an az_status and never aborts; the caller
+ * owns all structures passed in; one host thread drives a context; all device
+ * work of a context is ordered on the context's own HIP stream.  Pointers named
+ * d_* are device pointers (from az_gpu_malloc or any HIP allocation); all
+ * other pointers are host memory.
+ */
+#ifndef AZIMUTH_GPU_H
+#define AZIMUTH_GPU_H
+
+#include "azimuth_types.h"
+#include "azimuth_scene.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* A ray of a batch query: the reference ray3 {Origin, Direction, T}
+ * (ray.h:1-5) plus an upper distance bound used by the any-hit query.
+ * Direction need not be unit length. 32 bytes. */
+typedef struct {
+    f32 Origin[3];
+    f32 Time;           /* ray3.T: shutter time in [0,1) for motion blur */
+    f32 Direction[3];
+    f32 TMax;           /* any-hit: occluded iff a hit with t < TMax exists */
+} az_ray;
+
+/* Closest-hit record, 16 bytes.  Prim is the primitive id (below) or
+ * AZ_MISS; t is the world distance along Direction as the reference reports
+ * it (ray_hit.t, ray.h:7-13); Beta/Gamma are the barycentrics of P1 and P2
+ * computed by the reference's Cramer's-rule test (main.cpp:154-189), 0 for
+ * spheres.
+ *
+ * Primitive ids: the triangles of scene->Meshes[m] are numbered after those of
+ * meshes 0..m-1, in the order of that mesh's triangle entity array (the order
+ * its BVH leaves have, left to right); spheres follow all triangles as
+ * triangle_count + UniqueEntityID. */
+typedef struct {
+    u32 Prim;
+    f32 t;
+    f32 Beta;
+    f32 Gamma;
+} az_hit;
+
+#define AZ_MISS 0xFFFFFFFFu
+
+/* Per-render options the reference fixes elsewhere. */
+typedef struct {
+    u64 Seed;               /* per-pixel PCG32 streams are seeded from (Seed, pixel index) */
+    u32 SampleCountOverride;/* 0: camera->SampleCount; else used instead (still truncated to a square) */
+    u32 RowBegin, RowEnd;   /* scanline range to render; RowEnd 0 = all rows */
+    u32 AccumulateF64;      /* 1: accumulate in double (long reference renders) */
+    u32 GuideEnable;        /* 1: mix the learned guide with BSDF sampling */
+    f32 GuideFraction;      /* one-sample MIS selection probability of the guide */
+    u32 Reserved[3];
+} az_render_opts;
+
+typedef struct {
+    f64 LastKernelMs;       /* device time of the last trace/render call's kernels */
+    u64 NodesVisited;       /* summed over the last trace call (if counting enabled) */
+    u64 TrianglesTested;
+    u64 PrimitivesTested;
+    u32 NodeBytes;          /* bytes fetched per node visit */
+    u32 TriangleBytes;      /* bytes fetched per triangle test */
+    u32 BvhWidth;
+    u32 StackDepth;
+    u32 BlockSize;
+    u32 NodeCount;
+    u32 LeafCount;
+    u32 TriangleCount;
+    u32 SphereCount;
+    u32 InstanceCount;
+} az_stats;
+
+typedef struct az_gpu_ctx az_gpu_ctx;
+
+/* Context on HIP device `DeviceOrdinal`; creates the context's stream. */
+az_status az_gpu_create(az_gpu_ctx **Out, int DeviceOrdinal);
+void      az_gpu_destroy(az_gpu_ctx *Ctx);
+/* Text of the last error of this context (static storage, never NULL). */
+const char *az_gpu_last_error(const az_gpu_ctx *Ctx);
+
+/* Flattens the scene (entities, meshes, materials, lights, textures, env map)
+ * into device arrays and builds the device BVH.  Replaces any earlier upload.
+ * Reads Scene and Epsilons only during the call. */
+az_status az_gpu_upload_scene(az_gpu_ctx *Ctx, const scene *Scene, const epsilon_values *Epsilons);
+
+/* Batch closest hit for the reference query RayEntityIntersection(BVHRoot,
+ * Cull, Epsilons->Intersection) (main.cpp:53-385), N rays. */
+az_status az_gpu_trace_closest(az_gpu_ctx *Ctx, const az_ray *d_Rays, u32 N, az_hit *d_Hits,
+                               b32 BackfaceCulling);
+/* Batch occlusion in the form of the reference shadow query (cull off,
+ * epsilon 0, main.cpp:697-712): Prim of d_Hits[i] is AZ_MISS when no
+ * primitive is hit with t < TMax, else the id of one such primitive with its
+ * t and barycentrics. */
+az_status az_gpu_trace_any(az_gpu_ctx *Ctx, const az_ray *d_Rays, u32 N, az_hit *d_Hits);
+
+/* The render call: fills Pixels (host, Width*Height entries, or the rows of
+ * Opts->RowBegin..RowEnd) exactly as RenderScene's pixel buffer before
+ * OutputPixelsToFile (main.cpp:1814-1880).  Opts may be NULL. */
+az_status az_gpu_render(az_gpu_ctx *Ctx, const camera *Camera, filter_type Filter,
+                        b32 BackfaceCulling, const az_render_opts *Opts, pixel *Pixels);
+
+/* Device-resident variant: accumulates into d_Pixels (Width*Height pixel,
+ * zeroed by the caller) without a copy back; d_PixelsF64 when AccumulateF64. */
+az_status az_gpu_render_device(az_gpu_ctx *Ctx, const camera *Camera, filter_type Filter,
+                               b32 BackfaceCulling, const az_render_opts *Opts, void *d_Pixels);
+
+az_status az_gpu_get_stats(az_gpu_ctx *Ctx, az_stats *Stats);
+/* Enables per-ray node/triangle counting in trace calls (slower). */
+az_status az_gpu_set_counting(az_gpu_ctx *Ctx, b32 Enable);
+
+/* Launch parameters normally chosen by the autotuner (0 = keep current). */
+az_status az_gpu_set_launch(az_gpu_ctx *Ctx, u32 BlockSize, u32 Reserved);
+
+/* Device memory helpers, ordered on the context stream. */
+az_status az_gpu_malloc(az_gpu_ctx *Ctx, void **d_Ptr, u64 Bytes);
+az_status az_gpu_free(az_gpu_ctx *Ctx, void *d_Ptr);
+az_status az_gpu_upload(az_gpu_ctx *Ctx, void *d_Dst, const void *Src, u64 Bytes);
+az_status az_gpu_download(az_gpu_ctx *Ctx, void *Dst, const void *d_Src, u64 Bytes);
+az_status az_gpu_memset(az_gpu_ctx *Ctx, void *d_Dst, int Value, u64 Bytes);
+az_status az_gpu_synchronize(az_gpu_ctx *Ctx);
+
+/* Per-launch kernel timing: while enabled, every trace call records a HIP
+ * event pair around its kernel on the context stream.  Collect waits for the
+ * stream and returns (up to Capacity) the durations in milliseconds of the
+ * launches recorded since the previous collect, in launch order. */
+az_status az_gpu_record_launch_times(az_gpu_ctx *Ctx, b32 Enable);
+az_status az_gpu_collect_launch_times(az_gpu_ctx *Ctx, f64 *Milliseconds, u32 Capacity, u32 *Count);
+
+/* Event timing of an arbitrary sequence of calls on the context stream. */
+az_status az_gpu_timer_start(az_gpu_ctx *Ctx);
+az_status az_gpu_timer_stop(az_gpu_ctx *Ctx, f64 *Milliseconds);
+
+/* RenderScene-shaped entry: uploads Scene, renders Camera with the box
+ * filter (main.cpp:1847) and writes Pixels.  ThreadCount is accepted for
+ * signature compatibility and ignored. */
+b32 az_render_scene(scene *Scene, camera *Camera, epsilon_values *Epsilons, u32 ThreadCount,
+                    b32 BackfaceCulling, pixel *Pixels);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AZIMUTH_GPU_H */

@@ -1,0 +1,86 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the CPU oracle (liboracle.so),
+a restatement of the azimuth render hot path used to check the GPU library.
+Only tests, the smoke check and the benchmark's CPU-baseline leg import it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from azimuth_amd import native as N
+
+_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_DIR, "liboracle.so")
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built (make -C oracle)")
+        lib = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        lib.azo_trace_closest.argtypes = [P, P, C.c_uint32, P, C.c_uint32, C.c_float, C.c_uint32]
+        lib.azo_trace_any.argtypes = [P, P, C.c_uint32, P, C.c_uint32]
+        lib.azo_brute_closest.argtypes = [P, P, C.c_uint32, P, C.c_uint32, C.c_float, C.c_uint32]
+        lib.azo_render.argtypes = [P, C.POINTER(N.camera), C.POINTER(N.epsilon_values), C.c_int, C.c_uint32,
+                                   C.POINTER(N.az_render_opts), P, C.c_uint32]
+        lib.azo_render_f64.argtypes = lib.azo_render.argtypes
+        lib.azo_rng_draw.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+        lib.azo_rng_draw.restype = C.c_float
+        lib.azo_pcg32_at.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+        lib.azo_pcg32_at.restype = C.c_uint32
+        lib.azo_perlin_gradient.argtypes = [C.c_uint32, P]
+        lib.azo_perlin_sample.argtypes = [C.POINTER(N.texture), C.c_float, C.c_float, C.c_float]
+        lib.azo_perlin_sample.restype = C.c_float
+        lib.azo_evaluate_brdf.argtypes = [C.POINTER(N.brdf), P, P, P, C.c_float, P, P]
+        _lib = lib
+    return _lib
+
+
+def threads_default():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def trace(scene, rays, any_hit=False, cull=False, eps=0.0, threads=None, brute=False):
+    lib = load()
+    rays = np.ascontiguousarray(rays, dtype=N.RAY_DTYPE)
+    out = np.zeros(rays.shape[0], dtype=N.HIT_DTYPE)
+    th = threads or threads_default()
+    if brute:
+        lib.azo_brute_closest(scene.ptr, rays.ctypes.data, rays.shape[0], out.ctypes.data, 1 if cull else 0,
+                              float(eps), th)
+    elif any_hit:
+        lib.azo_trace_any(scene.ptr, rays.ctypes.data, rays.shape[0], out.ctypes.data, th)
+    else:
+        lib.azo_trace_closest(scene.ptr, rays.ctypes.data, rays.shape[0], out.ctypes.data, 1 if cull else 0,
+                              float(eps), th)
+    return out
+
+
+def render(scene, camera, filt=N.FilterType_Box, cull=False, seed=0, spp=0, row_begin=0, row_end=0,
+           eps=(0.0, 0.000275), threads=None, f64=False):
+    lib = load()
+    o = N.az_render_opts()
+    o.Seed = int(seed)
+    o.SampleCountOverride = int(spp)
+    o.RowBegin, o.RowEnd = int(row_begin), int(row_end)
+    e = N.epsilon_values(*eps)
+    W, H = camera.Resolution.x, camera.Resolution.y
+    th = threads or threads_default()
+    if f64:
+        out = np.zeros((W * H, 4), dtype=np.float64)
+        lib.azo_render_f64(scene.ptr, C.byref(camera), C.byref(e), int(filt), 1 if cull else 0, C.byref(o),
+                           out.ctypes.data, th)
+        return out
+    out = np.zeros(W * H, dtype=N.PIXEL_DTYPE)
+    lib.azo_render(scene.ptr, C.byref(camera), C.byref(e), int(filt), 1 if cull else 0, C.byref(o),
+                   out.ctypes.data, th)
+    return out
+
+
+def rng_draw(seed, pixel, sample, node, dim):
+    return load().azo_rng_draw(seed, pixel, sample, node, dim)
