@@ -25,6 +25,8 @@ constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kMaxLeafItems = 16;
 constexpr uint32_t kMaxLeafFirst = (1u << 27) - 1;
+// Traversal stack entries per lane kept in scratch below the LDS ring.
+constexpr int kSpill = 80;
 
 inline AZ_HD bool ref_is_leaf(uint32_t r) { return (r & kLeafBit) != 0u; }
 inline AZ_HD uint32_t leaf_first(uint32_t r) { return r & 0x07FFFFFFu; }

@@ -192,18 +192,30 @@ __device__ inline int node_test(const Node4 *__restrict__ nodes, uint32_t idx, f
     return n;
 }
 
-// Per-lane traversal stack in LDS, laid out [depth][lane] so that lanes at
-// the same depth touch consecutive words.
-template <int DEPTH, int BLOCK>
-struct LdsStack {
+// Per-lane traversal stack: the top RING entries live in LDS, laid out
+// [slot][lane] so that lanes at the same depth touch consecutive words; older
+// entries are spilled to a private (scratch) array.  Entry i of the logical
+// stack sits in ring slot i % RING while it is among the top RING entries,
+// else in spill[i - RING].  Capacity RING + SPILL covers the deepest stack
+// the uploaded BVH can need (checked at upload).
+
+
+template <int RING, int BLOCK>
+struct RingStack {
     uint32_t *base;   // &smem[0][lane]
+    uint32_t spill[kSpill];
     __device__ inline void push(int &sp, uint32_t v) {
-        base[sp * BLOCK] = v;
+        uint32_t *slot = base + (sp & (RING - 1)) * BLOCK;
+        if (sp >= RING) spill[sp - RING] = *slot;
+        *slot = v;
         ++sp;
     }
     __device__ inline uint32_t pop(int &sp) {
         --sp;
-        return base[sp * BLOCK];
+        uint32_t *slot = base + (sp & (RING - 1)) * BLOCK;
+        uint32_t v = *slot;
+        if (sp >= RING) *slot = spill[sp - RING];
+        return v;
     }
 };
 
@@ -215,9 +227,11 @@ struct TraceCounters {
 
 // Two-level walk.  ANY: stop at the first primitive with eps < t < tMax
 // (world t); otherwise the closest hit with eps < t (tMax ignored).
-template <bool ANY, bool COUNT, int DEPTH, int BLOCK>
+template <bool ANY, bool COUNT, int RING, int BLOCK>
 __device__ inline bool trace_ray(const DeviceGeometry &G, const Ray3 &r, float tMax, float eps, bool cull,
-                                 LdsStack<DEPTH, BLOCK> stack, HitRec &out, TraceCounters &cnt) {
+                                 uint32_t *ldsLane, HitRec &out, TraceCounters &cnt) {
+    RingStack<RING, BLOCK> stack;
+    stack.base = ldsLane;
     const float dirLen = len3(r.dx, r.dy, r.dz);
     float bestW = ANY ? tMax : kFltMax;
     bool found = false;

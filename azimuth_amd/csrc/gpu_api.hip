@@ -268,14 +268,12 @@ az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
     C->stats.NodeBytes = sizeof(Node4);
     C->stats.TriangleBytes = sizeof(TriPacked);
     C->stats.BvhWidth = 4;
-    // The traversal kernels come with 32 or 64 stack entries per lane.
-    if (info.stackNeeded <= 32) C->launch.depth = 32;
-    else if (info.stackNeeded <= 64) C->launch.depth = 64;
-    else {
-        C->lastError = "BVH needs a deeper traversal stack than 64 entries";
+    // Stack capacity of the kernels: the LDS ring plus the scratch spill.
+    if (info.stackNeeded > (uint32_t)(16 + kSpill)) {
+        C->lastError = "BVH needs a deeper traversal stack than the kernels hold";
         return AZ_ERR_UNSUPPORTED;
     }
-    C->stats.StackDepth = (u32)C->launch.depth;
+    C->stats.StackDepth = info.stackNeeded;
     C->stats.BlockSize = (u32)C->launch.block;
     return AZ_OK;
 }
@@ -393,13 +391,16 @@ az_status az_gpu_set_counting(az_gpu_ctx *C, b32 on) {
     return AZ_OK;
 }
 
-az_status az_gpu_set_launch(az_gpu_ctx *C, u32 block, u32) {
+az_status az_gpu_set_launch(az_gpu_ctx *C, u32 block, u32 ring, s32 waves) {
     if (!C) return AZ_ERR_INVALID_ARGUMENT;
-    if (block) {
-        if (!trace_variant_exists(C->launch.depth, (int)block)) return AZ_ERR_INVALID_ARGUMENT;
-        C->launch.block = (int)block;
-        C->stats.BlockSize = block;
-    }
+    int b = block ? (int)block : C->launch.block;
+    int r = ring ? (int)ring : C->launch.ring;
+    int w = waves >= 0 ? (int)waves : C->launch.waves;
+    if (!trace_variant_exists(r, b, w)) return AZ_ERR_INVALID_ARGUMENT;
+    C->launch.block = b;
+    C->launch.ring = r;
+    C->launch.waves = w;
+    C->stats.BlockSize = (u32)b;
     return AZ_OK;
 }
 
@@ -441,6 +442,12 @@ az_status az_gpu_memset(az_gpu_ctx *C, void *dst, int v, u64 bytes) {
 az_status az_gpu_synchronize(az_gpu_ctx *C) {
     if (!C) return AZ_ERR_INVALID_ARGUMENT;
     AZ_HIP(C, hipStreamSynchronize(C->stream));
+    return AZ_OK;
+}
+
+az_status az_gpu_device_copy(az_gpu_ctx *C, void *dst, const void *src, u64 bytes) {
+    if (!C || (bytes && (!dst || !src)) || (bytes % 16) != 0) return AZ_ERR_INVALID_ARGUMENT;
+    AZ_HIP(C, launch_copy16(src, dst, bytes, C->stream));
     return AZ_OK;
 }
 

@@ -8,17 +8,21 @@
 
 namespace az {
 
-// Stack depth per lane and workgroup size of the traversal kernels.  Depth
-// must cover the deepest stack the uploaded BVH can need.
+// LDS ring entries per lane and workgroup size of the traversal kernels.  The
+// ring plus the scratch spill (kSpill) must cover the deepest stack the
+// uploaded BVH can need.
 struct TraceLaunch {
-    int depth = 32;
-    int block = 128;
+    int ring = 16;
+    int block = 256;
+    int waves = 6;   // minimum waves per SIMD the register allocation targets (0: compiler's choice)
 };
 
-bool trace_variant_exists(int depth, int block);
+bool trace_variant_exists(int ring, int block, int waves);
 
 hipError_t launch_trace(const TraceLaunch &L, bool any, bool count, const DeviceGeometry &G, const az_ray *rays,
                         uint32_t n, az_hit *hits, float eps, int cull, unsigned long long *counters,
                         hipStream_t s);
+
+hipError_t launch_copy16(const void *src, void *dst, size_t bytes, hipStream_t s);
 
 }  // namespace az

@@ -1,7 +1,8 @@
 // trace.hip -- batch closest-hit and any-hit kernels over az_ray arrays.
 //
 // One ray per lane; rays and hits move as 32- and 16-byte records; the BVH is
-// walked with an LDS stack of DEPTH entries per lane.
+// walked with a per-lane ring stack of RING entries in LDS (older entries in
+// scratch).
 #include <hip/hip_runtime.h>
 
 #include "azimuth_gpu.h"
@@ -10,11 +11,13 @@
 
 namespace az {
 
-template <bool ANY, bool COUNT, int DEPTH, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_trace(DeviceGeometry G, const az_ray *__restrict__ rays, uint32_t n,
+// WAVES > 0 asks the compiler to fit WAVES waves per SIMD (register budget
+// 512/WAVES VGPRs); 0 leaves the allocation to the compiler.
+template <bool ANY, bool COUNT, int RING, int BLOCK, int WAVES>
+__global__ __launch_bounds__(BLOCK, (WAVES > 0 ? WAVES : 1)) void k_trace(DeviceGeometry G, const az_ray *__restrict__ rays, uint32_t n,
                                                 az_hit *__restrict__ hits, float eps, int cull,
                                                 unsigned long long *__restrict__ counters) {
-    __shared__ uint32_t smem[DEPTH * BLOCK];
+    __shared__ uint32_t smem[RING * BLOCK];
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
@@ -27,10 +30,10 @@ __global__ __launch_bounds__(BLOCK) void k_trace(DeviceGeometry G, const az_ray 
     r.dx = b.x;
     r.dy = b.y;
     r.dz = b.z;
-    LdsStack<DEPTH, BLOCK> st{smem + threadIdx.x};
     HitRec h;
     TraceCounters c{0u, 0u, 0u};
-    bool hit = trace_ray<ANY, COUNT, DEPTH, BLOCK>(G, r, b.w, ANY ? 0.0f : eps, ANY ? false : (cull != 0), st, h, c);
+    bool hit = trace_ray<ANY, COUNT, RING, BLOCK>(G, r, b.w, ANY ? 0.0f : eps, ANY ? false : (cull != 0),
+                                                  smem + threadIdx.x, h, c);
     uint4 o;
     o.x = hit ? h.prim : AZ_MISS;
     o.y = __float_as_uint(hit ? h.t : 0.0f);
@@ -44,27 +47,34 @@ __global__ __launch_bounds__(BLOCK) void k_trace(DeviceGeometry G, const az_ray 
     }
 }
 
-template <bool ANY, bool COUNT, int DEPTH, int BLOCK>
+template <bool ANY, bool COUNT, int RING, int BLOCK, int WAVES>
 static hipError_t launch_variant(const DeviceGeometry &G, const az_ray *rays, uint32_t n, az_hit *hits, float eps,
                                  int cull, unsigned long long *counters, hipStream_t s) {
     dim3 grid((n + BLOCK - 1) / BLOCK);
-    hipLaunchKernelGGL((k_trace<ANY, COUNT, DEPTH, BLOCK>), grid, dim3(BLOCK), 0, s, G, rays, n, hits, eps, cull,
-                       counters);
+    hipLaunchKernelGGL((k_trace<ANY, COUNT, RING, BLOCK, WAVES>), grid, dim3(BLOCK), 0, s, G, rays, n, hits, eps,
+                       cull, counters);
     return hipGetLastError();
 }
 
+// The compiled (ring, block, waves) variants.
+#define AZ_TRACE_VARIANTS(X) \
+    X(16, 64, 0)             \
+    X(16, 64, 6)             \
+    X(16, 64, 8)             \
+    X(16, 128, 0)            \
+    X(16, 128, 6)            \
+    X(16, 256, 0)            \
+    X(16, 256, 6)            \
+    X(32, 64, 0)             \
+    X(32, 128, 0)
+
 template <bool ANY, bool COUNT>
-static hipError_t launch_depth(const TraceLaunch &L, const DeviceGeometry &G, const az_ray *rays, uint32_t n,
-                               az_hit *hits, float eps, int cull, unsigned long long *counters, hipStream_t s) {
-#define AZ_CASE(D, B)                                                                           \
-    if (L.depth == D && L.block == B)                                                           \
-        return launch_variant<ANY, COUNT, D, B>(G, rays, n, hits, eps, cull, counters, s);
-    AZ_CASE(32, 64)
-    AZ_CASE(32, 128)
-    AZ_CASE(32, 256)
-    AZ_CASE(64, 64)
-    AZ_CASE(64, 128)
-    AZ_CASE(64, 256)
+static hipError_t launch_ring(const TraceLaunch &L, const DeviceGeometry &G, const az_ray *rays, uint32_t n,
+                              az_hit *hits, float eps, int cull, unsigned long long *counters, hipStream_t s) {
+#define AZ_CASE(R, B, W)                                                                        \
+    if (L.ring == R && L.block == B && L.waves == W)                                            \
+        return launch_variant<ANY, COUNT, R, B, W>(G, rays, n, hits, eps, cull, counters, s);
+    AZ_TRACE_VARIANTS(AZ_CASE)
 #undef AZ_CASE
     return hipErrorInvalidValue;
 }
@@ -74,17 +84,35 @@ hipError_t launch_trace(const TraceLaunch &L, bool any, bool count, const Device
                         hipStream_t s) {
     if (n == 0) return hipSuccess;
     if (any) {
-        return count ? launch_depth<true, true>(L, G, rays, n, hits, eps, cull, counters, s)
-                     : launch_depth<true, false>(L, G, rays, n, hits, eps, cull, counters, s);
+        return count ? launch_ring<true, true>(L, G, rays, n, hits, eps, cull, counters, s)
+                     : launch_ring<true, false>(L, G, rays, n, hits, eps, cull, counters, s);
     }
-    return count ? launch_depth<false, true>(L, G, rays, n, hits, eps, cull, counters, s)
-                 : launch_depth<false, false>(L, G, rays, n, hits, eps, cull, counters, s);
+    return count ? launch_ring<false, true>(L, G, rays, n, hits, eps, cull, counters, s)
+                 : launch_ring<false, false>(L, G, rays, n, hits, eps, cull, counters, s);
 }
 
-}  // namespace az
-
-namespace az {
-bool trace_variant_exists(int depth, int block) {
-    return (depth == 32 || depth == 64) && (block == 64 || block == 128 || block == 256);
+bool trace_variant_exists(int ring, int block, int waves) {
+#define AZ_CASE(R, B, W) \
+    if (ring == R && block == B && waves == W) return true;
+    AZ_TRACE_VARIANTS(AZ_CASE)
+#undef AZ_CASE
+    return false;
 }
+
+// Plain 16-byte-per-lane copy: the calibration kernel for HBM counters.
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy16(const void *src, void *dst, size_t bytes, hipStream_t s) {
+    size_t n = bytes / 16;
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)blocks), dim3(256), 0, s, (const uint4 *)src, (uint4 *)dst, n);
+    return hipGetLastError();
+}
+
 }  // namespace az

@@ -217,13 +217,14 @@ EXPORTS = {
     "az_gpu_render_device": (s32, [_P, C.POINTER(camera), s32, u32, C.POINTER(az_render_opts), _P]),
     "az_gpu_get_stats": (s32, [_P, C.POINTER(az_stats)]),
     "az_gpu_set_counting": (s32, [_P, u32]),
-    "az_gpu_set_launch": (s32, [_P, u32, u32]),
+    "az_gpu_set_launch": (s32, [_P, u32, u32, s32]),
     "az_gpu_malloc": (s32, [_P, C.POINTER(_P), u64]),
     "az_gpu_free": (s32, [_P, _P]),
     "az_gpu_upload": (s32, [_P, _P, _P, u64]),
     "az_gpu_download": (s32, [_P, _P, _P, u64]),
     "az_gpu_memset": (s32, [_P, _P, C.c_int, u64]),
     "az_gpu_synchronize": (s32, [_P]),
+    "az_gpu_device_copy": (s32, [_P, _P, _P, u64]),
     "az_gpu_record_launch_times": (s32, [_P, u32]),
     "az_gpu_collect_launch_times": (s32, [_P, C.POINTER(f64), u32, C.POINTER(u32)]),
     "az_gpu_timer_start": (s32, [_P]),

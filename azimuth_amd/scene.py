@@ -293,11 +293,14 @@ class GpuContext:
     def set_counting(self, on):
         N.check(self.lib.az_gpu_set_counting(self.h, 1 if on else 0), "az_gpu_set_counting", self.h)
 
-    def set_block(self, block):
-        N.check(self.lib.az_gpu_set_launch(self.h, int(block), 0), "az_gpu_set_launch", self.h)
+    def set_launch(self, block=0, ring=0, waves=-1):
+        N.check(self.lib.az_gpu_set_launch(self.h, int(block), int(ring), int(waves)), "az_gpu_set_launch", self.h)
 
     def synchronize(self):
         N.check(self.lib.az_gpu_synchronize(self.h), "az_gpu_synchronize", self.h)
+
+    def device_copy(self, d_dst, d_src, nbytes):
+        N.check(self.lib.az_gpu_device_copy(self.h, d_dst.ptr, d_src.ptr, int(nbytes)), "az_gpu_device_copy", self.h)
 
     def record_launch_times(self, on=True):
         N.check(self.lib.az_gpu_record_launch_times(self.h, 1 if on else 0), "az_gpu_record_launch_times", self.h)

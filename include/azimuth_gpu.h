@@ -124,8 +124,12 @@ az_status az_gpu_get_stats(az_gpu_ctx *Ctx, az_stats *Stats);
 /* Enables per-ray node/triangle counting in trace calls (slower). */
 az_status az_gpu_set_counting(az_gpu_ctx *Ctx, b32 Enable);
 
-/* Launch parameters normally chosen by the autotuner (0 = keep current). */
-az_status az_gpu_set_launch(az_gpu_ctx *Ctx, u32 BlockSize, u32 Reserved);
+/* Traversal launch parameters normally chosen by the autotuner: workgroup
+ * size (64, 128, 256), LDS stack entries per lane (16, 32) and the waves per
+ * SIMD the register allocation targets (0 = compiler's choice, 6, 8).  0 (-1
+ * for MinWaves) keeps the current value; combinations that were not compiled
+ * return AZ_ERR_INVALID_ARGUMENT. */
+az_status az_gpu_set_launch(az_gpu_ctx *Ctx, u32 BlockSize, u32 LdsStackEntries, s32 MinWaves);
 
 /* Device memory helpers, ordered on the context stream. */
 az_status az_gpu_malloc(az_gpu_ctx *Ctx, void **d_Ptr, u64 Bytes);
@@ -134,6 +138,9 @@ az_status az_gpu_upload(az_gpu_ctx *Ctx, void *d_Dst, const void *Src, u64 Bytes
 az_status az_gpu_download(az_gpu_ctx *Ctx, void *Dst, const void *d_Src, u64 Bytes);
 az_status az_gpu_memset(az_gpu_ctx *Ctx, void *d_Dst, int Value, u64 Bytes);
 az_status az_gpu_synchronize(az_gpu_ctx *Ctx);
+/* Copies Bytes (a multiple of 16) with a plain streaming kernel on the
+ * context stream: the reference point for HBM bandwidth and counters. */
+az_status az_gpu_device_copy(az_gpu_ctx *Ctx, void *d_Dst, const void *d_Src, u64 Bytes);
 
 /* Per-launch kernel timing: while enabled, every trace call records a HIP
  * event pair around its kernel on the context stream.  Collect waits for the
