@@ -191,3 +191,89 @@ def cornell_scene(width=256, height=256, spp=16, max_depth=8, light_mode="mesh",
     sc = b.build(meta={"name": "cornell"})
     cam = look_at_camera((0.5, 0.5, 2.0), (0.5, 0.5, 0.0), (0, 1, 0), width, height, 50.0, spp=spp)
     return sc, cam
+
+
+# ---------------------------------------------------------------------------
+# feature scene: every light type, every BRDF model, textures, bump maps,
+# glossy mirrors, dielectrics with absorption, motion blur and an
+# environment map, at a small resolution
+
+
+def procedural_envmap(width=256, height=128, sun_peak=5000.0, sun_sigma_deg=0.5, sun_dir=(0.3, 0.8, 0.5)):
+    """Sky gradient plus a Gaussian sun lobe, as an RGB F32 lat-long map in
+    the reference's (u, v) = ((pi - atan2(z, x)) / 2pi, acos(y) / pi)
+    parametrization; texel values are in 0..255 units (the reference divides
+    by Normalizer = 255)."""
+    v = (np.arange(height) + 0.5) / height
+    u = (np.arange(width) + 0.5) / width
+    theta = v * np.pi
+    phi = np.pi - u * 2 * np.pi
+    th, ph = np.meshgrid(theta, phi, indexing="ij")
+    d = np.stack([np.sin(th) * np.cos(ph), np.cos(th), np.sin(th) * np.sin(ph)], -1)
+    sky_t = np.clip(d[..., 1], 0, 1)[..., None]
+    sky = (1 - sky_t) * np.array([0.9, 0.85, 0.8]) + sky_t * np.array([0.35, 0.55, 0.95])
+    sky = sky * 0.6 * (d[..., 1:2] > -0.05) + (d[..., 1:2] <= -0.05) * np.array([0.15, 0.12, 0.1])
+    sdir = np.asarray(sun_dir, np.float64)
+    sdir /= np.linalg.norm(sdir)
+    ang = np.arccos(np.clip(d @ sdir, -1, 1))
+    sun = sun_peak * np.exp(-0.5 * (ang / np.radians(sun_sigma_deg)) ** 2)
+    img = (sky + sun[..., None] * np.array([1.0, 0.95, 0.85])) * 255.0
+    return img.astype(np.float32)
+
+
+def feature_scene(width=48, height=48, spp=4, max_depth=5, seed=3, env=True):
+    from .scene import directional_light, point_light
+    g = np.random.default_rng(seed)
+    b = SceneBuilder(max_depth=max_depth, path_trace=True, importance_sampling=True, ambient=(0.02, 0.02, 0.03),
+                     background=(0.1, 0.1, 0.2))
+    mats = []
+    for bt in range(7):
+        mats.append(b.add_material(make_material(kd=g.uniform(0.2, 0.8, 3), ks=g.uniform(0.1, 0.5, 3),
+                                                 ka=(0.2, 0.2, 0.2), exponent=float(g.uniform(5, 80)), brdf_type=bt,
+                                                 brdf_ior=1.5)))
+    glossy = b.add_material(make_material(kd=(0.1, 0.1, 0.1), mirror=(0.6, 0.6, 0.6), glossiness=0.1))
+    glass = b.add_material(make_material(kd=(0, 0, 0), transparency=(0.8, 0.9, 0.7), ior=1.45))
+    lamp = b.add_material(make_material(kd=(0, 0, 0)))
+    checker = (np.indices((16, 16)).sum(0) % 2).astype(np.float32)
+    img = np.stack([checker * 200 + 30, checker * 60 + 60, (1 - checker) * 180 + 20], -1).astype(np.uint8)
+    t2d, keep = image_texture(img, blend=N.TextureBlendMode_ReplaceDiffuse, filt=N.TextureFilter_Bilinear)
+    tex_img = b.add_texture(t2d)
+    tex_bump = b.add_texture(perlin_texture(seed + 1, scaling=3.0, bump=True, bump_multiplier=0.3,
+                                            blend=N.TextureBlendMode_BlendDiffuse))
+    tex_vein = b.add_texture(perlin_texture(seed + 2, scaling=5.0, appearance=N.TexturePerlinAppearance_Vein,
+                                            blend=N.TextureBlendMode_ReplaceAll))
+    # floor with uvs and the image texture
+    fv = np.array([(-4, 0, -4), (4, 0, -4), (4, 0, 4), (-4, 0, 4)], np.float32)
+    ff = np.array([[0, 2, 1], [0, 3, 2]], np.uint32)
+    uv = np.array([(0, 0), (2, 0), (2, 2), (0, 2)], np.float32)
+    b.add_mesh(fv, ff, material=mats[5], uvs=uv, texture=tex_img)
+    # spheres: one per BRDF, a glossy mirror, glass, bumped, veined, moving
+    for k in range(7):
+        ang = 2 * np.pi * k / 7
+        b.add_sphere((2.2 * np.cos(ang), 0.45, 2.2 * np.sin(ang)), 0.45, material=mats[k])
+    b.add_sphere((0, 0.6, 0), 0.6, material=glossy)
+    b.add_sphere((0.9, 0.35, 0.9), 0.35, material=glass)
+    b.add_sphere((-0.9, 0.35, -0.9), 0.35, material=mats[2], texture=tex_bump)
+    b.add_sphere((-0.9, 0.35, 0.9), 0.35, material=mats[4], texture=tex_vein)
+    b.add_sphere((0.9, 0.3, -0.9), 0.3, material=mats[6], motion=(0.4, 0.0, 0.0))
+    # a smooth-shaded mesh with a transform and motion
+    v, f = displaced_icosphere(2, seed)
+    T = np.eye(4)
+    T[:3, :3] *= 0.5
+    T[:3, 3] = (0.0, 1.6, 1.6)
+    b.add_mesh(v, f, material=mats[3], transform=T, motion=(0.0, 0.2, 0.0), smooth=True, texture=tex_bump)
+    # lights
+    b.add_light(point_light((2, 4, 2), (6, 6, 6)))
+    b.add_light(area_light((-1, 3.5, -1), (4, 4, 4), (0.5, 0, 0), (0, 0, 0.5), (0, -1, 0)))
+    b.add_light(directional_light((-0.3, -1, -0.2), (0.4, 0.4, 0.4)))
+    b.add_light(spot_light((0, 4, 0), (20, 20, 20), (0, -1, 0), 60.0, 40.0))
+    b.add_sphere((-2.5, 2.5, -2.5), 0.25, material=lamp, emitter=True, radiance=(8, 7, 6))
+    lv = np.array([(2.5, 3.0, -2.5), (3.0, 3.0, -2.5), (3.0, 3.0, -2.0), (2.5, 3.0, -2.0)], np.float32)
+    b.add_mesh(lv, np.array([[0, 1, 2], [0, 2, 3]], np.uint32), material=lamp, emitter=True, radiance=(6, 6, 8))
+    if env:
+        envimg = procedural_envmap(64, 32)
+        te, keep_env = image_texture(envimg, filt=N.TextureFilter_Nearest)
+        b.set_environment_map(te)
+    sc = b.build(meta={"name": "features"})
+    cam = look_at_camera((0.0, 3.0, 6.5), (0.0, 0.5, 0.0), (0, 1, 0), width, height, 45.0, spp=spp)
+    return sc, cam

@@ -76,7 +76,7 @@ struct alignas(16) TopItem {
     float motion[3];
     float radius;          // sphere only
     float center[3];       // sphere only (object space)
-    float pad;
+    uint32_t rank;         // position of the entity among the reference BVH's leaves (left to right)
 };
 
 struct DeviceGeometry {

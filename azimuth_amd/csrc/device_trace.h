@@ -83,9 +83,10 @@ __device__ inline float world_t(const Affine &fwd, float px, float py, float pz,
 }
 
 // Triangle test of main.cpp:154-189 against the ray (o, d).  Accepts when the
-// barycentrics pass the reference bounds and eps < t < tBest.
+// barycentrics pass the reference bounds and eps < t; the caller compares t
+// with its current best.
 __device__ inline bool tri_test(const TriPacked &tp, float ox, float oy, float oz, float dx, float dy,
-                                float dz, float eps, float tBest, float &t, float &beta, float &gamma) {
+                                float dz, float eps, float &t, float &beta, float &gamma) {
     const float A = tp.a, B = tp.b, C = dx;
     const float D = tp.d, E = tp.e, F = dy;
     const float G = tp.g, H = tp.h, I = dz;
@@ -101,7 +102,15 @@ __device__ inline bool tri_test(const TriPacked &tp, float ox, float oy, float o
     float alpha = 1.0f - beta - gamma;
     if (!(alpha > kBaryLower)) return false;
     t = detT / detM;
-    return t > eps && t < tBest;
+    return t > eps;
+}
+
+// The reference picks the closest hit by a recursive minimum that keeps the
+// right child on equal t (main.cpp:303-307), so among exactly equal t the
+// right-most leaf of its BVH wins.  Ranks order the leaves left to right:
+// the primitive id inside a mesh, the entity's leaf position at the top level.
+__device__ inline bool closer(float t, uint32_t rank, float bestT, uint32_t bestRank) {
+    return t < bestT || (t == bestT && rank > bestRank);
 }
 
 // Sphere test of main.cpp:64-103 on an item; returns the world t (and the
@@ -234,6 +243,7 @@ __device__ inline bool trace_ray(const DeviceGeometry &G, const Ray3 &r, float t
     stack.base = ldsLane;
     const float dirLen = len3(r.dx, r.dy, r.dz);
     float bestW = ANY ? tMax : kFltMax;
+    uint32_t bestRank = ANY ? kEmpty : 0u;   // ANY: a hit must be strictly below tMax
     bool found = false;
     out.prim = kEmpty;
     out.item = kEmpty;
@@ -263,8 +273,9 @@ __device__ inline bool trace_ray(const DeviceGeometry &G, const Ray3 &r, float t
                 const TopItem &it = G.items[curItem];
                 float px = ox + (dx * localBest), py = oy + (dy * localBest), pz = oz + (dz * localBest);
                 float w = world_t(it.fwd, px, py, pz, mx, my, mz, sox, soy, soz, dirLen);
-                if (w > eps && w < bestW) {
+                if (w > eps && closer(w, it.rank, bestW, bestRank)) {
                     bestW = w;
+                    bestRank = it.rank;
                     found = true;
                     out.prim = localPrim;
                     out.item = curItem;
@@ -303,8 +314,9 @@ __device__ inline bool trace_ray(const DeviceGeometry &G, const Ray3 &r, float t
             if (COUNT) cnt.prims++;
             if (it.kind == kItemSphere) {
                 float tw, tl;
-                if (sphere_test(it, r, eps, dirLen, tw, tl) && tw < bestW) {
+                if (sphere_test(it, r, eps, dirLen, tw, tl) && closer(tw, it.rank, bestW, bestRank)) {
                     bestW = tw;
+                    bestRank = it.rank;
                     found = true;
                     out.prim = G.triTotal + it.entityId;
                     out.item = itemIdx;
@@ -345,7 +357,8 @@ __device__ inline bool trace_ray(const DeviceGeometry &G, const Ray3 &r, float t
                     if (!(dx * fn[0] + dy * fn[1] + dz * fn[2] < 0.0f)) continue;
                 }
                 float t, b, g;
-                if (tri_test(tp, ox, oy, oz, dx, dy, dz, eps, localBest, t, b, g)) {
+                if (tri_test(tp, ox, oy, oz, dx, dy, dz, eps, t, b, g) &&
+                    (ANY ? t < localBest : closer(t, tp.prim, localBest, localPrim))) {
                     if (ANY) {
                         const TopItem &it = G.items[curItem];
                         float px = ox + (dx * t), py = oy + (dy * t), pz = oz + (dz * t);

@@ -178,6 +178,7 @@ az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
         TopItem it;
         memset(&it, 0, sizeof(it));
         it.entityId = e->UniqueEntityID;
+        it.rank = (uint32_t)items0.size();   // `tops` lists the reference leaves left to right
         it.inv = toAffine(e->InvTransformationMatrix);
         it.fwd = toAffine(e->TransformationMatrix);
         it.motion[0] = e->MotionVector.x;

@@ -56,6 +56,8 @@ struct UploadInfo {
 
 }  // namespace az
 
+void az_release_render_buffers(void *buffers);
+
 struct az_gpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -73,11 +75,14 @@ struct az_gpu_ctx {
     az::TraceLaunch launch;
     az::DeviceBuffer counters;
     az::RenderState render;
+    az::DevScene devScene{};
+    void *renderBuffers = nullptr;   // owned; released by az_release_render_buffers
     az_stats stats{};
     bool recordLaunches = false;
     std::vector<hipEvent_t> launchEvents;   // pairs: start, stop
     size_t launchUsed = 0;                  // pairs in use since the last collect
     ~az_gpu_ctx() {
+        az_release_render_buffers(renderBuffers);
         for (hipEvent_t e : launchEvents) (void)hipEventDestroy(e);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);

@@ -1,28 +1,1176 @@
 // render.hip -- the wavefront path tracer behind az_gpu_render.
+//
+// The reference renders each sample with a depth-first recursion that splits
+// at every vertex into NEE shadow rays, one hemisphere scatter ray, a mirror
+// ray and, at dielectrics, a refraction and a reflection ray
+// (TraceRay, main.cpp:966-1133).  Here that tree is expanded breadth first:
+// every level of the tree is one queue of PathRay records in HBM, and each
+// level runs three kernels:
+//   extend  : closest hit of every queued ray (device_trace.h)
+//   shade   : hit reconstruction, emission/ambient, NEE shadow-ray records,
+//             children for the next level with their throughput weights
+//   connect : occlusion of the shadow rays; unoccluded ones add their
+//             contribution to their sample
+// followed, per chunk of samples, by a splat into the pixel buffer (box or
+// Gaussian, main.cpp:1209-1254).  Random numbers are addressed by
+// (seed, pixel, sample, tree node, draw), so the breadth-first order draws the
+// same numbers as the depth-first CPU oracle.
 #include <hip/hip_runtime.h>
 
-#include "azimuth_gpu.h"
-#include "gpu_context.h"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
 
-az_status az_upload_shading(az_gpu_ctx *C, const scene *S, const az::UploadInfo &info) {
-    (void)S;
-    (void)info;
-    C->render.ready = false;
+#include "azimuth_gpu.h"
+#include "device_trace.h"
+#include "gpu_context.h"
+#include "shade_math.h"
+
+namespace az {
+
+constexpr int kRenderBlock = 256;
+constexpr int kRenderRing = 16;
+
+// One slot per lane that wants one: wave-aggregated append to a counter.
+__device__ inline uint32_t wave_append(uint32_t *counter, bool want) {
+    uint64_t mask = __ballot(want);
+    if (!want) return 0xFFFFFFFFu;
+    uint32_t lo = (uint32_t)mask, hi = (uint32_t)(mask >> 32);
+    uint32_t below = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+    int leader = __builtin_ctzll(mask);
+    uint32_t base = 0;
+    if (below == 0) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + below;
+}
+
+__device__ inline void add3(float *a, V3 c) {
+    atomicAdd(a + 0, c.x);
+    atomicAdd(a + 1, c.y);
+    atomicAdd(a + 2, c.z);
+}
+
+// ---- generate: camera rays (main.cpp:1135-1178) ----------------------------
+__global__ __launch_bounds__(kRenderBlock) void k_generate(DevScene S, CameraSetup cam, ChunkParams cp,
+                                                           PathRay *__restrict__ q, float *__restrict__ acc,
+                                                           uint32_t n) {
+    uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t pixLocal = i / cp.spp, sl = i % cp.spp;
+    uint32_t row = cp.row0 + pixLocal / cam.width, col = pixLocal % cam.width;
+    uint32_t pixel = row * cam.width + col;
+    uint32_t s = cp.s0 + sl;
+    uint32_t sx = s % cam.sppSqrt, sy = s / cam.sppSqrt;
+    float offx = 0.5f * cam.invSppSqrt + sx * cam.invSppSqrt;
+    float offy = 0.5f * cam.invSppSqrt + sy * cam.invSppSqrt;
+    float scx = col + offx, scy = row + offy;
+    Rng rng = make_rng(S.pcgTable, cp.seed, pixel, s, 0u);
+    float lx = rng.uniform() - 0.5f;
+    float ly = rng.uniform() - 0.5f;
+    lx *= cam.aperture;
+    ly *= cam.aperture;
+    V3 ax = v3(cam.ax), ay = v3(cam.ay);
+    V3 lox = ax * lx, loy = ay * ly;
+    V3 dir = normalize(v3(cam.base) + scx * v3(cam.stepX) - scy * v3(cam.stepY) - lox - loy);
+    V3 org = v3(cam.pos) + lox + loy;
+    float time = rng.uniform();
+    PathRay r;
+    r.ox = org.x;
+    r.oy = org.y;
+    r.oz = org.z;
+    r.time = time;
+    r.dx = dir.x;
+    r.dy = dir.y;
+    r.dz = dir.z;
+    r.node = child_node(0u, 0u);
+    r.wx = r.wy = r.wz = 1.0f;
+    r.sample = i;
+    r.cx = r.cy = r.cz = 0.0f;
+    r.flags = 0u;
+    q[i] = r;
+    acc[3 * i + 0] = 0.0f;
+    acc[3 * i + 1] = 0.0f;
+    acc[3 * i + 2] = 0.0f;
+}
+
+// ---- extend: closest hit of every queued ray ---------------------------------
+__global__ __launch_bounds__(kRenderBlock, 6) void k_extend(DeviceGeometry G, const PathRay *__restrict__ q,
+                                                           uint32_t n, PathHit *__restrict__ hits, float eps,
+                                                           int cull) {
+    __shared__ uint32_t smem[kRenderRing * kRenderBlock];
+    uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
+    if (i >= n) return;
+    const PathRay &pr = q[i];
+    Ray3 r;
+    r.ox = pr.ox;
+    r.oy = pr.oy;
+    r.oz = pr.oz;
+    r.time = pr.time;
+    r.dx = pr.dx;
+    r.dy = pr.dy;
+    r.dz = pr.dz;
+    HitRec h;
+    TraceCounters c{0u, 0u, 0u};
+    bool hit = trace_ray<false, false, kRenderRing, kRenderBlock>(G, r, 0.0f, eps, cull != 0, smem + threadIdx.x, h, c);
+    PathHit o;
+    o.prim = hit ? h.prim : kEmpty;
+    o.item = h.item;
+    o.t = h.t;
+    o.tl = h.tl;
+    o.beta = h.beta;
+    o.gamma = h.gamma;
+    o.pad0 = 0.0f;
+    o.pad1 = 0.0f;
+    hits[i] = o;
+}
+
+// ---- connect: shadow rays ------------------------------------------------------
+__global__ __launch_bounds__(kRenderBlock, 6) void k_connect(DeviceGeometry G, const ShadowRay *__restrict__ sq,
+                                                            uint32_t n, float *__restrict__ acc) {
+    __shared__ uint32_t smem[kRenderRing * kRenderBlock];
+    uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
+    if (i >= n) return;
+    const ShadowRay &s = sq[i];
+    Ray3 r;
+    r.ox = s.ox;
+    r.oy = s.oy;
+    r.oz = s.oz;
+    r.time = s.time;
+    r.dx = s.dx;
+    r.dy = s.dy;
+    r.dz = s.dz;
+    HitRec h;
+    TraceCounters c{0u, 0u, 0u};
+    bool reaches;
+    if (s.mode == kShadowAny) {
+        // Occluded by any hit with 0 < t < tmax (main.cpp:697-712, 751-752).
+        reaches = !trace_ray<true, false, kRenderRing, kRenderBlock>(G, r, s.tmax, 0.0f, false, smem + threadIdx.x, h, c);
+    } else {
+        // Entity lights: the closest hit must be the light itself (main.cpp:704-708).
+        bool hit = trace_ray<false, false, kRenderRing, kRenderBlock>(G, r, 0.0f, 0.0f, false, smem + threadIdx.x, h, c);
+        reaches = !hit || G.items[h.item].entityId == s.entityId;
+    }
+    if (reaches) add3(acc + 3 * s.sample, v3(s.cx, s.cy, s.cz));
+}
+
+// ---- shade ------------------------------------------------------------------------
+struct ShadeOut {
+    PathRay *next;
+    ShadowRay *shadow;
+    uint32_t *counters;   // [0] next queue, [1] shadow queue
+    uint32_t capacity, shadowCapacity;
+};
+
+__device__ inline void emit_child(const ShadeOut &O, bool want, const PathRay &c) {
+    uint32_t idx = wave_append(O.counters + 0, want);
+    if (want && idx < O.capacity) O.next[idx] = c;
+}
+
+__device__ inline void emit_shadow(const ShadeOut &O, bool want, const ShadowRay &s) {
+    uint32_t idx = wave_append(O.counters + 1, want);
+    if (want && idx < O.shadowCapacity) O.shadow[idx] = s;
+}
+
+__device__ inline ShadowRay make_shadow(V3 o, V3 d, float time, float tmax, V3 c, uint32_t sample, uint32_t mode,
+                                        uint32_t entityId) {
+    ShadowRay s;
+    s.ox = o.x;
+    s.oy = o.y;
+    s.oz = o.z;
+    s.time = time;
+    s.dx = d.x;
+    s.dy = d.y;
+    s.dz = d.z;
+    s.tmax = tmax;
+    s.cx = c.x;
+    s.cy = c.y;
+    s.cz = c.z;
+    s.sample = sample;
+    s.mode = mode;
+    s.entityId = entityId;
+    s.pad0 = 0u;
+    s.pad1 = 0u;
+    return s;
+}
+
+__device__ inline PathRay make_child(V3 o, V3 d, float time, uint32_t node, V3 w, uint32_t sample, V3 coef,
+                                     uint32_t depth, bool inside) {
+    PathRay r;
+    r.ox = o.x;
+    r.oy = o.y;
+    r.oz = o.z;
+    r.time = time;
+    r.dx = d.x;
+    r.dy = d.y;
+    r.dz = d.z;
+    r.node = node;
+    r.wx = w.x;
+    r.wy = w.y;
+    r.wz = w.z;
+    r.sample = sample;
+    r.cx = coef.x;
+    r.cy = coef.y;
+    r.cz = coef.z;
+    r.flags = (depth & 0xFFu) | (inside ? 0x100u : 0u);
+    return r;
+}
+
+// Bump mapping of main.cpp:323-377 on the hit normal N.
+__device__ inline V3 bump(const DevScene &S, const DevTexture &T, V3 N, V3 P, float u, float v, V3 Pu, V3 Pv) {
+    if (T.type == 0) {
+        V3 a = sample_texture_2d(S, T, u, v, false);
+        V3 b = sample_texture_2d(S, T, u + T.invW, v, false);
+        V3 c = sample_texture_2d(S, T, u, v + T.invH, false);
+        V3 du = b - a, dv = c - a;
+        float Du = ((du.x + du.y + du.z) / 3.0f) * T.bumpMul;
+        float Dv = ((dv.x + dv.y + dv.z) / 3.0f) * T.bumpMul;
+        V3 dPu = Pu + Du * N;
+        V3 dPv = Pv + Dv * N;
+        return normalize(cross(dPv, dPu));
+    }
+    const float e = 0.001f;
+    float D = sample_perlin(S, T, P);
+    float Dx = sample_perlin(S, T, v3(P.x + e, P.y, P.z));
+    float Dy = sample_perlin(S, T, v3(P.x, P.y + e, P.z));
+    float Dz = sample_perlin(S, T, v3(P.x, P.y, P.z + e));
+    V3 g = v3((Dx - D) / e, (Dy - D) / e, (Dz - D) / e);
+    N = N - g * T.bumpMul;
+    return normalize(N);
+}
+
+__global__ __launch_bounds__(kRenderBlock) void k_shade(DevScene S, DeviceGeometry G, CameraSetup cam, ChunkParams cp,
+                                                        const PathRay *__restrict__ q,
+                                                        const PathHit *__restrict__ hits, uint32_t n,
+                                                        float *__restrict__ acc, ShadeOut O) {
+    uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
+    if (i >= n) return;
+    const PathRay r = q[i];
+    const PathHit h = hits[i];
+    const uint32_t depth = r.flags & 0xFFu;
+    const bool inside = (r.flags & 0x100u) != 0u;
+    const uint32_t pixLocal = r.sample / cp.spp;
+    const uint32_t pixel = cp.row0 * cam.width + pixLocal;
+    const uint32_t s = cp.s0 + r.sample % cp.spp;
+    float *a = acc + 3 * r.sample;
+    V3 w = v3(r.wx, r.wy, r.wz);
+    V3 D = v3(r.dx, r.dy, r.dz);
+
+    if (h.prim == kEmpty) {
+        if (depth == 0u) {
+            // Camera-ray miss: SampleBackgroundColor (main.cpp:1180-1207).
+            V3 bg;
+            if (S.useEnv) {
+                float th = Acos(D.y);
+                float ph = atan2f(D.z, D.x);
+                float u = (-ph + kPiF) / (2.0f * kPiF), v = th / kPiF;
+                bg = sample_texture(S, S.env, v3(u, v, 1.0f), u, v);
+            } else if (S.useBackgroundTexture) {
+                uint32_t row = pixel / cam.width, col = pixel % cam.width;
+                float u = (float)col / (float)((int32_t)cam.width - 1), v = (float)row / (float)((int32_t)cam.height - 1);
+                bg = sample_texture(S, S.background, v3(u, v, 1.0f), u, v);
+            } else {
+                bg = v3(S.backgroundColor);
+            }
+            add3(a, bg);
+        }
+        return;
+    }
+    // Beer attenuation over this ray's segment (main.cpp:916-928, 951-958).
+    if (r.cx != 0.0f || r.cy != 0.0f || r.cz != 0.0f)
+        w = had(w, v3(Exp(r.cx * h.t), Exp(r.cy * h.t), Exp(r.cz * h.t)));
+
+    const TopItem &it = G.items[h.item];
+    const DevItemShade sh = S.itemShade[h.item];
+    if (sh.isEmitter) {                                   // main.cpp:984-985
+        add3(a, had(w, v3(sh.radiance)));
+        return;
+    }
+
+    // ---- hit reconstruction (main.cpp:64-146, 147-243, 244-281, 323-382) --------
+    float mx = r.time * it.motion[0], my = r.time * it.motion[1], mz = r.time * it.motion[2];
+    float sox = r.ox - mx, soy = r.oy - my, soz = r.oz - mz;
+    float lox, loy, loz, ldx, ldy, ldz, sc;
+    local_ray(it.inv, sox, soy, soz, r.dx, r.dy, r.dz, lox, loy, loz, ldx, ldy, ldz, sc);
+    const V3 motion = v3(mx, my, mz);
+    V3 P, N;
+    float u, v;
+    const bool textured = sh.texture != 0u;
+    const DevTexture *tex = textured ? &S.textures[sh.texture] : nullptr;
+    if (sh.kind == kItemSphere) {
+        V3 hp = v3(lox + (ldx * h.tl), loy + (ldy * h.tl), loz + (ldz * h.tl));
+        V3 lhp = hp - v3(it.center);
+        V3 hn = normalize(lhp);
+        P = point_to_world(hp, it.fwd) + motion;
+        float theta = Acos(hn.y);
+        float phi = atan2f(hn.z, hn.x);
+        float twoPi = kPiF * 2.0f;
+        u = (-phi + kPiF) / twoPi;
+        v = theta / kPiF;
+        if (textured) {
+            N = hn;
+            if (tex->bump) {
+                float pyPi = kPiF * lhp.y, rPi = kPiF * it.radius;
+                V3 Pv = v3(cosf(phi) * pyPi, -sinf(theta) * rPi, sinf(phi) * pyPi);
+                V3 Pu = v3(twoPi * lhp.z, 0.0f, -twoPi * lhp.x);
+                N = bump(S, *tex, N, P, u, v, Pu, Pv);
+            }
+            N = normal_to_world(N, it.inv);
+        } else {
+            N = normal_to_world(hn, it.inv);
+        }
+    } else {
+        const DevPrimAttr &pa = S.prims[h.prim];
+        V3 Pl = v3(lox + (ldx * h.tl), loy + (ldy * h.tl), loz + (ldz * h.tl));
+        float beta = h.beta, gamma = h.gamma;
+        float alpha = 1.0f - beta - gamma;
+        V3 Nl = pa.smooth ? alpha * v3(pa.n0) + beta * v3(pa.n1) + gamma * v3(pa.n2) : v3(pa.fn);
+        u = (alpha * pa.uv0[0]) + (beta * pa.uv1[0]) + (gamma * pa.uv2[0]);
+        v = (alpha * pa.uv0[1]) + (beta * pa.uv1[1]) + (gamma * pa.uv2[1]);
+        if (textured && tex->bump) {
+            float u0 = pa.uv0[0], v0 = pa.uv0[1], u1 = pa.uv1[0], v1 = pa.uv1[1], u2 = pa.uv2[0], v2 = pa.uv2[1];
+            V3 P0 = v3(pa.p0), P1 = v3(pa.p1), P2 = v3(pa.p2);
+            float div = ((u1 - u0) * (v2 - v0)) - ((v1 - v0) * (u2 - u0));
+            div = 1.0f / div;
+            V3 Pu = (v2 - v0) * (P1 - P0) + (v0 - v1) * (P2 - P0);
+            Pu = Pu * div;
+            V3 Pv = (u0 - u2) * (P1 - P0) + (u1 - u0) * (P2 - P0);
+            Pv = Pv * div;
+            Nl = bump(S, *tex, Nl, Pl, u, v, Pu, Pv);
+        }
+        P = point_to_world(Pl, it.fwd) + motion;
+        N = normal_to_world(Nl, it.inv);
+    }
+
+    // ---- the vertex (main.cpp:986-1128) ------------------------------------------------
+    const DevMaterial M = S.materials[sh.material];
+    V3 kd = v3(M.kd);
+    bool ignoreAmbient = false;
+    if (textured) {
+        V3 ts = sample_texture(S, *tex, P, u, v);
+        if (tex->blend == 0) {            // ReplaceAll: DC starts at 1, no ambient
+            kd = ts;
+            add3(a, had(w, ts));
+            ignoreAmbient = true;
+        } else if (tex->blend == 1) {
+            kd = ts;
+        } else {
+            kd = kd + ts;
+            kd = kd * 0.5f;
+        }
+    }
+    Rng rng = make_rng(S.pcgTable, cp.seed, pixel, s, r.node);
+    const float eps = S.epsShadow;
+
+    // Lights (main.cpp:609-717).
+    for (uint32_t li = 0; li < S.lightCount; ++li) {
+        const DevLight &L = S.lights[li];
+        V3 Ld = v3(0.0f, 0.0f, 0.0f), Li = v3(L.intensity);
+        float falloff = 1.0f, ldist = 1.0f, invPdf = 1.0f, tmax = 1.0f - eps;
+        uint32_t mode = kShadowAny;
+        if (L.type == kLightPoint) {
+            Ld = v3(L.pos) - P;
+            ldist = length(Ld);
+        } else if (L.type == kLightArea) {
+            float r0 = rng.uniform();
+            float r1 = rng.uniform();
+            V3 lp = v3(L.pos) + v3(L.e0) * r0 + v3(L.e1) * r1;
+            falloff = dot(normalize(lp - P), v3(L.normal));
+            if (falloff < 0.0f) falloff *= -1.0f;
+            Ld = lp - P;
+            ldist = length(Ld);
+        } else if (L.type == kLightDirectional) {
+            Ld = normalize(-v3(L.dir));
+            tmax = INFINITY;
+        } else if (L.type == kLightSpot) {
+            Ld = v3(L.pos) - P;
+            ldist = length(Ld);
+            float ct = dot(normalize(-Ld), normalize(v3(L.dir)));
+            if (ct < L.cosCov) falloff = 0.0f;
+            else if (ct > L.cosFall) falloff = 1.0f;
+            else {
+                falloff = (ct - L.cosCov) / (L.cosFall - L.cosCov);
+                falloff = powf(falloff, 4.0f);
+                if (falloff < 0.0f) falloff = 0.0f;
+                else if (falloff > 1.0f) falloff = 1.0f;
+            }
+        } else {
+            // SampleEntityLight (main.cpp:392-459).
+            V3 hpl = point_to_world(P, L.inv);
+            V3 dl;
+            if (L.entityKind == (int32_t)kItemSphere) {
+                V3 Dv = v3(L.center) - hpl;
+                float rod = L.radius / length(Dv);
+                float cosMax = sqrtf(1.0f - sq(rod));
+                float cosT = 1.0f + rng.uniform() * (cosMax - 1.0f);
+                float sinT = sqrtf(1.0f - sq(cosT));
+                float phi = 2.0f * kPiF * rng.uniform();
+                float cp_ = cosf(phi), sp_ = sinf(phi);
+                V3 W = normalize(Dv), U, Vv;
+                orthonormal_basis(W, U, Vv);
+                dl = normalize(U * sinT * cp_ + Vv * sinT * sp_ + W * cosT);
+                invPdf = 2.0f * kPiF * (1.0f - cosMax);
+            } else {
+                uint32_t idx = (uint32_t)(rng.uniform() * (L.triCount));
+                idx = idx < L.triCount - 1u ? idx : L.triCount - 1u;
+                const DevLightTri &T = S.lightTris[L.triOffset + idx];
+                float r1 = sqrtf(rng.uniform());
+                float r2 = rng.uniform();
+                V3 Q = (1.0f - r2) * v3(T.p0) + r2 * v3(T.p1);
+                V3 Pp = (1.0f - r1) * v3(T.p2) + r1 * Q;
+                V3 wi = Pp - hpl;
+                dl = normalize(wi);
+                float ca = dot(dl, v3(T.n));
+                if (ca < 0.0f) ca *= -1.0f;
+                invPdf = (L.totalArea * ca) / sq(length(wi));
+            }
+            Ld = vector_to_world(dl, L.fwd);
+            Li = v3(L.radiance);
+            mode = kShadowEntity;
+        }
+        if (ldist == 0.0f) ldist = 0.001f;
+        V3 c;
+        bool want = !inside && light_term(M, kd, Li, Ld, D, N, ldist, falloff, invPdf, c);
+        if (want) c = had(w, c);
+        emit_shadow(O, want, make_shadow(P + (Ld * eps), Ld, r.time, tmax, c, r.sample, mode, L.entityId));
+    }
+    // Environment NEE, cosine-distributed (main.cpp:719-778).
+    if (S.useEnv) {
+        float phi = 2.0f * kPiF * rng.uniform();
+        float cp_ = cosf(phi), sp_ = sinf(phi);
+        float theta = Asin(sqrtf(rng.uniform()));
+        float ct = cosf(theta), st = sinf(theta);
+        V3 U, Vv;
+        orthonormal_basis(N, U, Vv);
+        V3 L = normalize(U * st * cp_ + Vv * st * sp_ + N * ct);
+        float eu = (-atan2f(L.z, L.x) + kPiF) / (2.0f * kPiF);
+        float ev = Acos(L.y) / kPiF;
+        V3 I = sample_texture(S, S.env, v3(eu, ev, 1.0f), eu, ev);
+        float ca = dot(L, N);
+        if (ca == 0.0f) ca = 0.000001f;
+        float invPdf = kPiF / ca;
+        V3 c;
+        bool want = light_term(M, kd, I, L, D, N, 1.0f, 1.0f, invPdf, c);
+        if (want) c = had(w, c);
+        emit_shadow(O, want, make_shadow(P + (L * eps), L, r.time, INFINITY, c, r.sample, kShadowAny, 0u));
+    }
+    const bool childOk = depth + 1u <= S.maxDepth;
+    const V3 zero = v3(0.0f, 0.0f, 0.0f);
+    // Hemisphere scatter (main.cpp:1024-1083).
+    if (S.pathTrace) {
+        float theta = S.importance ? Asin(sqrtf(rng.uniform())) : Acos(rng.uniform());
+        float ct = cosf(theta), st = sinf(theta);
+        float phi = 2.0f * kPiF * rng.uniform();
+        float cp_ = cosf(phi), sp_ = sinf(phi);
+        V3 U, Vv;
+        orthonormal_basis(N, U, Vv);
+        V3 Sd = normalize(U * st * cp_ + Vv * st * sp_ + N * ct);
+        float invPdf;
+        if (S.importance) {
+            float cs = dot(Sd, N);
+            if (cs == 0.0f) cs = 0.000001f;
+            invPdf = kPiF / cs;
+        } else {
+            invPdf = kPiF * 2.0f;
+        }
+        V3 cw;
+        bool want = childOk && scatter_weight(M, kd, Sd, D, N, invPdf, cw);
+        if (want) cw = had(w, cw);
+        emit_child(O, want, make_child(P + (Sd * eps), Sd, r.time, child_node(r.node, 0u), cw, r.sample, zero,
+                                       depth + 1u, inside));
+    }
+    // Ambient (main.cpp:1088-1099).
+    const bool transparent = (M.flags & kMatTransparent) != 0u;
+    if ((!inside || !transparent) && !ignoreAmbient) add3(a, had(w, had(v3(M.ka), v3(S.ambient))));
+    // Mirror (main.cpp:781-854).
+    if (M.flags & kMatMirror) {
+        V3 R = reflect(-D, N);
+        if (M.gloss != 0.0f) {
+            V3 rp = v3(fabsf(R.x), fabsf(R.y), fabsf(R.z));
+            if (rp.x < rp.y) {
+                if (rp.x < rp.z) { rp = R; rp.x = rp.x < 0.0f ? -1.0f : 1.0f; }
+                else { rp = R; rp.z = rp.z < 0.0f ? -1.0f : 1.0f; }
+            } else {
+                if (rp.y < rp.z) { rp = R; rp.y = rp.y < 0.0f ? -1.0f : 1.0f; }
+                else { rp = R; rp.z = rp.z < 0.0f ? -1.0f : 1.0f; }
+            }
+            rp = normalize(rp);
+            V3 uu = cross(rp, R);
+            V3 vv = cross(uu, R);
+            float x1 = rng.uniform();
+            float x2 = rng.uniform();
+            R = R + 2.0f * M.gloss * (uu * (x1 - 0.5f) + vv * (x2 - 0.5f));
+            R = normalize(R);
+        }
+        emit_child(O, childOk, make_child(P + (R * 0.000825f), R, r.time, child_node(r.node, 1u),
+                                          had(w, v3(M.km)), r.sample, v3(r.cx, r.cy, r.cz), depth + 1u, inside));
+    }
+    // Dielectric (main.cpp:856-964): both refraction and reflection are traced.
+    if (transparent) {
+        V3 tr = v3(M.tr);
+        float to = M.ior, from = 1.0f;
+        if (inside) {
+            from = M.ior;
+            to = 1.0f;
+        }
+        V3 Nn = N;
+        float cosT = -dot(D, Nn);
+        if (cosT < 0.0f) {
+            Nn = -N;
+            cosT = -dot(D, Nn);
+        }
+        float ratio = from / to;
+        float delta = 1.0f - sq(ratio) * (1.0f - sq(cosT));
+        float refl;
+        bool refract = delta > 0.0f;
+        PathRay rc{};
+        if (!refract) {
+            refl = 1.0f;
+        } else {
+            float cosPhi = sqrtf(delta);
+            float cosAir = inside ? cosPhi : cosT;
+            float inv = 1.0f - cosAir;
+            float r0 = sq((from - to) / (from + to));
+            refl = r0 + (1.0f - r0) * powf(inv, 5.0f);
+            V3 T = (D + Nn * cosT) * ratio - Nn * cosPhi;
+            T = normalize(T);
+            V3 coef = inside ? zero : v3(tr.x - 1.0f, tr.y - 1.0f, tr.z - 1.0f);
+            rc = make_child(P + (T * 0.000100f), T, r.time, child_node(r.node, 2u), w * (1.0f - refl), r.sample, coef,
+                            depth + 1u, !inside);
+        }
+        emit_child(O, childOk && refract, rc);
+        V3 R = reflect(-D, Nn);
+        V3 coef = inside ? v3(tr.x - 1.0f, tr.y - 1.0f, tr.z - 1.0f) : zero;
+        emit_child(O, childOk, make_child(P + (R * 0.000100f), R, r.time, child_node(r.node, 3u), w * refl, r.sample,
+                                          coef, depth + 1u, inside));
+    }
+}
+
+// ---- splat (main.cpp:1209-1254) -------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kRenderBlock) void k_splat_box(CameraSetup cam, ChunkParams cp, const float *__restrict__ acc,
+                                                            T *__restrict__ px) {
+    uint32_t p = blockIdx.x * kRenderBlock + threadIdx.x;
+    if (p >= cp.rows * cam.width) return;
+    uint32_t pixel = cp.row0 * cam.width + p;
+    T cr = px[4 * pixel], cg = px[4 * pixel + 1], cb = px[4 * pixel + 2], cw = px[4 * pixel + 3];
+    for (uint32_t k = 0; k < cp.spp; ++k) {
+        const float *c = acc + 3 * ((size_t)p * cp.spp + k);
+        cr += (T)c[0];
+        cg += (T)c[1];
+        cb += (T)c[2];
+        cw += (T)1.0f;
+    }
+    px[4 * pixel] = cr;
+    px[4 * pixel + 1] = cg;
+    px[4 * pixel + 2] = cb;
+    px[4 * pixel + 3] = cw;
+}
+
+// Gaussian: each destination pixel gathers the samples of the (up to) 3x3
+// source pixels of this chunk whose footprint covers it.
+template <typename T>
+__global__ __launch_bounds__(kRenderBlock) void k_splat_gauss(CameraSetup cam, ChunkParams cp,
+                                                              const float *__restrict__ acc, T *__restrict__ px,
+                                                              uint32_t dstRow0, uint32_t dstRows) {
+    uint32_t p = blockIdx.x * kRenderBlock + threadIdx.x;
+    if (p >= dstRows * cam.width) return;
+    int32_t gy = (int32_t)(dstRow0 + p / cam.width), gx = (int32_t)(p % cam.width);
+    uint32_t pixel = (uint32_t)gy * cam.width + (uint32_t)gx;
+    T cr = px[4 * pixel], cg = px[4 * pixel + 1], cb = px[4 * pixel + 2], cw = px[4 * pixel + 3];
+    const int32_t K = cam.kernel;
+    const int32_t r0 = (int32_t)cp.row0, r1 = (int32_t)(cp.row0 + cp.rows) - 1;
+    for (int32_t row = gy - K; row <= gy + K; ++row) {
+        if (row < r0 || row > r1) continue;
+        for (int32_t col = gx - K; col <= gx + K; ++col) {
+            if (col < 0 || col >= (int32_t)cam.width) continue;
+            uint32_t pl = (uint32_t)(row - r0) * cam.width + (uint32_t)col;
+            for (uint32_t k = 0; k < cp.spp; ++k) {
+                uint32_t s = cp.s0 + k;
+                uint32_t sx = s % cam.sppSqrt, sy = s / cam.sppSqrt;
+                float scx = col + (0.5f * cam.invSppSqrt + sx * cam.invSppSqrt);
+                float scy = row + (0.5f * cam.invSppSqrt + sy * cam.invSppSqrt);
+                float dx = scx - (gx + 0.5f), dy = scy - (gy + 0.5f);
+                float d2 = sq(dx) + sq(dy);
+                float g = cam.oneOverTwoPiSigma2 * Exp(cam.negOneOverTwoSigma2 * d2);
+                const float *c = acc + 3 * ((size_t)pl * cp.spp + k);
+                cr += (T)(c[0] * g);
+                cg += (T)(c[1] * g);
+                cb += (T)(c[2] * g);
+                cw += (T)g;
+            }
+        }
+    }
+    px[4 * pixel] = cr;
+    px[4 * pixel + 1] = cg;
+    px[4 * pixel + 2] = cb;
+    px[4 * pixel + 3] = cw;
+}
+
+}  // namespace az
+
+using namespace az;
+
+// ---- host side ---------------------------------------------------------------------------
+namespace {
+
+#define AZ_HIPR(ctx, call)                                                  \
+    do {                                                                    \
+        hipError_t _e = (call);                                             \
+        if (_e != hipSuccess) {                                             \
+            (ctx)->lastError = std::string(#call ": ") + hipGetErrorString(_e); \
+            return AZ_ERR_DEVICE;                                           \
+        }                                                                   \
+    } while (0)
+
+// PCG32 jump table: entry k holds mult^(2^k) and sum_{i<2^k} mult^i.
+std::vector<uint64_t> pcgTable() {
+    std::vector<uint64_t> t(128);
+    uint64_t m = kPcgMult, g = 1u;
+    for (int k = 0; k < 64; ++k) {
+        t[2 * k] = m;
+        t[2 * k + 1] = g;
+        g = (m + 1u) * g;
+        m = m * m;
+    }
+    return t;
+}
+
+// Perlin gradient table drawn from PCG32(0x5045524C, 256), three draws per
+// entry, mapped to [-1,1) and normalized.
+std::vector<float> perlinTable() {
+    uint64_t state = 0u, inc = (256u << 1u) | 1u;
+    auto next = [&]() {
+        uint64_t old = state;
+        state = old * kPcgMult + inc;
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((0u - rot) & 31u));
+    };
+    next();
+    state += 0x5045524CULL;
+    next();
+    auto uni = [&]() { return (float)(next() >> 8) * (1.0f / 16777216.0f); };
+    std::vector<float> g(3 * 256);
+    for (int i = 0; i < 256; ++i) {
+        float a = uni(), b = uni(), c = uni();
+        float x = (a - 0.5f) * 2.0f, y = (b - 0.5f) * 2.0f, z = (c - 0.5f) * 2.0f;
+        float l = sqrtf(x * x + y * y + z * z);
+        if (l != 0.0f) {
+            x /= l;
+            y /= l;
+            z /= l;
+        } else {
+            x = y = z = 0.0f;
+        }
+        g[3 * i] = x;
+        g[3 * i + 1] = y;
+        g[3 * i + 2] = z;
+    }
+    return g;
+}
+
+Affine affineOf(const mat4 &m) {
+    Affine a;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 4; ++j) a.m[i][j] = m.E[i][j];
+    return a;
+}
+
+inline uint32_t bitsOf(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return u;
+}
+
+DevTexture devTexture(const az_texture &t, std::vector<uint8_t> &texels, std::vector<uint32_t> &perms) {
+    DevTexture d;
+    memset(&d, 0, sizeof(d));
+    d.type = (int32_t)t.Type;
+    d.bumpMul = t.BumpMapMultiplier;
+    d.bump = t.BumpMap ? 1 : 0;
+    d.wrap = (int32_t)t.WrapMode;
+    d.blend = (int32_t)t.BlendMode;
+    d.filter = (int32_t)t.Filter;
+    if (t.Type == TextureType_2D) {
+        d.pixelFormat = (int32_t)t.PixelFormat;
+        d.width = (int32_t)t.Width;
+        d.height = (int32_t)t.Height;
+        d.invW = t.InvWidth;
+        d.invH = t.InvHeight;
+        d.normalizer = t.Normalizer;
+        size_t texel = t.PixelFormat == PixelFormat_RGB_F32 ? 12 : 3;
+        size_t bytes = (size_t)t.Width * t.Height * texel;
+        while (texels.size() % 16) texels.push_back(0);
+        d.dataOffset = (uint32_t)texels.size();
+        if (t.Pixels && bytes) texels.insert(texels.end(), (const uint8_t *)t.Pixels, (const uint8_t *)t.Pixels + bytes);
+        else texels.resize(texels.size() + std::max<size_t>(bytes, 12), 0);
+    } else {
+        d.perlinAppearance = (int32_t)t.PerlinAppearance;
+        d.scaling = t.ScalingFactor;
+        d.permOffset = (uint32_t)perms.size();
+        for (int i = 0; i < 256; ++i) perms.push_back(t.Perm[i] & 255u);
+    }
+    return d;
+}
+
+template <typename T>
+hipError_t uploadVec(DeviceBuffer &b, const std::vector<T> &v, hipStream_t s) {
+    size_t n = std::max<size_t>(1, v.size()) * sizeof(T);
+    hipError_t e = b.alloc(n);
+    if (e != hipSuccess) return e;
+    if (!v.empty()) e = hipMemcpyAsync(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+    return e;
+}
+
+}  // namespace
+
+struct RenderBuffers {
+    DeviceBuffer materials, textures, texels, perms, lights, lightTris, prims, itemShade, perlin, pcg;
+    DeviceBuffer q0, q1, hits, shadow, acc, counters, pixels;
+};
+
+static RenderBuffers &buffersOf(az_gpu_ctx *C) {
+    // One set of buffers per context, owned by the context's render state.
+    static_assert(sizeof(RenderState) >= 1, "");
+    if (!C->renderBuffers) C->renderBuffers = new RenderBuffers();
+    return *reinterpret_cast<RenderBuffers *>(C->renderBuffers);
+}
+
+void az_release_render_buffers(void *p) { delete reinterpret_cast<RenderBuffers *>(p); }
+
+az_status az_upload_shading(az_gpu_ctx *C, const scene *S, const UploadInfo &info) {
+    RenderBuffers &B = buffersOf(C);
+    DevScene &D = C->devScene;
+    memset(&D, 0, sizeof(D));
+    // Materials.
+    std::vector<DevMaterial> mats(std::max<u32>(1, S->MaterialCount));
+    for (u32 i = 0; i < S->MaterialCount; ++i) {
+        const material &m = S->Materials[i];
+        DevMaterial &d = mats[i];
+        memset(&d, 0, sizeof(d));
+        const vec3 *src[5] = {&m.DiffuseReflectance, &m.SpecularReflectance, &m.AmbientReflectance,
+                              &m.MirrorReflectance, &m.Transparency};
+        float *dst[5] = {d.kd, d.ks, d.ka, d.km, d.tr};
+        for (int k = 0; k < 5; ++k) {
+            dst[k][0] = src[k]->x;
+            dst[k][1] = src[k]->y;
+            dst[k][2] = src[k]->z;
+        }
+        d.ior = m.RefractionIndex;
+        d.gloss = m.Glossiness;
+        d.exponent = m.BRDF.Exponent;
+        d.brdfIor = m.BRDF.RefractiveIndex;
+        d.brdfType = (int32_t)m.BRDF.Type;
+        if (bitsOf(m.Transparency.x) | bitsOf(m.Transparency.y) | bitsOf(m.Transparency.z)) d.flags |= kMatTransparent;
+        if (bitsOf(m.MirrorReflectance.x) | bitsOf(m.MirrorReflectance.y) | bitsOf(m.MirrorReflectance.z))
+            d.flags |= kMatMirror;
+    }
+    // Textures (index 0 unused: entities use 1-based indices).
+    std::vector<uint8_t> texels;
+    std::vector<uint32_t> perms;
+    std::vector<DevTexture> texs(1 + S->TextureCount);
+    memset(texs.data(), 0, sizeof(DevTexture));
+    for (u32 i = 0; i < S->TextureCount; ++i) texs[1 + i] = devTexture(S->Textures[i], texels, perms);
+    if (S->UseEnvironmentMap) D.env = devTexture(S->EnvironmentMap, texels, perms);
+    if (S->UseBackgroundTexture) D.background = devTexture(S->BackgroundTexture, texels, perms);
+    if (texels.empty()) texels.resize(16, 0);
+    if (perms.empty()) perms.resize(1, 0);
+    // Lights.
+    std::vector<DevLight> lights(std::max<u32>(1, S->LightCount));
+    std::vector<DevLightTri> ltris;
+    for (u32 i = 0; i < S->LightCount; ++i) {
+        const light &L = S->Lights[i];
+        DevLight &d = lights[i];
+        memset(&d, 0, sizeof(d));
+        d.type = (int32_t)L.Type;
+        auto cp3 = [](float *o, const vec3 &v) {
+            o[0] = v.x;
+            o[1] = v.y;
+            o[2] = v.z;
+        };
+        switch (L.Type) {
+        case LightType_PointLight:
+            cp3(d.pos, L.Position);
+            cp3(d.intensity, L.Intensity);
+            break;
+        case LightType_AreaLight:
+            cp3(d.pos, L.Position);
+            cp3(d.intensity, L.Intensity);
+            cp3(d.e0, L.EdgeVectors[0]);
+            cp3(d.e1, L.EdgeVectors[1]);
+            cp3(d.normal, L.Normal);
+            break;
+        case LightType_DirectionalLight:
+            cp3(d.dir, L.Direction);
+            cp3(d.intensity, L.Intensity);
+            break;
+        case LightType_SpotLight:
+            cp3(d.pos, L.Position);
+            cp3(d.intensity, L.Intensity);
+            cp3(d.dir, L.SpotDirection);
+            d.cosCov = L.CosHalfCoverage;
+            d.cosFall = L.CosHalfFallOff;
+            break;
+        case LightType_EntityLight: {
+            const entity &E = L.Entity;
+            d.entityId = E.UniqueEntityID;
+            d.inv = affineOf(E.InvTransformationMatrix);
+            d.fwd = affineOf(E.TransformationMatrix);
+            cp3(d.radiance, E.Radiance);
+            if (E.Type == EntityType_Sphere) {
+                d.entityKind = (int32_t)kItemSphere;
+                cp3(d.center, E.Center);
+                d.radius = E.Radius;
+            } else if (E.Type == EntityType_MeshInstance) {
+                d.entityKind = (int32_t)kItemInstance;
+                const mesh *M = E.Mesh;
+                d.triOffset = (uint32_t)ltris.size();
+                d.triCount = M->TriangleCount;
+                d.totalArea = M->TotalArea;
+                for (u32 k = 0; k < M->TriangleCount; ++k) {
+                    DevLightTri t;
+                    cp3(t.p0, M->Triangles[k].P0);
+                    cp3(t.p1, M->Triangles[k].P1);
+                    cp3(t.p2, M->Triangles[k].P2);
+                    cp3(t.n, M->Triangles[k].N);
+                    ltris.push_back(t);
+                }
+            } else {
+                C->lastError = "entity light of an unsupported entity type";
+                return AZ_ERR_UNSUPPORTED;
+            }
+        } break;
+        }
+    }
+    if (ltris.empty()) ltris.resize(1);
+    // Per-triangle shading attributes in primitive-id order.
+    std::vector<DevPrimAttr> prims(std::max<u32>(1, info.triTotal));
+    for (u32 m = 0; m < S->MeshCount; ++m) {
+        const std::vector<const entity *> &tl = info.meshTriEntities[m];
+        for (size_t k = 0; k < tl.size(); ++k) {
+            const entity *e = tl[k];
+            DevPrimAttr &p = prims[info.meshPrimBase[m] + k];
+            auto cp3 = [](float *o, const vec3 &v) {
+                o[0] = v.x;
+                o[1] = v.y;
+                o[2] = v.z;
+            };
+            cp3(p.n0, e->N0);
+            cp3(p.n1, e->N1);
+            cp3(p.n2, e->N2);
+            cp3(p.fn, e->FaceNormal);
+            p.uv0[0] = e->UV0.x;
+            p.uv0[1] = e->UV0.y;
+            p.uv1[0] = e->UV1.x;
+            p.uv1[1] = e->UV1.y;
+            p.uv2[0] = e->UV2.x;
+            p.uv2[1] = e->UV2.y;
+            cp3(p.p0, e->P0);
+            cp3(p.p1, e->P1);
+            cp3(p.p2, e->P2);
+            p.smooth = e->ShadingMode == ShadingMode_Smooth ? 1 : 0;
+        }
+    }
+    // Per top-level item.
+    std::vector<DevItemShade> items(info.topEntities.size());
+    for (size_t k = 0; k < items.size(); ++k) {
+        const entity *e = info.topEntities[k];
+        DevItemShade &d = items[k];
+        memset(&d, 0, sizeof(d));
+        d.material = e->MaterialIndex < S->MaterialCount ? e->MaterialIndex : 0u;
+        d.texture = e->TextureIndex <= S->TextureCount ? e->TextureIndex : 0u;
+        d.isEmitter = e->IsEmitter ? 1u : 0u;
+        d.kind = e->Type == EntityType_Sphere ? kItemSphere : kItemInstance;
+        d.radiance[0] = e->Radiance.x;
+        d.radiance[1] = e->Radiance.y;
+        d.radiance[2] = e->Radiance.z;
+    }
+    std::vector<float> grad = perlinTable();
+    std::vector<uint64_t> pcg = pcgTable();
+    hipStream_t s = C->stream;
+    AZ_HIPR(C, uploadVec(B.materials, mats, s));
+    AZ_HIPR(C, uploadVec(B.textures, texs, s));
+    AZ_HIPR(C, uploadVec(B.texels, texels, s));
+    AZ_HIPR(C, uploadVec(B.perms, perms, s));
+    AZ_HIPR(C, uploadVec(B.lights, lights, s));
+    AZ_HIPR(C, uploadVec(B.lightTris, ltris, s));
+    AZ_HIPR(C, uploadVec(B.prims, prims, s));
+    AZ_HIPR(C, uploadVec(B.itemShade, items, s));
+    AZ_HIPR(C, uploadVec(B.perlin, grad, s));
+    AZ_HIPR(C, uploadVec(B.pcg, pcg, s));
+    AZ_HIPR(C, hipStreamSynchronize(s));
+    D.materials = (const DevMaterial *)B.materials.ptr;
+    D.textures = (const DevTexture *)B.textures.ptr;
+    D.texels = (const uint8_t *)B.texels.ptr;
+    D.perms = (const uint32_t *)B.perms.ptr;
+    D.lights = (const DevLight *)B.lights.ptr;
+    D.lightTris = (const DevLightTri *)B.lightTris.ptr;
+    D.prims = (const DevPrimAttr *)B.prims.ptr;
+    D.itemShade = (const DevItemShade *)B.itemShade.ptr;
+    D.perlinGrad = (const float *)B.perlin.ptr;
+    D.pcgTable = (const uint64_t *)B.pcg.ptr;
+    D.lightCount = S->LightCount;
+    D.maxDepth = S->MaxRecursionDepth;
+    D.pathTrace = S->PathTrace ? 1u : 0u;
+    D.importance = S->ImportanceSampling ? 1u : 0u;
+    D.useEnv = S->UseEnvironmentMap ? 1u : 0u;
+    D.useBackgroundTexture = S->UseBackgroundTexture ? 1u : 0u;
+    D.ambient[0] = S->AmbientLight.x;
+    D.ambient[1] = S->AmbientLight.y;
+    D.ambient[2] = S->AmbientLight.z;
+    D.backgroundColor[0] = S->BackgroundColor.x;
+    D.backgroundColor[1] = S->BackgroundColor.y;
+    D.backgroundColor[2] = S->BackgroundColor.z;
+    D.epsIntersection = C->eps.Intersection;
+    D.epsShadow = C->eps.ShadowRay;
+    D.triTotal = info.triTotal;
+    if (S->MaxRecursionDepth > 200) {
+        C->lastError = "MaxRecursionDepth above 200 is not supported";
+        return AZ_ERR_UNSUPPORTED;
+    }
+    C->render.ready = true;
+    return AZ_OK;
+}
+
+// Camera frame exactly as ProduceImageInLinearSpace sets it up
+// (main.cpp:1265-1347), in single precision on the host.
+static CameraSetup setupCamera(const camera *Cam, u32 sppOverride) {
+    struct H3 {
+        float x, y, z;
+    };
+    auto mk = [](float x, float y, float z) { return H3{x, y, z}; };
+    auto sub = [&](H3 a, H3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); };
+    auto add = [&](H3 a, H3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); };
+    auto mul = [&](H3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); };
+    auto smul = [&](float s, H3 a) { return mk(s * a.x, s * a.y, s * a.z); };
+    auto crs = [&](H3 a, H3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); };
+    auto nrm = [&](H3 a) {
+        float l = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z);
+        if (l != 0.0f) return mk(a.x / l, a.y / l, a.z / l);
+        return mk(0.0f, 0.0f, 0.0f);
+    };
+    auto from = [&](const vec3 &v) { return mk(v.x, v.y, v.z); };
+    CameraSetup c;
+    memset(&c, 0, sizeof(c));
+    H3 gaze = from(Cam->Gaze);
+    if (Cam->GazeIsPoint) gaze = sub(from(Cam->Gaze), from(Cam->Position));
+    H3 gn = nrm(gaze);
+    H3 az = mk(-gn.x, -gn.y, -gn.z);
+    H3 ax, ay;
+    if (Cam->IsLeftHanded) {
+        ax = nrm(crs(nrm(from(Cam->Up)), gn));
+        ay = crs(ax, az);
+    } else {
+        ax = nrm(crs(gn, nrm(from(Cam->Up))));
+        ay = crs(az, ax);
+    }
+    float fdnd = 1.0f;
+    if (Cam->FocusDistance != 0.0f) fdnd = Cam->FocusDistance / Cam->NearDistance;
+    c.aperture = Cam->ApertureSize;
+    c.width = Cam->Resolution.x;
+    c.height = Cam->Resolution.y;
+    float r = Cam->NearPlane.x, l = Cam->NearPlane.y, b = Cam->NearPlane.z, t = Cam->NearPlane.w;
+    if (Cam->Type == CameraType_Simple) {
+        float hh = Cam->NearDistance * tanf((Cam->FovY / 2.0f / 180.0f) * kPiF);
+        float aspect = (float)c.width / (float)c.height;
+        float wh = aspect * hh;
+        t = +hh;
+        b = -hh;
+        r = +wh;
+        l = -wh;
+    }
+    float psx = (r - l) / (float)c.width, psy = (t - b) / (float)c.height;
+    H3 sx = mul(mul(ax, psx), fdnd), sy = mul(mul(ay, psy), fdnd);
+    H3 pos = from(Cam->Position);
+    H3 tl = add(add(add(pos, smul(Cam->NearDistance, gn)), smul(l, ax)), smul(t, ay));
+    H3 base = mul(sub(tl, pos), fdnd);
+    float *dsts[6] = {c.base, c.pos, c.ax, c.ay, c.stepX, c.stepY};
+    H3 srcs[6] = {base, pos, ax, ay, sx, sy};
+    for (int k = 0; k < 6; ++k) {
+        dsts[k][0] = srcs[k].x;
+        dsts[k][1] = srcs[k].y;
+        dsts[k][2] = srcs[k].z;
+    }
+    u32 spp = sppOverride ? sppOverride : Cam->SampleCount;
+    c.sppSqrt = (u32)sqrtf((float)spp);
+    if (c.sppSqrt == 0) c.sppSqrt = 1;
+    c.invSppSqrt = 1.0f / c.sppSqrt;
+    float radius = 1.5f;
+    float sigma = radius / 3.0f;
+    float twoS2 = 2.0f * (sigma * sigma);
+    c.negOneOverTwoSigma2 = -1.0f / twoS2;
+    c.oneOverTwoPiSigma2 = 1.0f / (twoS2 * (float)kPiD);
+    c.kernel = (int32_t)ceilf(radius - 0.5f);
+    return c;
+}
+
+static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter, b32 Cull, const az_render_opts *Opts,
+                            void *d_px, bool f64) {
+    RenderBuffers &B = buffersOf(C);
+    az_render_opts o;
+    memset(&o, 0, sizeof(o));
+    if (Opts) o = *Opts;
+    CameraSetup cam = setupCamera(Cam, o.SampleCountOverride);
+    if (cam.width == 0 || cam.height == 0) return AZ_ERR_INVALID_ARGUMENT;
+    u32 row0 = o.RowBegin, row1 = o.RowEnd ? std::min(o.RowEnd, cam.height) : cam.height;
+    if (row0 >= row1) return AZ_OK;
+    const u32 rows = row1 - row0;
+    const u32 nPix = rows * cam.width;
+    const u32 sppTotal = cam.sppSqrt * cam.sppSqrt;
+    // Samples per chunk: about 2M camera samples per wavefront.
+    u32 chunkSpp = std::max<u32>(1u, std::min<u32>(sppTotal, (u32)(2u << 20) / std::max<u32>(1u, nPix)));
+    DevScene &S = C->devScene;
+    const u32 shadowPerRay = S.lightCount + (S.useEnv ? 1u : 0u);
+    hipStream_t st = C->stream;
+    AZ_HIPR(C, B.counters.alloc(64));
+    uint32_t *dCount = (uint32_t *)B.counters.ptr;
+    u32 s0 = 0;
+    // Device memory the queues of one chunk may take before the chunk is
+    // split into fewer samples per pixel.
+    const size_t budget = (size_t)48 << 30;
+    while (s0 < sppTotal) {
+        u32 spp = std::min(chunkSpp, sppTotal - s0);
+        u32 n = nPix * spp;
+        AZ_HIPR(C, B.q0.alloc((size_t)n * sizeof(PathRay)));
+        AZ_HIPR(C, B.acc.alloc((size_t)n * 3 * sizeof(float)));
+        u32 cap = n, shadowCap = 0;
+        ChunkParams cp;
+        memset(&cp, 0, sizeof(cp));
+        cp.seed = o.Seed;
+        cp.row0 = row0;
+        cp.rows = rows;
+        cp.s0 = s0;
+        cp.spp = spp;
+        cp.capacity = cap;
+        cp.shadowCapacity = shadowCap;
+        cp.cull = Cull ? 1u : 0u;
+        cp.gaussian = Filter == FilterType_Gaussian ? 1u : 0u;
+        DeviceBuffer *bufA = &B.q0, *bufB = &B.q1;
+        float *acc = (float *)B.acc.ptr;
+        hipLaunchKernelGGL(k_generate, dim3((n + kRenderBlock - 1) / kRenderBlock), dim3(kRenderBlock), 0, st, S, cam,
+                           cp, (PathRay *)bufA->ptr, acc, n);
+        AZ_HIPR(C, hipGetLastError());
+        u32 count = n;
+        bool overflow = false;
+        for (u32 depth = 0; depth <= S.maxDepth && count > 0; ++depth) {
+            // A vertex has at most four children (scatter, mirror, refraction,
+            // reflection) and one shadow ray per light plus the environment,
+            // so these capacities always hold the next level.
+            cap = 4u * count;
+            shadowCap = std::max<u32>(1u, shadowPerRay * count);
+            size_t need = (size_t)cap * sizeof(PathRay) + (size_t)count * (sizeof(PathRay) + sizeof(PathHit)) +
+                          (size_t)shadowCap * sizeof(ShadowRay);
+            if (need > budget || (size_t)count * 4 > 0xFFFFFFFFull) {
+                overflow = true;
+                break;
+            }
+            AZ_HIPR(C, bufB->alloc((size_t)cap * sizeof(PathRay)));
+            AZ_HIPR(C, B.hits.alloc((size_t)count * sizeof(PathHit)));
+            AZ_HIPR(C, B.shadow.alloc((size_t)shadowCap * sizeof(ShadowRay)));
+            PathRay *qa = (PathRay *)bufA->ptr, *qb = (PathRay *)bufB->ptr;
+            cp.capacity = cap;
+            cp.shadowCapacity = shadowCap;
+            dim3 grid((count + kRenderBlock - 1) / kRenderBlock);
+            hipLaunchKernelGGL(k_extend, grid, dim3(kRenderBlock), 0, st, C->dg, qa, count, (PathHit *)B.hits.ptr,
+                               S.epsIntersection, (int)cp.cull);
+            AZ_HIPR(C, hipGetLastError());
+            AZ_HIPR(C, hipMemsetAsync(dCount, 0, 8, st));
+            ShadeOut O{qb, (ShadowRay *)B.shadow.ptr, dCount, cap, shadowCap};
+            hipLaunchKernelGGL(k_shade, grid, dim3(kRenderBlock), 0, st, S, C->dg, cam, cp, qa,
+                               (const PathHit *)B.hits.ptr, count, acc, O);
+            AZ_HIPR(C, hipGetLastError());
+            uint32_t hc[2];
+            AZ_HIPR(C, hipMemcpyAsync(hc, dCount, 8, hipMemcpyDeviceToHost, st));
+            AZ_HIPR(C, hipStreamSynchronize(st));
+            if (hc[0] > cap || hc[1] > shadowCap) {
+                overflow = true;
+                break;
+            }
+            if (hc[1]) {
+                hipLaunchKernelGGL(k_connect, dim3((hc[1] + kRenderBlock - 1) / kRenderBlock), dim3(kRenderBlock), 0,
+                                   st, C->dg, (const ShadowRay *)B.shadow.ptr, hc[1], acc);
+                AZ_HIPR(C, hipGetLastError());
+            }
+            count = hc[0];
+            std::swap(bufA, bufB);
+        }
+        if (overflow) {
+            if (chunkSpp == 1) {
+                C->lastError = "path queue overflow with one sample per chunk";
+                return AZ_ERR_OUT_OF_MEMORY;
+            }
+            chunkSpp = std::max<u32>(1u, chunkSpp / 2);
+            continue;   // the chunk is redone with fewer samples; its accumulators are rebuilt
+        }
+        if (!cp.gaussian) {
+            dim3 g((nPix + kRenderBlock - 1) / kRenderBlock);
+            if (f64) hipLaunchKernelGGL(k_splat_box<double>, g, dim3(kRenderBlock), 0, st, cam, cp, acc, (double *)d_px);
+            else hipLaunchKernelGGL(k_splat_box<float>, g, dim3(kRenderBlock), 0, st, cam, cp, acc, (float *)d_px);
+        } else {
+            u32 d0 = row0 > (u32)cam.kernel ? row0 - (u32)cam.kernel : 0u;
+            u32 d1 = std::min(cam.height, row1 + (u32)cam.kernel);
+            u32 dn = (d1 - d0) * cam.width;
+            dim3 g((dn + kRenderBlock - 1) / kRenderBlock);
+            if (f64)
+                hipLaunchKernelGGL(k_splat_gauss<double>, g, dim3(kRenderBlock), 0, st, cam, cp, acc, (double *)d_px, d0,
+                                   d1 - d0);
+            else
+                hipLaunchKernelGGL(k_splat_gauss<float>, g, dim3(kRenderBlock), 0, st, cam, cp, acc, (float *)d_px, d0,
+                                   d1 - d0);
+        }
+        AZ_HIPR(C, hipGetLastError());
+        s0 += spp;
+    }
     return AZ_OK;
 }
 
 extern "C" {
 
-az_status az_gpu_render(az_gpu_ctx *C, const camera *, filter_type, b32, const az_render_opts *, pixel *) {
-    if (!C) return AZ_ERR_INVALID_ARGUMENT;
-    return AZ_ERR_UNSUPPORTED;
+az_status az_gpu_render_device(az_gpu_ctx *C, const camera *Cam, filter_type Filter, b32 Cull,
+                               const az_render_opts *Opts, void *d_px) {
+    if (!C || !Cam || !d_px) return AZ_ERR_INVALID_ARGUMENT;
+    if (!C->hasScene || !C->render.ready) return AZ_ERR_NO_SCENE;
+    AZ_HIPR(C, hipSetDevice(C->device));
+    AZ_HIPR(C, hipEventRecord(C->ev0, C->stream));
+    az_status st = renderImpl(C, Cam, Filter, Cull, Opts, d_px, Opts && Opts->AccumulateF64);
+    AZ_HIPR(C, hipEventRecord(C->ev1, C->stream));
+    C->lastStart = C->ev0;
+    C->lastStop = C->ev1;
+    C->pendingTiming = true;
+    return st;
 }
 
-az_status az_gpu_render_device(az_gpu_ctx *C, const camera *, filter_type, b32, const az_render_opts *, void *) {
-    if (!C) return AZ_ERR_INVALID_ARGUMENT;
-    return AZ_ERR_UNSUPPORTED;
+az_status az_gpu_render(az_gpu_ctx *C, const camera *Cam, filter_type Filter, b32 Cull, const az_render_opts *Opts,
+                        pixel *Pixels) {
+    if (!C || !Cam || !Pixels) return AZ_ERR_INVALID_ARGUMENT;
+    if (!C->hasScene || !C->render.ready) return AZ_ERR_NO_SCENE;
+    if (Opts && Opts->AccumulateF64) return AZ_ERR_INVALID_ARGUMENT;   // f64 output is device-only
+    RenderBuffers &B = buffersOf(C);
+    size_t bytes = (size_t)Cam->Resolution.x * Cam->Resolution.y * sizeof(pixel);
+    AZ_HIPR(C, hipSetDevice(C->device));
+    AZ_HIPR(C, B.pixels.alloc(bytes));
+    AZ_HIPR(C, hipMemcpyAsync(B.pixels.ptr, Pixels, bytes, hipMemcpyHostToDevice, C->stream));
+    az_status st = az_gpu_render_device(C, Cam, Filter, Cull, Opts, B.pixels.ptr);
+    if (st != AZ_OK) return st;
+    AZ_HIPR(C, hipMemcpyAsync(Pixels, B.pixels.ptr, bytes, hipMemcpyDeviceToHost, C->stream));
+    AZ_HIPR(C, hipStreamSynchronize(C->stream));
+    return AZ_OK;
 }
 
-b32 az_render_scene(scene *, camera *, epsilon_values *, u32, b32, pixel *) { return 0; }
+b32 az_render_scene(scene *Scene, camera *Camera, epsilon_values *Epsilons, u32 ThreadCount, b32 BackfaceCulling,
+                    pixel *Pixels) {
+    (void)ThreadCount;
+    if (!Scene || !Camera || !Epsilons || !Pixels) return 0;
+    az_gpu_ctx *C = nullptr;
+    if (az_gpu_create(&C, 0) != AZ_OK) return 0;
+    b32 ok = 0;
+    if (az_gpu_upload_scene(C, Scene, Epsilons) == AZ_OK) {
+        memset(Pixels, 0, sizeof(pixel) * Camera->Resolution.x * Camera->Resolution.y);
+        ok = az_gpu_render(C, Camera, FilterType_Box, BackfaceCulling, nullptr, Pixels) == AZ_OK;
+    }
+    az_gpu_destroy(C);
+    return ok;
+}
 
 }  // extern "C"

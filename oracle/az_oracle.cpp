@@ -1064,8 +1064,9 @@ inline az_hit toHit(const Hit &h, bool hit) {
 }
 
 // Brute force: the same per-primitive tests over every primitive, with the
-// instance transform arithmetic of main.cpp:244-281 and the minimum taken by
-// strict `<` in primitive order.
+// instance transform arithmetic of main.cpp:244-281.  Leaves are visited left
+// to right and a later leaf wins an exact tie (`<=`), which is what the
+// reference's recursive minimum with ties to the right child yields.
 bool bruteClosest(const Ctx &C, Hit *H, const Ray &ray, bool cull, f32 eps) {
     *H = freshHit();
     bool any = false;
@@ -1086,7 +1087,7 @@ bool bruteClosest(const Ctx &C, Hit *H, const Ray &ray, bool cull, f32 eps) {
     for (const entity *E : tops) {
         if (E->Type == EntityType_Sphere) {
             Hit h = freshHit();
-            if (intersect(C, &h, ray, E, cull, eps, -1) && h.t < H->t) {
+            if (intersect(C, &h, ray, E, cull, eps, -1) && h.t <= H->t) {
                 *H = h;
                 any = true;
             }
@@ -1101,7 +1102,7 @@ bool bruteClosest(const Ctx &C, Hit *H, const Ray &ray, bool cull, f32 eps) {
             const entity *base = C.triBase[mi];
             for (u32 k = 0; k < E->Mesh->TriangleCount; ++k) {
                 Hit h = freshHit();
-                if (intersect(C, &h, lr, base + k, cull, eps, mi) && h.t < best.t) {
+                if (intersect(C, &h, lr, base + k, cull, eps, mi) && h.t <= best.t) {
                     best = h;
                     found = true;
                 }
@@ -1110,7 +1111,7 @@ bool bruteClosest(const Ctx &C, Hit *H, const Ray &ray, bool cull, f32 eps) {
                 V wp = pointToWorld(best.P, E->TransformationMatrix);
                 wp = wp + motion;
                 f32 wt = length(wp - r.o - motion) / length(r.d);
-                if (wt > eps && wt < H->t) {
+                if (wt > eps && wt <= H->t) {
                     *H = best;
                     H->N = normalToWorld(best.N, E->InvTransformationMatrix);
                     H->P = wp;

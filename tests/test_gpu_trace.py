@@ -166,3 +166,27 @@ def test_full_size_icosphere_sampled(gpu):
         assert exact == 1.0
         h = got["Prim"] != N.AZ_MISS
         assert np.all(got["t"][h] > 0)
+
+
+def test_exact_ties_follow_reference_order(gpu):
+    """Rays through the exact seams of the Cornell box (camera on the box's
+    symmetry axis): equal t on two triangles resolves to the right-most leaf
+    of the reference BVH, as its recursive minimum does."""
+    sc, cam = configs.cornell_scene(64, 64, spp=1, with_spheres=False)
+    gpu.upload_scene(sc)
+    n = 64
+    u = (np.arange(n) + 0.5) / n
+    # Targets on the four seams of the back wall and the floor/right-wall seam.
+    pts = np.concatenate([np.stack([u, np.zeros(n), np.zeros(n)], 1), np.stack([np.zeros(n), u, np.zeros(n)], 1),
+                          np.stack([np.ones(n), u, np.zeros(n)], 1), np.stack([np.ones(n), np.zeros(n), u], 1)])
+    eye = np.array([0.5, 0.5, 2.0], np.float32)
+    r = np.zeros(pts.shape[0], dtype=N.RAY_DTYPE)
+    r["Origin"] = eye
+    r["Direction"] = (pts.astype(np.float32) - eye)
+    r["TMax"] = np.inf
+    got = gpu.trace(r)
+    want = oracle.trace(sc, r)
+    brute = oracle.trace(sc, r, brute=True)
+    assert np.array_equal(want["Prim"], brute["Prim"])
+    assert np.array_equal(got["Prim"], want["Prim"]), f"{int((got['Prim'] != want['Prim']).sum())} tie mismatches"
+    assert np.array_equal(got["t"], want["t"])
