@@ -8,6 +8,7 @@
 // (bvh.cpp:145-190, 279-307).  The partition is run on an index permutation
 // and the entities are moved once at the end; the resulting order equals the
 // reference's in-place swapping because both apply the same swaps.
+#include "azimuth_gpu.h"
 #include "azimuth_scene.h"
 
 #include <cmath>
@@ -593,12 +594,13 @@ u32 az_scene_triangle_count(const scene *Scene) {
 extern "C" {
 // Sizes of the ABI structures, for binding layers to check their mirrors:
 // entity, mesh, material, light, az_texture, camera, scene, az_mesh_desc,
-// az_sphere_desc, az_scene_params.
+// az_sphere_desc, az_scene_params, az_render_opts, az_stats, az_ray, az_hit.
 u32 az_abi_sizes(u32 *Out, u32 Capacity) {
     const u32 sizes[] = {(u32)sizeof(entity),        (u32)sizeof(mesh),           (u32)sizeof(material),
                          (u32)sizeof(light),         (u32)sizeof(az_texture),     (u32)sizeof(camera),
                          (u32)sizeof(scene),         (u32)sizeof(az_mesh_desc),   (u32)sizeof(az_sphere_desc),
-                         (u32)sizeof(az_scene_params)};
+                         (u32)sizeof(az_scene_params), (u32)sizeof(az_render_opts), (u32)sizeof(az_stats),
+                         (u32)sizeof(az_ray),        (u32)sizeof(az_hit)};
     const u32 n = (u32)(sizeof(sizes) / sizeof(sizes[0]));
     for (u32 i = 0; i < n && i < Capacity; ++i) Out[i] = sizes[i];
     return n;

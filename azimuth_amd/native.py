@@ -185,7 +185,8 @@ PIXEL_DTYPE = np.dtype([("Color", "<f4", 3), ("TotalWeight", "<f4")])
 ABI_SIZES = [("entity", 288), ("mesh", 200), ("material", C.sizeof(material)), ("light", C.sizeof(light)),
              ("texture", C.sizeof(texture)), ("camera", C.sizeof(camera)), ("scene", 2744),
              ("az_mesh_desc", C.sizeof(az_mesh_desc)), ("az_sphere_desc", C.sizeof(az_sphere_desc)),
-             ("az_scene_params", C.sizeof(az_scene_params))]
+             ("az_scene_params", C.sizeof(az_scene_params)), ("az_render_opts", C.sizeof(az_render_opts)),
+             ("az_stats", C.sizeof(az_stats)), ("az_ray", RAY_DTYPE.itemsize), ("az_hit", HIT_DTYPE.itemsize)]
 
 _P = C.c_void_p
 # Exported functions: name -> (restype, argtypes).  Every name here is also
