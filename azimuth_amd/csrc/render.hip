@@ -239,7 +239,7 @@ __device__ inline V3 bump(const DevScene &S, const DevTexture &T, V3 N, V3 P, fl
     return normalize(N);
 }
 
-__global__ __launch_bounds__(kRenderBlock) void k_shade(DevScene S, DeviceGeometry G, CameraSetup cam, ChunkParams cp,
+__global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeometry G, CameraSetup cam, ChunkParams cp,
                                                         const PathRay *__restrict__ q,
                                                         const PathHit *__restrict__ hits, uint32_t n,
                                                         float *__restrict__ acc, ShadeOut O) {
@@ -432,6 +432,9 @@ __global__ __launch_bounds__(kRenderBlock) void k_shade(DevScene S, DeviceGeomet
         V3 c;
         bool want = !inside && light_term(M, kd, Li, Ld, D, N, ldist, falloff, invPdf, c);
         if (want) c = had(w, c);
+        // A contribution of exactly zero (outside a spot cone, a black
+        // material) adds nothing whatever the occlusion: no shadow ray.
+        want = want && (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f);
         emit_shadow(O, want, make_shadow(P + (Ld * eps), Ld, r.time, tmax, c, r.sample, mode, L.entityId));
     }
     // Environment NEE, cosine-distributed (main.cpp:719-778).
@@ -452,6 +455,7 @@ __global__ __launch_bounds__(kRenderBlock) void k_shade(DevScene S, DeviceGeomet
         V3 c;
         bool want = light_term(M, kd, I, L, D, N, 1.0f, 1.0f, invPdf, c);
         if (want) c = had(w, c);
+        want = want && (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f);
         emit_shadow(O, want, make_shadow(P + (L * eps), L, r.time, INFINITY, c, r.sample, kShadowAny, 0u));
     }
     const bool childOk = depth + 1u <= S.maxDepth;

@@ -1,0 +1,92 @@
+"""Render throughput of the wavefront path tracer on one GPU:
+
+    python -m azimuth_amd.render_bench --workload cornell|interior|motion
+        [--spp N] [--scale S] [--cpu-seconds T]
+
+Prints one JSON line: Msamples/s (width * height * spp_eff / render time,
+spp_eff = floor(sqrt(spp))^2 as the reference renders), the render time from
+HIP events around the whole render call, and optionally the CPU oracle's rate
+on a bounded strip of rows (the same camera, spp and seed), extrapolated
+linearly to the full image.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import configs, native as N, workloads
+from .scene import GpuContext
+
+
+def build(name, scale, spp):
+    if name == "cornell":
+        sc, cam = configs.cornell_scene(int(256 * scale), int(256 * scale), spp=spp or 16, max_depth=8)
+        filt = N.FilterType_Box
+    elif name == "interior":
+        sc, cam = workloads.interior_scene(int(1920 * scale), int(1080 * scale), spp=spp or 64, max_depth=6)
+        filt = N.FilterType_Gaussian
+    elif name == "motion":
+        sc, cam = workloads.motion_scene(int(1280 * scale), int(720 * scale), spp=spp or 128, max_depth=10)
+        filt = N.FilterType_Box
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return sc, cam, filt
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="cornell")
+    ap.add_argument("--spp", type=int, default=0)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--repeat", type=int, default=1)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=0.0, help="time the CPU oracle on a strip of rows")
+    a = ap.parse_args(argv)
+    sc, cam, filt = build(a.workload, a.scale, a.spp)
+    W, H = cam.Resolution.x, cam.Resolution.y
+    spp_eff = int(math.isqrt(cam.SampleCount)) ** 2
+    ctx = GpuContext(0)
+    t0 = time.time()
+    ctx.upload_scene(sc)
+    upload_s = time.time() - t0
+    ctx.render(cam, filt=filt, seed=a.seed, spp=1, row_end=8)   # warm-up: kernels loaded, buffers sized
+    best = None
+    for _ in range(a.repeat):
+        ctx.timer_start()
+        px = ctx.render(cam, filt=filt, seed=a.seed)
+        ms = ctx.timer_stop()
+        best = ms if best is None else min(best, ms)
+    img = px["Color"] / np.maximum(px["TotalWeight"], 1e-30)[:, None]
+    out = {"workload": a.workload, "width": W, "height": H, "spp": spp_eff, "triangles": sc.triangle_count,
+           "render_ms": round(best, 3), "msamples_per_s": round(W * H * spp_eff / (best * 1e-3) / 1e6, 3),
+           "upload_s": round(upload_s, 2), "mean_radiance": [round(float(x), 6) for x in img.mean(0)],
+           "filter": "gaussian" if filt == N.FilterType_Gaussian else "box"}
+    if a.cpu_seconds > 0:
+        import oracle
+        threads = max(1, min(16, os.cpu_count() or 1))
+        rows = 1
+        t0 = time.time()
+        oracle.render(sc, cam, filt=filt, seed=a.seed, row_begin=H // 2, row_end=H // 2 + rows, threads=threads)
+        dt = time.time() - t0
+        rows = int(max(1, min(H, rows * a.cpu_seconds / max(dt, 1e-3))))
+        t0 = time.time()
+        oracle.render(sc, cam, filt=filt, seed=a.seed, row_begin=(H - rows) // 2, row_end=(H - rows) // 2 + rows,
+                      threads=threads)
+        dt = time.time() - t0
+        rate = W * rows * spp_eff / dt / 1e6
+        out["cpu_baseline"] = {"value": round(rate, 5), "unit": "Msamples/s", "cores": threads, "kind": "port",
+                               "sample": f"{rows} middle rows x {W} px x {spp_eff} spp in {dt:.1f}s; a full "
+                                         f"image extrapolates to {W * H * spp_eff / (rate * 1e6):.0f}s"}
+        out["speedup_vs_cpu"] = round(out["msamples_per_s"] / rate, 1)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()
