@@ -87,6 +87,7 @@ struct DevScene {
     float epsIntersection, epsShadow;
     uint32_t triTotal;
     uint32_t pad;
+    float boxMin[3], boxInv[3];      // scene box: guide positions are (P - min) * inv
 };
 
 // ---- render queues --------------------------------------------------------
@@ -118,7 +119,8 @@ struct alignas(16) ShadowRay {
     uint32_t sample;
     uint32_t mode;                   // kShadowAny or kShadowEntity
     uint32_t entityId;               // kShadowEntity: the light's UniqueEntityID
-    uint32_t pad0, pad1;
+    uint32_t rec;                    // guide training record of this ray, or kEmpty
+    uint32_t pad1;
 };
 
 struct CameraSetup {                 // main.cpp:1265-1347

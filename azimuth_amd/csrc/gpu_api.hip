@@ -229,6 +229,15 @@ az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
     }
     info.topEntities = topOrdered;
     info.maxTlasStack = ts.maxStack;
+    for (int k = 0; k < 3; ++k) {
+        info.boxMin[k] = INFINITY;
+        info.boxMax[k] = -INFINITY;
+    }
+    for (const BuildBox &b : tboxes)
+        for (int k = 0; k < 3; ++k) {
+            info.boxMin[k] = std::min(info.boxMin[k], b.lo[k]);
+            info.boxMax[k] = std::max(info.boxMax[k], b.hi[k]);
+        }
     info.stackNeeded = ts.maxStack + 1 + info.maxBlasStack;
 
     // ---- upload ---------------------------------------------------------------

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "azimuth_gpu.h"
+#include "azimuth_guide.h"
 #include "device_render.h"
 #include "device_scene.h"
 #include "kernels.h"
@@ -52,11 +53,16 @@ struct UploadInfo {
     uint32_t triTotal = 0;
     uint32_t maxBlasStack = 0, maxTlasStack = 0, stackNeeded = 0;
     uint32_t leafCount = 0;
+    float boxMin[3] = {0.0f, 0.0f, 0.0f}, boxMax[3] = {1.0f, 1.0f, 1.0f};   // scene bounds
 };
 
 }  // namespace az
 
 void az_release_render_buffers(void *buffers);
+
+// Guide calls ordered on a caller's stream (the renderer's).
+az_status az_guide_infer_on_stream(az_guide *G, const void *d_Queries, u32 N, float *d_Lobes, hipStream_t s);
+az_status az_guide_train_on_stream(az_guide *G, const void *d_Samples, u32 N, hipStream_t s);
 
 struct az_gpu_ctx {
     int device = 0;
@@ -77,11 +83,13 @@ struct az_gpu_ctx {
     az::RenderState render;
     az::DevScene devScene{};
     void *renderBuffers = nullptr;   // owned; released by az_release_render_buffers
+    az_guide *guide = nullptr;       // owned; the learned guide of guided renders
     az_stats stats{};
     bool recordLaunches = false;
     std::vector<hipEvent_t> launchEvents;   // pairs: start, stop
     size_t launchUsed = 0;                  // pairs in use since the last collect
     ~az_gpu_ctx() {
+        if (guide) az_guide_destroy(guide);
         az_release_render_buffers(renderBuffers);
         for (hipEvent_t e : launchEvents) (void)hipEventDestroy(e);
         if (ev0) (void)hipEventDestroy(ev0);

@@ -1,0 +1,662 @@
+// guide.hip -- the learned path guide: an MLP from an encoded hit point to a
+// mixture of 8 von Mises-Fisher lobes, with BF16 MFMA inference and training
+// kernels (FP32 accumulation, FP32 master weights, Adam).  No counterpart in
+// the reference renderer.
+//
+// Network: 128 inputs (16 sin/cos frequencies of the box-normalized position,
+// the normal, the roughness), 4 hidden layers of 64 (ReLU), 40 outputs
+// (8 logits, 8 raw mean directions, 8 log-concentrations).
+//
+// Matrix layout.  Activations are kept transposed, features x samples, and a
+// wave owns 16 samples.  A layer computes Y^T = W^T X^T with
+// v_mfma_f32_16x16x16bf16_1k: A = a 16x16 block of W^T (out x in), B = a
+// 16x16 block of X^T (in x samples).  The 16x16 result block in the
+// accumulator layout (lane l holds rows 4(l/16)..+3 of column l%16) is the B
+// operand layout (lane l holds k = 4(l/16)..+3 of column l%16), so each
+// layer's output tiles feed the next layer without moving data.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "azimuth_gpu.h"
+#include "azimuth_guide.h"
+#include "gpu_context.h"
+#include "guide.h"
+
+namespace az {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// ---- parameter layout ------------------------------------------------------
+__host__ __device__ constexpr int layer_in(int l) { return l == 0 ? kGuideIn : kGuideHid; }
+__host__ __device__ constexpr int layer_out(int l) { return l == 4 ? kGuideOut : kGuideHid; }
+__host__ __device__ constexpr int layer_out_pad(int l) { return l == 4 ? kGuideOutPad : kGuideHid; }
+// FP32 master: W_l [in][out] then b_l [out], layer after layer.
+__host__ __device__ constexpr int w_off(int l) {
+    return l == 0 ? 0 : w_off(l - 1) + layer_in(l - 1) * layer_out(l - 1) + layer_out(l - 1);
+}
+__host__ __device__ constexpr int b_off(int l) { return w_off(l) + layer_in(l) * layer_out(l); }
+constexpr int kParamCount = w_off(5);
+// BF16 copies, padded outputs: W^T [out_pad][in] and W [in][out_pad].
+__host__ __device__ constexpr int h_off(int l) {
+    return l == 0 ? 0 : h_off(l - 1) + layer_in(l - 1) * layer_out_pad(l - 1);
+}
+constexpr int kHalfCount = h_off(5);
+
+// ---- bf16 and MFMA ------------------------------------------------------------
+__device__ inline short f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7F800000u) == 0x7F800000u) return (short)(u >> 16);   // inf/nan
+    u += 0x7FFFu + ((u >> 16) & 1u);                                 // round to nearest even
+    return (short)(u >> 16);
+}
+__device__ inline float bf2f(short h) { return __uint_as_float(((uint32_t)(uint16_t)h) << 16); }
+
+__device__ inline v4f mfma16(v4s a, v4s b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+
+__device__ inline v4s load4(const short *p) { return *reinterpret_cast<const v4s *>(p); }
+
+// ---- input encoding ----------------------------------------------------------------
+// Feature f < 96: frequency j = f/6, coordinate c = (f%6)/2, sin (f even) or
+// cos (f odd) of pi * 2^j * p[c]; 96..98 the normal, 99 the roughness, then 0.
+__device__ inline float encode_feature(int f, const float *p, const float *n, float rough) {
+    if (f < 96) {
+        int j = f / 6, c = (f % 6) >> 1;
+        float v = (float)(1u << j) * p[c];
+        return (f & 1) ? cospif(v) : sinpif(v);
+    }
+    if (f < 99) return n[f - 96];
+    if (f == 99) return rough;
+    return 0.0f;
+}
+
+// The 8 B-operand tiles of the encoded input for this lane's sample.
+__device__ inline void encode_tiles(const float *p, const float *n, float rough, int lane, v4s in[8]) {
+    const int kq = 4 * (lane >> 4);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        v4s v;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = f2bf(encode_feature(16 * t + kq + i, p, n, rough));
+        in[t] = v;
+    }
+}
+
+// One dense layer for the wave's 16 samples: out tiles (NO of them) from in
+// tiles (NI), A blocks read from W^T [out_pad][in] at `wt`, bias from `b`.
+template <int NI, int NO, bool RELU>
+__device__ inline void layer_fwd(const short *__restrict__ wt, const float *__restrict__ b, int in, const v4s *x,
+                                 v4f *acc, v4s *y, int lane) {
+    const int row = lane & 15, kq = 4 * (lane >> 4);
+#pragma unroll
+    for (int mt = 0; mt < NO; ++mt) {
+        v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kt = 0; kt < NI; ++kt) {
+            v4s a = load4(wt + (size_t)(16 * mt + row) * in + 16 * kt + kq);
+            c = mfma16(a, x[kt], c);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float v = c[j] + b[16 * mt + kq + j];
+            if (RELU) v = v > 0.0f ? v : 0.0f;
+            c[j] = v;
+        }
+        acc[mt] = c;
+        if (y) {
+            v4s h;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) h[j] = f2bf(c[j]);
+            y[mt] = h;
+        }
+    }
+}
+
+// Mixture parameters from the 40 raw outputs of one sample.
+__device__ inline void lobes_from_outputs(const float *o, float *lobes /* 8 x {w, mx, my, mz, k} */) {
+    float m = o[0];
+    for (int k = 1; k < 8; ++k) m = fmaxf(m, o[k]);
+    float e[8], s = 0.0f;
+    for (int k = 0; k < 8; ++k) {
+        e[k] = expf(o[k] - m);
+        s += e[k];
+    }
+    for (int k = 0; k < 8; ++k) {
+        float x = o[8 + 3 * k], y = o[9 + 3 * k], z = o[10 + 3 * k];
+        float l = sqrtf(x * x + y * y + z * z);
+        float inv = l > 1e-12f ? 1.0f / l : 0.0f;
+        float kr = fminf(fmaxf(o[32 + k], kGuideKMin), kGuideKMax);
+        lobes[5 * k + 0] = e[k] / s;
+        lobes[5 * k + 1] = x * inv;
+        lobes[5 * k + 2] = y * inv;
+        lobes[5 * k + 3] = z * inv;
+        lobes[5 * k + 4] = expf(kr);
+    }
+}
+
+// ---- kernels ----------------------------------------------------------------------------
+
+// Lane-layout check of the MFMA builtin: D = A * B for 16x16 matrices given
+// row-major in memory, loaded in the operand layouts this file assumes.
+__global__ __launch_bounds__(64) void k_mfma_selftest(const float *A, const float *B, float *D) {
+    int lane = threadIdx.x;
+    int row = lane & 15, kq = 4 * (lane >> 4);
+    v4s a, b;
+    for (int i = 0; i < 4; ++i) {
+        a[i] = f2bf(A[row * 16 + kq + i]);     // A[m = row][k = kq + i]
+        b[i] = f2bf(B[(kq + i) * 16 + row]);   // B[k = kq + i][n = row]
+    }
+    v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+    c = mfma16(a, b, c);
+    for (int j = 0; j < 4; ++j) D[(kq + j) * 16 + row] = c[j];   // D[m = kq + j][n = row]
+}
+
+// Inference: per query (p01, n, rough) -> 8 lobes x {w, mu, kappa}.
+__global__ __launch_bounds__(256) void k_guide_infer(GuideWeights W, const GuideQuery *__restrict__ q, uint32_t n,
+                                                     float *__restrict__ lobes) {
+    __shared__ float outs[4][16][kGuideOutPad];
+    // Every wave reaches the barrier below; waves past the end work on a
+    // clamped copy of the last query and store nothing.
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t base = (blockIdx.x * 4 + wave) * 16;
+    const uint32_t s = base + (lane & 15);
+    const bool valid = s < n;
+    GuideQuery g = q[valid ? s : n - 1];
+    v4s x[8], h1[4], h2[4];
+    v4f acc[4];
+    encode_tiles(g.p, g.n, g.rough, lane, x);
+    layer_fwd<8, 4, true>(W.wt + h_off(0), W.bias + 0 * kGuideBiasStride, kGuideIn, x, acc, h1, lane);
+    layer_fwd<4, 4, true>(W.wt + h_off(1), W.bias + 1 * kGuideBiasStride, kGuideHid, h1, acc, h2, lane);
+    layer_fwd<4, 4, true>(W.wt + h_off(2), W.bias + 2 * kGuideBiasStride, kGuideHid, h2, acc, h1, lane);
+    layer_fwd<4, 4, true>(W.wt + h_off(3), W.bias + 3 * kGuideBiasStride, kGuideHid, h1, acc, h2, lane);
+    layer_fwd<4, 3, false>(W.wt + h_off(4), W.bias + 4 * kGuideBiasStride, kGuideHid, h2, acc, nullptr, lane);
+    const int col = lane & 15, kq = 4 * (lane >> 4);
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) outs[wave][col][16 * t + kq + j] = acc[t][j];
+    __syncthreads();
+    if (lane < 16 && valid) {
+        float lb[40];
+        lobes_from_outputs(outs[wave][lane], lb);
+        float *dst = lobes + (size_t)s * 40;
+        for (int k = 0; k < 40; ++k) dst[k] = lb[k];
+    }
+}
+
+// Training pass 1: forward, loss gradient, backward through the activations.
+// Writes H_l^T and dY_l^T (features x batch, bf16) for the weight-gradient
+// GEMMs, the bias gradients (atomics) and the per-sample loss.
+__global__ __launch_bounds__(256) void k_guide_train_pass1(GuideWeights W, const GuideSample *__restrict__ smp,
+                                                           uint32_t n, float invN, GuideTrainBuffers T) {
+    __shared__ float outs[4][16][kGuideOutPad];
+    // n is a multiple of 16 (checked on the host), so a wave is either wholly
+    // inside the batch or wholly past it; the latter computes on a clamped
+    // copy of the last sample, reaches every barrier and stores nothing.
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t base = (blockIdx.x * 4 + wave) * 16;
+    const bool active = base < n;
+    const int col = lane & 15, kq = 4 * (lane >> 4);
+    const uint32_t s = active ? base + col : n - 1;
+    const GuideSample g = smp[s];
+    v4s x[8], h[4][4];
+    v4f acc[4], out[3];
+    encode_tiles(g.p, g.n, g.rough, lane, x);
+    layer_fwd<8, 4, true>(W.wt + h_off(0), W.bias + 0 * kGuideBiasStride, kGuideIn, x, acc, h[0], lane);
+    layer_fwd<4, 4, true>(W.wt + h_off(1), W.bias + 1 * kGuideBiasStride, kGuideHid, h[0], acc, h[1], lane);
+    layer_fwd<4, 4, true>(W.wt + h_off(2), W.bias + 2 * kGuideBiasStride, kGuideHid, h[1], acc, h[2], lane);
+    layer_fwd<4, 4, true>(W.wt + h_off(3), W.bias + 3 * kGuideBiasStride, kGuideHid, h[2], acc, h[3], lane);
+    layer_fwd<4, 3, false>(W.wt + h_off(4), W.bias + 4 * kGuideBiasStride, kGuideHid, h[3], out, nullptr, lane);
+    // Store H^T (the inputs of every layer) for the weight gradients.
+    auto storeT = [&](short *dst, int tile, v4s v) {
+        if (!active) return;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) dst[(size_t)(16 * tile + kq + j) * T.batch + s] = v[j];
+    };
+#pragma unroll
+    for (int t = 0; t < 8; ++t) storeT(T.hT + T.hOff[0], t, x[t]);
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) storeT(T.hT + T.hOff[l + 1], t, h[l][t]);
+    // Output gradient per sample (lanes 0..15 own one sample each).
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) outs[wave][col][16 * t + kq + j] = out[t][j];
+    __syncthreads();
+    if (lane < 16) {
+        float *o = outs[wave][lane];
+        float lb[40];
+        lobes_from_outputs(o, lb);
+        float w[3] = {g.dir[0], g.dir[1], g.dir[2]};
+        float lp[8], mx = -INFINITY;
+        for (int k = 0; k < 8; ++k) {
+            float kap = lb[5 * k + 4];
+            float c = lb[5 * k + 1] * w[0] + lb[5 * k + 2] * w[1] + lb[5 * k + 3] * w[2];
+            lp[k] = logf(kap) - logf(2.0f * 3.14159265f) - log1pf(-expf(-2.0f * kap)) + kap * (c - 1.0f) +
+                    logf(fmaxf(lb[5 * k], 1e-30f));
+            mx = fmaxf(mx, lp[k]);
+        }
+        float se = 0.0f;
+        for (int k = 0; k < 8; ++k) se += expf(lp[k] - mx);
+        float lq = mx + logf(se);
+        if (active) T.loss[s] = -g.wgt * lq;
+        float gq = -g.wgt * invN;                        // dL / dlog q
+        float d[kGuideOutPad];
+        for (int k = 0; k < kGuideOutPad; ++k) d[k] = 0.0f;
+        for (int k = 0; k < 8; ++k) {
+            float r = expf(lp[k] - lq);
+            float pi = lb[5 * k], kap = lb[5 * k + 4];
+            d[k] = gq * (r - pi);
+            float rx = o[8 + 3 * k], ry = o[9 + 3 * k], rz = o[10 + 3 * k];
+            float rn = fmaxf(sqrtf(rx * rx + ry * ry + rz * rz), 1e-12f);
+            float c = lb[5 * k + 1] * w[0] + lb[5 * k + 2] * w[1] + lb[5 * k + 3] * w[2];
+            for (int a = 0; a < 3; ++a) d[8 + 3 * k + a] = gq * r * kap * (w[a] - c * lb[5 * k + 1 + a]) / rn;
+            float kr = o[32 + k];
+            bool inside = kr > kGuideKMin && kr < kGuideKMax;
+            float dl = 1.0f / kap - 2.0f / expm1f(2.0f * kap) + (c - 1.0f);
+            d[32 + k] = inside ? gq * r * dl * kap : 0.0f;
+        }
+        for (int k = 0; k < kGuideOutPad; ++k) o[k] = d[k];
+    }
+    __syncthreads();
+    // dY of the output layer as B tiles; bias gradient; store dY^T.
+    v4f dy[4];
+    v4s dyb[4];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) dy[t][j] = outs[wave][col][16 * t + kq + j];
+    }
+    auto biasGrad = [&](int l, int t, v4f v) {
+        // Sum over the 16 samples (lanes with the same lane/16) and add.
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float x = v[j];
+            for (int off = 8; off > 0; off >>= 1) x += __shfl_xor(x, off, 16);
+            if (col == 0 && active) atomicAdd(T.grad + b_off(l) + 16 * t + kq + j, x);
+        }
+    };
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) dyb[t][j] = f2bf(dy[t][j]);
+        storeT(T.dyT + T.dyOff[4], t, dyb[t]);
+        if (16 * t + kq < kGuideOut) biasGrad(4, t, dy[t]);
+    }
+    // Backward: dX^T = W dY^T (A = W [in][out_pad]), masked by ReLU'.
+    const int row = lane & 15;
+    auto back = [&](int l, int NO, const v4s *dyIn, const v4s *hPrev, v4s *dyOut, v4f *dyF) {
+        const int out = layer_out_pad(l);
+        for (int mt = 0; mt < 4; ++mt) {
+            v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int kt = 0; kt < NO; ++kt) {
+                v4s a = load4(W.wb + h_off(l) + (size_t)(16 * mt + row) * out + 16 * kt + kq);
+                c = mfma16(a, dyIn[kt], c);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) c[j] = bf2f(hPrev[mt][j]) > 0.0f ? c[j] : 0.0f;
+            dyF[mt] = c;
+            v4s hb;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) hb[j] = f2bf(c[j]);
+            dyOut[mt] = hb;
+        }
+    };
+    v4s g4[4], g3[4];
+    v4f gf[4];
+    back(4, 3, dyb, h[3], g4, gf);                      // dY of layer 3 (hidden 4)
+    for (int t = 0; t < 4; ++t) {
+        storeT(T.dyT + T.dyOff[3], t, g4[t]);
+        biasGrad(3, t, gf[t]);
+    }
+    back(3, 4, g4, h[2], g3, gf);
+    for (int t = 0; t < 4; ++t) {
+        storeT(T.dyT + T.dyOff[2], t, g3[t]);
+        biasGrad(2, t, gf[t]);
+    }
+    back(2, 4, g3, h[1], g4, gf);
+    for (int t = 0; t < 4; ++t) {
+        storeT(T.dyT + T.dyOff[1], t, g4[t]);
+        biasGrad(1, t, gf[t]);
+    }
+    back(1, 4, g4, h[0], g3, gf);
+    for (int t = 0; t < 4; ++t) {
+        storeT(T.dyT + T.dyOff[0], t, g3[t]);
+        biasGrad(0, t, gf[t]);
+    }
+}
+
+// Training pass 2: dW_l [in][out] = sum over the batch of H_l^T (in x batch)
+// times dY_l (batch x out).  One wave per 16x16 output tile and K-slice;
+// slices are added with atomics.  A[m = in][k = sample] is read from H^T,
+// B[k = sample][n = out] from dY^T.
+__global__ __launch_bounds__(64) void k_guide_train_dw(GuideTrainBuffers T, float *__restrict__ grad, int layer,
+                                                       uint32_t kSlice) {
+    const int lane = threadIdx.x;
+    const int in = layer_in(layer), out = layer_out(layer);
+    const int tilesN = (out + 15) / 16;
+    const int tile = blockIdx.x, mt = tile / tilesN, nt = tile % tilesN;
+    const uint32_t k0 = blockIdx.y * kSlice, k1 = min(k0 + kSlice, T.batch);
+    const int row = lane & 15, kq = 4 * (lane >> 4);
+    const short *hT = T.hT + T.hOff[layer] + (size_t)(16 * mt + row) * T.batch;
+    const int ocol = 16 * nt + row;
+    const short *dT = T.dyT + T.dyOff[layer] + (size_t)ocol * T.batch;
+    v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t k = k0; k < k1; k += 16) {
+        v4s a = load4(hT + k + kq);
+        v4s b = load4(dT + k + kq);
+        c = mfma16(a, b, c);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        int i = 16 * mt + kq + j, o = 16 * nt + row;
+        if (i < in && o < out) atomicAdd(grad + w_off(layer) + (size_t)i * out + o, c[j]);
+    }
+}
+
+// Adam on the FP32 masters, then the BF16 copies in both layouts.
+__global__ __launch_bounds__(256) void k_guide_adam(float *__restrict__ p, float *__restrict__ g, float *__restrict__ m,
+                                                    float *__restrict__ v, float lr, float b1, float b2, float eps,
+                                                    float bc1, float bc2, int count) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= count) return;
+    float gr = g[i];
+    float mm = b1 * m[i] + (1.0f - b1) * gr;
+    float vv = b2 * v[i] + (1.0f - b2) * gr * gr;
+    m[i] = mm;
+    v[i] = vv;
+    p[i] -= lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
+    g[i] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_guide_refresh(const float *__restrict__ p, short *__restrict__ wt,
+                                                       short *__restrict__ wb, float *__restrict__ bias) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    for (int l = 0; l < 5; ++l) {
+        const int in = layer_in(l), out = layer_out(l), op = layer_out_pad(l);
+        if (i < in * op) {
+            int r = i / in, c = i % in;          // W^T [out_pad][in]
+            wt[h_off(l) + i] = f2bf(r < out ? p[w_off(l) + c * out + r] : 0.0f);
+            int r2 = i / op, c2 = i % op;        // W [in][out_pad]
+            wb[h_off(l) + i] = f2bf(c2 < out ? p[w_off(l) + r2 * out + c2] : 0.0f);
+        }
+        if (i < kGuideBiasStride) bias[l * kGuideBiasStride + i] = i < out ? p[b_off(l) + i] : 0.0f;
+    }
+}
+
+}  // namespace az
+
+using namespace az;
+
+// ---- host ----------------------------------------------------------------------------------
+struct az_guide {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DeviceBuffer params, grad, m, v, wt, wb, bias, samples, hT, dyT, loss, queries, lobes, scratch;
+    uint32_t step = 0;
+    uint32_t batchCap = 0;
+    std::string lastError;
+    float lr = 1e-3f;
+};
+
+#define AZ_GHIP(g, call)                                                       \
+    do {                                                                       \
+        hipError_t _e = (call);                                                \
+        if (_e != hipSuccess) {                                                \
+            (g)->lastError = std::string(#call ": ") + hipGetErrorString(_e);  \
+            return AZ_ERR_DEVICE;                                              \
+        }                                                                      \
+    } while (0)
+
+static GuideWeights weightsOf(az_guide *G) {
+    GuideWeights w;
+    w.wt = (const short *)G->wt.ptr;
+    w.wb = (const short *)G->wb.ptr;
+    w.bias = (const float *)G->bias.ptr;
+    return w;
+}
+
+static az_status refresh(az_guide *G) {
+    int n = kGuideIn * kGuideHid;   // the largest layer
+    hipLaunchKernelGGL(k_guide_refresh, dim3((n + 255) / 256), dim3(256), 0, G->stream, (const float *)G->params.ptr,
+                       (short *)G->wt.ptr, (short *)G->wb.ptr, (float *)G->bias.ptr);
+    AZ_GHIP(G, hipGetLastError());
+    return AZ_OK;
+}
+
+extern "C" {
+
+u32 az_guide_param_count(void) { return (u32)kParamCount; }
+
+az_status az_guide_create(az_guide **Out, int dev, u64 Seed) {
+    if (!Out) return AZ_ERR_INVALID_ARGUMENT;
+    *Out = nullptr;
+    az_guide *G = new (std::nothrow) az_guide();
+    if (!G) return AZ_ERR_OUT_OF_MEMORY;
+    G->device = dev;
+    if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&G->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete G;
+        return AZ_ERR_DEVICE;
+    }
+    // He-normal initial weights from a seeded generator (Box-Muller on
+    // splitmix64), zero biases.
+    std::vector<float> p(kParamCount, 0.0f);
+    uint64_t st = Seed * 0x9E3779B97F4A7C15ull + 1;
+    auto u01 = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    };
+    for (int l = 0; l < 5; ++l) {
+        double sd = std::sqrt(2.0 / layer_in(l));
+        for (int i = 0; i < layer_in(l) * layer_out(l); ++i) {
+            double a = u01(), b = u01();
+            p[w_off(l) + i] = (float)(sd * std::sqrt(-2.0 * std::log(a)) * std::cos(2.0 * M_PI * b));
+        }
+    }
+    az_status s = az_guide_set_params(G, p.data(), kParamCount);
+    if (s != AZ_OK) {
+        az_guide_destroy(G);
+        return s;
+    }
+    *Out = G;
+    return AZ_OK;
+}
+
+void az_guide_destroy(az_guide *G) {
+    if (!G) return;
+    (void)hipSetDevice(G->device);
+    (void)hipStreamSynchronize(G->stream);
+    G->params.release();
+    if (G->stream) (void)hipStreamDestroy(G->stream);
+    delete G;
+}
+
+const char *az_guide_last_error(const az_guide *G) { return G ? G->lastError.c_str() : "no guide"; }
+
+az_status az_guide_set_params(az_guide *G, const f32 *Params, u32 Count) {
+    if (!G || !Params || Count != (u32)kParamCount) return AZ_ERR_INVALID_ARGUMENT;
+    AZ_GHIP(G, hipSetDevice(G->device));
+    size_t pb = (size_t)kParamCount * sizeof(float);
+    AZ_GHIP(G, G->params.alloc(pb));
+    AZ_GHIP(G, G->grad.alloc(pb));
+    AZ_GHIP(G, G->m.alloc(pb));
+    AZ_GHIP(G, G->v.alloc(pb));
+    AZ_GHIP(G, G->wt.alloc((size_t)kHalfCount * sizeof(short)));
+    AZ_GHIP(G, G->wb.alloc((size_t)kHalfCount * sizeof(short)));
+    AZ_GHIP(G, G->bias.alloc((size_t)5 * kGuideBiasStride * sizeof(float)));
+    AZ_GHIP(G, hipMemcpyAsync(G->params.ptr, Params, pb, hipMemcpyHostToDevice, G->stream));
+    AZ_GHIP(G, hipMemsetAsync(G->grad.ptr, 0, pb, G->stream));
+    AZ_GHIP(G, hipMemsetAsync(G->m.ptr, 0, pb, G->stream));
+    AZ_GHIP(G, hipMemsetAsync(G->v.ptr, 0, pb, G->stream));
+    G->step = 0;
+    az_status s = refresh(G);
+    if (s != AZ_OK) return s;
+    AZ_GHIP(G, hipStreamSynchronize(G->stream));
+    return AZ_OK;
+}
+
+az_status az_guide_get_params(az_guide *G, f32 *Params, u32 Count) {
+    if (!G || !Params || Count != (u32)kParamCount) return AZ_ERR_INVALID_ARGUMENT;
+    AZ_GHIP(G, hipMemcpyAsync(Params, G->params.ptr, (size_t)kParamCount * sizeof(float), hipMemcpyDeviceToHost,
+                              G->stream));
+    AZ_GHIP(G, hipStreamSynchronize(G->stream));
+    return AZ_OK;
+}
+
+az_status az_guide_get_grad(az_guide *G, f32 *Grad, u32 Count) {
+    if (!G || !Grad || Count != (u32)kParamCount) return AZ_ERR_INVALID_ARGUMENT;
+    AZ_GHIP(G, hipMemcpyAsync(Grad, G->grad.ptr, (size_t)kParamCount * sizeof(float), hipMemcpyDeviceToHost,
+                              G->stream));
+    AZ_GHIP(G, hipStreamSynchronize(G->stream));
+    return AZ_OK;
+}
+
+az_status az_guide_infer_device(az_guide *G, const void *d_Queries, u32 N, f32 *d_Lobes) {
+    if (!G || (N && (!d_Queries || !d_Lobes))) return AZ_ERR_INVALID_ARGUMENT;
+    if (N == 0) return AZ_OK;
+    dim3 grid((N + 63) / 64);
+    hipLaunchKernelGGL(k_guide_infer, grid, dim3(256), 0, G->stream, weightsOf(G), (const GuideQuery *)d_Queries, N,
+                       d_Lobes);
+    AZ_GHIP(G, hipGetLastError());
+    return AZ_OK;
+}
+
+az_status az_guide_infer(az_guide *G, const f32 *Queries, u32 N, f32 *Lobes) {
+    if (!G || (N && (!Queries || !Lobes))) return AZ_ERR_INVALID_ARGUMENT;
+    if (N == 0) return AZ_OK;
+    AZ_GHIP(G, G->queries.alloc((size_t)N * sizeof(GuideQuery)));
+    AZ_GHIP(G, G->lobes.alloc((size_t)N * 40 * sizeof(float)));
+    AZ_GHIP(G, hipMemcpyAsync(G->queries.ptr, Queries, (size_t)N * sizeof(GuideQuery), hipMemcpyHostToDevice, G->stream));
+    az_status s = az_guide_infer_device(G, G->queries.ptr, N, (f32 *)G->lobes.ptr);
+    if (s != AZ_OK) return s;
+    AZ_GHIP(G, hipMemcpyAsync(Lobes, G->lobes.ptr, (size_t)N * 40 * sizeof(float), hipMemcpyDeviceToHost, G->stream));
+    AZ_GHIP(G, hipStreamSynchronize(G->stream));
+    return AZ_OK;
+}
+
+// One training step on N device-resident samples (N a multiple of 16):
+// gradients of the mean loss, then Adam unless ApplyUpdate is 0.
+az_status az_guide_train_device(az_guide *G, const void *d_Samples, u32 N, b32 ApplyUpdate, f32 *MeanLoss) {
+    if (!G || !d_Samples || N == 0 || (N % 16) != 0) return AZ_ERR_INVALID_ARGUMENT;
+    AZ_GHIP(G, hipSetDevice(G->device));
+    if (N > G->batchCap) {
+        AZ_GHIP(G, G->hT.alloc((size_t)N * (kGuideIn + 4 * kGuideHid) * sizeof(short)));
+        AZ_GHIP(G, G->dyT.alloc((size_t)N * (4 * kGuideHid + kGuideOutPad) * sizeof(short)));
+        AZ_GHIP(G, G->loss.alloc((size_t)N * sizeof(float)));
+        G->batchCap = N;
+    }
+    GuideTrainBuffers T;
+    T.batch = N;
+    T.hT = (short *)G->hT.ptr;
+    T.dyT = (short *)G->dyT.ptr;
+    T.loss = (float *)G->loss.ptr;
+    T.grad = (float *)G->grad.ptr;
+    size_t off = 0;
+    for (int l = 0; l < 5; ++l) {
+        T.hOff[l] = off;
+        off += (size_t)layer_in(l) * N;
+    }
+    off = 0;
+    for (int l = 0; l < 5; ++l) {
+        T.dyOff[l] = off;
+        off += (size_t)layer_out_pad(l) * N;
+    }
+    dim3 g1((N + 63) / 64);
+    hipLaunchKernelGGL(k_guide_train_pass1, g1, dim3(256), 0, G->stream, weightsOf(G), (const GuideSample *)d_Samples, N,
+                       1.0f / (float)N, T);
+    AZ_GHIP(G, hipGetLastError());
+    const uint32_t kSlice = 1024;
+    for (int l = 0; l < 5; ++l) {
+        int tiles = ((layer_in(l) + 15) / 16) * ((layer_out(l) + 15) / 16);
+        dim3 g2(tiles, (N + kSlice - 1) / kSlice);
+        hipLaunchKernelGGL(k_guide_train_dw, g2, dim3(64), 0, G->stream, T, (float *)G->grad.ptr, l, kSlice);
+        AZ_GHIP(G, hipGetLastError());
+    }
+    if (MeanLoss) {
+        std::vector<float> h(N);
+        AZ_GHIP(G, hipMemcpyAsync(h.data(), G->loss.ptr, (size_t)N * sizeof(float), hipMemcpyDeviceToHost, G->stream));
+        AZ_GHIP(G, hipStreamSynchronize(G->stream));
+        double s = 0.0;
+        for (float x : h) s += x;
+        *MeanLoss = (f32)(s / N);
+    }
+    if (ApplyUpdate) {
+        G->step++;
+        float bc1 = 1.0f - powf(0.9f, (float)G->step), bc2 = 1.0f - powf(0.999f, (float)G->step);
+        hipLaunchKernelGGL(k_guide_adam, dim3((kParamCount + 255) / 256), dim3(256), 0, G->stream, (float *)G->params.ptr,
+                           (float *)G->grad.ptr, (float *)G->m.ptr, (float *)G->v.ptr, G->lr, 0.9f, 0.999f, 1e-8f, bc1,
+                           bc2, kParamCount);
+        AZ_GHIP(G, hipGetLastError());
+        az_status s = refresh(G);
+        if (s != AZ_OK) return s;
+    }
+    return AZ_OK;
+}
+
+az_status az_guide_train(az_guide *G, const f32 *Samples, u32 N, b32 ApplyUpdate, f32 *MeanLoss) {
+    if (!G || !Samples || N == 0 || (N % 16) != 0) return AZ_ERR_INVALID_ARGUMENT;
+    AZ_GHIP(G, G->samples.alloc((size_t)N * sizeof(GuideSample)));
+    AZ_GHIP(G, hipMemcpyAsync(G->samples.ptr, Samples, (size_t)N * sizeof(GuideSample), hipMemcpyHostToDevice,
+                              G->stream));
+    az_status s = az_guide_train_device(G, G->samples.ptr, N, ApplyUpdate, MeanLoss);
+    if (s != AZ_OK) return s;
+    AZ_GHIP(G, hipStreamSynchronize(G->stream));
+    return AZ_OK;
+}
+
+az_status az_guide_zero_grad(az_guide *G) {
+    if (!G) return AZ_ERR_INVALID_ARGUMENT;
+    AZ_GHIP(G, hipMemsetAsync(G->grad.ptr, 0, (size_t)kParamCount * sizeof(float), G->stream));
+    return AZ_OK;
+}
+
+az_status az_guide_set_learning_rate(az_guide *G, f32 Lr) {
+    if (!G || !(Lr > 0.0f)) return AZ_ERR_INVALID_ARGUMENT;
+    G->lr = Lr;
+    return AZ_OK;
+}
+
+}  // extern "C"
+
+// The renderer's calls: the guide's kernels ordered on the render stream.
+az_status az_guide_infer_on_stream(az_guide *G, const void *d_Queries, u32 N, float *d_Lobes, hipStream_t s) {
+    hipStream_t own = G->stream;
+    G->stream = s;
+    az_status st = az_guide_infer_device(G, d_Queries, N, d_Lobes);
+    G->stream = own;
+    return st;
+}
+
+az_status az_guide_train_on_stream(az_guide *G, const void *d_Samples, u32 N, hipStream_t s) {
+    hipStream_t own = G->stream;
+    G->stream = s;
+    az_status st = az_guide_train_device(G, d_Samples, N, 1, nullptr);
+    G->stream = own;
+    return st;
+}
+
+extern "C" {
+
+az_status az_guide_mfma_selftest(az_guide *G, const f32 *A, const f32 *B, f32 *D) {
+    if (!G || !A || !B || !D) return AZ_ERR_INVALID_ARGUMENT;
+    AZ_GHIP(G, G->scratch.alloc(3 * 256 * sizeof(float)));
+    float *d = (float *)G->scratch.ptr;
+    AZ_GHIP(G, hipMemcpyAsync(d, A, 256 * sizeof(float), hipMemcpyHostToDevice, G->stream));
+    AZ_GHIP(G, hipMemcpyAsync(d + 256, B, 256 * sizeof(float), hipMemcpyHostToDevice, G->stream));
+    hipLaunchKernelGGL(k_mfma_selftest, dim3(1), dim3(64), 0, G->stream, d, d + 256, d + 512);
+    AZ_GHIP(G, hipGetLastError());
+    AZ_GHIP(G, hipMemcpyAsync(D, d + 512, 256 * sizeof(float), hipMemcpyDeviceToHost, G->stream));
+    AZ_GHIP(G, hipStreamSynchronize(G->stream));
+    return AZ_OK;
+}
+
+}  // extern "C"

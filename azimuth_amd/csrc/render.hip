@@ -25,6 +25,8 @@
 #include "azimuth_gpu.h"
 #include "device_trace.h"
 #include "gpu_context.h"
+#include "azimuth_guide.h"
+#include "guide.h"
 #include "shade_math.h"
 
 namespace az {
@@ -127,7 +129,8 @@ __global__ __launch_bounds__(kRenderBlock, 6) void k_extend(DeviceGeometry G, co
 
 // ---- connect: shadow rays ------------------------------------------------------
 __global__ __launch_bounds__(kRenderBlock, 6) void k_connect(DeviceGeometry G, const ShadowRay *__restrict__ sq,
-                                                            uint32_t n, float *__restrict__ acc) {
+                                                            uint32_t n, float *__restrict__ acc,
+                                                            GuideSample *__restrict__ recs) {
     __shared__ uint32_t smem[kRenderRing * kRenderBlock];
     uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
     if (i >= n) return;
@@ -152,9 +155,68 @@ __global__ __launch_bounds__(kRenderBlock, 6) void k_connect(DeviceGeometry G, c
         reaches = !hit || G.items[h.item].entityId == s.entityId;
     }
     if (reaches) add3(acc + 3 * s.sample, v3(s.cx, s.cy, s.cz));
+    // A guide training sample whose direction is blocked saw no radiance.
+    if (s.rec != kEmpty && !reaches) recs[s.rec].wgt = 0.0f;
 }
 
 // ---- shade ------------------------------------------------------------------------
+// Learned guide inputs and training outputs of the shade kernel.
+struct GuideIO {
+    const float *lobes;              // per queued ray: 8 x {w, mu, kappa}, or null (plain sampling)
+    GuideSample *recs;               // training samples, or null when not training
+    uint32_t *recCount;
+    uint32_t recCap;
+    float alpha;                     // probability of drawing from the guide
+};
+
+// Guide query of a hit: position in the scene box, normal, roughness.
+__device__ inline float guide_roughness(const DevMaterial &M) {
+    bool spec = M.ks[0] != 0.0f || M.ks[1] != 0.0f || M.ks[2] != 0.0f;
+    if (!spec) return (M.flags & kMatMirror) && M.kd[0] == 0.0f && M.kd[1] == 0.0f && M.kd[2] == 0.0f ? 0.0f : 1.0f;
+    return 2.0f / (M.exponent + 2.0f);
+}
+
+__device__ inline void guide_position(const DevScene &S, V3 P, float *p) {
+    p[0] = fminf(fmaxf((P.x - S.boxMin[0]) * S.boxInv[0], 0.0f), 1.0f);
+    p[1] = fminf(fmaxf((P.y - S.boxMin[1]) * S.boxInv[1], 0.0f), 1.0f);
+    p[2] = fminf(fmaxf((P.z - S.boxMin[2]) * S.boxInv[2], 0.0f), 1.0f);
+}
+
+// Mixture pdf q(w) of 8 vMF lobes, each kappa / (2 pi (1 - e^-2kappa)) *
+// exp(kappa (mu.w - 1)) (Jakob 2012).
+__device__ inline float guide_pdf(const float *lb, V3 w) {
+    float q = 0.0f;
+    for (int k = 0; k < 8; ++k) {
+        const float *l = lb + 5 * k;
+        float kap = l[4];
+        float c = l[1] * w.x + l[2] * w.y + l[3] * w.z;
+        q += l[0] * kap / (2.0f * kPiF * -expm1f(-2.0f * kap)) * expf(kap * (c - 1.0f));
+    }
+    return q;
+}
+
+// Draws a direction from the mixture with three uniforms: lobe choice, then
+// Jakob's inversion for the cosine to the mean and a uniform azimuth.
+__device__ inline V3 guide_sample(const float *lb, float u0, float u1, float u2) {
+    int k = 0;
+    float c = lb[0];
+    while (k < 7 && u0 >= c) {
+        ++k;
+        c += lb[5 * k];
+    }
+    const float *l = lb + 5 * k;
+    V3 mu = v3(l[1], l[2], l[3]);
+    float kap = l[4];
+    float xi = 1.0f - u1;                                            // (0, 1]
+    float wz = 1.0f + logf(xi + (1.0f - xi) * expf(-2.0f * kap)) / kap;
+    wz = fminf(fmaxf(wz, -1.0f), 1.0f);
+    float s = sqrtf(fmaxf(0.0f, 1.0f - wz * wz));
+    V3 t, b;
+    orthonormal_basis(mu, t, b);
+    float ph = 2.0f * kPiF * u2;
+    return normalize(mu * wz + t * (s * cosf(ph)) + b * (s * sinf(ph)));
+}
+
 struct ShadeOut {
     PathRay *next;
     ShadowRay *shadow;
@@ -189,7 +251,7 @@ __device__ inline ShadowRay make_shadow(V3 o, V3 d, float time, float tmax, V3 c
     s.sample = sample;
     s.mode = mode;
     s.entityId = entityId;
-    s.pad0 = 0u;
+    s.rec = kEmpty;
     s.pad1 = 0u;
     return s;
 }
@@ -239,62 +301,16 @@ __device__ inline V3 bump(const DevScene &S, const DevTexture &T, V3 N, V3 P, fl
     return normalize(N);
 }
 
-__global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeometry G, CameraSetup cam, ChunkParams cp,
-                                                        const PathRay *__restrict__ q,
-                                                        const PathHit *__restrict__ hits, uint32_t n,
-                                                        float *__restrict__ acc, ShadeOut O) {
-    uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
-    if (i >= n) return;
-    const PathRay r = q[i];
-    const PathHit h = hits[i];
-    const uint32_t depth = r.flags & 0xFFu;
-    const bool inside = (r.flags & 0x100u) != 0u;
-    const uint32_t pixLocal = r.sample / cp.spp;
-    const uint32_t pixel = cp.row0 * cam.width + pixLocal;
-    const uint32_t s = cp.s0 + r.sample % cp.spp;
-    float *a = acc + 3 * r.sample;
-    V3 w = v3(r.wx, r.wy, r.wz);
-    V3 D = v3(r.dx, r.dy, r.dz);
-
-    if (h.prim == kEmpty) {
-        if (depth == 0u) {
-            // Camera-ray miss: SampleBackgroundColor (main.cpp:1180-1207).
-            V3 bg;
-            if (S.useEnv) {
-                float th = Acos(D.y);
-                float ph = atan2f(D.z, D.x);
-                float u = (-ph + kPiF) / (2.0f * kPiF), v = th / kPiF;
-                bg = sample_texture(S, S.env, v3(u, v, 1.0f), u, v);
-            } else if (S.useBackgroundTexture) {
-                uint32_t row = pixel / cam.width, col = pixel % cam.width;
-                float u = (float)col / (float)((int32_t)cam.width - 1), v = (float)row / (float)((int32_t)cam.height - 1);
-                bg = sample_texture(S, S.background, v3(u, v, 1.0f), u, v);
-            } else {
-                bg = v3(S.backgroundColor);
-            }
-            add3(a, bg);
-        }
-        return;
-    }
-    // Beer attenuation over this ray's segment (main.cpp:916-928, 951-958).
-    if (r.cx != 0.0f || r.cy != 0.0f || r.cz != 0.0f)
-        w = had(w, v3(Exp(r.cx * h.t), Exp(r.cy * h.t), Exp(r.cz * h.t)));
-
+// Hit point, shading normal and uv of a closest hit, as RayEntityIntersection
+// leaves them in ray_hit (main.cpp:64-146, 147-243, 244-281, 323-382).
+__device__ inline void reconstruct(const DevScene &S, const DeviceGeometry &G, const PathRay &r, const PathHit &h,
+                                   const DevItemShade &sh, V3 &P, V3 &N, float &u, float &v) {
     const TopItem &it = G.items[h.item];
-    const DevItemShade sh = S.itemShade[h.item];
-    if (sh.isEmitter) {                                   // main.cpp:984-985
-        add3(a, had(w, v3(sh.radiance)));
-        return;
-    }
-
-    // ---- hit reconstruction (main.cpp:64-146, 147-243, 244-281, 323-382) --------
     float mx = r.time * it.motion[0], my = r.time * it.motion[1], mz = r.time * it.motion[2];
     float sox = r.ox - mx, soy = r.oy - my, soz = r.oz - mz;
     float lox, loy, loz, ldx, ldy, ldz, sc;
     local_ray(it.inv, sox, soy, soz, r.dx, r.dy, r.dz, lox, loy, loz, ldx, ldy, ldz, sc);
     const V3 motion = v3(mx, my, mz);
-    V3 P, N;
-    float u, v;
     const bool textured = sh.texture != 0u;
     const DevTexture *tex = textured ? &S.textures[sh.texture] : nullptr;
     if (sh.kind == kItemSphere) {
@@ -341,6 +357,61 @@ __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeo
         P = point_to_world(Pl, it.fwd) + motion;
         N = normal_to_world(Nl, it.inv);
     }
+
+}
+
+__global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeometry G, CameraSetup cam, ChunkParams cp,
+                                                        const PathRay *__restrict__ q,
+                                                        const PathHit *__restrict__ hits, uint32_t n,
+                                                        float *__restrict__ acc, ShadeOut O, GuideIO GI) {
+    uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
+    if (i >= n) return;
+    const PathRay r = q[i];
+    const PathHit h = hits[i];
+    const uint32_t depth = r.flags & 0xFFu;
+    const bool inside = (r.flags & 0x100u) != 0u;
+    const uint32_t pixLocal = r.sample / cp.spp;
+    const uint32_t pixel = cp.row0 * cam.width + pixLocal;
+    const uint32_t s = cp.s0 + r.sample % cp.spp;
+    float *a = acc + 3 * r.sample;
+    V3 w = v3(r.wx, r.wy, r.wz);
+    V3 D = v3(r.dx, r.dy, r.dz);
+
+    if (h.prim == kEmpty) {
+        if (depth == 0u) {
+            // Camera-ray miss: SampleBackgroundColor (main.cpp:1180-1207).
+            V3 bg;
+            if (S.useEnv) {
+                float th = Acos(D.y);
+                float ph = atan2f(D.z, D.x);
+                float u = (-ph + kPiF) / (2.0f * kPiF), v = th / kPiF;
+                bg = sample_texture(S, S.env, v3(u, v, 1.0f), u, v);
+            } else if (S.useBackgroundTexture) {
+                uint32_t row = pixel / cam.width, col = pixel % cam.width;
+                float u = (float)col / (float)((int32_t)cam.width - 1), v = (float)row / (float)((int32_t)cam.height - 1);
+                bg = sample_texture(S, S.background, v3(u, v, 1.0f), u, v);
+            } else {
+                bg = v3(S.backgroundColor);
+            }
+            add3(a, bg);
+        }
+        return;
+    }
+    // Beer attenuation over this ray's segment (main.cpp:916-928, 951-958).
+    if (r.cx != 0.0f || r.cy != 0.0f || r.cz != 0.0f)
+        w = had(w, v3(Exp(r.cx * h.t), Exp(r.cy * h.t), Exp(r.cz * h.t)));
+
+    const DevItemShade sh = S.itemShade[h.item];
+    if (sh.isEmitter) {                                   // main.cpp:984-985
+        add3(a, had(w, v3(sh.radiance)));
+        return;
+    }
+
+    V3 P, N;
+    float u, v;
+    reconstruct(S, G, r, h, sh, P, N, u, v);
+    const bool textured = sh.texture != 0u;
+    const DevTexture *tex = textured ? &S.textures[sh.texture] : nullptr;
 
     // ---- the vertex (main.cpp:986-1128) ------------------------------------------------
     const DevMaterial M = S.materials[sh.material];
@@ -437,26 +508,67 @@ __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeo
         want = want && (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f);
         emit_shadow(O, want, make_shadow(P + (Ld * eps), Ld, r.time, tmax, c, r.sample, mode, L.entityId));
     }
-    // Environment NEE, cosine-distributed (main.cpp:719-778).
+    // Environment NEE, cosine-distributed (main.cpp:719-778).  With the
+    // guide, the direction comes from the guide with probability alpha and
+    // from the cosine distribution otherwise, weighted by the mixture pdf
+    // (one-sample MIS, balance heuristic; Veach 1997), which keeps the
+    // estimator unbiased: the cosine term covers every direction with
+    // non-zero contribution.
     if (S.useEnv) {
-        float phi = 2.0f * kPiF * rng.uniform();
-        float cp_ = cosf(phi), sp_ = sinf(phi);
-        float theta = Asin(sqrtf(rng.uniform()));
-        float ct = cosf(theta), st = sinf(theta);
-        V3 U, Vv;
-        orthonormal_basis(N, U, Vv);
-        V3 L = normalize(U * st * cp_ + Vv * st * sp_ + N * ct);
+        const float *lb = GI.lobes ? GI.lobes + (size_t)i * 40 : nullptr;
+        V3 L;
+        bool fromGuide = false;
+        if (lb) fromGuide = rng.uniform() < GI.alpha;
+        if (fromGuide) {
+            float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
+            L = guide_sample(lb, u0, u1, u2);
+        } else {
+            float phi = 2.0f * kPiF * rng.uniform();
+            float cp_ = cosf(phi), sp_ = sinf(phi);
+            float theta = Asin(sqrtf(rng.uniform()));
+            float ct = cosf(theta), st = sinf(theta);
+            V3 U, Vv;
+            orthonormal_basis(N, U, Vv);
+            L = normalize(U * st * cp_ + Vv * st * sp_ + N * ct);
+        }
         float eu = (-atan2f(L.z, L.x) + kPiF) / (2.0f * kPiF);
         float ev = Acos(L.y) / kPiF;
         V3 I = sample_texture(S, S.env, v3(eu, ev, 1.0f), eu, ev);
         float ca = dot(L, N);
-        if (ca == 0.0f) ca = 0.000001f;
-        float invPdf = kPiF / ca;
+        float invPdf, pdf = 0.0f;
+        if (lb) {
+            pdf = GI.alpha * guide_pdf(lb, L) + (1.0f - GI.alpha) * fmaxf(ca, 0.0f) / kPiF;
+            invPdf = pdf > 0.0f ? 1.0f / pdf : 0.0f;
+        } else {
+            if (ca == 0.0f) ca = 0.000001f;
+            invPdf = kPiF / ca;
+        }
         V3 c;
         bool want = light_term(M, kd, I, L, D, N, 1.0f, 1.0f, invPdf, c);
         if (want) c = had(w, c);
         want = want && (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f);
-        emit_shadow(O, want, make_shadow(P + (L * eps), L, r.time, INFINITY, c, r.sample, kShadowAny, 0u));
+        ShadowRay sr = make_shadow(P + (L * eps), L, r.time, INFINITY, c, r.sample, kShadowAny, 0u);
+        if (GI.recs) {
+            // Training sample: direction, weight = luminance(L_env) cos / pdf;
+            // connect zeroes the weight when the direction is blocked.
+            uint32_t idx = wave_append(GI.recCount, want);
+            if (want && idx < GI.recCap) {
+                GuideSample g;
+                guide_position(S, P, g.p);
+                g.n[0] = N.x;
+                g.n[1] = N.y;
+                g.n[2] = N.z;
+                g.rough = guide_roughness(M);
+                g.dir[0] = L.x;
+                g.dir[1] = L.y;
+                g.dir[2] = L.z;
+                g.wgt = (0.27f * I.x + 0.67f * I.y + 0.06f * I.z) * fmaxf(ca, 0.0f) * invPdf;
+                g.pad = 0.0f;
+                GI.recs[idx] = g;
+                sr.rec = idx;
+            }
+        }
+        emit_shadow(O, want, sr);
     }
     const bool childOk = depth + 1u <= S.maxDepth;
     const V3 zero = v3(0.0f, 0.0f, 0.0f);
@@ -548,6 +660,45 @@ __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeo
         emit_child(O, childOk, make_child(P + (R * 0.000100f), R, r.time, child_node(r.node, 3u), w * refl, r.sample,
                                           coef, depth + 1u, inside));
     }
+}
+
+// ---- learned guide: queries and training batches ------------------------------------------
+__global__ __launch_bounds__(kRenderBlock) void k_guide_query(DevScene S, DeviceGeometry G, const PathRay *__restrict__ q,
+                                                              const PathHit *__restrict__ hits, uint32_t n,
+                                                              GuideQuery *__restrict__ out) {
+    uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
+    if (i >= n) return;
+    const PathRay r = q[i];
+    const PathHit h = hits[i];
+    GuideQuery g;
+    memset(&g, 0, sizeof(g));
+    if (h.prim != kEmpty) {
+        const DevItemShade sh = S.itemShade[h.item];
+        if (!sh.isEmitter) {
+            V3 P, N;
+            float u, v;
+            reconstruct(S, G, r, h, sh, P, N, u, v);
+            guide_position(S, P, g.p);
+            g.n[0] = N.x;
+            g.n[1] = N.y;
+            g.n[2] = N.z;
+            g.rough = guide_roughness(S.materials[sh.material]);
+        }
+    }
+    out[i] = g;
+}
+
+// A training batch: n records picked by a hash of (seed, slot).
+__global__ __launch_bounds__(kRenderBlock) void k_guide_gather(const GuideSample *__restrict__ recs, uint32_t count,
+                                                               GuideSample *__restrict__ batch, uint32_t n,
+                                                               uint64_t seed) {
+    uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
+    if (i >= n) return;
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    batch[i] = recs[(uint32_t)(z % count)];
 }
 
 // ---- splat (main.cpp:1209-1254) -------------------------------------------------------
@@ -732,6 +883,7 @@ hipError_t uploadVec(DeviceBuffer &b, const std::vector<T> &v, hipStream_t s) {
 struct RenderBuffers {
     DeviceBuffer materials, textures, texels, perms, lights, lightTris, prims, itemShade, perlin, pcg;
     DeviceBuffer q0, q1, hits, shadow, acc, counters, pixels;
+    DeviceBuffer queries, lobes, recs, batch;   // learned guide
 };
 
 static RenderBuffers &buffersOf(az_gpu_ctx *C) {
@@ -929,6 +1081,11 @@ az_status az_upload_shading(az_gpu_ctx *C, const scene *S, const UploadInfo &inf
     D.epsIntersection = C->eps.Intersection;
     D.epsShadow = C->eps.ShadowRay;
     D.triTotal = info.triTotal;
+    for (int k = 0; k < 3; ++k) {
+        float ext = info.boxMax[k] - info.boxMin[k];
+        D.boxMin[k] = info.boxMin[k];
+        D.boxInv[k] = ext > 0.0f ? 1.0f / ext : 0.0f;
+    }
     if (S->MaxRecursionDepth > 200) {
         C->lastError = "MaxRecursionDepth above 200 is not supported";
         return AZ_ERR_UNSUPPORTED;
@@ -1022,22 +1179,45 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
     const u32 rows = row1 - row0;
     const u32 nPix = rows * cam.width;
     const u32 sppTotal = cam.sppSqrt * cam.sppSqrt;
-    // Samples per chunk: about 2M camera samples per wavefront.
-    u32 chunkSpp = std::max<u32>(1u, std::min<u32>(sppTotal, (u32)(2u << 20) / std::max<u32>(1u, nPix)));
     DevScene &S = C->devScene;
+    // The learned guide: a fresh network per render, trained on the
+    // environment samples of the first trainSpp samples per pixel.
+    const bool guided = o.GuideEnable != 0 && S.useEnv;
+    const u32 trainSpp = guided ? (o.GuideTrainSpp ? o.GuideTrainSpp : 32u) : 0u;
+    const u32 stepsPerChunk = o.GuideStepsPerChunk ? o.GuideStepsPerChunk : 8u;
+    const float alpha = o.GuideFraction > 0.0f && o.GuideFraction < 1.0f ? o.GuideFraction : 0.5f;
+    const u32 recCap = 8u << 20;
+    if (guided) {
+        if (C->guide) az_guide_destroy(C->guide);
+        C->guide = nullptr;
+        az_status gs = az_guide_create(&C->guide, C->device, o.Seed ^ 0x67756964ull);
+        if (gs != AZ_OK) {
+            C->lastError = "guide creation failed";
+            return gs;
+        }
+        AZ_HIPR(C, B.recs.alloc((size_t)recCap * sizeof(GuideSample)));
+        AZ_HIPR(C, B.batch.alloc((size_t)65536 * sizeof(GuideSample)));
+    }
+    // Samples per chunk: about 2M camera samples per wavefront; while the
+    // guide trains, at least 8 chunks so that it is updated often.
+    u32 chunkSpp = std::max<u32>(1u, std::min<u32>(sppTotal, (u32)(2u << 20) / std::max<u32>(1u, nPix)));
     const u32 shadowPerRay = S.lightCount + (S.useEnv ? 1u : 0u);
     hipStream_t st = C->stream;
     AZ_HIPR(C, B.counters.alloc(64));
     uint32_t *dCount = (uint32_t *)B.counters.ptr;
     u32 s0 = 0;
+    u32 trainRound = 0;
     // Device memory the queues of one chunk may take before the chunk is
     // split into fewer samples per pixel.
     const size_t budget = (size_t)48 << 30;
     while (s0 < sppTotal) {
+        const bool training = guided && s0 < trainSpp;
         u32 spp = std::min(chunkSpp, sppTotal - s0);
+        if (training) spp = std::min(spp, std::max<u32>(1u, std::min(trainSpp - s0, trainSpp / 8u)));
         u32 n = nPix * spp;
         AZ_HIPR(C, B.q0.alloc((size_t)n * sizeof(PathRay)));
         AZ_HIPR(C, B.acc.alloc((size_t)n * 3 * sizeof(float)));
+        if (training) AZ_HIPR(C, hipMemsetAsync(dCount + 2, 0, 4, st));
         u32 cap = n, shadowCap = 0;
         ChunkParams cp;
         memset(&cp, 0, sizeof(cp));
@@ -1064,7 +1244,8 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             cap = 4u * count;
             shadowCap = std::max<u32>(1u, shadowPerRay * count);
             size_t need = (size_t)cap * sizeof(PathRay) + (size_t)count * (sizeof(PathRay) + sizeof(PathHit)) +
-                          (size_t)shadowCap * sizeof(ShadowRay);
+                          (size_t)shadowCap * sizeof(ShadowRay) +
+                          (guided ? (size_t)count * (sizeof(GuideQuery) + 40 * sizeof(float)) : 0);
             if (need > budget || (size_t)count * 4 > 0xFFFFFFFFull) {
                 overflow = true;
                 break;
@@ -1079,10 +1260,26 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             hipLaunchKernelGGL(k_extend, grid, dim3(kRenderBlock), 0, st, C->dg, qa, count, (PathHit *)B.hits.ptr,
                                S.epsIntersection, (int)cp.cull);
             AZ_HIPR(C, hipGetLastError());
+            GuideIO GI{nullptr, nullptr, dCount + 2, recCap, alpha};
+            if (guided) {
+                // Query every hit, then one batched MFMA inference.
+                AZ_HIPR(C, B.queries.alloc((size_t)count * sizeof(GuideQuery)));
+                AZ_HIPR(C, B.lobes.alloc((size_t)count * 40 * sizeof(float)));
+                hipLaunchKernelGGL(k_guide_query, grid, dim3(kRenderBlock), 0, st, S, C->dg, qa,
+                                   (const PathHit *)B.hits.ptr, count, (GuideQuery *)B.queries.ptr);
+                AZ_HIPR(C, hipGetLastError());
+                az_status gs = az_guide_infer_on_stream(C->guide, B.queries.ptr, count, (float *)B.lobes.ptr, st);
+                if (gs != AZ_OK) {
+                    C->lastError = "guide inference failed";
+                    return gs;
+                }
+                GI.lobes = (const float *)B.lobes.ptr;
+                if (training) GI.recs = (GuideSample *)B.recs.ptr;
+            }
             AZ_HIPR(C, hipMemsetAsync(dCount, 0, 8, st));
             ShadeOut O{qb, (ShadowRay *)B.shadow.ptr, dCount, cap, shadowCap};
             hipLaunchKernelGGL(k_shade, grid, dim3(kRenderBlock), 0, st, S, C->dg, cam, cp, qa,
-                               (const PathHit *)B.hits.ptr, count, acc, O);
+                               (const PathHit *)B.hits.ptr, count, acc, O, GI);
             AZ_HIPR(C, hipGetLastError());
             uint32_t hc[2];
             AZ_HIPR(C, hipMemcpyAsync(hc, dCount, 8, hipMemcpyDeviceToHost, st));
@@ -1093,7 +1290,7 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             }
             if (hc[1]) {
                 hipLaunchKernelGGL(k_connect, dim3((hc[1] + kRenderBlock - 1) / kRenderBlock), dim3(kRenderBlock), 0,
-                                   st, C->dg, (const ShadowRay *)B.shadow.ptr, hc[1], acc);
+                                   st, C->dg, (const ShadowRay *)B.shadow.ptr, hc[1], acc, (GuideSample *)B.recs.ptr);
                 AZ_HIPR(C, hipGetLastError());
             }
             count = hc[0];
@@ -1124,6 +1321,28 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
                                    d1 - d0);
         }
         AZ_HIPR(C, hipGetLastError());
+        if (training) {
+            // Adam steps on batches gathered from this chunk's samples.
+            uint32_t rc = 0;
+            AZ_HIPR(C, hipMemcpyAsync(&rc, dCount + 2, 4, hipMemcpyDeviceToHost, st));
+            AZ_HIPR(C, hipStreamSynchronize(st));
+            u32 R = std::min(rc, recCap);
+            if (R >= 1024) {
+                u32 bn = std::min<u32>(65536u, R) & ~15u;
+                for (u32 k = 0; k < stepsPerChunk; ++k) {
+                    uint64_t seed = (o.Seed + 0x9E3779B97F4A7C15ull) * (uint64_t)(trainRound * 131 + k + 1);
+                    hipLaunchKernelGGL(k_guide_gather, dim3((bn + kRenderBlock - 1) / kRenderBlock), dim3(kRenderBlock),
+                                       0, st, (const GuideSample *)B.recs.ptr, R, (GuideSample *)B.batch.ptr, bn, seed);
+                    AZ_HIPR(C, hipGetLastError());
+                    az_status gs = az_guide_train_on_stream(C->guide, B.batch.ptr, bn, st);
+                    if (gs != AZ_OK) {
+                        C->lastError = "guide training failed";
+                        return gs;
+                    }
+                }
+            }
+            ++trainRound;
+        }
         s0 += spp;
     }
     return AZ_OK;

@@ -1,0 +1,91 @@
+"""Equal-time comparison of the guided and the plain wavefront tracer on the
+architectural interior (environment with a sun lobe, 8 spots):
+
+    python -m azimuth_amd.guide_experiment [--scale S] [--ref-spp 4096]
+        [--spp 64] [--seed 1]
+
+The reference is a plain render with --ref-spp samples per pixel accumulated
+in double precision.  The plain tracer renders --spp samples per pixel (time
+T); the guided tracer (environment directions from the learned guide mixed
+50/50 with the reference's cosine sampling, trained online during its first
+32 samples per pixel) renders the largest square spp whose measured time stays
+within T.  Because both estimators are unbiased, MSE scales as 1/time, and
+the report also gives the time-normalized ratio (MSE_plain * T_plain) /
+(MSE_guided * T_guided).  One JSON line is printed.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import native as N, workloads
+from .scene import GpuContext
+
+
+def resolve(px):
+    return px[:, :3] / np.maximum(px[:, 3:4], 1e-300)
+
+
+def timed(ctx, cam, filt, seed, spp, guide):
+    ctx.synchronize()
+    t0 = time.time()
+    px = ctx.render_f64(cam, filt=filt, seed=seed, spp=spp, guide=guide)
+    return resolve(px), time.time() - t0
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--scale", type=float, default=0.25)
+    ap.add_argument("--ref-spp", type=int, default=4096)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=1)
+    a = ap.parse_args(argv)
+    W, H = int(1920 * a.scale), int(1080 * a.scale)
+    sc, cam = workloads.interior_scene(W, H, spp=a.spp, max_depth=6)
+    filt = N.FilterType_Gaussian
+    ctx = GpuContext(0)
+    ctx.upload_scene(sc)
+    ctx.render(cam, filt=filt, spp=1, row_end=4)                      # warm-up
+    t0 = time.time()
+    ref = resolve(ctx.render_f64(cam, filt=filt, seed=1000 + a.seed, spp=a.ref_spp))
+    t_ref = time.time() - t0
+    print(f"reference {a.ref_spp} spp in {t_ref:.1f}s", file=sys.stderr, flush=True)
+    plain, t_plain = timed(ctx, cam, filt, a.seed, a.spp, False)
+    mse_plain = float(np.mean((plain - ref) ** 2))
+    print(f"plain {a.spp} spp {t_plain:.2f}s mse {mse_plain:.4e}", file=sys.stderr, flush=True)
+    # Guided at the same spp gives its cost per sample; then the largest
+    # square spp that fits in the plain time.
+    g_same, t_g_same = timed(ctx, cam, filt, a.seed, a.spp, True)
+    mse_g_same = float(np.mean((g_same - ref) ** 2))
+    print(f"guided {a.spp} spp {t_g_same:.2f}s mse {mse_g_same:.4e}", file=sys.stderr, flush=True)
+    s = int(math.isqrt(max(1, int(a.spp * t_plain / t_g_same))))
+    spp_eq = max(1, s * s)
+    g_eq, t_g_eq = timed(ctx, cam, filt, a.seed + 7, spp_eq, True)
+    while t_g_eq > t_plain * 1.02 and s > 1:
+        s -= 1
+        spp_eq = s * s
+        g_eq, t_g_eq = timed(ctx, cam, filt, a.seed + 7, spp_eq, True)
+    mse_g_eq = float(np.mean((g_eq - ref) ** 2))
+    print(f"guided {spp_eq} spp {t_g_eq:.2f}s mse {mse_g_eq:.4e}", file=sys.stderr, flush=True)
+    out = {
+        "experiment": "equal-time MSE vs reference, guided vs plain",
+        "workload": f"architectural interior {W}x{H} ({sc.triangle_count} triangles), Gaussian filter, depth 6",
+        "reference": {"spp": a.ref_spp, "accumulation": "f64", "seconds": round(t_ref, 2)},
+        "plain": {"spp": a.spp, "seconds": round(t_plain, 3), "mse": mse_plain},
+        "guided_same_spp": {"spp": a.spp, "seconds": round(t_g_same, 3), "mse": mse_g_same},
+        "guided_equal_time": {"spp": spp_eq, "seconds": round(t_g_eq, 3), "mse": mse_g_eq},
+        "mse_ratio_equal_time": round(mse_plain / mse_g_eq, 3),
+        "mse_ratio_time_normalized": round((mse_plain * t_plain) / (mse_g_eq * t_g_eq), 3),
+        "mse_ratio_same_spp": round(mse_plain / mse_g_same, 3),
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -167,7 +167,8 @@ class az_scene_params(C.Structure):
 
 class az_render_opts(C.Structure):
     _fields_ = [("Seed", u64), ("SampleCountOverride", u32), ("RowBegin", u32), ("RowEnd", u32),
-                ("AccumulateF64", u32), ("GuideEnable", u32), ("GuideFraction", f32), ("Reserved", u32 * 3)]
+                ("AccumulateF64", u32), ("GuideEnable", u32), ("GuideFraction", f32), ("GuideTrainSpp", u32),
+                ("GuideStepsPerChunk", u32), ("Reserved", u32)]
 
 
 class az_stats(C.Structure):
@@ -231,6 +232,21 @@ EXPORTS = {
     "az_gpu_timer_start": (s32, [_P]),
     "az_gpu_timer_stop": (s32, [_P, C.POINTER(f64)]),
     "az_render_scene": (u32, [_P, C.POINTER(camera), C.POINTER(epsilon_values), u32, u32, _P]),
+    # azimuth_guide.h
+    "az_guide_param_count": (u32, []),
+    "az_guide_create": (s32, [C.POINTER(_P), C.c_int, u64]),
+    "az_guide_destroy": (None, [_P]),
+    "az_guide_last_error": (C.c_char_p, [_P]),
+    "az_guide_set_params": (s32, [_P, _P, u32]),
+    "az_guide_get_params": (s32, [_P, _P, u32]),
+    "az_guide_get_grad": (s32, [_P, _P, u32]),
+    "az_guide_zero_grad": (s32, [_P]),
+    "az_guide_set_learning_rate": (s32, [_P, f32]),
+    "az_guide_infer": (s32, [_P, _P, u32, _P]),
+    "az_guide_infer_device": (s32, [_P, _P, u32, _P]),
+    "az_guide_train": (s32, [_P, _P, u32, u32, C.POINTER(f32)]),
+    "az_guide_train_device": (s32, [_P, _P, u32, u32, C.POINTER(f32)]),
+    "az_guide_mfma_selftest": (s32, [_P, _P, _P, _P]),
 }
 
 _lib = None

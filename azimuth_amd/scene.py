@@ -334,6 +334,24 @@ class GpuContext:
                                        out.ctypes.data), "az_gpu_render", self.h)
         return out
 
+    def render_f64(self, camera, filt=N.FilterType_Box, seed=0, spp=0, guide=False, guide_fraction=0.5):
+        """Device render accumulated in double precision; returns (W*H, 4)
+        float64 (colour sums, weight sums)."""
+        W, H = camera.Resolution.x, camera.Resolution.y
+        d = self.alloc(W * H * 32)
+        N.check(self.lib.az_gpu_memset(self.h, d.ptr, 0, W * H * 32), "az_gpu_memset", self.h)
+        o = N.az_render_opts()
+        o.Seed = int(seed)
+        o.SampleCountOverride = int(spp)
+        o.AccumulateF64 = 1
+        o.GuideEnable = 1 if guide else 0
+        o.GuideFraction = float(guide_fraction)
+        N.check(self.lib.az_gpu_render_device(self.h, C.byref(camera), int(filt), 0, C.byref(o), d.ptr),
+                "az_gpu_render_device", self.h)
+        out = d.download(np.float64, W * H * 4).reshape(W * H, 4)
+        d.free()
+        return out
+
     def close(self):
         if self.h:
             self.lib.az_gpu_destroy(self.h)

@@ -64,9 +64,13 @@ typedef struct {
     u32 SampleCountOverride;/* 0: camera->SampleCount; else used instead (still truncated to a square) */
     u32 RowBegin, RowEnd;   /* scanline range to render; RowEnd 0 = all rows */
     u32 AccumulateF64;      /* 1: accumulate in double (long reference renders) */
-    u32 GuideEnable;        /* 1: mix the learned guide with BSDF sampling */
-    f32 GuideFraction;      /* one-sample MIS selection probability of the guide */
-    u32 Reserved[3];
+    u32 GuideEnable;        /* 1: sample environment directions from the learned guide mixed
+                               with the reference's cosine sampling (one-sample MIS); the
+                               guide starts from seeded weights and trains online */
+    f32 GuideFraction;      /* probability of drawing the direction from the guide (0 = 0.5) */
+    u32 GuideTrainSpp;      /* samples per pixel rendered while training (0 = 32) */
+    u32 GuideStepsPerChunk; /* Adam steps after each training chunk (0 = 8) */
+    u32 Reserved;
 } az_render_opts;
 
 typedef struct {

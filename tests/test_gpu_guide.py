@@ -1,0 +1,185 @@
+"""The guide's BF16 MFMA kernels against the float64 numpy reference
+(oracle/guide_ref.py), through the C ABI.
+
+Tolerances: inference is compared with a reference that rounds weights and
+activations to bf16 as the kernels do (then only the fp32 summation order
+differs: 1e-3 relative); gradients are compared with the float64 reference,
+whose difference is the bf16 rounding of activations and back-propagated
+gradients (relative L2 error per layer below 5%, cosine above 0.995)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import guide_ref as R
+from azimuth_amd.guide import Guide, pack_params, unpack_params
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(x):
+    a = np.asarray(x, np.float32).copy()
+    u = a.view(np.uint32)
+    u += (np.uint32(0x7FFF) + ((u >> 16) & 1)).astype(np.uint32)
+    u &= np.uint32(0xFFFF0000)
+    return u.view(np.float32).astype(np.float64)
+
+
+def forward_bf16(Ws, bs, x):
+    h = bf16(x)
+    for l in range(4):
+        h = bf16(np.maximum(h @ bf16(Ws[l]) + bs[l].astype(np.float32), 0.0))
+    return h @ bf16(Ws[4]) + bs[4]
+
+
+@pytest.fixture(scope="module")
+def guide():
+    g = Guide(0, seed=3)
+    yield g
+    g.close()
+
+
+def random_batch(n, seed):
+    g = np.random.default_rng(seed)
+    p = g.random((n, 3)).astype(np.float32)
+    nrm = g.normal(size=(n, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    rough = g.random(n).astype(np.float32)
+    d = g.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return p, nrm.astype(np.float32), rough, d.astype(np.float32), (g.random(n) + 0.2).astype(np.float32)
+
+
+def test_mfma_lane_layout(guide):
+    g = np.random.default_rng(0)
+    A, B = g.normal(size=(16, 16)), g.normal(size=(16, 16))
+    D = guide.mfma_selftest(A, B)
+    np.testing.assert_allclose(D, bf16(A) @ bf16(B), rtol=1e-5, atol=1e-5)
+
+
+def test_inference_matches_reference(guide):
+    Ws, bs = R.init_params(7)
+    g = np.random.default_rng(1)
+    bs = [b + g.normal(0, 0.2, b.shape).astype(np.float32) for b in bs]
+    guide.set_params(pack_params(Ws, bs))
+    p, nrm, rough, _, _ = random_batch(1000, 2)     # not a multiple of 16 or 64
+    lobes = guide.infer(p, nrm, rough)
+    x = R.encode(p, nrm, rough)
+    o = forward_bf16(Ws, bs, x)
+    pi, mu, kappa, _, _ = R.mixture(o)
+    np.testing.assert_allclose(lobes[:, :, 0], pi, rtol=2e-3, atol=1e-5)
+    np.testing.assert_allclose(lobes[:, :, 1:4], mu, rtol=2e-3, atol=2e-4)
+    np.testing.assert_allclose(lobes[:, :, 4], kappa, rtol=2e-3)
+    # and the float64 network agrees to bf16 precision
+    o64, _ = R.forward(Ws, bs, x)
+    pi64 = R.mixture(o64)[0]
+    assert np.max(np.abs(lobes[:, :, 0] - pi64)) < 0.05
+
+
+def grads_bf16(Ws, bs, x, w, wgt):
+    """The float64 reference's backward with bf16 rounding where the kernels
+    round: activations, the output gradient and each back-propagated
+    gradient before it enters the next product; bias gradients from the
+    unrounded values."""
+    n = x.shape[0]
+    hs = [bf16(x)]
+    for l in range(4):
+        hs.append(bf16(np.maximum(hs[-1] @ bf16(Ws[l]) + bs[l].astype(np.float32), 0.0)))
+    o = hs[-1] @ bf16(Ws[4]) + bs[4]
+    pi, mu, kappa, raw, rn = R.mixture(o)
+    lp = R.vmf_log_pdf(mu, kappa, w) + np.log(pi)
+    m = lp.max(1, keepdims=True)
+    lq = (m + np.log(np.exp(lp - m).sum(1, keepdims=True)))[:, 0]
+    r = np.exp(lp - lq[:, None])
+    gq = (-wgt / n)[:, None]
+    do = np.zeros_like(o)
+    do[:, :8] = gq * (r - pi)
+    c = np.einsum("nkd,nd->nk", mu, w)
+    do[:, 8:32] = (gq[:, :, None] * (r * kappa)[:, :, None] * (w[:, None, :] - c[:, :, None] * mu) /
+                   np.maximum(rn, 1e-12)).reshape(n, 24)
+    kr = o[:, 32:40]
+    inside = (kr > R.KMIN) & (kr < R.KMAX)
+    do[:, 32:40] = gq * r * (1 / kappa - 2 / np.expm1(2 * kappa) + (c - 1)) * kappa * inside
+    dWs, dbs = [None] * 5, [None] * 5
+    dy = do
+    for l in range(4, -1, -1):
+        dbs[l] = dy.sum(0)
+        dyb = bf16(dy)
+        dWs[l] = hs[l].T @ dyb
+        if l > 0:
+            dy = (dyb @ bf16(Ws[l]).T) * (hs[l] > 0)
+    return dWs, dbs
+
+
+def test_gradients_match_reference(guide):
+    Ws, bs = R.init_params(11)
+    g = np.random.default_rng(4)
+    bs = [b + g.normal(0, 0.1, b.shape).astype(np.float32) for b in bs]
+    guide.set_params(pack_params(Ws, bs))
+    p, nrm, rough, d, w = random_batch(4096, 5)
+    guide.zero_grad()
+    loss_gpu = guide.train(p, nrm, rough, d, w, update=False)
+    gg = unpack_params(guide.grad())
+    x = R.encode(p, nrm, rough)
+    d64, w64 = d.astype(np.float64), w.astype(np.float64)
+    loss, dWs, dbs = R.loss_and_grads([W.astype(np.float64) for W in Ws], [b.astype(np.float64) for b in bs], x,
+                                      d64, w64)
+    eWs, ebs = grads_bf16(Ws, bs, x, d64, w64)
+    assert loss_gpu == pytest.approx(loss, rel=2e-2)
+    for l in range(5):
+        for got, want, emu in ((gg[0][l], dWs[l], eWs[l]), (gg[1][l], dbs[l], ebs[l])):
+            err = np.linalg.norm(got - want) / max(np.linalg.norm(want), 1e-20)
+            cos = float(np.dot(got.ravel(), want.ravel()) / (np.linalg.norm(got) * np.linalg.norm(want) + 1e-30))
+            err_emu = np.linalg.norm(got - emu) / max(np.linalg.norm(emu), 1e-20)
+            print(f"layer {l}: vs float64 rel err {err:.4f} cos {cos:.5f}; vs bf16-rounded reference {err_emu:.2e}",
+                  flush=True)
+            assert err_emu < 5e-3
+            assert err < 0.15 and cos > 0.99
+    guide.zero_grad()
+
+
+def test_adam_step_closed_form(guide):
+    """One update from zero moments moves every parameter by
+    -lr * g / (|g| + eps') (bias-corrected first Adam step)."""
+    Ws, bs = R.init_params(2)
+    p0 = pack_params(Ws, bs)
+    guide.set_params(p0)
+    p, nrm, rough, d, w = random_batch(1024, 6)
+    guide.zero_grad()
+    guide.train(p, nrm, rough, d, w, update=False)
+    gr = guide.grad().astype(np.float64)
+    guide.set_params(p0)                  # resets moments and gradient
+    guide.train(p, nrm, rough, d, w, update=True)
+    p1 = guide.params().astype(np.float64)
+    want = p0 - 1e-3 * gr / (np.abs(gr) + 1e-8)
+    big = np.abs(gr) > 1e-6
+    np.testing.assert_allclose(p1[big], want[big], rtol=0, atol=2e-5)
+
+
+def test_learns_a_position_dependent_lobe(guide):
+    """Directions drawn from vMF(mu(p), kappa=20) with unit weights: the
+    trained mixture's loss approaches the target's entropy and its dominant
+    lobe points along mu(p)."""
+    guide.set_params(pack_params(*R.init_params(5)))
+    g = np.random.default_rng(9)
+
+    def batch(n):
+        p = g.random((n, 3)).astype(np.float32)
+        mu = np.stack([p[:, 0] - 0.5, np.ones(n), p[:, 2] - 0.5], 1)
+        mu /= np.linalg.norm(mu, axis=1, keepdims=True)
+        d = R.sample_vmf(mu, np.full(n, 20.0), g.random(n), g.random(n))
+        nrm = np.tile([0.0, 1.0, 0.0], (n, 1))
+        return p, nrm.astype(np.float32), np.full(n, 0.5, np.float32), d.astype(np.float32), np.ones(n, np.float32), mu
+
+    losses = []
+    for it in range(400):
+        p, nrm, rough, d, w, _ = batch(4096)
+        losses.append(guide.train(p, nrm, rough, d, w, update=True))
+    entropy = -(math.log(20 / (2 * math.pi)) + 20 * (1 / math.tanh(20) - 1 / 20 - 1))
+    print(f"loss {losses[0]:.3f} -> {np.mean(losses[-20:]):.3f}, target entropy {entropy:.3f}", flush=True)
+    assert np.mean(losses[-20:]) < entropy + 0.25
+    p, nrm, rough, d, w, mu = batch(512)
+    lobes = guide.infer(p, nrm, rough)
+    k = np.argmax(lobes[:, :, 0], 1)
+    dom = lobes[np.arange(512), k, 1:4]
+    assert np.mean(np.sum(dom * mu, 1)) > 0.95
