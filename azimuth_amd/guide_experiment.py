@@ -5,11 +5,12 @@ architectural interior (environment with a sun lobe, 8 spots):
         [--spp 64] [--seed 1]
 
 The reference is a plain render with --ref-spp samples per pixel accumulated
-in double precision.  The plain tracer renders --spp samples per pixel (time
+in double precision (chunks of 256 spp with distinct seeds).  The plain tracer renders --spp samples per pixel (time
 T); the guided tracer (environment directions from the learned guide mixed
 50/50 with the reference's cosine sampling, trained online during its first
 32 samples per pixel) renders the largest square spp whose measured time stays
-within T.  Because both estimators are unbiased, MSE scales as 1/time, and
+within T.  Times are wall-clock around the render call (guide creation and
+training included), after a device synchronize.  Because both estimators are unbiased, MSE scales as 1/time, and
 the report also gives the time-normalized ratio (MSE_plain * T_plain) /
 (MSE_guided * T_guided).  One JSON line is printed.
 """
@@ -32,15 +33,25 @@ def resolve(px):
 
 
 def timed(ctx, cam, filt, seed, spp, guide):
-    ctx.synchronize()
-    t0 = time.time()
     px = ctx.render_f64(cam, filt=filt, seed=seed, spp=spp, guide=guide)
-    return resolve(px), time.time() - t0
+    return resolve(px), ctx.last_render_s
+
+
+def reference(ctx, cam, filt, seed, spp):
+    """Plain render of spp samples per pixel in double precision, as chunks
+    of 256 spp with distinct seeds (sums of colour and weight add up)."""
+    chunk = min(spp, 256)
+    acc = None
+    for i in range(spp // chunk):
+        px = ctx.render_f64(cam, filt=filt, seed=seed + i, spp=chunk)
+        acc = px if acc is None else acc + px
+        print(f"reference chunk {i + 1}/{spp // chunk} ({ctx.last_render_s:.1f}s)", file=sys.stderr, flush=True)
+    return resolve(acc)
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--scale", type=float, default=0.25)
+    ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--ref-spp", type=int, default=4096)
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--seed", type=int, default=1)
@@ -52,7 +63,7 @@ def main(argv=None):
     ctx.upload_scene(sc)
     ctx.render(cam, filt=filt, spp=1, row_end=4)                      # warm-up
     t0 = time.time()
-    ref = resolve(ctx.render_f64(cam, filt=filt, seed=1000 + a.seed, spp=a.ref_spp))
+    ref = reference(ctx, cam, filt, 1000 * (a.seed + 1), a.ref_spp)
     t_ref = time.time() - t0
     print(f"reference {a.ref_spp} spp in {t_ref:.1f}s", file=sys.stderr, flush=True)
     plain, t_plain = timed(ctx, cam, filt, a.seed, a.spp, False)

@@ -1,7 +1,7 @@
 """Render throughput of the wavefront path tracer on one GPU:
 
     python -m azimuth_amd.render_bench --workload cornell|interior|motion
-        [--spp N] [--scale S] [--cpu-seconds T]
+        [--spp N] [--scale S] [--guide] [--cpu-seconds T]
 
 Prints one JSON line: Msamples/s (width * height * spp_eff / render time,
 spp_eff = floor(sqrt(spp))^2 as the reference renders), the render time from
@@ -46,6 +46,7 @@ def main(argv=None):
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--repeat", type=int, default=1)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--guide", action="store_true", help="render with the learned path guide")
     ap.add_argument("--cpu-seconds", type=float, default=0.0, help="time the CPU oracle on a strip of rows")
     a = ap.parse_args(argv)
     sc, cam, filt = build(a.workload, a.scale, a.spp)
@@ -59,14 +60,14 @@ def main(argv=None):
     best = None
     for _ in range(a.repeat):
         ctx.timer_start()
-        px = ctx.render(cam, filt=filt, seed=a.seed)
+        px = ctx.render(cam, filt=filt, seed=a.seed, guide=a.guide)
         ms = ctx.timer_stop()
         best = ms if best is None else min(best, ms)
     img = px["Color"] / np.maximum(px["TotalWeight"], 1e-30)[:, None]
     out = {"workload": a.workload, "width": W, "height": H, "spp": spp_eff, "triangles": sc.triangle_count,
            "render_ms": round(best, 3), "msamples_per_s": round(W * H * spp_eff / (best * 1e-3) / 1e6, 3),
            "upload_s": round(upload_s, 2), "mean_radiance": [round(float(x), 6) for x in img.mean(0)],
-           "filter": "gaussian" if filt == N.FilterType_Gaussian else "box"}
+           "filter": "gaussian" if filt == N.FilterType_Gaussian else "box", "guide": bool(a.guide)}
     if a.cpu_seconds > 0:
         import oracle
         threads = max(1, min(16, os.cpu_count() or 1))

@@ -7,6 +7,7 @@ batch closest/any hit, render).  Errors raise `native.AzError`.
 from __future__ import annotations
 
 import ctypes as C
+import time
 
 import numpy as np
 
@@ -336,7 +337,8 @@ class GpuContext:
 
     def render_f64(self, camera, filt=N.FilterType_Box, seed=0, spp=0, guide=False, guide_fraction=0.5):
         """Device render accumulated in double precision; returns (W*H, 4)
-        float64 (colour sums, weight sums)."""
+        float64 (colour sums, weight sums).  The wall time of the render call
+        (guide training included) is left in last_render_s."""
         W, H = camera.Resolution.x, camera.Resolution.y
         d = self.alloc(W * H * 32)
         N.check(self.lib.az_gpu_memset(self.h, d.ptr, 0, W * H * 32), "az_gpu_memset", self.h)
@@ -346,8 +348,12 @@ class GpuContext:
         o.AccumulateF64 = 1
         o.GuideEnable = 1 if guide else 0
         o.GuideFraction = float(guide_fraction)
+        self.synchronize()
+        t0 = time.perf_counter()
         N.check(self.lib.az_gpu_render_device(self.h, C.byref(camera), int(filt), 0, C.byref(o), d.ptr),
                 "az_gpu_render_device", self.h)
+        self.synchronize()
+        self.last_render_s = time.perf_counter() - t0
         out = d.download(np.float64, W * H * 4).reshape(W * H, 4)
         d.free()
         return out

@@ -45,8 +45,8 @@ def main():
     ap.add_argument("--profile", action="store_true", help="short run for profilers: no CPU baseline, no any-hit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark runs on exactly one GPU; multi-GPU sharding is described "
-                                   "in DESIGN.md and not run"}))
+        print(json.dumps({"error": "this benchmark runs on exactly one GPU; sharding across GPUs is not "
+                                   "built"}))
         sys.exit(2)
 
     from azimuth_amd import configs, native as N
@@ -147,11 +147,15 @@ def main():
         a.free()
         b.free()
 
+    # HBM-side bytes per closest-hit launch, from the rocprofv3 FETCH_SIZE and
+    # WRITE_SIZE runs summarized in profiles/ (scaled by a plain copy kernel).
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_icosphere_closest.json")
+    tpath = os.path.join(ROOT, "profiles", "trace_icosphere_closest.md")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            for ln in f:
+                if ln.startswith("traffic_bytes_per_launch"):
+                    traffic = int(ln.split("=")[1])
 
     cpu = None
     if not (args.no_cpu_baseline or args.profile):
