@@ -34,6 +34,41 @@ namespace az {
 constexpr int kRenderBlock = 256;
 constexpr int kRenderRing = 16;
 
+// ---- segmented queues -------------------------------------------------------------
+// Appends to one counter serialize in the memory system (about 11 ns each on
+// MI355X whatever the wave count), so a producer kernel appends through
+// kSegs counters, each on its own 128-byte line: block b writes into segment
+// b % nseg, a fixed slice of the queue whose capacity covers the most its
+// blocks can emit.  A scan then turns the segment counts into offsets, and a
+// consumer maps its dense index i to the slot of the i-th record.
+constexpr uint32_t kSegs = 256;
+constexpr uint32_t kSegStride = 32;          // uint32 per counter: 128 bytes apart
+
+struct SegIn {
+    const uint32_t *offsets;                 // nseg + 1 exclusive prefix of the segment counts
+    uint32_t nseg, segCap;                   // nseg == 1: the queue is dense
+};
+
+// Slot of dense index i: the segment s with offsets[s] <= i < offsets[s + 1].
+__device__ inline uint32_t seg_slot(const uint32_t *off, uint32_t nseg, uint32_t segCap, uint32_t i) {
+    if (nseg == 1u) return i;
+    uint32_t lo = 0u, hi = nseg;
+    while (hi - lo > 1u) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo * segCap + (i - off[lo]);
+}
+
+// Stages a queue's offsets in LDS; every thread of the block must call it.
+__device__ inline const uint32_t *stage_offsets(const SegIn &in, uint32_t *soff) {
+    if (in.nseg == 1u) return nullptr;
+    for (uint32_t k = threadIdx.x; k <= in.nseg; k += blockDim.x) soff[k] = in.offsets[k];
+    __syncthreads();
+    return soff;
+}
+
 // One slot per lane that wants one: wave-aggregated append to a counter.
 __device__ inline uint32_t wave_append(uint32_t *counter, bool want) {
     uint64_t mask = __ballot(want);
@@ -97,13 +132,15 @@ __global__ __launch_bounds__(kRenderBlock) void k_generate(DevScene S, CameraSet
 }
 
 // ---- extend: closest hit of every queued ray ---------------------------------
-__global__ __launch_bounds__(kRenderBlock, 6) void k_extend(DeviceGeometry G, const PathRay *__restrict__ q,
+__global__ __launch_bounds__(kRenderBlock, 6) void k_extend(DeviceGeometry G, const PathRay *__restrict__ q, SegIn qin,
                                                            uint32_t n, PathHit *__restrict__ hits, float eps,
                                                            int cull) {
     __shared__ uint32_t smem[kRenderRing * kRenderBlock];
+    __shared__ uint32_t soff[kSegs + 1];
+    const uint32_t *off = stage_offsets(qin, soff);
     uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
     if (i >= n) return;
-    const PathRay &pr = q[i];
+    const PathRay &pr = q[seg_slot(off, qin.nseg, qin.segCap, i)];
     Ray3 r;
     r.ox = pr.ox;
     r.oy = pr.oy;
@@ -129,12 +166,14 @@ __global__ __launch_bounds__(kRenderBlock, 6) void k_extend(DeviceGeometry G, co
 
 // ---- connect: shadow rays ------------------------------------------------------
 __global__ __launch_bounds__(kRenderBlock, 6) void k_connect(DeviceGeometry G, const ShadowRay *__restrict__ sq,
-                                                            uint32_t n, float *__restrict__ acc,
+                                                            SegIn sin, uint32_t n, float *__restrict__ acc,
                                                             GuideSample *__restrict__ recs) {
     __shared__ uint32_t smem[kRenderRing * kRenderBlock];
+    __shared__ uint32_t soff[kSegs + 1];
+    const uint32_t *off = stage_offsets(sin, soff);
     uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
     if (i >= n) return;
-    const ShadowRay &s = sq[i];
+    const ShadowRay &s = sq[seg_slot(off, sin.nseg, sin.segCap, i)];
     Ray3 r;
     r.ox = s.ox;
     r.oy = s.oy;
@@ -163,9 +202,9 @@ __global__ __launch_bounds__(kRenderBlock, 6) void k_connect(DeviceGeometry G, c
 // Learned guide inputs and training outputs of the shade kernel.
 struct GuideIO {
     const float *lobes;              // per queued ray: 8 x {w, mu, kappa}, or null (plain sampling)
-    GuideSample *recs;               // training samples, or null when not training
-    uint32_t *recCount;
-    uint32_t recCap;
+    GuideSample *recs;               // training samples (kSegs segments), or null when not training
+    uint32_t *recCounters;           // kSegs counters, kSegStride apart
+    uint32_t recSegCap;
     float alpha;                     // probability of drawing from the guide
 };
 
@@ -217,21 +256,23 @@ __device__ inline V3 guide_sample(const float *lb, float u0, float u1, float u2)
     return normalize(mu * wz + t * (s * cosf(ph)) + b * (s * sinf(ph)));
 }
 
+// Outputs of shade: the next level's rays and the shadow rays, both
+// segmented (block b appends to segment b % nseg).
 struct ShadeOut {
     PathRay *next;
     ShadowRay *shadow;
-    uint32_t *counters;   // [0] next queue, [1] shadow queue
-    uint32_t capacity, shadowCapacity;
+    uint32_t *childCounters, *shadowCounters;     // kSegs counters each, kSegStride apart
+    uint32_t nseg, childSegCap, shadowSegCap;
 };
 
-__device__ inline void emit_child(const ShadeOut &O, bool want, const PathRay &c) {
-    uint32_t idx = wave_append(O.counters + 0, want);
-    if (want && idx < O.capacity) O.next[idx] = c;
+__device__ inline void emit_child(const ShadeOut &O, uint32_t seg, bool want, const PathRay &c) {
+    uint32_t idx = wave_append(O.childCounters + seg * kSegStride, want);
+    if (want && idx < O.childSegCap) O.next[(size_t)seg * O.childSegCap + idx] = c;
 }
 
-__device__ inline void emit_shadow(const ShadeOut &O, bool want, const ShadowRay &s) {
-    uint32_t idx = wave_append(O.counters + 1, want);
-    if (want && idx < O.shadowCapacity) O.shadow[idx] = s;
+__device__ inline void emit_shadow(const ShadeOut &O, uint32_t seg, bool want, const ShadowRay &s) {
+    uint32_t idx = wave_append(O.shadowCounters + seg * kSegStride, want);
+    if (want && idx < O.shadowSegCap) O.shadow[(size_t)seg * O.shadowSegCap + idx] = s;
 }
 
 __device__ inline ShadowRay make_shadow(V3 o, V3 d, float time, float tmax, V3 c, uint32_t sample, uint32_t mode,
@@ -361,12 +402,15 @@ __device__ inline void reconstruct(const DevScene &S, const DeviceGeometry &G, c
 }
 
 __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeometry G, CameraSetup cam, ChunkParams cp,
-                                                        const PathRay *__restrict__ q,
+                                                        const PathRay *__restrict__ q, SegIn qin,
                                                         const PathHit *__restrict__ hits, uint32_t n,
                                                         float *__restrict__ acc, ShadeOut O, GuideIO GI) {
+    __shared__ uint32_t soff[kSegs + 1];
+    const uint32_t *off = stage_offsets(qin, soff);
     uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
     if (i >= n) return;
-    const PathRay r = q[i];
+    const uint32_t seg = blockIdx.x % O.nseg;
+    const PathRay r = q[seg_slot(off, qin.nseg, qin.segCap, i)];
     const PathHit h = hits[i];
     const uint32_t depth = r.flags & 0xFFu;
     const bool inside = (r.flags & 0x100u) != 0u;
@@ -506,7 +550,7 @@ __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeo
         // A contribution of exactly zero (outside a spot cone, a black
         // material) adds nothing whatever the occlusion: no shadow ray.
         want = want && (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f);
-        emit_shadow(O, want, make_shadow(P + (Ld * eps), Ld, r.time, tmax, c, r.sample, mode, L.entityId));
+        emit_shadow(O, seg, want, make_shadow(P + (Ld * eps), Ld, r.time, tmax, c, r.sample, mode, L.entityId));
     }
     // Environment NEE, cosine-distributed (main.cpp:719-778).  With the
     // guide, the direction comes from the guide with probability alpha and
@@ -551,8 +595,10 @@ __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeo
         if (GI.recs) {
             // Training sample: direction, weight = luminance(L_env) cos / pdf;
             // connect zeroes the weight when the direction is blocked.
-            uint32_t idx = wave_append(GI.recCount, want);
-            if (want && idx < GI.recCap) {
+            const uint32_t rseg = blockIdx.x % kSegs;
+            uint32_t idx = wave_append(GI.recCounters + rseg * kSegStride, want);
+            if (want && idx < GI.recSegCap) {
+                idx += rseg * GI.recSegCap;
                 GuideSample g;
                 guide_position(S, P, g.p);
                 g.n[0] = N.x;
@@ -568,7 +614,7 @@ __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeo
                 sr.rec = idx;
             }
         }
-        emit_shadow(O, want, sr);
+        emit_shadow(O, seg, want, sr);
     }
     const bool childOk = depth + 1u <= S.maxDepth;
     const V3 zero = v3(0.0f, 0.0f, 0.0f);
@@ -592,12 +638,15 @@ __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeo
         V3 cw;
         bool want = childOk && scatter_weight(M, kd, Sd, D, N, invPdf, cw);
         if (want) cw = had(w, cw);
-        emit_child(O, want, make_child(P + (Sd * eps), Sd, r.time, child_node(r.node, 0u), cw, r.sample, zero,
+        emit_child(O, seg, want, make_child(P + (Sd * eps), Sd, r.time, child_node(r.node, 0u), cw, r.sample, zero,
                                        depth + 1u, inside));
     }
     // Ambient (main.cpp:1088-1099).
     const bool transparent = (M.flags & kMatTransparent) != 0u;
-    if ((!inside || !transparent) && !ignoreAmbient) add3(a, had(w, had(v3(M.ka), v3(S.ambient))));
+    if ((!inside || !transparent) && !ignoreAmbient) {
+        V3 am = had(w, had(v3(M.ka), v3(S.ambient)));
+        if (am.x != 0.0f || am.y != 0.0f || am.z != 0.0f) add3(a, am);
+    }
     // Mirror (main.cpp:781-854).
     if (M.flags & kMatMirror) {
         V3 R = reflect(-D, N);
@@ -618,7 +667,7 @@ __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeo
             R = R + 2.0f * M.gloss * (uu * (x1 - 0.5f) + vv * (x2 - 0.5f));
             R = normalize(R);
         }
-        emit_child(O, childOk, make_child(P + (R * 0.000825f), R, r.time, child_node(r.node, 1u),
+        emit_child(O, seg, childOk, make_child(P + (R * 0.000825f), R, r.time, child_node(r.node, 1u),
                                           had(w, v3(M.km)), r.sample, v3(r.cx, r.cy, r.cz), depth + 1u, inside));
     }
     // Dielectric (main.cpp:856-964): both refraction and reflection are traced.
@@ -654,21 +703,23 @@ __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeo
             rc = make_child(P + (T * 0.000100f), T, r.time, child_node(r.node, 2u), w * (1.0f - refl), r.sample, coef,
                             depth + 1u, !inside);
         }
-        emit_child(O, childOk && refract, rc);
+        emit_child(O, seg, childOk && refract, rc);
         V3 R = reflect(-D, Nn);
         V3 coef = inside ? v3(tr.x - 1.0f, tr.y - 1.0f, tr.z - 1.0f) : zero;
-        emit_child(O, childOk, make_child(P + (R * 0.000100f), R, r.time, child_node(r.node, 3u), w * refl, r.sample,
+        emit_child(O, seg, childOk, make_child(P + (R * 0.000100f), R, r.time, child_node(r.node, 3u), w * refl, r.sample,
                                           coef, depth + 1u, inside));
     }
 }
 
 // ---- learned guide: queries and training batches ------------------------------------------
 __global__ __launch_bounds__(kRenderBlock) void k_guide_query(DevScene S, DeviceGeometry G, const PathRay *__restrict__ q,
-                                                              const PathHit *__restrict__ hits, uint32_t n,
+                                                              SegIn qin, const PathHit *__restrict__ hits, uint32_t n,
                                                               GuideQuery *__restrict__ out) {
+    __shared__ uint32_t soff[kSegs + 1];
+    const uint32_t *off = stage_offsets(qin, soff);
     uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
     if (i >= n) return;
-    const PathRay r = q[i];
+    const PathRay r = q[seg_slot(off, qin.nseg, qin.segCap, i)];
     const PathHit h = hits[i];
     GuideQuery g;
     memset(&g, 0, sizeof(g));
@@ -688,17 +739,65 @@ __global__ __launch_bounds__(kRenderBlock) void k_guide_query(DevScene S, Device
     out[i] = g;
 }
 
-// A training batch: n records picked by a hash of (seed, slot).
-__global__ __launch_bounds__(kRenderBlock) void k_guide_gather(const GuideSample *__restrict__ recs, uint32_t count,
-                                                               GuideSample *__restrict__ batch, uint32_t n,
-                                                               uint64_t seed) {
+// A training batch: n of the count recorded samples, picked by a hash of
+// (seed, slot) (splitmix64 finalizer).
+__global__ __launch_bounds__(kRenderBlock) void k_guide_gather(const GuideSample *__restrict__ recs, SegIn rin,
+                                                               uint32_t count, GuideSample *__restrict__ batch,
+                                                               uint32_t n, uint64_t seed) {
+    __shared__ uint32_t soff[kSegs + 1];
+    const uint32_t *off = stage_offsets(rin, soff);
     uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
     if (i >= n) return;
     uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(i + 1);
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     z ^= z >> 31;
-    batch[i] = recs[(uint32_t)(z % count)];
+    batch[i] = recs[seg_slot(off, rin.nseg, rin.segCap, (uint32_t)(z % count))];
+}
+
+// Segment counts -> offsets (one block per queue, one thread per segment):
+// offsets[s] = sum of the clamped counts of segments < s, offsets[nseg] and
+// *total = their sum.  The counters are cleared for the next producer.
+struct SegScan {
+    uint32_t *counters[2];
+    uint32_t *offsets[2];
+    uint32_t segCap[2];
+    uint32_t nseg[2];
+    uint32_t *total;                 // total[q] for queue q
+};
+
+__global__ __launch_bounds__(kSegs) void k_seg_scan(SegScan P) {
+    __shared__ uint32_t part[kSegs / 64];
+    const uint32_t q = blockIdx.x, t = threadIdx.x, nseg = P.nseg[q];
+    uint32_t c = 0u;
+    if (t < nseg) {
+        c = min(P.counters[q][t * kSegStride], P.segCap[q]);
+        P.counters[q][t * kSegStride] = 0u;
+    }
+    // Inclusive scan within the wave, then across the block's waves.
+    const uint32_t lane = t & 63u, wave = t >> 6;
+    uint32_t x = c;
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63u) part[wave] = x;
+    __syncthreads();
+    uint32_t before = 0u;
+    for (uint32_t w = 0; w < wave; ++w) before += part[w];
+    const uint32_t incl = before + x;
+    if (t < nseg) P.offsets[q][t] = incl - c;
+    if (t == nseg - 1u) {
+        P.offsets[q][nseg] = incl;
+        P.total[q] = incl;
+    }
+}
+
+// Clears n words (a fill kernel rather than a memset node, so that the same
+// code can be captured in a graph).
+__global__ void k_clear_u32(uint32_t *p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
 // ---- splat (main.cpp:1209-1254) -------------------------------------------------------
@@ -1203,8 +1302,21 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
     u32 chunkSpp = std::max<u32>(1u, std::min<u32>(sppTotal, (u32)(2u << 20) / std::max<u32>(1u, nPix)));
     const u32 shadowPerRay = S.lightCount + (S.useEnv ? 1u : 0u);
     hipStream_t st = C->stream;
-    AZ_HIPR(C, B.counters.alloc(64));
-    uint32_t *dCount = (uint32_t *)B.counters.ptr;
+    // Counters and offsets of the segmented queues: child, shadow and
+    // training-record counters (kSegs each, one 128-byte line apart), the
+    // totals, and the offsets of the current and next path queues, of the
+    // shadow queue and of the training records.
+    const size_t cntWords = (size_t)kSegs * kSegStride, offWords = kSegs + 1;
+    const size_t ctlWords = 3 * cntWords + 16 + 4 * offWords;
+    AZ_HIPR(C, B.counters.alloc(ctlWords * sizeof(uint32_t)));
+    uint32_t *ctl = (uint32_t *)B.counters.ptr;
+    uint32_t *childCnt = ctl, *shadowCnt = ctl + cntWords, *recCnt = ctl + 2 * cntWords;
+    uint32_t *dTotal = ctl + 3 * cntWords;
+    uint32_t *offQ[2] = {dTotal + 16, dTotal + 16 + offWords};
+    uint32_t *offShadow = dTotal + 16 + 2 * offWords, *offRec = dTotal + 16 + 3 * offWords;
+    hipLaunchKernelGGL(k_clear_u32, dim3((u32)((ctlWords + 255) / 256)), dim3(256), 0, st, ctl, (u32)ctlWords);
+    AZ_HIPR(C, hipGetLastError());
+    const u32 recSegCap = recCap / kSegs;
     u32 s0 = 0;
     u32 trainRound = 0;
     // Device memory the queues of one chunk may take before the chunk is
@@ -1217,8 +1329,6 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
         u32 n = nPix * spp;
         AZ_HIPR(C, B.q0.alloc((size_t)n * sizeof(PathRay)));
         AZ_HIPR(C, B.acc.alloc((size_t)n * 3 * sizeof(float)));
-        if (training) AZ_HIPR(C, hipMemsetAsync(dCount + 2, 0, 4, st));
-        u32 cap = n, shadowCap = 0;
         ChunkParams cp;
         memset(&cp, 0, sizeof(cp));
         cp.seed = o.Seed;
@@ -1226,46 +1336,50 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
         cp.rows = rows;
         cp.s0 = s0;
         cp.spp = spp;
-        cp.capacity = cap;
-        cp.shadowCapacity = shadowCap;
+        cp.capacity = n;
+        cp.shadowCapacity = 0;
         cp.cull = Cull ? 1u : 0u;
         cp.gaussian = Filter == FilterType_Gaussian ? 1u : 0u;
         DeviceBuffer *bufA = &B.q0, *bufB = &B.q1;
+        int offA = 0;
         float *acc = (float *)B.acc.ptr;
         hipLaunchKernelGGL(k_generate, dim3((n + kRenderBlock - 1) / kRenderBlock), dim3(kRenderBlock), 0, st, S, cam,
                            cp, (PathRay *)bufA->ptr, acc, n);
         AZ_HIPR(C, hipGetLastError());
+        SegIn qin{nullptr, 1u, n};   // camera rays are dense
         u32 count = n;
         bool overflow = false;
         for (u32 depth = 0; depth <= S.maxDepth && count > 0; ++depth) {
-            // A vertex has at most four children (scatter, mirror, refraction,
-            // reflection) and one shadow ray per light plus the environment,
-            // so these capacities always hold the next level.
-            cap = 4u * count;
-            shadowCap = std::max<u32>(1u, shadowPerRay * count);
-            size_t need = (size_t)cap * sizeof(PathRay) + (size_t)count * (sizeof(PathRay) + sizeof(PathHit)) +
-                          (size_t)shadowCap * sizeof(ShadowRay) +
+            // Block b of shade appends to segment b % nseg; a segment's slice
+            // holds what its blocks can emit: at most four children
+            // (scatter, mirror, refraction, reflection) and one shadow ray
+            // per light plus the environment per ray.
+            const u32 blocks = (count + kRenderBlock - 1) / kRenderBlock;
+            const u32 nseg = std::min<u32>(kSegs, blocks);
+            const u32 perSeg = (blocks + nseg - 1) / nseg * kRenderBlock;
+            const u32 childSegCap = 4u * perSeg, shadowSegCap = std::max<u32>(1u, shadowPerRay * perSeg);
+            const size_t cap = (size_t)nseg * childSegCap, shadowCap = (size_t)nseg * shadowSegCap;
+            size_t need = cap * sizeof(PathRay) + (size_t)count * (sizeof(PathRay) + sizeof(PathHit)) +
+                          shadowCap * sizeof(ShadowRay) +
                           (guided ? (size_t)count * (sizeof(GuideQuery) + 40 * sizeof(float)) : 0);
-            if (need > budget || (size_t)count * 4 > 0xFFFFFFFFull) {
+            if (need > budget || cap > 0xFFFFFFFFull || shadowCap > 0xFFFFFFFFull) {
                 overflow = true;
                 break;
             }
-            AZ_HIPR(C, bufB->alloc((size_t)cap * sizeof(PathRay)));
+            AZ_HIPR(C, bufB->alloc(cap * sizeof(PathRay)));
             AZ_HIPR(C, B.hits.alloc((size_t)count * sizeof(PathHit)));
-            AZ_HIPR(C, B.shadow.alloc((size_t)shadowCap * sizeof(ShadowRay)));
+            AZ_HIPR(C, B.shadow.alloc(shadowCap * sizeof(ShadowRay)));
             PathRay *qa = (PathRay *)bufA->ptr, *qb = (PathRay *)bufB->ptr;
-            cp.capacity = cap;
-            cp.shadowCapacity = shadowCap;
-            dim3 grid((count + kRenderBlock - 1) / kRenderBlock);
-            hipLaunchKernelGGL(k_extend, grid, dim3(kRenderBlock), 0, st, C->dg, qa, count, (PathHit *)B.hits.ptr,
+            dim3 grid(blocks);
+            hipLaunchKernelGGL(k_extend, grid, dim3(kRenderBlock), 0, st, C->dg, qa, qin, count, (PathHit *)B.hits.ptr,
                                S.epsIntersection, (int)cp.cull);
             AZ_HIPR(C, hipGetLastError());
-            GuideIO GI{nullptr, nullptr, dCount + 2, recCap, alpha};
+            GuideIO GI{nullptr, nullptr, recCnt, recSegCap, alpha};
             if (guided) {
                 // Query every hit, then one batched MFMA inference.
                 AZ_HIPR(C, B.queries.alloc((size_t)count * sizeof(GuideQuery)));
                 AZ_HIPR(C, B.lobes.alloc((size_t)count * 40 * sizeof(float)));
-                hipLaunchKernelGGL(k_guide_query, grid, dim3(kRenderBlock), 0, st, S, C->dg, qa,
+                hipLaunchKernelGGL(k_guide_query, grid, dim3(kRenderBlock), 0, st, S, C->dg, qa, qin,
                                    (const PathHit *)B.hits.ptr, count, (GuideQuery *)B.queries.ptr);
                 AZ_HIPR(C, hipGetLastError());
                 az_status gs = az_guide_infer_on_stream(C->guide, B.queries.ptr, count, (float *)B.lobes.ptr, st);
@@ -1276,29 +1390,39 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
                 GI.lobes = (const float *)B.lobes.ptr;
                 if (training) GI.recs = (GuideSample *)B.recs.ptr;
             }
-            AZ_HIPR(C, hipMemsetAsync(dCount, 0, 8, st));
-            ShadeOut O{qb, (ShadowRay *)B.shadow.ptr, dCount, cap, shadowCap};
-            hipLaunchKernelGGL(k_shade, grid, dim3(kRenderBlock), 0, st, S, C->dg, cam, cp, qa,
+            ShadeOut O{qb, (ShadowRay *)B.shadow.ptr, childCnt, shadowCnt, nseg, childSegCap, shadowSegCap};
+            hipLaunchKernelGGL(k_shade, grid, dim3(kRenderBlock), 0, st, S, C->dg, cam, cp, qa, qin,
                                (const PathHit *)B.hits.ptr, count, acc, O, GI);
             AZ_HIPR(C, hipGetLastError());
+            SegScan sc;
+            sc.counters[0] = childCnt;
+            sc.counters[1] = shadowCnt;
+            sc.offsets[0] = offQ[1 - offA];
+            sc.offsets[1] = offShadow;
+            sc.segCap[0] = childSegCap;
+            sc.segCap[1] = shadowSegCap;
+            sc.nseg[0] = sc.nseg[1] = nseg;
+            sc.total = dTotal;
+            hipLaunchKernelGGL(k_seg_scan, dim3(2), dim3(kSegs), 0, st, sc);
+            AZ_HIPR(C, hipGetLastError());
             uint32_t hc[2];
-            AZ_HIPR(C, hipMemcpyAsync(hc, dCount, 8, hipMemcpyDeviceToHost, st));
+            AZ_HIPR(C, hipMemcpyAsync(hc, dTotal, 8, hipMemcpyDeviceToHost, st));
             AZ_HIPR(C, hipStreamSynchronize(st));
-            if (hc[0] > cap || hc[1] > shadowCap) {
-                overflow = true;
-                break;
-            }
             if (hc[1]) {
+                SegIn sin{offShadow, nseg, shadowSegCap};
                 hipLaunchKernelGGL(k_connect, dim3((hc[1] + kRenderBlock - 1) / kRenderBlock), dim3(kRenderBlock), 0,
-                                   st, C->dg, (const ShadowRay *)B.shadow.ptr, hc[1], acc, (GuideSample *)B.recs.ptr);
+                                   st, C->dg, (const ShadowRay *)B.shadow.ptr, sin, hc[1], acc,
+                                   (GuideSample *)B.recs.ptr);
                 AZ_HIPR(C, hipGetLastError());
             }
             count = hc[0];
+            offA = 1 - offA;
+            qin = SegIn{offQ[offA], nseg, childSegCap};
             std::swap(bufA, bufB);
         }
         if (overflow) {
             if (chunkSpp == 1) {
-                C->lastError = "path queue overflow with one sample per chunk";
+                C->lastError = "path queue exceeds the memory budget with one sample per chunk";
                 return AZ_ERR_OUT_OF_MEMORY;
             }
             chunkSpp = std::max<u32>(1u, chunkSpp / 2);
@@ -1323,16 +1447,26 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
         AZ_HIPR(C, hipGetLastError());
         if (training) {
             // Adam steps on batches gathered from this chunk's samples.
-            uint32_t rc = 0;
-            AZ_HIPR(C, hipMemcpyAsync(&rc, dCount + 2, 4, hipMemcpyDeviceToHost, st));
+            SegScan sc;
+            memset(&sc, 0, sizeof(sc));
+            sc.counters[0] = recCnt;
+            sc.offsets[0] = offRec;
+            sc.segCap[0] = recSegCap;
+            sc.nseg[0] = kSegs;
+            sc.total = dTotal + 2;
+            hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(kSegs), 0, st, sc);
+            AZ_HIPR(C, hipGetLastError());
+            uint32_t R = 0;
+            AZ_HIPR(C, hipMemcpyAsync(&R, dTotal + 2, 4, hipMemcpyDeviceToHost, st));
             AZ_HIPR(C, hipStreamSynchronize(st));
-            u32 R = std::min(rc, recCap);
             if (R >= 1024) {
                 u32 bn = std::min<u32>(65536u, R) & ~15u;
+                SegIn rin{offRec, kSegs, recSegCap};
                 for (u32 k = 0; k < stepsPerChunk; ++k) {
                     uint64_t seed = (o.Seed + 0x9E3779B97F4A7C15ull) * (uint64_t)(trainRound * 131 + k + 1);
                     hipLaunchKernelGGL(k_guide_gather, dim3((bn + kRenderBlock - 1) / kRenderBlock), dim3(kRenderBlock),
-                                       0, st, (const GuideSample *)B.recs.ptr, R, (GuideSample *)B.batch.ptr, bn, seed);
+                                       0, st, (const GuideSample *)B.recs.ptr, rin, R, (GuideSample *)B.batch.ptr, bn,
+                                       seed);
                     AZ_HIPR(C, hipGetLastError());
                     az_status gs = az_guide_train_on_stream(C->guide, B.batch.ptr, bn, st);
                     if (gs != AZ_OK) {
