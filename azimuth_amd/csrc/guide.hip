@@ -16,6 +16,7 @@
 // layer's output tiles feed the next layer without moving data.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -60,27 +61,49 @@ __device__ inline v4f mfma16(v4s a, v4s b, v4f c) { return __builtin_amdgcn_mfma
 __device__ inline v4s load4(const short *p) { return *reinterpret_cast<const v4s *>(p); }
 
 // ---- input encoding ----------------------------------------------------------------
-// Feature f < 96: frequency j = f/6, coordinate c = (f%6)/2, sin (f even) or
-// cos (f odd) of pi * 2^j * p[c]; 96..98 the normal, 99 the roughness, then 0.
-__device__ inline float encode_feature(int f, const float *p, const float *n, float rough) {
-    if (f < 96) {
-        int j = f / 6, c = (f % 6) >> 1;
-        float v = (float)(1u << j) * p[c];
-        return (f & 1) ? cospif(v) : sinpif(v);
-    }
-    if (f < 99) return n[f - 96];
-    if (f == 99) return rough;
-    return 0.0f;
-}
-
-// The 8 B-operand tiles of the encoded input for this lane's sample.
+// Features 0..95 are sin and cos of pi * 2^j * p[c] for 16 frequencies j and
+// the 3 coordinates c; the order follows the B-operand lanes: the lanes of
+// quarter q = lane / 16 hold features 16 t + 4 q + i (t < 6, i < 4), and with
+// m = 4 t + i that feature is frequency 4 q + m / 6 of coordinate (m % 6) / 2,
+// sin for even m % 6, cos for odd.  A lane evaluates sinpi/cospi at its
+// lowest frequency 2^(4q) and doubles three times (sin 2a = 2 sin a cos a,
+// cos 2a = (cos a - sin a)(cos a + sin a)), which stays within a few 1e-7 of
+// the direct values, far below bf16 rounding.  Features 96..98 are the
+// normal, 99 the roughness, 100..127 zero.
 __device__ inline void encode_tiles(const float *p, const float *n, float rough, int lane, v4s in[8]) {
-    const int kq = 4 * (lane >> 4);
+    const int q = lane >> 4;
+    const float scale = (float)(1u << (4 * q));
+    float S[4][3], Co[4][3];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
+    for (int c = 0; c < 3; ++c) {
+        const float th = scale * p[c];
+        S[0][c] = sinpif(th);
+        Co[0][c] = cospif(th);
+#pragma unroll
+        for (int j = 1; j < 4; ++j) {
+            const float a = S[j - 1][c], b = Co[j - 1][c];
+            S[j][c] = 2.0f * a * b;
+            Co[j][c] = (b - a) * (b + a);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
         v4s v;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] = f2bf(encode_feature(16 * t + kq + i, p, n, rough));
+        for (int i = 0; i < 4; ++i) {
+            const int m = 4 * t + i, j = m / 6, combo = m % 6, c = combo >> 1;
+            v[i] = f2bf((combo & 1) ? Co[j][c] : S[j][c]);
+        }
+        in[t] = v;
+    }
+#pragma unroll
+    for (int t = 6; t < 8; ++t) {
+        v4s v;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int f = 16 * t + 4 * q + i;
+            v[i] = f2bf(f < 99 ? n[f - 96] : (f == 99 ? rough : 0.0f));
+        }
         in[t] = v;
     }
 }
@@ -155,35 +178,115 @@ __global__ __launch_bounds__(64) void k_mfma_selftest(const float *A, const floa
 }
 
 // Inference: per query (p01, n, rough) -> 8 lobes x {w, mu, kappa}.
-__global__ __launch_bounds__(256) void k_guide_infer(GuideWeights W, const GuideQuery *__restrict__ q, uint32_t n,
-                                                     float *__restrict__ lobes) {
-    __shared__ float outs[4][16][kGuideOutPad];
-    // Every wave reaches the barrier below; waves past the end work on a
-    // clamped copy of the last query and store nothing.
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t base = (blockIdx.x * 4 + wave) * 16;
-    const uint32_t s = base + (lane & 15);
-    const bool valid = s < n;
-    GuideQuery g = q[valid ? s : n - 1];
-    v4s x[8], h1[4], h2[4];
-    v4f acc[4];
-    encode_tiles(g.p, g.n, g.rough, lane, x);
-    layer_fwd<8, 4, true>(W.wt + h_off(0), W.bias + 0 * kGuideBiasStride, kGuideIn, x, acc, h1, lane);
-    layer_fwd<4, 4, true>(W.wt + h_off(1), W.bias + 1 * kGuideBiasStride, kGuideHid, h1, acc, h2, lane);
-    layer_fwd<4, 4, true>(W.wt + h_off(2), W.bias + 2 * kGuideBiasStride, kGuideHid, h2, acc, h1, lane);
-    layer_fwd<4, 4, true>(W.wt + h_off(3), W.bias + 3 * kGuideBiasStride, kGuideHid, h1, acc, h2, lane);
-    layer_fwd<4, 3, false>(W.wt + h_off(4), W.bias + 4 * kGuideBiasStride, kGuideHid, h2, acc, nullptr, lane);
-    const int col = lane & 15, kq = 4 * (lane >> 4);
+// A block stages every layer's weights in LDS once, as A fragments (a wave
+// reads one fragment as 64 consecutive 8-byte words: no bank conflicts),
+// then loops over groups of 256 queries; a wave takes 64 of them, four
+// 16-sample column tiles that share each A fragment.
+constexpr int kInferGroups = 4;
+constexpr int kInferSamples = 4 * 16 * kInferGroups;   // per block iteration
+
+struct InferLds {
+    short w[kHalfCount];
+    float bias[5 * kGuideBiasStride];
+    float outs[4][16][kGuideOutPad];
+};
+
+template <int NI, int NO, bool RELU>
+__device__ inline void layer_lds(const short *w, const float *b, const v4s (*x)[8], v4s (*y)[8], int lane) {
+    const int kq = 4 * (lane >> 4);
 #pragma unroll
-    for (int t = 0; t < 3; ++t)
+    for (int mt = 0; mt < NO; ++mt) {
+        v4f c[kInferGroups];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) outs[wave][col][16 * t + kq + j] = acc[t][j];
+        for (int g = 0; g < kInferGroups; ++g) c[g] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kt = 0; kt < NI; ++kt) {
+            const v4s a = *reinterpret_cast<const v4s *>(w + ((mt * NI + kt) * 64 + lane) * 4);
+#pragma unroll
+            for (int g = 0; g < kInferGroups; ++g) c[g] = mfma16(a, x[g][kt], c[g]);
+        }
+#pragma unroll
+        for (int g = 0; g < kInferGroups; ++g) {
+            v4s h;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float v = c[g][j] + b[16 * mt + kq + j];
+                if (RELU) v = v > 0.0f ? v : 0.0f;
+                h[j] = f2bf(v);
+            }
+            y[g][mt] = h;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256, 2) void k_guide_infer(GuideWeights W, const GuideQuery *__restrict__ q, uint32_t n,
+                                                        float *__restrict__ lobes) {
+    __shared__ InferLds L;
+    for (int k = threadIdx.x; k < kHalfCount / 8; k += 256)
+        reinterpret_cast<uint4 *>(L.w)[k] = reinterpret_cast<const uint4 *>(W.wf)[k];
+    for (int k = threadIdx.x; k < 5 * kGuideBiasStride; k += 256) L.bias[k] = W.bias[k];
     __syncthreads();
-    if (lane < 16 && valid) {
-        float lb[40];
-        lobes_from_outputs(outs[wave][lane], lb);
-        float *dst = lobes + (size_t)s * 40;
-        for (int k = 0; k < 40; ++k) dst[k] = lb[k];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, kq = 4 * (lane >> 4);
+    const uint32_t groups = (n + kInferSamples - 1) / kInferSamples;
+    // The loop bounds are uniform over the block, so every wave reaches every
+    // barrier; queries past n are computed on a clamped copy and not stored.
+    for (uint32_t it = blockIdx.x; it < groups; it += gridDim.x) {
+        const uint32_t base = it * kInferSamples + wave * (16 * kInferGroups);
+        v4s x[kInferGroups][8], h1[kInferGroups][8], h2[kInferGroups][8];
+#pragma unroll
+        for (int g = 0; g < kInferGroups; ++g) {
+            const uint32_t s = base + 16 * g + col;
+            const GuideQuery Q = q[s < n ? s : n - 1];
+            encode_tiles(Q.p, Q.n, Q.rough, lane, x[g]);
+        }
+        layer_lds<8, 4, true>(L.w + h_off(0), L.bias + 0 * kGuideBiasStride, x, h1, lane);
+        layer_lds<4, 4, true>(L.w + h_off(1), L.bias + 1 * kGuideBiasStride, h1, h2, lane);
+        layer_lds<4, 4, true>(L.w + h_off(2), L.bias + 2 * kGuideBiasStride, h2, h1, lane);
+        layer_lds<4, 4, true>(L.w + h_off(3), L.bias + 3 * kGuideBiasStride, h1, h2, lane);
+        const short *w4 = L.w + h_off(4);
+        const float *b4 = L.bias + 4 * kGuideBiasStride;
+#pragma unroll
+        for (int g = 0; g < kInferGroups; ++g) {
+            // Output layer of this 16-sample tile, staged in LDS.
+#pragma unroll
+            for (int mt = 0; mt < 3; ++mt) {
+                v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int kt = 0; kt < 4; ++kt)
+                    c = mfma16(*reinterpret_cast<const v4s *>(w4 + ((mt * 4 + kt) * 64 + lane) * 4), h2[g][kt], c);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) L.outs[wave][col][16 * mt + kq + j] = c[j] + b4[16 * mt + kq + j];
+            }
+            __syncthreads();
+            // Lobe k of sample sl for the 128 (sample, lobe) items, two per
+            // lane; consecutive lanes write consecutive 20-byte lobes.
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int item = lane + 64 * r, sl = item >> 3, k = item & 7;
+                const uint32_t s = base + 16 * g + sl;
+                const float *o = L.outs[wave][sl];
+                float m = o[0];
+#pragma unroll
+                for (int e = 1; e < 8; ++e) m = fmaxf(m, o[e]);
+                float sum = 0.0f;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) sum += expf(o[e] - m);
+                const float x0 = o[8 + 3 * k], y0 = o[9 + 3 * k], z0 = o[10 + 3 * k];
+                const float len = sqrtf(x0 * x0 + y0 * y0 + z0 * z0);
+                const float inv = len > 1e-12f ? 1.0f / len : 0.0f;
+                const float kr = fminf(fmaxf(o[32 + k], kGuideKMin), kGuideKMax);
+                if (s < n) {
+                    float *dst = lobes + (size_t)s * 40 + 5 * k;
+                    dst[0] = expf(o[k] - m) / sum;
+                    dst[1] = x0 * inv;
+                    dst[2] = y0 * inv;
+                    dst[3] = z0 * inv;
+                    dst[4] = expf(kr);
+                }
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -375,7 +478,8 @@ __global__ __launch_bounds__(256) void k_guide_adam(float *__restrict__ p, float
 }
 
 __global__ __launch_bounds__(256) void k_guide_refresh(const float *__restrict__ p, short *__restrict__ wt,
-                                                       short *__restrict__ wb, float *__restrict__ bias) {
+                                                       short *__restrict__ wb, short *__restrict__ wf,
+                                                       float *__restrict__ bias) {
     int i = blockIdx.x * 256 + threadIdx.x;
     for (int l = 0; l < 5; ++l) {
         const int in = layer_in(l), out = layer_out(l), op = layer_out_pad(l);
@@ -384,6 +488,10 @@ __global__ __launch_bounds__(256) void k_guide_refresh(const float *__restrict__
             wt[h_off(l) + i] = f2bf(r < out ? p[w_off(l) + c * out + r] : 0.0f);
             int r2 = i / op, c2 = i % op;        // W [in][out_pad]
             wb[h_off(l) + i] = f2bf(c2 < out ? p[w_off(l) + r2 * out + c2] : 0.0f);
+            // A fragment (mt, kt), lane, value e: W^T[16 mt + lane % 16][16 kt + 4 (lane / 16) + e]
+            int frag = i >> 8, ln = (i >> 2) & 63, e = i & 3, nkt = in / 16;
+            int r3 = 16 * (frag / nkt) + (ln & 15), c3 = 16 * (frag % nkt) + 4 * (ln >> 4) + e;
+            wf[h_off(l) + i] = f2bf(r3 < out ? p[w_off(l) + c3 * out + r3] : 0.0f);
         }
         if (i < kGuideBiasStride) bias[l * kGuideBiasStride + i] = i < out ? p[b_off(l) + i] : 0.0f;
     }
@@ -397,9 +505,10 @@ using namespace az;
 struct az_guide {
     int device = 0;
     hipStream_t stream = nullptr;
-    DeviceBuffer params, grad, m, v, wt, wb, bias, samples, hT, dyT, loss, queries, lobes, scratch;
+    DeviceBuffer params, grad, m, v, wt, wb, wf, bias, samples, hT, dyT, loss, queries, lobes, scratch;
     uint32_t step = 0;
     uint32_t batchCap = 0;
+    int cuCount = 256;
     std::string lastError;
     float lr = 1e-3f;
 };
@@ -417,6 +526,7 @@ static GuideWeights weightsOf(az_guide *G) {
     GuideWeights w;
     w.wt = (const short *)G->wt.ptr;
     w.wb = (const short *)G->wb.ptr;
+    w.wf = (const short *)G->wf.ptr;
     w.bias = (const float *)G->bias.ptr;
     return w;
 }
@@ -424,7 +534,7 @@ static GuideWeights weightsOf(az_guide *G) {
 static az_status refresh(az_guide *G) {
     int n = kGuideIn * kGuideHid;   // the largest layer
     hipLaunchKernelGGL(k_guide_refresh, dim3((n + 255) / 256), dim3(256), 0, G->stream, (const float *)G->params.ptr,
-                       (short *)G->wt.ptr, (short *)G->wb.ptr, (float *)G->bias.ptr);
+                       (short *)G->wt.ptr, (short *)G->wb.ptr, (short *)G->wf.ptr, (float *)G->bias.ptr);
     AZ_GHIP(G, hipGetLastError());
     return AZ_OK;
 }
@@ -439,10 +549,12 @@ az_status az_guide_create(az_guide **Out, int dev, u64 Seed) {
     az_guide *G = new (std::nothrow) az_guide();
     if (!G) return AZ_ERR_OUT_OF_MEMORY;
     G->device = dev;
-    if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&G->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&G->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipDeviceGetAttribute(&G->cuCount, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
         delete G;
         return AZ_ERR_DEVICE;
     }
+    G->cuCount = std::max(1, G->cuCount);
     // He-normal initial weights from a seeded generator (Box-Muller on
     // splitmix64), zero biases.
     std::vector<float> p(kParamCount, 0.0f);
@@ -492,6 +604,7 @@ az_status az_guide_set_params(az_guide *G, const f32 *Params, u32 Count) {
     AZ_GHIP(G, G->v.alloc(pb));
     AZ_GHIP(G, G->wt.alloc((size_t)kHalfCount * sizeof(short)));
     AZ_GHIP(G, G->wb.alloc((size_t)kHalfCount * sizeof(short)));
+    AZ_GHIP(G, G->wf.alloc((size_t)kHalfCount * sizeof(short)));
     AZ_GHIP(G, G->bias.alloc((size_t)5 * kGuideBiasStride * sizeof(float)));
     AZ_GHIP(G, hipMemcpyAsync(G->params.ptr, Params, pb, hipMemcpyHostToDevice, G->stream));
     AZ_GHIP(G, hipMemsetAsync(G->grad.ptr, 0, pb, G->stream));
@@ -523,7 +636,10 @@ az_status az_guide_get_grad(az_guide *G, f32 *Grad, u32 Count) {
 az_status az_guide_infer_device(az_guide *G, const void *d_Queries, u32 N, f32 *d_Lobes) {
     if (!G || (N && (!d_Queries || !d_Lobes))) return AZ_ERR_INVALID_ARGUMENT;
     if (N == 0) return AZ_OK;
-    dim3 grid((N + 63) / 64);
+    // Persistent blocks: two per CU (the LDS a block holds allows two), each
+    // looping over groups of kInferSamples queries.
+    const u32 groups = (N + kInferSamples - 1) / kInferSamples;
+    dim3 grid(std::min<u32>(groups, 2u * (u32)G->cuCount));
     hipLaunchKernelGGL(k_guide_infer, grid, dim3(256), 0, G->stream, weightsOf(G), (const GuideQuery *)d_Queries, N,
                        d_Lobes);
     AZ_GHIP(G, hipGetLastError());

@@ -37,16 +37,29 @@ def init_params(seed=0):
     return Ws, bs
 
 
+def feature_map():
+    """Frequency j, coordinate c and cos flag of the 96 sinusoidal features.
+
+    Feature f = 16 t + 4 q + i (t < 6) belongs to the q-th quarter of the
+    MFMA operand lanes; with m = 4 t + i, it is frequency j = 4 q + m // 6,
+    coordinate (m % 6) // 2, sin for even m % 6 and cos for odd.  A lane
+    thus holds the four frequencies 4q..4q+3 of all three coordinates.
+    """
+    f = np.arange(96)
+    t, q, i = f // 16, (f % 16) // 4, f % 4
+    m = 4 * t + i
+    return 4 * q + m // 6, (m % 6) // 2, (m % 6) % 2
+
+
 def encode(p01, nrm, rough):
     """(n,3) positions in [0,1], (n,3) normals, (n,) roughness -> (n,128)."""
     p01 = np.asarray(p01, np.float32)
     n = p01.shape[0]
     x = np.zeros((n, IN), np.float64)
-    for j in range(FREQS):
-        v = (np.float32(2.0 ** j) * p01).astype(np.float64)     # exact in float32
-        r = np.mod(v, 2.0)
-        x[:, 6 * j + 0:6 * j + 6:2] = np.sin(np.pi * r)
-        x[:, 6 * j + 1:6 * j + 6:2] = np.cos(np.pi * r)
+    j, c, cosf = feature_map()
+    v = (np.float32(2.0) ** j.astype(np.float32))[None, :] * p01[:, c]      # exact in float32
+    r = np.mod(v.astype(np.float64), 2.0)
+    x[:, :96] = np.where(cosf[None, :] == 1, np.cos(np.pi * r), np.sin(np.pi * r))
     x[:, 96:99] = nrm
     x[:, 99] = rough
     return x

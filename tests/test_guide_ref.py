@@ -73,5 +73,11 @@ def test_encoding_layout():
     assert x.shape == (1, 128)
     # frequency 0, coordinate 1 (y = 0.5): sin(pi/2) = 1, cos = 0
     assert x[0, 2] == pytest.approx(1.0) and abs(x[0, 3]) < 1e-12
+    # frequency 2 of x = 0.25: sin(pi) = 0, cos(pi) = -1 at features 48, 49;
+    # frequency 15 of z = 0.75: cos(24576 pi) = 1 at feature 95
+    assert abs(x[0, 48]) < 1e-12 and x[0, 49] == pytest.approx(-1.0)
+    assert x[0, 95] == pytest.approx(1.0)
+    j, c, cs = R.feature_map()
+    assert len(set(zip(j.tolist(), c.tolist(), cs.tolist()))) == 96
     assert list(x[0, 96:100]) == [0.0, 1.0, 0.0, pytest.approx(0.3)]
     assert np.all(x[0, 100:] == 0.0)
