@@ -88,6 +88,10 @@ struct DevScene {
     uint32_t triTotal;
     uint32_t pad;
     float boxMin[3], boxInv[3];      // scene box: guide positions are (P - min) * inv
+    // Environment luminance distribution over the (u, v) cells of the map:
+    // row CDF (envH + 1 floats) then one column CDF per row (envW + 1 each).
+    const float *envCdf;
+    uint32_t envW, envH;
 };
 
 // ---- render queues --------------------------------------------------------
@@ -140,6 +144,7 @@ struct ChunkParams {
     uint32_t shadowCapacity;
     uint32_t cull;
     uint32_t gaussian;
+    uint32_t envLuminance;           // environment NEE from the luminance distribution
 };
 
 struct RenderState {

@@ -258,6 +258,54 @@ __device__ inline V3 guide_sample(const float *lb, float u0, float u1, float u2)
 
 // Outputs of shade: the next level's rays and the shadow rays, both
 // segmented (block b appends to segment b % nseg).
+// ---- environment luminance sampling ----------------------------------------------------
+// The environment map is looked up at u = (pi - atan2(z, x)) / 2pi,
+// v = acos(y) / pi (main.cpp:726-733); a (u, v) cell of the W x H grid spans
+// the solid angle 2 pi^2 sin(theta) du dv.  Cells are drawn by their
+// luminance (3x3 texel maximum, so that nearest and bilinear lookups are
+// both covered, plus a small floor) times sin(theta), then a uniform point
+// in the cell: pdf(w) = P(cell) W H / (2 pi^2 sin(theta)).
+
+// Largest k < n with cdf[k] <= u (cdf[0] = 0, cdf[n] = 1).
+__device__ inline uint32_t cdf_find(const float *cdf, uint32_t n, float u) {
+    uint32_t lo = 0u, hi = n;
+    while (hi - lo > 1u) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (cdf[mid] <= u) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ inline V3 env_lum_sample(const DevScene &S, float u0, float u1, float &pdf) {
+    const uint32_t W = S.envW, H = S.envH;
+    const float *rowCdf = S.envCdf;
+    const uint32_t r = cdf_find(rowCdf, H, u0);
+    const float pr = rowCdf[r + 1] - rowCdf[r];
+    const float fv = fminf(fmaxf((u0 - rowCdf[r]) / pr, 0.0f), 0.99999994f);
+    const float *colCdf = S.envCdf + (H + 1) + (size_t)r * (W + 1);
+    const uint32_t c = cdf_find(colCdf, W, u1);
+    const float pc = colCdf[c + 1] - colCdf[c];
+    const float fu = fminf(fmaxf((u1 - colCdf[c]) / pc, 0.0f), 0.99999994f);
+    const float u = ((float)c + fu) / (float)W, v = ((float)r + fv) / (float)H;
+    const float th = kPiF * v, ph = kPiF - 2.0f * kPiF * u;
+    const float st = sinf(th);
+    pdf = pr * pc * (float)W * (float)H / (2.0f * kPiF * kPiF * fmaxf(st, 1e-7f));
+    return v3(st * cosf(ph), cosf(th), st * sinf(ph));
+}
+
+__device__ inline float env_lum_pdf(const DevScene &S, V3 L) {
+    const uint32_t W = S.envW, H = S.envH;
+    const float u = (-atan2f(L.z, L.x) + kPiF) / (2.0f * kPiF);
+    const float v = acosf(fminf(fmaxf(L.y, -1.0f), 1.0f)) / kPiF;
+    const uint32_t r = min((uint32_t)fmaxf(v * (float)H, 0.0f), H - 1u);
+    const uint32_t c = min((uint32_t)fmaxf(u * (float)W, 0.0f), W - 1u);
+    const float *colCdf = S.envCdf + (H + 1) + (size_t)r * (W + 1);
+    const float p = (S.envCdf[r + 1] - S.envCdf[r]) * (colCdf[c + 1] - colCdf[c]);
+    const float st = sqrtf(fmaxf(0.0f, 1.0f - L.y * L.y));
+    return p * (float)W * (float)H / (2.0f * kPiF * kPiF * fmaxf(st, 1e-7f));
+}
+
 struct ShadeOut {
     PathRay *next;
     ShadowRay *shadow;
@@ -560,12 +608,17 @@ __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeo
     // non-zero contribution.
     if (S.useEnv) {
         const float *lb = GI.lobes ? GI.lobes + (size_t)i * 40 : nullptr;
+        const bool lum = cp.envLuminance != 0u;
         V3 L;
         bool fromGuide = false;
+        float pBase = 0.0f;
         if (lb) fromGuide = rng.uniform() < GI.alpha;
         if (fromGuide) {
             float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
             L = guide_sample(lb, u0, u1, u2);
+        } else if (lum) {
+            float u0 = rng.uniform(), u1 = rng.uniform();
+            L = env_lum_sample(S, u0, u1, pBase);
         } else {
             float phi = 2.0f * kPiF * rng.uniform();
             float cp_ = cosf(phi), sp_ = sinf(phi);
@@ -579,10 +632,14 @@ __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeo
         float ev = Acos(L.y) / kPiF;
         V3 I = sample_texture(S, S.env, v3(eu, ev, 1.0f), eu, ev);
         float ca = dot(L, N);
-        float invPdf, pdf = 0.0f;
+        float invPdf;
         if (lb) {
-            pdf = GI.alpha * guide_pdf(lb, L) + (1.0f - GI.alpha) * fmaxf(ca, 0.0f) / kPiF;
+            // One-sample MIS over the guide and the base distribution.
+            float base = lum ? (fromGuide ? env_lum_pdf(S, L) : pBase) : fmaxf(ca, 0.0f) / kPiF;
+            float pdf = GI.alpha * guide_pdf(lb, L) + (1.0f - GI.alpha) * base;
             invPdf = pdf > 0.0f ? 1.0f / pdf : 0.0f;
+        } else if (lum) {
+            invPdf = pBase > 0.0f ? 1.0f / pBase : 0.0f;
         } else {
             if (ca == 0.0f) ca = 0.000001f;
             invPdf = kPiF / ca;
@@ -983,7 +1040,72 @@ struct RenderBuffers {
     DeviceBuffer materials, textures, texels, perms, lights, lightTris, prims, itemShade, perlin, pcg;
     DeviceBuffer q0, q1, hits, shadow, acc, counters, pixels;
     DeviceBuffer queries, lobes, recs, batch;   // learned guide
+    DeviceBuffer envCdf;                        // environment luminance distribution
 };
+
+// Row and column CDFs of the environment map's luminance over its (u, v)
+// cells: cell (c, r) weighs the largest texel luminance among texels
+// c-1..c+1, r-1..r+1 (so that it covers what nearest and bilinear lookups
+// read there), plus 1e-4 of the mean as a floor (every direction stays
+// reachable), times sin(theta) at the row centre.
+static std::vector<float> envLuminanceCdf(const az_texture &t) {
+    const int W = (int)t.Width, H = (int)t.Height;
+    const bool f32 = t.PixelFormat == PixelFormat_RGB_F32;
+    std::vector<double> lum((size_t)W * H);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            size_t i = ((size_t)y * W + x) * 3;
+            double r, g, b;
+            if (f32) {
+                const float *p = (const float *)t.Pixels;
+                r = p[i], g = p[i + 1], b = p[i + 2];
+            } else {
+                const uint8_t *p = (const uint8_t *)t.Pixels;
+                r = p[i], g = p[i + 1], b = p[i + 2];
+            }
+            lum[(size_t)y * W + x] = std::fabs(0.2126 * r + 0.7152 * g + 0.0722 * b);
+        }
+    std::vector<double> cell((size_t)W * H);
+    double total = 0.0;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            double m = 0.0;
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    int yy = std::min(std::max(y + dy, 0), H - 1), xx = (x + dx + W) % W;
+                    m = std::max(m, lum[(size_t)yy * W + xx]);
+                }
+            cell[(size_t)y * W + x] = m;
+            total += m;
+        }
+    const double floorv = 1e-4 * std::max(total / ((double)W * H), 1e-30);
+    std::vector<float> out((size_t)(H + 1) + (size_t)H * (W + 1));
+    std::vector<double> rowMass(H);
+    for (int y = 0; y < H; ++y) {
+        const double st = std::sin(kPiD * (y + 0.5) / H);
+        double acc = 0.0;
+        float *col = out.data() + (H + 1) + (size_t)y * (W + 1);
+        col[0] = 0.0f;
+        std::vector<double> run(W + 1, 0.0);
+        for (int x = 0; x < W; ++x) {
+            acc += (cell[(size_t)y * W + x] + floorv) * st;
+            run[x + 1] = acc;
+        }
+        for (int x = 1; x <= W; ++x) col[x] = (float)(run[x] / acc);
+        col[W] = 1.0f;
+        rowMass[y] = acc;
+    }
+    double acc = 0.0;
+    out[0] = 0.0f;
+    double rt = 0.0;
+    for (int y = 0; y < H; ++y) rt += rowMass[y];
+    for (int y = 0; y < H; ++y) {
+        acc += rowMass[y];
+        out[y + 1] = (float)(acc / rt);
+    }
+    out[H] = 1.0f;
+    return out;
+}
 
 static RenderBuffers &buffersOf(az_gpu_ctx *C) {
     // One set of buffers per context, owned by the context's render state.
@@ -1170,6 +1292,16 @@ az_status az_upload_shading(az_gpu_ctx *C, const scene *S, const UploadInfo &inf
     D.pathTrace = S->PathTrace ? 1u : 0u;
     D.importance = S->ImportanceSampling ? 1u : 0u;
     D.useEnv = S->UseEnvironmentMap ? 1u : 0u;
+    D.envCdf = nullptr;
+    D.envW = D.envH = 0u;
+    if (S->UseEnvironmentMap && S->EnvironmentMap.Type == TextureType_2D && S->EnvironmentMap.Pixels &&
+        S->EnvironmentMap.Width > 0 && S->EnvironmentMap.Height > 0) {
+        std::vector<float> cdf = envLuminanceCdf(S->EnvironmentMap);
+        AZ_HIPR(C, uploadVec(B.envCdf, cdf, C->stream));
+        D.envCdf = (const float *)B.envCdf.ptr;
+        D.envW = S->EnvironmentMap.Width;
+        D.envH = S->EnvironmentMap.Height;
+    }
     D.useBackgroundTexture = S->UseBackgroundTexture ? 1u : 0u;
     D.ambient[0] = S->AmbientLight.x;
     D.ambient[1] = S->AmbientLight.y;
@@ -1282,6 +1414,12 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
     // The learned guide: a fresh network per render, trained on the
     // environment samples of the first trainSpp samples per pixel.
     const bool guided = o.GuideEnable != 0 && S.useEnv;
+    if (o.EnvSampling > AZ_ENV_LUMINANCE) return AZ_ERR_INVALID_ARGUMENT;
+    const bool envLum = o.EnvSampling == AZ_ENV_LUMINANCE && S.useEnv;
+    if (envLum && !S.envCdf) {
+        C->lastError = "luminance sampling needs an image environment map";
+        return AZ_ERR_UNSUPPORTED;
+    }
     const u32 trainSpp = guided ? (o.GuideTrainSpp ? o.GuideTrainSpp : 32u) : 0u;
     const u32 stepsPerChunk = o.GuideStepsPerChunk ? o.GuideStepsPerChunk : 8u;
     const float alpha = o.GuideFraction > 0.0f && o.GuideFraction < 1.0f ? o.GuideFraction : 0.5f;
@@ -1340,6 +1478,7 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
         cp.shadowCapacity = 0;
         cp.cull = Cull ? 1u : 0u;
         cp.gaussian = Filter == FilterType_Gaussian ? 1u : 0u;
+        cp.envLuminance = envLum ? 1u : 0u;
         DeviceBuffer *bufA = &B.q0, *bufB = &B.q1;
         int offA = 0;
         float *acc = (float *)B.acc.ptr;

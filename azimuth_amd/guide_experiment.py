@@ -12,7 +12,9 @@ T); the guided tracer (environment directions from the learned guide mixed
 within T.  Times are wall-clock around the render call (guide creation and
 training included), after a device synchronize.  Because both estimators are unbiased, MSE scales as 1/time, and
 the report also gives the time-normalized ratio (MSE_plain * T_plain) /
-(MSE_guided * T_guided).  One JSON line is printed.
+(MSE_guided * T_guided).  For comparison it also renders with the
+environment map's luminance distribution, alone and as the guide's base
+distribution.  One JSON line is printed.
 """
 from __future__ import annotations
 
@@ -32,8 +34,8 @@ def resolve(px):
     return px[:, :3] / np.maximum(px[:, 3:4], 1e-300)
 
 
-def timed(ctx, cam, filt, seed, spp, guide):
-    px = ctx.render_f64(cam, filt=filt, seed=seed, spp=spp, guide=guide)
+def timed(ctx, cam, filt, seed, spp, guide, env=N.AZ_ENV_COSINE):
+    px = ctx.render_f64(cam, filt=filt, seed=seed, spp=spp, guide=guide, env_sampling=env)
     return resolve(px), ctx.last_render_s
 
 
@@ -83,6 +85,14 @@ def main(argv=None):
         g_eq, t_g_eq = timed(ctx, cam, filt, a.seed + 7, spp_eq, True)
     mse_g_eq = float(np.mean((g_eq - ref) ** 2))
     print(f"guided {spp_eq} spp {t_g_eq:.2f}s mse {mse_g_eq:.4e}", file=sys.stderr, flush=True)
+    # Beside the learned guide: the environment map's luminance distribution
+    # (no learning; samples the sun directly), alone and as the guide's base.
+    lum, t_lum = timed(ctx, cam, filt, a.seed, a.spp, False, N.AZ_ENV_LUMINANCE)
+    mse_lum = float(np.mean((lum - ref) ** 2))
+    glum, t_glum = timed(ctx, cam, filt, a.seed, a.spp, True, N.AZ_ENV_LUMINANCE)
+    mse_glum = float(np.mean((glum - ref) ** 2))
+    print(f"luminance {a.spp} spp {t_lum:.2f}s mse {mse_lum:.4e}; guide+luminance {t_glum:.2f}s mse {mse_glum:.4e}",
+          file=sys.stderr, flush=True)
     out = {
         "experiment": "equal-time MSE vs reference, guided vs plain",
         "workload": f"architectural interior {W}x{H} ({sc.triangle_count} triangles), Gaussian filter, depth 6",
@@ -93,6 +103,8 @@ def main(argv=None):
         "mse_ratio_equal_time": round(mse_plain / mse_g_eq, 3),
         "mse_ratio_time_normalized": round((mse_plain * t_plain) / (mse_g_eq * t_g_eq), 3),
         "mse_ratio_same_spp": round(mse_plain / mse_g_same, 3),
+        "luminance_env_sampling": {"spp": a.spp, "seconds": round(t_lum, 3), "mse": mse_lum},
+        "guided_over_luminance": {"spp": a.spp, "seconds": round(t_glum, 3), "mse": mse_glum},
     }
     print(json.dumps(out), flush=True)
     ctx.close()

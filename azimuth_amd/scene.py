@@ -322,8 +322,9 @@ class GpuContext:
         return ms.value
 
     def render(self, camera, filt=N.FilterType_Box, cull=False, seed=0, spp=0, row_begin=0, row_end=0,
-               guide=False, guide_fraction=0.5):
+               guide=False, guide_fraction=0.5, env_sampling=N.AZ_ENV_COSINE):
         o = N.az_render_opts()
+        o.EnvSampling = int(env_sampling)
         o.Seed = int(seed)
         o.SampleCountOverride = int(spp)
         o.RowBegin, o.RowEnd = int(row_begin), int(row_end)
@@ -335,7 +336,8 @@ class GpuContext:
                                        out.ctypes.data), "az_gpu_render", self.h)
         return out
 
-    def render_f64(self, camera, filt=N.FilterType_Box, seed=0, spp=0, guide=False, guide_fraction=0.5):
+    def render_f64(self, camera, filt=N.FilterType_Box, seed=0, spp=0, guide=False, guide_fraction=0.5,
+                   env_sampling=N.AZ_ENV_COSINE):
         """Device render accumulated in double precision; returns (W*H, 4)
         float64 (colour sums, weight sums).  The wall time of the render call
         (guide training included) is left in last_render_s."""
@@ -346,6 +348,7 @@ class GpuContext:
         o.Seed = int(seed)
         o.SampleCountOverride = int(spp)
         o.AccumulateF64 = 1
+        o.EnvSampling = int(env_sampling)
         o.GuideEnable = 1 if guide else 0
         o.GuideFraction = float(guide_fraction)
         self.synchronize()

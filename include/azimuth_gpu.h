@@ -70,8 +70,14 @@ typedef struct {
     f32 GuideFraction;      /* probability of drawing the direction from the guide (0 = 0.5) */
     u32 GuideTrainSpp;      /* samples per pixel rendered while training (0 = 32) */
     u32 GuideStepsPerChunk; /* Adam steps after each training chunk (0 = 8) */
-    u32 Reserved;
+    u32 EnvSampling;        /* environment next-event directions: AZ_ENV_COSINE (the reference's
+                               cosine distribution about the normal) or AZ_ENV_LUMINANCE (the
+                               environment map's luminance x sin(theta) distribution; unbiased,
+                               different random directions) */
 } az_render_opts;
+
+#define AZ_ENV_COSINE    0u
+#define AZ_ENV_LUMINANCE 1u
 
 typedef struct {
     f64 LastKernelMs;       /* device time of the last trace/render call's kernels */

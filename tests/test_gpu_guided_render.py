@@ -1,6 +1,8 @@
-"""Guided rendering: the one-sample MIS estimator with the learned guide must
-stay unbiased (its mean image converges to the plain tracer's) and should
-reduce variance on a scene lit by a small, bright sun."""
+"""Guided rendering and environment luminance sampling: both replace the
+reference's cosine-distributed environment directions by another
+distribution (the guide in a one-sample MIS mixture; the map's luminance
+CDF), so both must stay unbiased (their mean image converges to the plain
+tracer's) and should reduce variance on a scene lit by a small, bright sun."""
 import numpy as np
 import pytest
 
@@ -31,3 +33,37 @@ def test_guided_interior_unbiased_and_lower_variance(gpu):
     assert np.all(np.isfinite(guided))
     assert abs(m_g - m_ref) < 5 * se + 1e-3 * m_ref
     assert mse_g < mse_p
+
+
+def test_luminance_environment_sampling_unbiased_and_lower_variance(gpu):
+    sc, cam = workloads.interior_scene(96, 54, spp=64, max_depth=3, shells=12)
+    gpu.upload_scene(sc)
+    filt = N.FilterType_Box
+    ref = resolve(gpu.render_f64(cam, filt=filt, seed=100, spp=2304))
+    plain = resolve(gpu.render_f64(cam, filt=filt, seed=1, spp=64))
+    lum = resolve(gpu.render_f64(cam, filt=filt, seed=1, spp=64, env_sampling=N.AZ_ENV_LUMINANCE))
+    both = resolve(gpu.render_f64(cam, filt=filt, seed=1, spp=64, guide=True, env_sampling=N.AZ_ENV_LUMINANCE))
+    mse_p = float(np.mean((plain - ref) ** 2))
+    mse_l = float(np.mean((lum - ref) ** 2))
+    mse_b = float(np.mean((both - ref) ** 2))
+    m_ref = float(ref.mean())
+    se_ref = float(np.sqrt(mse_p / plain.size)) * (64 / 2304) ** 0.5 * 4
+    print(f"mean ref {m_ref:.5f} luminance {lum.mean():.5f} guide+luminance {both.mean():.5f}; "
+          f"mse plain {mse_p:.4e} luminance {mse_l:.4e} guide+luminance {mse_b:.4e}", flush=True)
+    assert np.all(np.isfinite(lum)) and np.all(np.isfinite(both))
+    # The reference itself carries Monte Carlo error (se_ref); the luminance
+    # renders' own error is small.
+    for img, mse in ((lum, mse_l), (both, mse_b)):
+        se = float(np.sqrt(mse / img.size)) * 4
+        assert abs(float(img.mean()) - m_ref) < 5 * (se + se_ref) + 1e-3 * m_ref
+    assert mse_l < mse_p
+
+
+def test_luminance_sampling_without_environment_changes_nothing(gpu):
+    from azimuth_amd import configs
+    sc, cam = configs.cornell_scene(24, 24, spp=4, max_depth=3)
+    gpu.upload_scene(sc)
+    a = gpu.render(cam, seed=2)
+    b = gpu.render(cam, seed=2, env_sampling=N.AZ_ENV_LUMINANCE)
+    assert np.array_equal(a["TotalWeight"], b["TotalWeight"])
+    np.testing.assert_allclose(a["Color"], b["Color"], rtol=1e-5, atol=1e-6)
