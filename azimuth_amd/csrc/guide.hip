@@ -375,21 +375,11 @@ __global__ __launch_bounds__(256) void k_guide_train_pass1(GuideWeights W, const
 #pragma unroll
         for (int j = 0; j < 4; ++j) dy[t][j] = outs[wave][col][16 * t + kq + j];
     }
-    auto biasGrad = [&](int l, int t, v4f v) {
-        // Sum over the 16 samples (lanes with the same lane/16) and add.
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            float x = v[j];
-            for (int off = 8; off > 0; off >>= 1) x += __shfl_xor(x, off, 16);
-            if (col == 0 && active) atomicAdd(T.grad + b_off(l) + 16 * t + kq + j, x);
-        }
-    };
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) dyb[t][j] = f2bf(dy[t][j]);
         storeT(T.dyT + T.dyOff[4], t, dyb[t]);
-        if (16 * t + kq < kGuideOut) biasGrad(4, t, dy[t]);
     }
     // Backward: dX^T = W dY^T (A = W [in][out_pad]), masked by ReLU'.
     const int row = lane & 15;
@@ -415,30 +405,29 @@ __global__ __launch_bounds__(256) void k_guide_train_pass1(GuideWeights W, const
     back(4, 3, dyb, h[3], g4, gf);                      // dY of layer 3 (hidden 4)
     for (int t = 0; t < 4; ++t) {
         storeT(T.dyT + T.dyOff[3], t, g4[t]);
-        biasGrad(3, t, gf[t]);
     }
     back(3, 4, g4, h[2], g3, gf);
     for (int t = 0; t < 4; ++t) {
         storeT(T.dyT + T.dyOff[2], t, g3[t]);
-        biasGrad(2, t, gf[t]);
     }
     back(2, 4, g3, h[1], g4, gf);
     for (int t = 0; t < 4; ++t) {
         storeT(T.dyT + T.dyOff[1], t, g4[t]);
-        biasGrad(1, t, gf[t]);
     }
     back(1, 4, g4, h[0], g3, gf);
     for (int t = 0; t < 4; ++t) {
         storeT(T.dyT + T.dyOff[0], t, g3[t]);
-        biasGrad(0, t, gf[t]);
     }
 }
 
-// Training pass 2: dW_l [in][out] = sum over the batch of H_l^T (in x batch)
-// times dY_l (batch x out).  One wave per 16x16 output tile and K-slice;
-// slices are added with atomics.  A[m = in][k = sample] is read from H^T,
-// B[k = sample][n = out] from dY^T.
-__global__ __launch_bounds__(64) void k_guide_train_dw(GuideTrainBuffers T, float *__restrict__ grad, int layer,
+// Training pass 2: the weight and bias gradients of one layer over one
+// K-slice of the batch.  A wave computes one 16x16 tile of dW_l [in][out] =
+// H_l^T (in x batch) x dY_l (batch x out) on MFMA (A[m = in][k = sample]
+// from H^T, B[k = sample][n = out] from dY^T) and writes it to the slice's
+// partial (laid out like the master parameters); waves of the first row
+// tile also sum dY over the slice, the bias gradient.  No atomics: the
+// partials are added in slice order by k_guide_reduce.
+__global__ __launch_bounds__(64) void k_guide_train_dw(GuideTrainBuffers T, float *__restrict__ part, int layer,
                                                        uint32_t kSlice) {
     const int lane = threadIdx.x;
     const int in = layer_in(layer), out = layer_out(layer);
@@ -450,16 +439,32 @@ __global__ __launch_bounds__(64) void k_guide_train_dw(GuideTrainBuffers T, floa
     const int ocol = 16 * nt + row;
     const short *dT = T.dyT + T.dyOff[layer] + (size_t)ocol * T.batch;
     v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+    float bsum = 0.0f;
     for (uint32_t k = k0; k < k1; k += 16) {
         v4s a = load4(hT + k + kq);
         v4s b = load4(dT + k + kq);
         c = mfma16(a, b, c);
+        bsum += (bf2f(b[0]) + bf2f(b[1])) + (bf2f(b[2]) + bf2f(b[3]));
     }
+    float *dst = part + (size_t)blockIdx.y * kParamCount;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         int i = 16 * mt + kq + j, o = 16 * nt + row;
-        if (i < in && o < out) atomicAdd(grad + w_off(layer) + (size_t)i * out + o, c[j]);
+        if (i < in && o < out) dst[w_off(layer) + i * out + o] = c[j];
     }
+    bsum += __shfl_xor(bsum, 16);
+    bsum += __shfl_xor(bsum, 32);
+    if (mt == 0 && lane < 16 && ocol < out) dst[b_off(layer) + ocol] = bsum;
+}
+
+// grad += the sum of the slices' partial gradients.
+__global__ __launch_bounds__(256) void k_guide_reduce(const float *__restrict__ part, uint32_t slices,
+                                                      float *__restrict__ grad) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= kParamCount) return;
+    float s = 0.0f;
+    for (uint32_t k = 0; k < slices; ++k) s += part[(size_t)k * kParamCount + p];
+    grad[p] += s;
 }
 
 // Adam on the FP32 masters, then the BF16 copies in both layouts.
@@ -505,7 +510,7 @@ using namespace az;
 struct az_guide {
     int device = 0;
     hipStream_t stream = nullptr;
-    DeviceBuffer params, grad, m, v, wt, wb, wf, bias, samples, hT, dyT, loss, queries, lobes, scratch;
+    DeviceBuffer params, grad, part, m, v, wt, wb, wf, bias, samples, hT, dyT, loss, queries, lobes, scratch;
     uint32_t step = 0;
     uint32_t batchCap = 0;
     int cuCount = 256;
@@ -690,13 +695,17 @@ az_status az_guide_train_device(az_guide *G, const void *d_Samples, u32 N, b32 A
     hipLaunchKernelGGL(k_guide_train_pass1, g1, dim3(256), 0, G->stream, weightsOf(G), (const GuideSample *)d_Samples, N,
                        1.0f / (float)N, T);
     AZ_GHIP(G, hipGetLastError());
-    const uint32_t kSlice = 1024;
+    const uint32_t kSlice = 1024, slices = (N + kSlice - 1) / kSlice;
+    AZ_GHIP(G, G->part.alloc((size_t)slices * kParamCount * sizeof(float)));
     for (int l = 0; l < 5; ++l) {
         int tiles = ((layer_in(l) + 15) / 16) * ((layer_out(l) + 15) / 16);
-        dim3 g2(tiles, (N + kSlice - 1) / kSlice);
-        hipLaunchKernelGGL(k_guide_train_dw, g2, dim3(64), 0, G->stream, T, (float *)G->grad.ptr, l, kSlice);
+        dim3 g2(tiles, slices);
+        hipLaunchKernelGGL(k_guide_train_dw, g2, dim3(64), 0, G->stream, T, (float *)G->part.ptr, l, kSlice);
         AZ_GHIP(G, hipGetLastError());
     }
+    hipLaunchKernelGGL(k_guide_reduce, dim3((kParamCount + 255) / 256), dim3(256), 0, G->stream,
+                       (const float *)G->part.ptr, slices, (float *)G->grad.ptr);
+    AZ_GHIP(G, hipGetLastError());
     if (MeanLoss) {
         std::vector<float> h(N);
         AZ_GHIP(G, hipMemcpyAsync(h.data(), G->loss.ptr, (size_t)N * sizeof(float), hipMemcpyDeviceToHost, G->stream));

@@ -120,8 +120,8 @@ def test_inference_matches_reference(guide):
 def grads_bf16(Ws, bs, x, w, wgt):
     """The float64 reference's backward with bf16 rounding where the kernels
     round: activations, the output gradient and each back-propagated
-    gradient before it enters the next product; bias gradients from the
-    unrounded values."""
+    gradient before it enters the next product; bias gradients are sums of
+    the rounded output gradients, as the weight-gradient kernel forms them."""
     n = x.shape[0]
     hs = [bf16(x)]
     for l in range(4):
@@ -144,8 +144,8 @@ def grads_bf16(Ws, bs, x, w, wgt):
     dWs, dbs = [None] * 5, [None] * 5
     dy = do
     for l in range(4, -1, -1):
-        dbs[l] = dy.sum(0)
         dyb = bf16(dy)
+        dbs[l] = dyb.sum(0)
         dWs[l] = hs[l].T @ dyb
         if l > 0:
             dy = (dyb @ bf16(Ws[l]).T) * (hs[l] > 0)
