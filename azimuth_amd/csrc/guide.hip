@@ -58,6 +58,10 @@ __device__ inline float bf2f(short h) { return __uint_as_float(((uint32_t)(uint1
 
 __device__ inline v4f mfma16(v4s a, v4s b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
 
+// Four floats to bf16, round to nearest even (two v_cvt_pk_bf16_f32).
+typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+__device__ inline v4s to_bf16x4(v4f x) { return __builtin_bit_cast(v4s, __builtin_convertvector(x, v4bf)); }
+
 __device__ inline v4s load4(const short *p) { return *reinterpret_cast<const v4s *>(p); }
 
 // ---- input encoding ----------------------------------------------------------------
@@ -70,9 +74,11 @@ __device__ inline v4s load4(const short *p) { return *reinterpret_cast<const v4s
 // cos 2a = (cos a - sin a)(cos a + sin a)), which stays within a few 1e-7 of
 // the direct values, far below bf16 rounding.  Features 96..98 are the
 // normal, 99 the roughness, 100..127 zero.
-__device__ inline void encode_tiles(const float *p, const float *n, float rough, int lane, v4s in[8]) {
+__device__ inline void encode_tiles(float px, float py, float pz, float nx, float ny, float nz, float rough, int lane,
+                                    v4s in[8]) {
     const int q = lane >> 4;
     const float scale = (float)(1u << (4 * q));
+    const float p[3] = {px, py, pz}, n[3] = {nx, ny, nz};
     float S[4][3], Co[4][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -88,23 +94,23 @@ __device__ inline void encode_tiles(const float *p, const float *n, float rough,
     }
 #pragma unroll
     for (int t = 0; t < 6; ++t) {
-        v4s v;
+        v4f v;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int m = 4 * t + i, j = m / 6, combo = m % 6, c = combo >> 1;
-            v[i] = f2bf((combo & 1) ? Co[j][c] : S[j][c]);
+            v[i] = (combo & 1) ? Co[j][c] : S[j][c];
         }
-        in[t] = v;
+        in[t] = to_bf16x4(v);
     }
 #pragma unroll
     for (int t = 6; t < 8; ++t) {
-        v4s v;
+        v4f v;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int f = 16 * t + 4 * q + i;
-            v[i] = f2bf(f < 99 ? n[f - 96] : (f == 99 ? rough : 0.0f));
+            v[i] = f < 99 ? n[f - 96] : (f == 99 ? rough : 0.0f);
         }
-        in[t] = v;
+        in[t] = to_bf16x4(v);
     }
 }
 
@@ -116,25 +122,18 @@ __device__ inline void layer_fwd(const short *__restrict__ wt, const float *__re
     const int row = lane & 15, kq = 4 * (lane >> 4);
 #pragma unroll
     for (int mt = 0; mt < NO; ++mt) {
-        v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+        v4f c = *reinterpret_cast<const v4f *>(b + 16 * mt + kq);     // the bias starts the sum
 #pragma unroll
         for (int kt = 0; kt < NI; ++kt) {
             v4s a = load4(wt + (size_t)(16 * mt + row) * in + 16 * kt + kq);
             c = mfma16(a, x[kt], c);
         }
+        if (RELU) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            float v = c[j] + b[16 * mt + kq + j];
-            if (RELU) v = v > 0.0f ? v : 0.0f;
-            c[j] = v;
+            for (int j = 0; j < 4; ++j) c[j] = fmaxf(c[j], 0.0f);
         }
         acc[mt] = c;
-        if (y) {
-            v4s h;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) h[j] = f2bf(c[j]);
-            y[mt] = h;
-        }
+        if (y) y[mt] = to_bf16x4(c);
     }
 }
 
@@ -185,7 +184,7 @@ __global__ __launch_bounds__(64) void k_mfma_selftest(const float *A, const floa
 constexpr int kInferGroups = 4;
 constexpr int kInferSamples = 4 * 16 * kInferGroups;   // per block iteration
 
-struct InferLds {
+struct alignas(16) InferLds {
     short w[kHalfCount];
     float bias[5 * kGuideBiasStride];
     float outs[4][16][kGuideOutPad];
@@ -196,9 +195,10 @@ __device__ inline void layer_lds(const short *w, const float *b, const v4s (*x)[
     const int kq = 4 * (lane >> 4);
 #pragma unroll
     for (int mt = 0; mt < NO; ++mt) {
+        const v4f bias = *reinterpret_cast<const v4f *>(b + 16 * mt + kq);   // the bias starts the sum
         v4f c[kInferGroups];
 #pragma unroll
-        for (int g = 0; g < kInferGroups; ++g) c[g] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int g = 0; g < kInferGroups; ++g) c[g] = bias;
 #pragma unroll
         for (int kt = 0; kt < NI; ++kt) {
             const v4s a = *reinterpret_cast<const v4s *>(w + ((mt * NI + kt) * 64 + lane) * 4);
@@ -207,14 +207,11 @@ __device__ inline void layer_lds(const short *w, const float *b, const v4s (*x)[
         }
 #pragma unroll
         for (int g = 0; g < kInferGroups; ++g) {
-            v4s h;
+            if (RELU) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                float v = c[g][j] + b[16 * mt + kq + j];
-                if (RELU) v = v > 0.0f ? v : 0.0f;
-                h[j] = f2bf(v);
+                for (int j = 0; j < 4; ++j) c[g][j] = fmaxf(c[g][j], 0.0f);
             }
-            y[g][mt] = h;
+            y[g][mt] = to_bf16x4(c[g]);
         }
     }
 }
@@ -237,8 +234,8 @@ __global__ __launch_bounds__(256, 2) void k_guide_infer(GuideWeights W, const Gu
 #pragma unroll
         for (int g = 0; g < kInferGroups; ++g) {
             const uint32_t s = base + 16 * g + col;
-            const GuideQuery Q = q[s < n ? s : n - 1];
-            encode_tiles(Q.p, Q.n, Q.rough, lane, x[g]);
+            const GuideQuery *Q = q + (s < n ? s : n - 1);
+            encode_tiles(Q->p[0], Q->p[1], Q->p[2], Q->n[0], Q->n[1], Q->n[2], Q->rough, lane, x[g]);
         }
         layer_lds<8, 4, true>(L.w + h_off(0), L.bias + 0 * kGuideBiasStride, x, h1, lane);
         layer_lds<4, 4, true>(L.w + h_off(1), L.bias + 1 * kGuideBiasStride, h1, h2, lane);
@@ -251,34 +248,38 @@ __global__ __launch_bounds__(256, 2) void k_guide_infer(GuideWeights W, const Gu
             // Output layer of this 16-sample tile, staged in LDS.
 #pragma unroll
             for (int mt = 0; mt < 3; ++mt) {
-                v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+                v4f c = *reinterpret_cast<const v4f *>(b4 + 16 * mt + kq);
 #pragma unroll
                 for (int kt = 0; kt < 4; ++kt)
                     c = mfma16(*reinterpret_cast<const v4s *>(w4 + ((mt * 4 + kt) * 64 + lane) * 4), h2[g][kt], c);
-#pragma unroll
-                for (int j = 0; j < 4; ++j) L.outs[wave][col][16 * mt + kq + j] = c[j] + b4[16 * mt + kq + j];
+                *reinterpret_cast<v4f *>(&L.outs[wave][col][16 * mt + kq]) = c;
             }
             __syncthreads();
             // Lobe k of sample sl for the 128 (sample, lobe) items, two per
-            // lane; consecutive lanes write consecutive 20-byte lobes.
+            // lane: the 8 lanes of a sample form its softmax with shuffles,
+            // and consecutive lanes write consecutive 20-byte lobes.
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
                 const int item = lane + 64 * r, sl = item >> 3, k = item & 7;
                 const uint32_t s = base + 16 * g + sl;
                 const float *o = L.outs[wave][sl];
-                float m = o[0];
-#pragma unroll
-                for (int e = 1; e < 8; ++e) m = fmaxf(m, o[e]);
-                float sum = 0.0f;
-#pragma unroll
-                for (int e = 0; e < 8; ++e) sum += expf(o[e] - m);
+                const float logit = o[k];
+                float m = logit;
+                m = fmaxf(m, __shfl_xor(m, 1, 8));
+                m = fmaxf(m, __shfl_xor(m, 2, 8));
+                m = fmaxf(m, __shfl_xor(m, 4, 8));
+                const float e = expf(logit - m);
+                float sum = e;
+                sum += __shfl_xor(sum, 1, 8);
+                sum += __shfl_xor(sum, 2, 8);
+                sum += __shfl_xor(sum, 4, 8);
                 const float x0 = o[8 + 3 * k], y0 = o[9 + 3 * k], z0 = o[10 + 3 * k];
                 const float len = sqrtf(x0 * x0 + y0 * y0 + z0 * z0);
                 const float inv = len > 1e-12f ? 1.0f / len : 0.0f;
                 const float kr = fminf(fmaxf(o[32 + k], kGuideKMin), kGuideKMax);
                 if (s < n) {
                     float *dst = lobes + (size_t)s * 40 + 5 * k;
-                    dst[0] = expf(o[k] - m) / sum;
+                    dst[0] = e / sum;
                     dst[1] = x0 * inv;
                     dst[2] = y0 * inv;
                     dst[3] = z0 * inv;
@@ -307,7 +308,7 @@ __global__ __launch_bounds__(256) void k_guide_train_pass1(GuideWeights W, const
     const GuideSample g = smp[s];
     v4s x[8], h[4][4];
     v4f acc[4], out[3];
-    encode_tiles(g.p, g.n, g.rough, lane, x);
+    encode_tiles(g.p[0], g.p[1], g.p[2], g.n[0], g.n[1], g.n[2], g.rough, lane, x);
     layer_fwd<8, 4, true>(W.wt + h_off(0), W.bias + 0 * kGuideBiasStride, kGuideIn, x, acc, h[0], lane);
     layer_fwd<4, 4, true>(W.wt + h_off(1), W.bias + 1 * kGuideBiasStride, kGuideHid, h[0], acc, h[1], lane);
     layer_fwd<4, 4, true>(W.wt + h_off(2), W.bias + 2 * kGuideBiasStride, kGuideHid, h[1], acc, h[2], lane);

@@ -1421,7 +1421,15 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
         return AZ_ERR_UNSUPPORTED;
     }
     const u32 trainSpp = guided ? (o.GuideTrainSpp ? o.GuideTrainSpp : 32u) : 0u;
-    const u32 stepsPerChunk = o.GuideStepsPerChunk ? o.GuideStepsPerChunk : 8u;
+    // Training chunks hold at most trainSpp / 8 samples per pixel (and at
+    // most the chunk size); by default the Adam steps are spread over them
+    // so that training takes about kGuideSteps steps whatever the image size.
+    const u32 chunkSpp0 = std::max<u32>(1u, std::min<u32>(sppTotal, (u32)(2u << 20) / std::max<u32>(1u, nPix)));
+    const u32 trainChunkSpp = std::max<u32>(1u, std::min<u32>(chunkSpp0, trainSpp / 8u));
+    const u32 trainChunks = trainSpp ? (std::min(trainSpp, sppTotal) + trainChunkSpp - 1) / trainChunkSpp : 1u;
+    const u32 kGuideSteps = 256;
+    const u32 stepsPerChunk =
+        o.GuideStepsPerChunk ? o.GuideStepsPerChunk : std::max<u32>(1u, (kGuideSteps + trainChunks - 1) / trainChunks);
     const float alpha = o.GuideFraction > 0.0f && o.GuideFraction < 1.0f ? o.GuideFraction : 0.5f;
     const u32 recCap = 8u << 20;
     if (guided) {

@@ -83,6 +83,10 @@ def main(argv=None):
         s -= 1
         spp_eq = s * s
         g_eq, t_g_eq = timed(ctx, cam, filt, a.seed + 7, spp_eq, True)
+    # The next square up, if it also fits in the plain render's time.
+    g_up, t_up = timed(ctx, cam, filt, a.seed + 7, (s + 1) * (s + 1), True)
+    if t_up <= t_plain * 1.02:
+        spp_eq, g_eq, t_g_eq = (s + 1) * (s + 1), g_up, t_up
     mse_g_eq = float(np.mean((g_eq - ref) ** 2))
     print(f"guided {spp_eq} spp {t_g_eq:.2f}s mse {mse_g_eq:.4e}", file=sys.stderr, flush=True)
     # Beside the learned guide: the environment map's luminance distribution
