@@ -414,6 +414,61 @@ az_status az_gpu_set_launch(az_gpu_ctx *C, u32 block, u32 ring, s32 waves) {
     return AZ_OK;
 }
 
+az_status az_gpu_autotune_trace(az_gpu_ctx *C, const az_ray *rays, u32 n, u32 repeats, az_tune_result *out) {
+    if (!C || !rays || n == 0) return AZ_ERR_INVALID_ARGUMENT;
+    if (!C->hasScene) return AZ_ERR_NO_SCENE;
+    AZ_HIP(C, hipSetDevice(C->device));
+    repeats = std::max<u32>(1u, repeats);
+    TraceLaunch cands[32];
+    const int k = trace_variants(cands, 32);
+    DeviceBuffer hits;
+    AZ_HIP(C, hits.alloc((size_t)n * sizeof(az_hit)));
+    hipEvent_t a = nullptr, b = nullptr;
+    AZ_HIP(C, hipEventCreate(&a));
+    az_status st = AZ_OK;
+    if (hipEventCreate(&b) != hipSuccess) st = AZ_ERR_DEVICE;
+    double best = 1e300;
+    int bestIdx = 0;
+    az_tune_result res;
+    memset(&res, 0, sizeof(res));
+    for (int i = 0; i < k && st == AZ_OK; ++i) {
+        hipError_t e = launch_trace(cands[i], false, false, C->dg, rays, n, (az_hit *)hits.ptr, C->eps.Intersection, 0,
+                                    nullptr, C->stream);
+        if (e == hipSuccess) e = hipEventRecord(a, C->stream);
+        for (u32 r = 0; r < repeats && e == hipSuccess; ++r)
+            e = launch_trace(cands[i], false, false, C->dg, rays, n, (az_hit *)hits.ptr, C->eps.Intersection, 0,
+                             nullptr, C->stream);
+        if (e == hipSuccess) e = hipEventRecord(b, C->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+        if (e != hipSuccess) {
+            C->lastError = std::string("autotune: ") + hipGetErrorString(e);
+            st = AZ_ERR_DEVICE;
+            break;
+        }
+        const double per = (double)ms / repeats;
+        az_tune_entry &t = res.Entries[res.Count++];
+        t.Block = (u32)cands[i].block;
+        t.LdsStackEntries = (u32)cands[i].ring;
+        t.MinWaves = cands[i].waves;
+        t.ChunkSamples = 0;
+        t.Milliseconds = per;
+        if (per < best) {
+            best = per;
+            bestIdx = i;
+        }
+    }
+    (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (st != AZ_OK) return st;
+    C->launch = cands[bestIdx];
+    C->stats.BlockSize = (u32)cands[bestIdx].block;
+    res.Best = (u32)bestIdx;
+    if (out) *out = res;
+    return AZ_OK;
+}
+
 az_status az_gpu_malloc(az_gpu_ctx *C, void **p, u64 bytes) {
     if (!C || !p) return AZ_ERR_INVALID_ARGUMENT;
     AZ_HIP(C, hipSetDevice(C->device));

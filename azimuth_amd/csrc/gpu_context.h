@@ -79,6 +79,7 @@ struct az_gpu_ctx {
     az::DeviceGeometry dg{};
     std::vector<az::TopItem> hostItems;
     az::TraceLaunch launch;
+    az::RenderTuning renderTuning;
     az::DeviceBuffer counters;
     az::RenderState render;
     az::DevScene devScene{};

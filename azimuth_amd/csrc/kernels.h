@@ -18,6 +18,20 @@ struct TraceLaunch {
 };
 
 bool trace_variant_exists(int ring, int block, int waves);
+// The compiled traversal variants (at most cap of them).
+int trace_variants(TraceLaunch *out, int cap);
+
+// Launch parameters of the renderer: camera samples per wavefront chunk (the
+// queue tile) and the workgroup size / register target of its traversal
+// kernels (extend, connect).  Chosen per scene by az_gpu_autotune_render.
+struct RenderTuning {
+    uint32_t chunkSamples = 2u << 20;
+    int traceBlock = 256;
+    int traceWaves = 6;
+};
+
+bool render_trace_variant_exists(int block, int waves);
+int render_trace_variants(RenderTuning *out, int cap);   // chunkSamples left at its default
 
 hipError_t launch_trace(const TraceLaunch &L, bool any, bool count, const DeviceGeometry &G, const az_ray *rays,
                         uint32_t n, az_hit *hits, float eps, int cull, unsigned long long *counters,

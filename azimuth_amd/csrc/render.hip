@@ -132,13 +132,17 @@ __global__ __launch_bounds__(kRenderBlock) void k_generate(DevScene S, CameraSet
 }
 
 // ---- extend: closest hit of every queued ray ---------------------------------
-__global__ __launch_bounds__(kRenderBlock, 6) void k_extend(DeviceGeometry G, const PathRay *__restrict__ q, SegIn qin,
-                                                           uint32_t n, PathHit *__restrict__ hits, float eps,
-                                                           int cull) {
-    __shared__ uint32_t smem[kRenderRing * kRenderBlock];
+// The traversal kernels of the renderer come in a few (workgroup size,
+// register target) variants, chosen per scene by the autotuner.
+#define AZ_RENDER_TRACE_VARIANTS(X) X(256, 6) X(128, 6) X(256, 4) X(64, 8)
+
+template <int BLOCK, int WAVES>
+__global__ __launch_bounds__(BLOCK, WAVES) void k_extend(DeviceGeometry G, const PathRay *__restrict__ q, SegIn qin,
+                                                        uint32_t n, PathHit *__restrict__ hits, float eps, int cull) {
+    __shared__ uint32_t smem[kRenderRing * BLOCK];
     __shared__ uint32_t soff[kSegs + 1];
     const uint32_t *off = stage_offsets(qin, soff);
-    uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     const PathRay &pr = q[seg_slot(off, qin.nseg, qin.segCap, i)];
     Ray3 r;
@@ -151,7 +155,7 @@ __global__ __launch_bounds__(kRenderBlock, 6) void k_extend(DeviceGeometry G, co
     r.dz = pr.dz;
     HitRec h;
     TraceCounters c{0u, 0u, 0u};
-    bool hit = trace_ray<false, false, kRenderRing, kRenderBlock>(G, r, 0.0f, eps, cull != 0, smem + threadIdx.x, h, c);
+    bool hit = trace_ray<false, false, kRenderRing, BLOCK>(G, r, 0.0f, eps, cull != 0, smem + threadIdx.x, h, c);
     PathHit o;
     o.prim = hit ? h.prim : kEmpty;
     o.item = h.item;
@@ -165,13 +169,14 @@ __global__ __launch_bounds__(kRenderBlock, 6) void k_extend(DeviceGeometry G, co
 }
 
 // ---- connect: shadow rays ------------------------------------------------------
-__global__ __launch_bounds__(kRenderBlock, 6) void k_connect(DeviceGeometry G, const ShadowRay *__restrict__ sq,
-                                                            SegIn sin, uint32_t n, float *__restrict__ acc,
-                                                            GuideSample *__restrict__ recs) {
-    __shared__ uint32_t smem[kRenderRing * kRenderBlock];
+template <int BLOCK, int WAVES>
+__global__ __launch_bounds__(BLOCK, WAVES) void k_connect(DeviceGeometry G, const ShadowRay *__restrict__ sq, SegIn sin,
+                                                         uint32_t n, float *__restrict__ acc,
+                                                         GuideSample *__restrict__ recs) {
+    __shared__ uint32_t smem[kRenderRing * BLOCK];
     __shared__ uint32_t soff[kSegs + 1];
     const uint32_t *off = stage_offsets(sin, soff);
-    uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     const ShadowRay &s = sq[seg_slot(off, sin.nseg, sin.segCap, i)];
     Ray3 r;
@@ -187,15 +192,61 @@ __global__ __launch_bounds__(kRenderBlock, 6) void k_connect(DeviceGeometry G, c
     bool reaches;
     if (s.mode == kShadowAny) {
         // Occluded by any hit with 0 < t < tmax (main.cpp:697-712, 751-752).
-        reaches = !trace_ray<true, false, kRenderRing, kRenderBlock>(G, r, s.tmax, 0.0f, false, smem + threadIdx.x, h, c);
+        reaches = !trace_ray<true, false, kRenderRing, BLOCK>(G, r, s.tmax, 0.0f, false, smem + threadIdx.x, h, c);
     } else {
         // Entity lights: the closest hit must be the light itself (main.cpp:704-708).
-        bool hit = trace_ray<false, false, kRenderRing, kRenderBlock>(G, r, 0.0f, 0.0f, false, smem + threadIdx.x, h, c);
+        bool hit = trace_ray<false, false, kRenderRing, BLOCK>(G, r, 0.0f, 0.0f, false, smem + threadIdx.x, h, c);
         reaches = !hit || G.items[h.item].entityId == s.entityId;
     }
     if (reaches) add3(acc + 3 * s.sample, v3(s.cx, s.cy, s.cz));
     // A guide training sample whose direction is blocked saw no radiance.
     if (s.rec != kEmpty && !reaches) recs[s.rec].wgt = 0.0f;
+}
+
+static hipError_t launch_extend(const RenderTuning &T, const DeviceGeometry &G, const PathRay *q, SegIn qin, uint32_t n,
+                                PathHit *hits, float eps, int cull, hipStream_t st) {
+#define AZ_CASE(B, W)                                                                                   \
+    if (T.traceBlock == B && T.traceWaves == W) {                                                      \
+        hipLaunchKernelGGL((k_extend<B, W>), dim3((n + B - 1) / B), dim3(B), 0, st, G, q, qin, n, hits, eps, cull); \
+        return hipGetLastError();                                                                       \
+    }
+    AZ_RENDER_TRACE_VARIANTS(AZ_CASE)
+#undef AZ_CASE
+    return hipErrorInvalidValue;
+}
+
+static hipError_t launch_connect(const RenderTuning &T, const DeviceGeometry &G, const ShadowRay *sq, SegIn sin,
+                                 uint32_t n, float *acc, GuideSample *recs, hipStream_t st) {
+#define AZ_CASE(B, W)                                                                                    \
+    if (T.traceBlock == B && T.traceWaves == W) {                                                       \
+        hipLaunchKernelGGL((k_connect<B, W>), dim3((n + B - 1) / B), dim3(B), 0, st, G, sq, sin, n, acc, recs); \
+        return hipGetLastError();                                                                        \
+    }
+    AZ_RENDER_TRACE_VARIANTS(AZ_CASE)
+#undef AZ_CASE
+    return hipErrorInvalidValue;
+}
+
+int render_trace_variants(RenderTuning *out, int cap) {
+    int k = 0;
+#define AZ_CASE(B, W)                              \
+    if (k < cap) {                                 \
+        out[k] = RenderTuning{};                   \
+        out[k].traceBlock = B;                     \
+        out[k].traceWaves = W;                     \
+    }                                              \
+    ++k;
+    AZ_RENDER_TRACE_VARIANTS(AZ_CASE)
+#undef AZ_CASE
+    return k < cap ? k : cap;
+}
+
+bool render_trace_variant_exists(int block, int waves) {
+#define AZ_CASE(B, W) \
+    if (block == B && waves == W) return true;
+    AZ_RENDER_TRACE_VARIANTS(AZ_CASE)
+#undef AZ_CASE
+    return false;
 }
 
 // ---- shade ------------------------------------------------------------------------
@@ -1424,7 +1475,8 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
     // Training chunks hold at most trainSpp / 8 samples per pixel (and at
     // most the chunk size); by default the Adam steps are spread over them
     // so that training takes about kGuideSteps steps whatever the image size.
-    const u32 chunkSpp0 = std::max<u32>(1u, std::min<u32>(sppTotal, (u32)(2u << 20) / std::max<u32>(1u, nPix)));
+    const u32 chunkSamples = std::max<u32>(1u, C->renderTuning.chunkSamples);
+    const u32 chunkSpp0 = std::max<u32>(1u, std::min<u32>(sppTotal, chunkSamples / std::max<u32>(1u, nPix)));
     const u32 trainChunkSpp = std::max<u32>(1u, std::min<u32>(chunkSpp0, trainSpp / 8u));
     const u32 trainChunks = trainSpp ? (std::min(trainSpp, sppTotal) + trainChunkSpp - 1) / trainChunkSpp : 1u;
     const u32 kGuideSteps = 256;
@@ -1443,9 +1495,10 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
         AZ_HIPR(C, B.recs.alloc((size_t)recCap * sizeof(GuideSample)));
         AZ_HIPR(C, B.batch.alloc((size_t)65536 * sizeof(GuideSample)));
     }
-    // Samples per chunk: about 2M camera samples per wavefront; while the
-    // guide trains, at least 8 chunks so that it is updated often.
-    u32 chunkSpp = std::max<u32>(1u, std::min<u32>(sppTotal, (u32)(2u << 20) / std::max<u32>(1u, nPix)));
+    // Samples per pixel per chunk: the tuned chunk size (2M camera samples
+    // by default); while the guide trains, at least 8 chunks so that it is
+    // updated often.
+    u32 chunkSpp = chunkSpp0;
     const u32 shadowPerRay = S.lightCount + (S.useEnv ? 1u : 0u);
     hipStream_t st = C->stream;
     // Counters and offsets of the segmented queues: child, shadow and
@@ -1518,9 +1571,8 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             AZ_HIPR(C, B.shadow.alloc(shadowCap * sizeof(ShadowRay)));
             PathRay *qa = (PathRay *)bufA->ptr, *qb = (PathRay *)bufB->ptr;
             dim3 grid(blocks);
-            hipLaunchKernelGGL(k_extend, grid, dim3(kRenderBlock), 0, st, C->dg, qa, qin, count, (PathHit *)B.hits.ptr,
-                               S.epsIntersection, (int)cp.cull);
-            AZ_HIPR(C, hipGetLastError());
+            AZ_HIPR(C, launch_extend(C->renderTuning, C->dg, qa, qin, count, (PathHit *)B.hits.ptr, S.epsIntersection,
+                                     (int)cp.cull, st));
             GuideIO GI{nullptr, nullptr, recCnt, recSegCap, alpha};
             if (guided) {
                 // Query every hit, then one batched MFMA inference.
@@ -1557,10 +1609,8 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             AZ_HIPR(C, hipStreamSynchronize(st));
             if (hc[1]) {
                 SegIn sin{offShadow, nseg, shadowSegCap};
-                hipLaunchKernelGGL(k_connect, dim3((hc[1] + kRenderBlock - 1) / kRenderBlock), dim3(kRenderBlock), 0,
-                                   st, C->dg, (const ShadowRay *)B.shadow.ptr, sin, hc[1], acc,
-                                   (GuideSample *)B.recs.ptr);
-                AZ_HIPR(C, hipGetLastError());
+                AZ_HIPR(C, launch_connect(C->renderTuning, C->dg, (const ShadowRay *)B.shadow.ptr, sin, hc[1], acc,
+                                          (GuideSample *)B.recs.ptr, st));
             }
             count = hc[0];
             offA = 1 - offA;
@@ -1660,6 +1710,90 @@ az_status az_gpu_render(az_gpu_ctx *C, const camera *Cam, filter_type Filter, b3
     AZ_HIPR(C, hipMemcpyAsync(Pixels, B.pixels.ptr, bytes, hipMemcpyDeviceToHost, C->stream));
     AZ_HIPR(C, hipStreamSynchronize(C->stream));
     return AZ_OK;
+}
+
+az_status az_gpu_set_render_tuning(az_gpu_ctx *C, u32 chunkSamples, u32 traceBlock, s32 traceWaves) {
+    if (!C) return AZ_ERR_INVALID_ARGUMENT;
+    RenderTuning t = C->renderTuning;
+    if (chunkSamples) t.chunkSamples = chunkSamples;
+    if (traceBlock) t.traceBlock = (int)traceBlock;
+    if (traceWaves >= 0) t.traceWaves = traceWaves;
+    if (!render_trace_variant_exists(t.traceBlock, t.traceWaves)) return AZ_ERR_INVALID_ARGUMENT;
+    C->renderTuning = t;
+    return AZ_OK;
+}
+
+az_status az_gpu_autotune_render(az_gpu_ctx *C, const camera *Cam, filter_type Filter, b32 Cull,
+                                 const az_render_opts *Opts, az_tune_result *out) {
+    if (!C || !Cam) return AZ_ERR_INVALID_ARGUMENT;
+    if (!C->hasScene || !C->render.ready) return AZ_ERR_NO_SCENE;
+    AZ_HIPR(C, hipSetDevice(C->device));
+    az_render_opts o;
+    memset(&o, 0, sizeof(o));
+    if (Opts) o = *Opts;
+    // Enough samples per pixel that the largest candidate chunk is filled
+    // twice (up to the camera's own count), so chunking is part of what is
+    // timed.
+    const u32 nPix = Cam->Resolution.x * Cam->Resolution.y;
+    const u32 camSpp = o.SampleCountOverride ? o.SampleCountOverride : Cam->SampleCount;
+    u32 spp = std::max<u32>(1u, std::min<u32>(camSpp, (8u << 20) / std::max<u32>(1u, nPix)));
+    u32 sq = (u32)std::sqrt((double)spp);
+    o.SampleCountOverride = std::max<u32>(1u, sq * sq);
+    o.GuideEnable = 0;
+    o.AccumulateF64 = 0;
+    RenderBuffers &B = buffersOf(C);
+    const size_t bytes = (size_t)Cam->Resolution.x * Cam->Resolution.y * sizeof(pixel);
+    AZ_HIPR(C, B.pixels.alloc(bytes));
+    AZ_HIPR(C, hipMemsetAsync(B.pixels.ptr, 0, bytes, C->stream));
+    RenderTuning variants[16];
+    const int nv = render_trace_variants(variants, 16);
+    const uint32_t chunks[3] = {1u << 20, 2u << 20, 4u << 20};
+    const RenderTuning keep = C->renderTuning;
+    // Warm-up with the current setting: kernels loaded, queues allocated.
+    az_status st = renderImpl(C, Cam, Filter, Cull, &o, B.pixels.ptr, false);
+    if (st != AZ_OK) return st;
+    hipEvent_t a = nullptr, b = nullptr;
+    AZ_HIPR(C, hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return AZ_ERR_DEVICE;
+    }
+    az_tune_result res;
+    memset(&res, 0, sizeof(res));
+    double best = 1e300;
+    RenderTuning bestT = keep;
+    for (int v = 0; v < nv && st == AZ_OK; ++v)
+        for (uint32_t cs : chunks) {
+            RenderTuning t = variants[v];
+            t.chunkSamples = cs;
+            C->renderTuning = t;
+            float ms = 0.0f;
+            if (hipEventRecord(a, C->stream) != hipSuccess) st = AZ_ERR_DEVICE;
+            if (st == AZ_OK) st = renderImpl(C, Cam, Filter, Cull, &o, B.pixels.ptr, false);
+            if (st == AZ_OK && (hipEventRecord(b, C->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                                hipEventElapsedTime(&ms, a, b) != hipSuccess))
+                st = AZ_ERR_DEVICE;
+            if (st != AZ_OK) break;
+            if (res.Count < 32) {
+                az_tune_entry &e = res.Entries[res.Count];
+                e.Block = (u32)t.traceBlock;
+                e.LdsStackEntries = (u32)kRenderRing;
+                e.MinWaves = t.traceWaves;
+                e.ChunkSamples = cs;
+                e.Milliseconds = ms;
+                if (ms < best) {
+                    best = ms;
+                    bestT = t;
+                    res.Best = res.Count;
+                }
+                res.Count++;
+            }
+        }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    C->renderTuning = st == AZ_OK ? bestT : keep;
+    if (st == AZ_OK && out) *out = res;
+    return st;
 }
 
 b32 az_render_scene(scene *Scene, camera *Camera, epsilon_values *Epsilons, u32 ThreadCount, b32 BackfaceCulling,

@@ -600,7 +600,7 @@ u32 az_abi_sizes(u32 *Out, u32 Capacity) {
                          (u32)sizeof(light),         (u32)sizeof(az_texture),     (u32)sizeof(camera),
                          (u32)sizeof(scene),         (u32)sizeof(az_mesh_desc),   (u32)sizeof(az_sphere_desc),
                          (u32)sizeof(az_scene_params), (u32)sizeof(az_render_opts), (u32)sizeof(az_stats),
-                         (u32)sizeof(az_ray),        (u32)sizeof(az_hit)};
+                         (u32)sizeof(az_ray),        (u32)sizeof(az_hit),         (u32)sizeof(az_tune_result)};
     const u32 n = (u32)(sizeof(sizes) / sizeof(sizes[0]));
     for (u32 i = 0; i < n && i < Capacity; ++i) Out[i] = sizes[i];
     return n;

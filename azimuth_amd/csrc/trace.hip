@@ -99,6 +99,16 @@ bool trace_variant_exists(int ring, int block, int waves) {
     return false;
 }
 
+int trace_variants(TraceLaunch *out, int cap) {
+    int k = 0;
+#define AZ_CASE(R, B, W)                 \
+    if (k < cap) out[k] = TraceLaunch{R, B, W}; \
+    ++k;
+    AZ_TRACE_VARIANTS(AZ_CASE)
+#undef AZ_CASE
+    return k < cap ? k : cap;
+}
+
 // Plain 16-byte-per-lane copy: the calibration kernel for HBM counters.
 __global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
     size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;

@@ -181,6 +181,15 @@ class az_stats(C.Structure):
                 ("TriangleCount", u32), ("SphereCount", u32), ("InstanceCount", u32)]
 
 
+class az_tune_entry(C.Structure):
+    _fields_ = [("Block", u32), ("LdsStackEntries", u32), ("MinWaves", s32), ("ChunkSamples", u32),
+                ("Milliseconds", f64)]
+
+
+class az_tune_result(C.Structure):
+    _fields_ = [("Count", u32), ("Best", u32), ("Entries", az_tune_entry * 32)]
+
+
 # az_ray / az_hit as numpy record layouts (32 and 16 bytes).
 RAY_DTYPE = np.dtype([("Origin", "<f4", 3), ("Time", "<f4"), ("Direction", "<f4", 3), ("TMax", "<f4")])
 HIT_DTYPE = np.dtype([("Prim", "<u4"), ("t", "<f4"), ("Beta", "<f4"), ("Gamma", "<f4")])
@@ -190,7 +199,8 @@ ABI_SIZES = [("entity", 288), ("mesh", 200), ("material", C.sizeof(material)), (
              ("texture", C.sizeof(texture)), ("camera", C.sizeof(camera)), ("scene", 2744),
              ("az_mesh_desc", C.sizeof(az_mesh_desc)), ("az_sphere_desc", C.sizeof(az_sphere_desc)),
              ("az_scene_params", C.sizeof(az_scene_params)), ("az_render_opts", C.sizeof(az_render_opts)),
-             ("az_stats", C.sizeof(az_stats)), ("az_ray", RAY_DTYPE.itemsize), ("az_hit", HIT_DTYPE.itemsize)]
+             ("az_stats", C.sizeof(az_stats)), ("az_ray", RAY_DTYPE.itemsize), ("az_hit", HIT_DTYPE.itemsize),
+             ("az_tune_result", C.sizeof(az_tune_result))]
 
 _P = C.c_void_p
 # Exported functions: name -> (restype, argtypes).  Every name here is also
@@ -223,6 +233,9 @@ EXPORTS = {
     "az_gpu_get_stats": (s32, [_P, C.POINTER(az_stats)]),
     "az_gpu_set_counting": (s32, [_P, u32]),
     "az_gpu_set_launch": (s32, [_P, u32, u32, s32]),
+    "az_gpu_autotune_trace": (s32, [_P, _P, u32, u32, _P]),
+    "az_gpu_autotune_render": (s32, [_P, _P, s32, u32, _P, _P]),
+    "az_gpu_set_render_tuning": (s32, [_P, u32, u32, s32]),
     "az_gpu_malloc": (s32, [_P, C.POINTER(_P), u64]),
     "az_gpu_free": (s32, [_P, _P]),
     "az_gpu_upload": (s32, [_P, _P, _P, u64]),

@@ -336,6 +336,33 @@ class GpuContext:
                                        out.ctypes.data), "az_gpu_render", self.h)
         return out
 
+    @staticmethod
+    def _tune_list(res):
+        return [{"block": e.Block, "ring": e.LdsStackEntries, "waves": e.MinWaves, "chunk": e.ChunkSamples,
+                 "ms": e.Milliseconds} for e in res.Entries[:res.Count]], int(res.Best)
+
+    def autotune_trace(self, d_rays, n, repeats=3):
+        """Times every traversal variant on the device rays and keeps the
+        fastest; returns (entries, best index)."""
+        res = N.az_tune_result()
+        N.check(self.lib.az_gpu_autotune_trace(self.h, d_rays.ptr, int(n), int(repeats), C.byref(res)),
+                "az_gpu_autotune_trace", self.h)
+        return self._tune_list(res)
+
+    def autotune_render(self, camera, filt=N.FilterType_Box, cull=False, seed=0):
+        """Times a 1-spp render for every (chunk size, traversal variant) and
+        keeps the fastest; returns (entries, best index)."""
+        o = N.az_render_opts()
+        o.Seed = int(seed)
+        res = N.az_tune_result()
+        N.check(self.lib.az_gpu_autotune_render(self.h, C.byref(camera), int(filt), 1 if cull else 0, C.byref(o),
+                                                C.byref(res)), "az_gpu_autotune_render", self.h)
+        return self._tune_list(res)
+
+    def set_render_tuning(self, chunk=0, block=0, waves=-1):
+        N.check(self.lib.az_gpu_set_render_tuning(self.h, int(chunk), int(block), int(waves)),
+                "az_gpu_set_render_tuning", self.h)
+
     def render_f64(self, camera, filt=N.FilterType_Box, seed=0, spp=0, guide=False, guide_fraction=0.5,
                    env_sampling=N.AZ_ENV_COSINE):
         """Device render accumulated in double precision; returns (W*H, 4)

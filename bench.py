@@ -8,6 +8,8 @@ coherent batch of a 2048x2048 pinhole camera and the incoherent batch with
 origins uniform in the mesh box and directions uniform on the sphere.  The
 value is closest-hit Mrays/s over both batches.
 
+Before timing, the per-scene autotuner picks the traversal variant (LDS
+stack entries, workgroup size, register target) on both batches.
 Also reported: the live roofline of the traversal kernel (algorithmic bytes
 per launch from kernel-counted node visits and triangle tests, over the
 launch's HIP-event duration), any-hit rates, and the oracle's reference
@@ -63,12 +65,26 @@ def main():
     t0 = time.time()
     ctx.upload_scene(sc)
     log(f"upload + device BVH build {time.time() - t0:.1f}s")
-    if args.block or args.ring or args.waves >= 0:
-        ctx.set_launch(block=args.block, ring=args.ring, waves=args.waves)
     d_coh, d_inc = ctx.alloc(coh.nbytes), ctx.alloc(inc.nbytes)
     d_coh.upload(coh)
     d_inc.upload(inc)
     d_hits = ctx.alloc(max(n_coh, n_inc) * HIT_BYTES)
+    tuned = None
+    if args.block or args.ring or args.waves >= 0:
+        ctx.set_launch(block=args.block, ring=args.ring, waves=args.waves)
+    else:
+        # Per-scene autotune (untimed): every compiled traversal variant on
+        # both batches; the fastest is kept.
+        sample = np.concatenate([coh, inc])
+        d_tune = ctx.alloc(sample.nbytes)
+        d_tune.upload(sample)
+        entries, best = ctx.autotune_trace(d_tune, sample.shape[0], repeats=3)
+        d_tune.free()
+        tuned = {"candidates": len(entries), "ring": entries[best]["ring"], "block": entries[best]["block"],
+                 "waves": entries[best]["waves"],
+                 "ms_per_1m_rays": {f'{e["ring"]}/{e["block"]}/{e["waves"]}': round(e["ms"] * 1e6 / sample.shape[0], 4)
+                                    for e in entries}}
+        log(f"autotune: ring {tuned['ring']} block {tuned['block']} waves {tuned['waves']}")
 
     # Kernel-counted work per batch (an untimed counting launch).
     ctx.set_counting(True)
@@ -181,6 +197,7 @@ def main():
             "rays_per_step": int(rays_per_step),
             "bvh": f"{st.BvhWidth}-wide, {node_b} B nodes, {tri_b} B triangles, stack {st.StackDepth}",
             "block": int(st.BlockSize),
+            "autotune": tuned,
         },
         "roofline": {
             "bound": "hbm",

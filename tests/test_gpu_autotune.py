@@ -1,0 +1,64 @@
+"""The autotuner picks launch parameters by timing; every candidate it can
+pick must produce the same results: traversal variants bit-identical hits,
+render variants (traversal workgroup size, register target, chunk size) the
+oracle's image to the render tolerance."""
+import numpy as np
+import pytest
+
+import oracle
+from azimuth_amd import configs, native as N
+
+pytestmark = pytest.mark.gpu
+
+
+def test_trace_autotune_and_every_variant_identical(gpu):
+    sc = configs.icosphere_scene(5, seed=7)
+    rays = np.concatenate([configs.coherent_rays(64, 64), configs.incoherent_rays(4096, sc.meta["bbox"], seed=3)])
+    want = oracle.trace(sc, rays)
+    gpu.upload_scene(sc)
+    d = gpu.alloc(rays.nbytes)
+    d.upload(rays)
+    entries, best = gpu.autotune_trace(d, rays.shape[0], repeats=2)
+    print("trace candidates:", [(e["ring"], e["block"], e["waves"], round(e["ms"], 4)) for e in entries],
+          "best", best, flush=True)
+    assert len(entries) >= 2 and 0 <= best < len(entries)
+    assert all(e["ms"] > 0 for e in entries)
+    assert gpu.stats().BlockSize == entries[best]["block"]
+    for e in entries:
+        gpu.set_launch(block=e["block"], ring=e["ring"], waves=e["waves"])
+        got = gpu.trace(rays)
+        assert got.tobytes() == want.tobytes(), e
+    b = entries[best]
+    gpu.set_launch(block=b["block"], ring=b["ring"], waves=b["waves"])
+    d.free()
+
+
+def test_render_autotune_and_variants_match_oracle(gpu):
+    sc, cam = configs.cornell_scene(48, 48, spp=4, max_depth=6)
+    gpu.upload_scene(sc)
+    entries, best = gpu.autotune_render(cam, seed=3)
+    print("render candidates:", [(e["block"], e["waves"], e["chunk"], round(e["ms"], 3)) for e in entries],
+          "best", best, flush=True)
+    assert len(entries) >= 4 and 0 <= best < len(entries)
+    want = oracle.render(sc, cam, seed=3)
+    wr = want["Color"] / want["TotalWeight"][:, None]
+    scale = float(np.mean(np.abs(wr)))
+    seen = set()
+    for e in entries:
+        key = (e["block"], e["waves"])
+        if key in seen:
+            continue
+        seen.add(key)
+        # A small chunk forces several chunks per render.
+        gpu.set_render_tuning(chunk=4096, block=e["block"], waves=e["waves"])
+        got = gpu.render(cam, seed=3)
+        assert np.array_equal(got["TotalWeight"], want["TotalWeight"])
+        g = got["Color"] / got["TotalWeight"][:, None]
+        rmse = float(np.sqrt(np.mean((g - wr) ** 2))) / scale
+        assert rmse < 1e-3, (e, rmse)
+    gpu.set_render_tuning(chunk=2 << 20, block=256, waves=6)
+
+
+def test_set_render_tuning_rejects_unknown_variant(gpu):
+    with pytest.raises(N.AzError):
+        gpu.set_render_tuning(block=96, waves=6)
