@@ -234,42 +234,87 @@ struct TraceCounters {
     uint32_t prims;
 };
 
-// Two-level walk.  ANY: stop at the first primitive with eps < t < tMax
-// (world t); otherwise the closest hit with eps < t (tMax ignored).
-template <bool ANY, bool COUNT, int RING, int BLOCK>
-__device__ inline bool trace_ray(const DeviceGeometry &G, const Ray3 &r, float tMax, float eps, bool cull,
-                                 uint32_t *ldsLane, HitRec &out, TraceCounters &cnt) {
+// Two-level walk over a wave's range of rays with refill.  Lanes of a wave
+// take rays [begin, end) in order; a lane whose ray finishes takes the next
+// unclaimed one at the end of the same step (one ballot, no atomics), so a
+// wave keeps its lanes busy until the range runs out instead of waiting for
+// its longest ray.  MODE 0: closest hit with eps < t (tMax ignored); MODE 1:
+// any hit with eps < t < tMax (world t); MODE 2: per ray, as the pool says.
+// Pool provides
+//   bool fetch(uint32_t id, Ray3 &r, float &tMax)   -> true: any-hit ray
+//   void finish(uint32_t id, bool found, const HitRec &h)
+// Every lane of the wave must call it (it ends when the whole range is done).
+template <int MODE, bool COUNT, int RING, int BLOCK, class Pool>
+__device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t begin, uint32_t end, float eps,
+                                  bool cull, uint32_t *ldsLane, TraceCounters &cnt) {
     RingStack<RING, BLOCK> stack;
     stack.base = ldsLane;
-    const float dirLen = len3(r.dx, r.dy, r.dz);
-    float bestW = ANY ? tMax : kFltMax;
-    uint32_t bestRank = ANY ? kEmpty : 0u;   // ANY: a hit must be strictly below tMax
+    uint32_t next = begin;                          // wave-uniform: first unclaimed ray
+    uint32_t id = 0;
+    bool have = false;
+    Ray3 r{};
+    bool any = MODE == 1;
+    float dirLen = 1.0f, bestW = kFltMax;
+    uint32_t bestRank = 0;
     bool found = false;
-    out.prim = kEmpty;
-    out.item = kEmpty;
-    out.t = 0.0f;
-    out.tl = 0.0f;
-    out.beta = 0.0f;
-    out.gamma = 0.0f;
-
+    HitRec out{};
     // Current walk ray: world ray in the TLAS, instance-local ray in a BLAS.
-    float ox = r.ox, oy = r.oy, oz = r.oz;
-    float dx = r.dx, dy = r.dy, dz = r.dz;
-    float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+    float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, ix = 0.f, iy = 0.f, iz = 0.f;
     // Instance state while inside a BLAS.
     bool inBlas = false;
     uint32_t curItem = 0;
     float sox = 0.f, soy = 0.f, soz = 0.f, mx = 0.f, my = 0.f, mz = 0.f, dscale = 1.0f;
-    float localBest = kFltMax;      // closest local t inside the current BLAS
+    float localBest = kFltMax;                      // closest local t inside the current BLAS
     uint32_t localPrim = kEmpty;
     float localBeta = 0.f, localGamma = 0.f;
-
     int sp = 0;
-    uint32_t cur = G.tlasRoot;
+    uint32_t cur = kEmpty;
     for (;;) {
+        // Refill the lanes without a ray.
+        const uint64_t need = __ballot(!have);
+        if (need) {
+            const uint32_t rank =
+                __builtin_amdgcn_mbcnt_hi((uint32_t)(need >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)need, 0u));
+            if (!have && next + rank < end) {
+                id = next + rank;
+                have = true;
+                float tMax = kFltMax;
+                const bool a = pool.fetch(id, r, tMax);
+                any = MODE == 2 ? a : MODE == 1;
+                dirLen = len3(r.dx, r.dy, r.dz);
+                bestW = any ? tMax : kFltMax;
+                bestRank = any ? kEmpty : 0u;       // any: a hit must be strictly below tMax
+                found = false;
+                out.prim = kEmpty;
+                out.item = kEmpty;
+                out.t = 0.0f;
+                out.tl = 0.0f;
+                out.beta = 0.0f;
+                out.gamma = 0.0f;
+                ox = r.ox;
+                oy = r.oy;
+                oz = r.oz;
+                dx = r.dx;
+                dy = r.dy;
+                dz = r.dz;
+                ix = 1.0f / dx;
+                iy = 1.0f / dy;
+                iz = 1.0f / dz;
+                inBlas = false;
+                localBest = kFltMax;
+                localPrim = kEmpty;
+                sp = 0;
+                cur = G.tlasRoot;
+            }
+            next += (uint32_t)__popcll(need);
+        }
+        if (__ballot(have) == 0) break;
+        if (!have) continue;
+        // One step of this lane's walk.
+        bool finished = false, advanced = false;
         if (cur == kEmpty) {
             // Leaving a BLAS: convert its closest local hit (main.cpp:256-276).
-            if (!ANY && localPrim != kEmpty) {
+            if (!any && localPrim != kEmpty) {
                 const TopItem &it = G.items[curItem];
                 float px = ox + (dx * localBest), py = oy + (dy * localBest), pz = oz + (dz * localBest);
                 float w = world_t(it.fwd, px, py, pz, mx, my, mz, sox, soy, soz, dirLen);
@@ -305,7 +350,7 @@ __device__ inline bool trace_ray(const DeviceGeometry &G, const Ray3 &r, float t
                 if (n > 3) stack.push(sp, h.ref[3]);
                 if (n > 2) stack.push(sp, h.ref[2]);
                 if (n > 1) stack.push(sp, h.ref[1]);
-                continue;
+                advanced = true;
             }
         } else if (!inBlas) {
             // TLAS leaves hold exactly one item.
@@ -324,7 +369,7 @@ __device__ inline bool trace_ray(const DeviceGeometry &G, const Ray3 &r, float t
                     out.tl = tl;
                     out.beta = 0.0f;
                     out.gamma = 0.0f;
-                    if (ANY) return true;
+                    if (any) finished = true;
                 }
             } else {
                 mx = r.time * it.motion[0];
@@ -345,7 +390,7 @@ __device__ inline bool trace_ray(const DeviceGeometry &G, const Ray3 &r, float t
                 localPrim = kEmpty;
                 stack.push(sp, kEmpty);
                 cur = it.blasRoot;
-                continue;
+                advanced = true;
             }
         } else {
             const uint32_t first = leaf_first(cur), count = leaf_count(cur);
@@ -358,19 +403,21 @@ __device__ inline bool trace_ray(const DeviceGeometry &G, const Ray3 &r, float t
                 }
                 float t, b, g;
                 if (tri_test(tp, ox, oy, oz, dx, dy, dz, eps, t, b, g) &&
-                    (ANY ? t < localBest : closer(t, tp.prim, localBest, localPrim))) {
-                    if (ANY) {
+                    (any ? t < localBest : closer(t, tp.prim, localBest, localPrim))) {
+                    if (any) {
                         const TopItem &it = G.items[curItem];
                         float px = ox + (dx * t), py = oy + (dy * t), pz = oz + (dz * t);
                         float w = world_t(it.fwd, px, py, pz, mx, my, mz, sox, soy, soz, dirLen);
                         if (w > eps && w < bestW) {
+                            found = true;
                             out.prim = tp.prim;
                             out.item = curItem;
                             out.t = w;
                             out.tl = t;
                             out.beta = b;
                             out.gamma = g;
-                            return true;
+                            finished = true;
+                            break;
                         }
                     } else {
                         localBest = t;
@@ -381,10 +428,15 @@ __device__ inline bool trace_ray(const DeviceGeometry &G, const Ray3 &r, float t
                 }
             }
         }
-        if (sp == 0) break;
-        cur = stack.pop(sp);
+        if (!finished && !advanced) {
+            if (sp == 0) finished = true;
+            else cur = stack.pop(sp);
+        }
+        if (finished) {
+            pool.finish(id, found, out);
+            have = false;
+        }
     }
-    return found;
 }
 
 }  // namespace az

@@ -136,39 +136,85 @@ __global__ __launch_bounds__(kRenderBlock) void k_generate(DevScene S, CameraSet
 // register target) variants, chosen per scene by the autotuner.
 #define AZ_RENDER_TRACE_VARIANTS(X) X(256, 6) X(128, 6) X(256, 4) X(64, 8)
 
+// Path rays of a segmented queue; hits by dense index.
+struct ExtendPool {
+    const PathRay *q;
+    const uint32_t *off;
+    SegIn in;
+    PathHit *hits;
+    __device__ inline bool fetch(uint32_t id, Ray3 &r, float &tMax) {
+        const PathRay &pr = q[seg_slot(off, in.nseg, in.segCap, id)];
+        r.ox = pr.ox;
+        r.oy = pr.oy;
+        r.oz = pr.oz;
+        r.time = pr.time;
+        r.dx = pr.dx;
+        r.dy = pr.dy;
+        r.dz = pr.dz;
+        tMax = kFltMax;
+        return false;
+    }
+    __device__ inline void finish(uint32_t id, bool found, const HitRec &h) {
+        PathHit o;
+        o.prim = found ? h.prim : kEmpty;
+        o.item = h.item;
+        o.t = h.t;
+        o.tl = h.tl;
+        o.beta = h.beta;
+        o.gamma = h.gamma;
+        o.pad0 = 0.0f;
+        o.pad1 = 0.0f;
+        hits[id] = o;
+    }
+};
+
 template <int BLOCK, int WAVES>
 __global__ __launch_bounds__(BLOCK, WAVES) void k_extend(DeviceGeometry G, const PathRay *__restrict__ q, SegIn qin,
                                                         uint32_t n, PathHit *__restrict__ hits, float eps, int cull) {
     __shared__ uint32_t smem[kRenderRing * BLOCK];
     __shared__ uint32_t soff[kSegs + 1];
     const uint32_t *off = stage_offsets(qin, soff);
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const PathRay &pr = q[seg_slot(off, qin.nseg, qin.segCap, i)];
-    Ray3 r;
-    r.ox = pr.ox;
-    r.oy = pr.oy;
-    r.oz = pr.oz;
-    r.time = pr.time;
-    r.dx = pr.dx;
-    r.dy = pr.dy;
-    r.dz = pr.dz;
-    HitRec h;
+    const uint32_t begin = (blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)) * 64u * kRaysPerLane;
+    const uint32_t end = min(begin + 64u * kRaysPerLane, n);
+    ExtendPool pool{q, off, qin, hits};
     TraceCounters c{0u, 0u, 0u};
-    bool hit = trace_ray<false, false, kRenderRing, BLOCK>(G, r, 0.0f, eps, cull != 0, smem + threadIdx.x, h, c);
-    PathHit o;
-    o.prim = hit ? h.prim : kEmpty;
-    o.item = h.item;
-    o.t = h.t;
-    o.tl = h.tl;
-    o.beta = h.beta;
-    o.gamma = h.gamma;
-    o.pad0 = 0.0f;
-    o.pad1 = 0.0f;
-    hits[i] = o;
+    trace_pool<0, false, kRenderRing, BLOCK>(G, pool, begin, end, eps, cull != 0, smem + threadIdx.x, c);
 }
 
 // ---- connect: shadow rays ------------------------------------------------------
+// Shadow rays of a segmented queue: any hit below tmax for point, area,
+// spot, directional and environment lights (main.cpp:697-712, 751-752); the
+// closest hit, which must be the light itself, for entity lights
+// (main.cpp:704-708).  An unoccluded ray adds its contribution to its sample.
+struct ConnectPool {
+    const ShadowRay *sq;
+    const uint32_t *off;
+    SegIn in;
+    const DeviceGeometry *G;
+    float *acc;
+    GuideSample *recs;
+    __device__ inline const ShadowRay &at(uint32_t id) const { return sq[seg_slot(off, in.nseg, in.segCap, id)]; }
+    __device__ inline bool fetch(uint32_t id, Ray3 &r, float &tMax) {
+        const ShadowRay &s = at(id);
+        r.ox = s.ox;
+        r.oy = s.oy;
+        r.oz = s.oz;
+        r.time = s.time;
+        r.dx = s.dx;
+        r.dy = s.dy;
+        r.dz = s.dz;
+        tMax = s.tmax;
+        return s.mode == kShadowAny;
+    }
+    __device__ inline void finish(uint32_t id, bool found, const HitRec &h) {
+        const ShadowRay &s = at(id);
+        const bool reaches = s.mode == kShadowAny ? !found : (!found || G->items[h.item].entityId == s.entityId);
+        if (reaches) add3(acc + 3 * s.sample, v3(s.cx, s.cy, s.cz));
+        // A guide training sample whose direction is blocked saw no radiance.
+        if (s.rec != kEmpty && !reaches) recs[s.rec].wgt = 0.0f;
+    }
+};
+
 template <int BLOCK, int WAVES>
 __global__ __launch_bounds__(BLOCK, WAVES) void k_connect(DeviceGeometry G, const ShadowRay *__restrict__ sq, SegIn sin,
                                                          uint32_t n, float *__restrict__ acc,
@@ -176,38 +222,19 @@ __global__ __launch_bounds__(BLOCK, WAVES) void k_connect(DeviceGeometry G, cons
     __shared__ uint32_t smem[kRenderRing * BLOCK];
     __shared__ uint32_t soff[kSegs + 1];
     const uint32_t *off = stage_offsets(sin, soff);
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const ShadowRay &s = sq[seg_slot(off, sin.nseg, sin.segCap, i)];
-    Ray3 r;
-    r.ox = s.ox;
-    r.oy = s.oy;
-    r.oz = s.oz;
-    r.time = s.time;
-    r.dx = s.dx;
-    r.dy = s.dy;
-    r.dz = s.dz;
-    HitRec h;
+    const uint32_t begin = (blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)) * 64u * kRaysPerLane;
+    const uint32_t end = min(begin + 64u * kRaysPerLane, n);
+    ConnectPool pool{sq, off, sin, &G, acc, recs};
     TraceCounters c{0u, 0u, 0u};
-    bool reaches;
-    if (s.mode == kShadowAny) {
-        // Occluded by any hit with 0 < t < tmax (main.cpp:697-712, 751-752).
-        reaches = !trace_ray<true, false, kRenderRing, BLOCK>(G, r, s.tmax, 0.0f, false, smem + threadIdx.x, h, c);
-    } else {
-        // Entity lights: the closest hit must be the light itself (main.cpp:704-708).
-        bool hit = trace_ray<false, false, kRenderRing, BLOCK>(G, r, 0.0f, 0.0f, false, smem + threadIdx.x, h, c);
-        reaches = !hit || G.items[h.item].entityId == s.entityId;
-    }
-    if (reaches) add3(acc + 3 * s.sample, v3(s.cx, s.cy, s.cz));
-    // A guide training sample whose direction is blocked saw no radiance.
-    if (s.rec != kEmpty && !reaches) recs[s.rec].wgt = 0.0f;
+    trace_pool<2, false, kRenderRing, BLOCK>(G, pool, begin, end, 0.0f, false, smem + threadIdx.x, c);
 }
 
 static hipError_t launch_extend(const RenderTuning &T, const DeviceGeometry &G, const PathRay *q, SegIn qin, uint32_t n,
                                 PathHit *hits, float eps, int cull, hipStream_t st) {
 #define AZ_CASE(B, W)                                                                                   \
     if (T.traceBlock == B && T.traceWaves == W) {                                                      \
-        hipLaunchKernelGGL((k_extend<B, W>), dim3((n + B - 1) / B), dim3(B), 0, st, G, q, qin, n, hits, eps, cull); \
+        hipLaunchKernelGGL((k_extend<B, W>), dim3((n + B * kRaysPerLane - 1) / (B * kRaysPerLane)), dim3(B), 0, st, \
+                           G, q, qin, n, hits, eps, cull);                                                      \
         return hipGetLastError();                                                                       \
     }
     AZ_RENDER_TRACE_VARIANTS(AZ_CASE)
@@ -219,7 +246,8 @@ static hipError_t launch_connect(const RenderTuning &T, const DeviceGeometry &G,
                                  uint32_t n, float *acc, GuideSample *recs, hipStream_t st) {
 #define AZ_CASE(B, W)                                                                                    \
     if (T.traceBlock == B && T.traceWaves == W) {                                                       \
-        hipLaunchKernelGGL((k_connect<B, W>), dim3((n + B - 1) / B), dim3(B), 0, st, G, sq, sin, n, acc, recs); \
+        hipLaunchKernelGGL((k_connect<B, W>), dim3((n + B * kRaysPerLane - 1) / (B * kRaysPerLane)), dim3(B), 0,  \
+                           st, G, sq, sin, n, acc, recs);                                                        \
         return hipGetLastError();                                                                        \
     }
     AZ_RENDER_TRACE_VARIANTS(AZ_CASE)

@@ -1,6 +1,7 @@
 // trace.hip -- batch closest-hit and any-hit kernels over az_ray arrays.
 //
-// One ray per lane; rays and hits move as 32- and 16-byte records; the BVH is
+// A wave walks a range of rays, a lane taking the next ray when its current
+// one finishes; rays and hits move as 32- and 16-byte records; the BVH is
 // walked with a per-lane ring stack of RING entries in LDS (older entries in
 // scratch).
 #include <hip/hip_runtime.h>
@@ -11,46 +12,68 @@
 
 namespace az {
 
+// Rays of an az_ray array, hits into an az_hit array.
+struct ArrayPool {
+    const az_ray *rays;
+    az_hit *hits;
+    __device__ inline bool fetch(uint32_t id, Ray3 &r, float &tMax) {
+        const float4 *rp = reinterpret_cast<const float4 *>(rays + id);
+        float4 a = rp[0], b = rp[1];
+        r.ox = a.x;
+        r.oy = a.y;
+        r.oz = a.z;
+        r.time = a.w;
+        r.dx = b.x;
+        r.dy = b.y;
+        r.dz = b.z;
+        tMax = b.w;
+        return false;
+    }
+    __device__ inline void finish(uint32_t id, bool found, const HitRec &h) {
+        uint4 o;
+        o.x = found ? h.prim : AZ_MISS;
+        o.y = __float_as_uint(found ? h.t : 0.0f);
+        o.z = __float_as_uint(found ? h.beta : 0.0f);
+        o.w = __float_as_uint(found ? h.gamma : 0.0f);
+        reinterpret_cast<uint4 *>(hits)[id] = o;
+    }
+};
+
+// Adds a wave's traversal counters with one atomic per counter.
+__device__ inline void add_counters(unsigned long long *counters, const TraceCounters &c) {
+    unsigned long long v[3] = {c.nodes, c.tris, c.prims};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        unsigned long long x = v[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+        if ((threadIdx.x & 63) == 0) atomicAdd(counters + k, x);
+    }
+}
+
 // WAVES > 0 asks the compiler to fit WAVES waves per SIMD (register budget
-// 512/WAVES VGPRs); 0 leaves the allocation to the compiler.
+// 512/WAVES VGPRs); 0 leaves the allocation to the compiler.  Each wave
+// walks kRaysPerLane x 64 consecutive rays with refill (trace_pool).
 template <bool ANY, bool COUNT, int RING, int BLOCK, int WAVES>
 __global__ __launch_bounds__(BLOCK, (WAVES > 0 ? WAVES : 1)) void k_trace(DeviceGeometry G, const az_ray *__restrict__ rays, uint32_t n,
                                                 az_hit *__restrict__ hits, float eps, int cull,
                                                 unsigned long long *__restrict__ counters) {
     __shared__ uint32_t smem[RING * BLOCK];
-    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
-    float4 a = rp[0], b = rp[1];
-    Ray3 r;
-    r.ox = a.x;
-    r.oy = a.y;
-    r.oz = a.z;
-    r.time = a.w;
-    r.dx = b.x;
-    r.dy = b.y;
-    r.dz = b.z;
-    HitRec h;
+    const uint32_t wave = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+    const uint32_t begin = wave * 64u * kRaysPerLane;
+    if (begin >= n) return;                          // whole waves only; no barriers follow
+    const uint32_t end = min(begin + 64u * kRaysPerLane, n);
+    ArrayPool pool{rays, hits};
     TraceCounters c{0u, 0u, 0u};
-    bool hit = trace_ray<ANY, COUNT, RING, BLOCK>(G, r, b.w, ANY ? 0.0f : eps, ANY ? false : (cull != 0),
-                                                  smem + threadIdx.x, h, c);
-    uint4 o;
-    o.x = hit ? h.prim : AZ_MISS;
-    o.y = __float_as_uint(hit ? h.t : 0.0f);
-    o.z = __float_as_uint(hit ? h.beta : 0.0f);
-    o.w = __float_as_uint(hit ? h.gamma : 0.0f);
-    reinterpret_cast<uint4 *>(hits)[i] = o;
-    if (COUNT) {
-        atomicAdd(counters + 0, (unsigned long long)c.nodes);
-        atomicAdd(counters + 1, (unsigned long long)c.tris);
-        atomicAdd(counters + 2, (unsigned long long)c.prims);
-    }
+    trace_pool<ANY ? 1 : 0, COUNT, RING, BLOCK>(G, pool, begin, end, ANY ? 0.0f : eps, ANY ? false : (cull != 0),
+                                               smem + threadIdx.x, c);
+    if (COUNT) add_counters(counters, c);
 }
 
 template <bool ANY, bool COUNT, int RING, int BLOCK, int WAVES>
 static hipError_t launch_variant(const DeviceGeometry &G, const az_ray *rays, uint32_t n, az_hit *hits, float eps,
                                  int cull, unsigned long long *counters, hipStream_t s) {
-    dim3 grid((n + BLOCK - 1) / BLOCK);
+    const uint32_t perBlock = BLOCK * kRaysPerLane;
+    dim3 grid((n + perBlock - 1) / perBlock);
     hipLaunchKernelGGL((k_trace<ANY, COUNT, RING, BLOCK, WAVES>), grid, dim3(BLOCK), 0, s, G, rays, n, hits, eps,
                        cull, counters);
     return hipGetLastError();
