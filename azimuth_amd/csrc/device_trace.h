@@ -244,7 +244,7 @@ struct TraceCounters {
 //   bool fetch(uint32_t id, Ray3 &r, float &tMax)   -> true: any-hit ray
 //   void finish(uint32_t id, bool found, const HitRec &h)
 // Every lane of the wave must call it (it ends when the whole range is done).
-template <int MODE, bool COUNT, int RING, int BLOCK, class Pool>
+template <int MODE, bool COUNT, int RING, int BLOCK, bool WW, class Pool>
 __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t begin, uint32_t end, float eps,
                                   bool cull, uint32_t *ldsLane, TraceCounters &cnt) {
     RingStack<RING, BLOCK> stack;
@@ -310,9 +310,32 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
         }
         if (__ballot(have) == 0) break;
         if (!have) continue;
-        // One step of this lane's walk.
+        // WW ("while-while", Aila and Laine 2009): internal nodes first, a
+        // lane descending until it reaches a leaf, a BLAS exit marker or the
+        // end of its walk, so that the wave's node tests run together and so
+        // do the leaf tests that follow.  Otherwise ("if-if") one node or one
+        // leaf per step.  Which is faster depends on the scene.
         bool finished = false, advanced = false;
-        if (cur == kEmpty) {
+        while (WW && cur != kEmpty && !ref_is_leaf(cur)) {
+            if (COUNT) cnt.nodes++;
+            float tclip = inBlas ? localBest : bestW;
+            NodeHits h;
+            int n = node_test(G.nodes, cur, ox, oy, oz, ix, iy, iz, tclip, h);
+            if (n > 0) {
+                cur = h.ref[0];
+                if (n > 3) stack.push(sp, h.ref[3]);
+                if (n > 2) stack.push(sp, h.ref[2]);
+                if (n > 1) stack.push(sp, h.ref[1]);
+            } else if (sp == 0) {
+                finished = true;
+                break;
+            } else {
+                cur = stack.pop(sp);
+            }
+        }
+        // Then one leaf or exit marker (or, if-if, one node).
+        if (finished) {
+        } else if (cur == kEmpty) {
             // Leaving a BLAS: convert its closest local hit (main.cpp:256-276).
             if (!any && localPrim != kEmpty) {
                 const TopItem &it = G.items[curItem];
@@ -340,7 +363,7 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
             ix = 1.0f / dx;
             iy = 1.0f / dy;
             iz = 1.0f / dz;
-        } else if (!ref_is_leaf(cur)) {
+        } else if (!WW && !ref_is_leaf(cur)) {
             if (COUNT) cnt.nodes++;
             float tclip = inBlas ? localBest : bestW;
             NodeHits h;

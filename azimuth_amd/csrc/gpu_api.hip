@@ -401,15 +401,17 @@ az_status az_gpu_set_counting(az_gpu_ctx *C, b32 on) {
     return AZ_OK;
 }
 
-az_status az_gpu_set_launch(az_gpu_ctx *C, u32 block, u32 ring, s32 waves) {
+az_status az_gpu_set_launch(az_gpu_ctx *C, u32 block, u32 ring, s32 waves, s32 loop) {
     if (!C) return AZ_ERR_INVALID_ARGUMENT;
     int b = block ? (int)block : C->launch.block;
     int r = ring ? (int)ring : C->launch.ring;
     int w = waves >= 0 ? (int)waves : C->launch.waves;
-    if (!trace_variant_exists(r, b, w)) return AZ_ERR_INVALID_ARGUMENT;
+    int f = loop >= 0 ? (int)loop : C->launch.loop;
+    if (!trace_variant_exists(r, b, w, f)) return AZ_ERR_INVALID_ARGUMENT;
     C->launch.block = b;
     C->launch.ring = r;
     C->launch.waves = w;
+    C->launch.loop = f;
     C->stats.BlockSize = (u32)b;
     return AZ_OK;
 }
@@ -453,6 +455,7 @@ az_status az_gpu_autotune_trace(az_gpu_ctx *C, const az_ray *rays, u32 n, u32 re
         t.LdsStackEntries = (u32)cands[i].ring;
         t.MinWaves = cands[i].waves;
         t.ChunkSamples = 0;
+        t.LoopForm = (u32)cands[i].loop;
         t.Milliseconds = per;
         if (per < best) {
             best = per;

@@ -15,9 +15,10 @@ struct TraceLaunch {
     int ring = 16;
     int block = 256;
     int waves = 6;   // minimum waves per SIMD the register allocation targets (0: compiler's choice)
+    int loop = 0;    // traversal loop form: 0 one node or leaf per step, 1 while-while
 };
 
-bool trace_variant_exists(int ring, int block, int waves);
+bool trace_variant_exists(int ring, int block, int waves, int loop);
 // The compiled traversal variants (at most cap of them).
 int trace_variants(TraceLaunch *out, int cap);
 
@@ -28,9 +29,10 @@ struct RenderTuning {
     uint32_t chunkSamples = 2u << 20;
     int traceBlock = 256;
     int traceWaves = 6;
+    int traceLoop = 1;   // while-while
 };
 
-bool render_trace_variant_exists(int block, int waves);
+bool render_trace_variant_exists(int block, int waves, int loop);
 int render_trace_variants(RenderTuning *out, int cap);   // chunkSamples left at its default
 
 hipError_t launch_trace(const TraceLaunch &L, bool any, bool count, const DeviceGeometry &G, const az_ray *rays,

@@ -134,7 +134,8 @@ __global__ __launch_bounds__(kRenderBlock) void k_generate(DevScene S, CameraSet
 // ---- extend: closest hit of every queued ray ---------------------------------
 // The traversal kernels of the renderer come in a few (workgroup size,
 // register target) variants, chosen per scene by the autotuner.
-#define AZ_RENDER_TRACE_VARIANTS(X) X(256, 6) X(128, 6) X(256, 4) X(64, 8)
+#define AZ_RENDER_TRACE_VARIANTS(X) \
+    X(256, 6, 1) X(128, 6, 1) X(256, 4, 1) X(64, 8, 1) X(256, 6, 0) X(128, 6, 0)
 
 // Path rays of a segmented queue; hits by dense index.
 struct ExtendPool {
@@ -168,7 +169,7 @@ struct ExtendPool {
     }
 };
 
-template <int BLOCK, int WAVES>
+template <int BLOCK, int WAVES, bool WW>
 __global__ __launch_bounds__(BLOCK, WAVES) void k_extend(DeviceGeometry G, const PathRay *__restrict__ q, SegIn qin,
                                                         uint32_t n, PathHit *__restrict__ hits, float eps, int cull) {
     __shared__ uint32_t smem[kRenderRing * BLOCK];
@@ -178,7 +179,7 @@ __global__ __launch_bounds__(BLOCK, WAVES) void k_extend(DeviceGeometry G, const
     const uint32_t end = min(begin + 64u * kRaysPerLane, n);
     ExtendPool pool{q, off, qin, hits};
     TraceCounters c{0u, 0u, 0u};
-    trace_pool<0, false, kRenderRing, BLOCK>(G, pool, begin, end, eps, cull != 0, smem + threadIdx.x, c);
+    trace_pool<0, false, kRenderRing, BLOCK, WW>(G, pool, begin, end, eps, cull != 0, smem + threadIdx.x, c);
 }
 
 // ---- connect: shadow rays ------------------------------------------------------
@@ -215,7 +216,7 @@ struct ConnectPool {
     }
 };
 
-template <int BLOCK, int WAVES>
+template <int BLOCK, int WAVES, bool WW>
 __global__ __launch_bounds__(BLOCK, WAVES) void k_connect(DeviceGeometry G, const ShadowRay *__restrict__ sq, SegIn sin,
                                                          uint32_t n, float *__restrict__ acc,
                                                          GuideSample *__restrict__ recs) {
@@ -226,15 +227,15 @@ __global__ __launch_bounds__(BLOCK, WAVES) void k_connect(DeviceGeometry G, cons
     const uint32_t end = min(begin + 64u * kRaysPerLane, n);
     ConnectPool pool{sq, off, sin, &G, acc, recs};
     TraceCounters c{0u, 0u, 0u};
-    trace_pool<2, false, kRenderRing, BLOCK>(G, pool, begin, end, 0.0f, false, smem + threadIdx.x, c);
+    trace_pool<2, false, kRenderRing, BLOCK, WW>(G, pool, begin, end, 0.0f, false, smem + threadIdx.x, c);
 }
 
 static hipError_t launch_extend(const RenderTuning &T, const DeviceGeometry &G, const PathRay *q, SegIn qin, uint32_t n,
                                 PathHit *hits, float eps, int cull, hipStream_t st) {
-#define AZ_CASE(B, W)                                                                                   \
-    if (T.traceBlock == B && T.traceWaves == W) {                                                      \
-        hipLaunchKernelGGL((k_extend<B, W>), dim3((n + B * kRaysPerLane - 1) / (B * kRaysPerLane)), dim3(B), 0, st, \
-                           G, q, qin, n, hits, eps, cull);                                                      \
+#define AZ_CASE(B, W, F)                                                                                 \
+    if (T.traceBlock == B && T.traceWaves == W && T.traceLoop == F) {                                  \
+        hipLaunchKernelGGL((k_extend<B, W, F != 0>), dim3((n + B * kRaysPerLane - 1) / (B * kRaysPerLane)), \
+                           dim3(B), 0, st, G, q, qin, n, hits, eps, cull);                              \
         return hipGetLastError();                                                                       \
     }
     AZ_RENDER_TRACE_VARIANTS(AZ_CASE)
@@ -244,10 +245,10 @@ static hipError_t launch_extend(const RenderTuning &T, const DeviceGeometry &G, 
 
 static hipError_t launch_connect(const RenderTuning &T, const DeviceGeometry &G, const ShadowRay *sq, SegIn sin,
                                  uint32_t n, float *acc, GuideSample *recs, hipStream_t st) {
-#define AZ_CASE(B, W)                                                                                    \
-    if (T.traceBlock == B && T.traceWaves == W) {                                                       \
-        hipLaunchKernelGGL((k_connect<B, W>), dim3((n + B * kRaysPerLane - 1) / (B * kRaysPerLane)), dim3(B), 0,  \
-                           st, G, sq, sin, n, acc, recs);                                                        \
+#define AZ_CASE(B, W, F)                                                                                  \
+    if (T.traceBlock == B && T.traceWaves == W && T.traceLoop == F) {                                   \
+        hipLaunchKernelGGL((k_connect<B, W, F != 0>), dim3((n + B * kRaysPerLane - 1) / (B * kRaysPerLane)), \
+                           dim3(B), 0, st, G, sq, sin, n, acc, recs);                                    \
         return hipGetLastError();                                                                        \
     }
     AZ_RENDER_TRACE_VARIANTS(AZ_CASE)
@@ -257,11 +258,12 @@ static hipError_t launch_connect(const RenderTuning &T, const DeviceGeometry &G,
 
 int render_trace_variants(RenderTuning *out, int cap) {
     int k = 0;
-#define AZ_CASE(B, W)                              \
+#define AZ_CASE(B, W, F)                           \
     if (k < cap) {                                 \
         out[k] = RenderTuning{};                   \
         out[k].traceBlock = B;                     \
         out[k].traceWaves = W;                     \
+        out[k].traceLoop = F;                      \
     }                                              \
     ++k;
     AZ_RENDER_TRACE_VARIANTS(AZ_CASE)
@@ -269,9 +271,9 @@ int render_trace_variants(RenderTuning *out, int cap) {
     return k < cap ? k : cap;
 }
 
-bool render_trace_variant_exists(int block, int waves) {
-#define AZ_CASE(B, W) \
-    if (block == B && waves == W) return true;
+bool render_trace_variant_exists(int block, int waves, int loop) {
+#define AZ_CASE(B, W, F) \
+    if (block == B && waves == W && loop == F) return true;
     AZ_RENDER_TRACE_VARIANTS(AZ_CASE)
 #undef AZ_CASE
     return false;
@@ -1740,13 +1742,14 @@ az_status az_gpu_render(az_gpu_ctx *C, const camera *Cam, filter_type Filter, b3
     return AZ_OK;
 }
 
-az_status az_gpu_set_render_tuning(az_gpu_ctx *C, u32 chunkSamples, u32 traceBlock, s32 traceWaves) {
+az_status az_gpu_set_render_tuning(az_gpu_ctx *C, u32 chunkSamples, u32 traceBlock, s32 traceWaves, s32 traceLoop) {
     if (!C) return AZ_ERR_INVALID_ARGUMENT;
     RenderTuning t = C->renderTuning;
     if (chunkSamples) t.chunkSamples = chunkSamples;
     if (traceBlock) t.traceBlock = (int)traceBlock;
     if (traceWaves >= 0) t.traceWaves = traceWaves;
-    if (!render_trace_variant_exists(t.traceBlock, t.traceWaves)) return AZ_ERR_INVALID_ARGUMENT;
+    if (traceLoop >= 0) t.traceLoop = traceLoop;
+    if (!render_trace_variant_exists(t.traceBlock, t.traceWaves, t.traceLoop)) return AZ_ERR_INVALID_ARGUMENT;
     C->renderTuning = t;
     return AZ_OK;
 }
@@ -1808,6 +1811,7 @@ az_status az_gpu_autotune_render(az_gpu_ctx *C, const camera *Cam, filter_type F
                 e.LdsStackEntries = (u32)kRenderRing;
                 e.MinWaves = t.traceWaves;
                 e.ChunkSamples = cs;
+                e.LoopForm = (u32)t.traceLoop;
                 e.Milliseconds = ms;
                 if (ms < best) {
                     best = ms;

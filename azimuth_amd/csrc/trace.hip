@@ -52,8 +52,9 @@ __device__ inline void add_counters(unsigned long long *counters, const TraceCou
 
 // WAVES > 0 asks the compiler to fit WAVES waves per SIMD (register budget
 // 512/WAVES VGPRs); 0 leaves the allocation to the compiler.  Each wave
-// walks kRaysPerLane x 64 consecutive rays with refill (trace_pool).
-template <bool ANY, bool COUNT, int RING, int BLOCK, int WAVES>
+// walks kRaysPerLane x 64 consecutive rays with refill (trace_pool); WW
+// selects the while-while loop form.
+template <bool ANY, bool COUNT, int RING, int BLOCK, int WAVES, bool WW>
 __global__ __launch_bounds__(BLOCK, (WAVES > 0 ? WAVES : 1)) void k_trace(DeviceGeometry G, const az_ray *__restrict__ rays, uint32_t n,
                                                 az_hit *__restrict__ hits, float eps, int cull,
                                                 unsigned long long *__restrict__ counters) {
@@ -64,39 +65,41 @@ __global__ __launch_bounds__(BLOCK, (WAVES > 0 ? WAVES : 1)) void k_trace(Device
     const uint32_t end = min(begin + 64u * kRaysPerLane, n);
     ArrayPool pool{rays, hits};
     TraceCounters c{0u, 0u, 0u};
-    trace_pool<ANY ? 1 : 0, COUNT, RING, BLOCK>(G, pool, begin, end, ANY ? 0.0f : eps, ANY ? false : (cull != 0),
-                                               smem + threadIdx.x, c);
+    trace_pool<ANY ? 1 : 0, COUNT, RING, BLOCK, WW>(G, pool, begin, end, ANY ? 0.0f : eps,
+                                                   ANY ? false : (cull != 0), smem + threadIdx.x, c);
     if (COUNT) add_counters(counters, c);
 }
 
-template <bool ANY, bool COUNT, int RING, int BLOCK, int WAVES>
+template <bool ANY, bool COUNT, int RING, int BLOCK, int WAVES, bool WW>
 static hipError_t launch_variant(const DeviceGeometry &G, const az_ray *rays, uint32_t n, az_hit *hits, float eps,
                                  int cull, unsigned long long *counters, hipStream_t s) {
     const uint32_t perBlock = BLOCK * kRaysPerLane;
     dim3 grid((n + perBlock - 1) / perBlock);
-    hipLaunchKernelGGL((k_trace<ANY, COUNT, RING, BLOCK, WAVES>), grid, dim3(BLOCK), 0, s, G, rays, n, hits, eps,
+    hipLaunchKernelGGL((k_trace<ANY, COUNT, RING, BLOCK, WAVES, WW>), grid, dim3(BLOCK), 0, s, G, rays, n, hits, eps,
                        cull, counters);
     return hipGetLastError();
 }
 
-// The compiled (ring, block, waves) variants.
+// The compiled (ring, block, waves, loop form) variants.
 #define AZ_TRACE_VARIANTS(X) \
-    X(16, 64, 0)             \
-    X(16, 64, 6)             \
-    X(16, 64, 8)             \
-    X(16, 128, 0)            \
-    X(16, 128, 6)            \
-    X(16, 256, 0)            \
-    X(16, 256, 6)            \
-    X(32, 64, 0)             \
-    X(32, 128, 0)
+    X(16, 64, 0, 0)          \
+    X(16, 64, 6, 0)          \
+    X(16, 128, 0, 0)         \
+    X(16, 128, 6, 0)         \
+    X(16, 256, 0, 0)         \
+    X(16, 256, 6, 0)         \
+    X(16, 64, 6, 1)          \
+    X(16, 128, 0, 1)         \
+    X(16, 128, 6, 1)         \
+    X(16, 256, 0, 1)         \
+    X(16, 256, 6, 1)
 
 template <bool ANY, bool COUNT>
 static hipError_t launch_ring(const TraceLaunch &L, const DeviceGeometry &G, const az_ray *rays, uint32_t n,
                               az_hit *hits, float eps, int cull, unsigned long long *counters, hipStream_t s) {
-#define AZ_CASE(R, B, W)                                                                        \
-    if (L.ring == R && L.block == B && L.waves == W)                                            \
-        return launch_variant<ANY, COUNT, R, B, W>(G, rays, n, hits, eps, cull, counters, s);
+#define AZ_CASE(R, B, W, F)                                                                     \
+    if (L.ring == R && L.block == B && L.waves == W && L.loop == F)                             \
+        return launch_variant<ANY, COUNT, R, B, W, F != 0>(G, rays, n, hits, eps, cull, counters, s);
     AZ_TRACE_VARIANTS(AZ_CASE)
 #undef AZ_CASE
     return hipErrorInvalidValue;
@@ -114,9 +117,9 @@ hipError_t launch_trace(const TraceLaunch &L, bool any, bool count, const Device
                  : launch_ring<false, false>(L, G, rays, n, hits, eps, cull, counters, s);
 }
 
-bool trace_variant_exists(int ring, int block, int waves) {
-#define AZ_CASE(R, B, W) \
-    if (ring == R && block == B && waves == W) return true;
+bool trace_variant_exists(int ring, int block, int waves, int loop) {
+#define AZ_CASE(R, B, W, F) \
+    if (ring == R && block == B && waves == W && loop == F) return true;
     AZ_TRACE_VARIANTS(AZ_CASE)
 #undef AZ_CASE
     return false;
@@ -124,8 +127,8 @@ bool trace_variant_exists(int ring, int block, int waves) {
 
 int trace_variants(TraceLaunch *out, int cap) {
     int k = 0;
-#define AZ_CASE(R, B, W)                 \
-    if (k < cap) out[k] = TraceLaunch{R, B, W}; \
+#define AZ_CASE(R, B, W, F)                           \
+    if (k < cap) out[k] = TraceLaunch{R, B, W, F};    \
     ++k;
     AZ_TRACE_VARIANTS(AZ_CASE)
 #undef AZ_CASE

@@ -183,7 +183,7 @@ class az_stats(C.Structure):
 
 class az_tune_entry(C.Structure):
     _fields_ = [("Block", u32), ("LdsStackEntries", u32), ("MinWaves", s32), ("ChunkSamples", u32),
-                ("Milliseconds", f64)]
+                ("LoopForm", u32), ("Reserved", u32), ("Milliseconds", f64)]
 
 
 class az_tune_result(C.Structure):
@@ -232,10 +232,10 @@ EXPORTS = {
     "az_gpu_render_device": (s32, [_P, C.POINTER(camera), s32, u32, C.POINTER(az_render_opts), _P]),
     "az_gpu_get_stats": (s32, [_P, C.POINTER(az_stats)]),
     "az_gpu_set_counting": (s32, [_P, u32]),
-    "az_gpu_set_launch": (s32, [_P, u32, u32, s32]),
+    "az_gpu_set_launch": (s32, [_P, u32, u32, s32, s32]),
     "az_gpu_autotune_trace": (s32, [_P, _P, u32, u32, _P]),
     "az_gpu_autotune_render": (s32, [_P, _P, s32, u32, _P, _P]),
-    "az_gpu_set_render_tuning": (s32, [_P, u32, u32, s32]),
+    "az_gpu_set_render_tuning": (s32, [_P, u32, u32, s32, s32]),
     "az_gpu_malloc": (s32, [_P, C.POINTER(_P), u64]),
     "az_gpu_free": (s32, [_P, _P]),
     "az_gpu_upload": (s32, [_P, _P, _P, u64]),

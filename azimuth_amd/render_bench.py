@@ -61,7 +61,7 @@ def main(argv=None):
     if a.autotune:
         entries, best = ctx.autotune_render(cam, filt=filt, seed=a.seed)
         tuned = {"block": entries[best]["block"], "waves": entries[best]["waves"], "chunk": entries[best]["chunk"],
-                 "candidates": len(entries)}
+                 "loop": "while-while" if entries[best]["loop"] else "if-if", "candidates": len(entries)}
     # Warm-up: a full render, so that kernels are loaded and every queue
     # buffer has its steady-state size before the timed renders.
     ctx.render(cam, filt=filt, seed=a.seed, guide=a.guide)

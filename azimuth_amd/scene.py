@@ -294,8 +294,9 @@ class GpuContext:
     def set_counting(self, on):
         N.check(self.lib.az_gpu_set_counting(self.h, 1 if on else 0), "az_gpu_set_counting", self.h)
 
-    def set_launch(self, block=0, ring=0, waves=-1):
-        N.check(self.lib.az_gpu_set_launch(self.h, int(block), int(ring), int(waves)), "az_gpu_set_launch", self.h)
+    def set_launch(self, block=0, ring=0, waves=-1, loop=-1):
+        N.check(self.lib.az_gpu_set_launch(self.h, int(block), int(ring), int(waves), int(loop)),
+                "az_gpu_set_launch", self.h)
 
     def synchronize(self):
         N.check(self.lib.az_gpu_synchronize(self.h), "az_gpu_synchronize", self.h)
@@ -339,7 +340,7 @@ class GpuContext:
     @staticmethod
     def _tune_list(res):
         return [{"block": e.Block, "ring": e.LdsStackEntries, "waves": e.MinWaves, "chunk": e.ChunkSamples,
-                 "ms": e.Milliseconds} for e in res.Entries[:res.Count]], int(res.Best)
+                 "loop": e.LoopForm, "ms": e.Milliseconds} for e in res.Entries[:res.Count]], int(res.Best)
 
     def autotune_trace(self, d_rays, n, repeats=3):
         """Times every traversal variant on the device rays and keeps the
@@ -359,8 +360,8 @@ class GpuContext:
                                                 C.byref(res)), "az_gpu_autotune_render", self.h)
         return self._tune_list(res)
 
-    def set_render_tuning(self, chunk=0, block=0, waves=-1):
-        N.check(self.lib.az_gpu_set_render_tuning(self.h, int(chunk), int(block), int(waves)),
+    def set_render_tuning(self, chunk=0, block=0, waves=-1, loop=-1):
+        N.check(self.lib.az_gpu_set_render_tuning(self.h, int(chunk), int(block), int(waves), int(loop)),
                 "az_gpu_set_render_tuning", self.h)
 
     def render_f64(self, camera, filt=N.FilterType_Box, seed=0, spp=0, guide=False, guide_fraction=0.5,

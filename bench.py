@@ -42,6 +42,7 @@ def main():
     ap.add_argument("--block", type=int, default=0, help="workgroup size (0: library default)")
     ap.add_argument("--ring", type=int, default=0, help="LDS stack entries per lane (0: library default)")
     ap.add_argument("--waves", type=int, default=-1, help="register target in waves/SIMD (-1: library default)")
+    ap.add_argument("--loop", type=int, default=-1, help="traversal loop form, 0 if-if, 1 while-while (-1: default)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile", action="store_true", help="short run for profilers: no CPU baseline, no any-hit")
@@ -70,8 +71,8 @@ def main():
     d_inc.upload(inc)
     d_hits = ctx.alloc(max(n_coh, n_inc) * HIT_BYTES)
     tuned = None
-    if args.block or args.ring or args.waves >= 0:
-        ctx.set_launch(block=args.block, ring=args.ring, waves=args.waves)
+    if args.block or args.ring or args.waves >= 0 or args.loop >= 0:
+        ctx.set_launch(block=args.block, ring=args.ring, waves=args.waves, loop=args.loop)
     else:
         # Per-scene autotune (untimed): every compiled traversal variant on
         # both batches; the fastest is kept.
@@ -81,10 +82,10 @@ def main():
         entries, best = ctx.autotune_trace(d_tune, sample.shape[0], repeats=3)
         d_tune.free()
         tuned = {"candidates": len(entries), "ring": entries[best]["ring"], "block": entries[best]["block"],
-                 "waves": entries[best]["waves"],
-                 "ms_per_1m_rays": {f'{e["ring"]}/{e["block"]}/{e["waves"]}': round(e["ms"] * 1e6 / sample.shape[0], 4)
-                                    for e in entries}}
-        log(f"autotune: ring {tuned['ring']} block {tuned['block']} waves {tuned['waves']}")
+                 "waves": entries[best]["waves"], "loop": "while-while" if entries[best]["loop"] else "if-if",
+                 "ms_per_1m_rays": {f'{e["ring"]}/{e["block"]}/{e["waves"]}/{e["loop"]}':
+                                    round(e["ms"] * 1e6 / sample.shape[0], 4) for e in entries}}
+        log(f"autotune: ring {tuned['ring']} block {tuned['block']} waves {tuned['waves']} loop {tuned['loop']}")
 
     # Kernel-counted work per batch (an untimed counting launch).
     ctx.set_counting(True)

@@ -136,11 +136,13 @@ az_status az_gpu_get_stats(az_gpu_ctx *Ctx, az_stats *Stats);
 az_status az_gpu_set_counting(az_gpu_ctx *Ctx, b32 Enable);
 
 /* Traversal launch parameters, as az_gpu_autotune_trace chooses them: workgroup
- * size (64, 128, 256), LDS stack entries per lane (16, 32) and the waves per
- * SIMD the register allocation targets (0 = compiler's choice, 6, 8).  0 (-1
- * for MinWaves) keeps the current value; combinations that were not compiled
- * return AZ_ERR_INVALID_ARGUMENT. */
-az_status az_gpu_set_launch(az_gpu_ctx *Ctx, u32 BlockSize, u32 LdsStackEntries, s32 MinWaves);
+ * size (64, 128, 256), LDS stack entries per lane (16), the waves per SIMD
+ * the register allocation targets (0 = compiler's choice, 6) and the loop
+ * form (0: one node or leaf per step; 1: while-while, all of a lane's node
+ * steps before its next leaf).  0 (-1 for MinWaves and LoopForm) keeps the
+ * current value; combinations that were not compiled return
+ * AZ_ERR_INVALID_ARGUMENT. */
+az_status az_gpu_set_launch(az_gpu_ctx *Ctx, u32 BlockSize, u32 LdsStackEntries, s32 MinWaves, s32 LoopForm);
 
 /* Autotuning.  Each candidate launch configuration is timed with HIP events
  * on the context stream; the fastest becomes the context's setting (and
@@ -151,6 +153,8 @@ typedef struct {
     u32 LdsStackEntries;  /* LDS stack entries per lane (trace tuning) */
     s32 MinWaves;         /* register target in waves per SIMD (0: compiler's choice) */
     u32 ChunkSamples;     /* camera samples per wavefront chunk (render tuning) */
+    u32 LoopForm;         /* 0: one node or leaf per step; 1: while-while */
+    u32 Reserved;
     f64 Milliseconds;     /* measured time of the candidate */
 } az_tune_entry;
 
@@ -162,7 +166,7 @@ typedef struct {
 
 /* Traversal: times closest-hit batches of d_Rays (N rays, Repeats launches
  * per candidate after one warm-up) for every compiled (LDS stack entries,
- * workgroup size, register target) variant. */
+ * workgroup size, register target, loop form) variant. */
 az_status az_gpu_autotune_trace(az_gpu_ctx *Ctx, const az_ray *d_Rays, u32 N, u32 Repeats, az_tune_result *Out);
 
 /* Renderer: times a render of Camera (Opts' rows and seed; guide off) at
@@ -172,8 +176,10 @@ az_status az_gpu_autotune_trace(az_gpu_ctx *Ctx, const az_ray *d_Rays, u32 N, u3
 az_status az_gpu_autotune_render(az_gpu_ctx *Ctx, const camera *Camera, filter_type Filter, b32 BackfaceCulling,
                                  const az_render_opts *Opts, az_tune_result *Out);
 /* Sets the renderer's chunk size (camera samples; 0 keeps it) and its
- * traversal workgroup size / register target (0 and -1 keep them). */
-az_status az_gpu_set_render_tuning(az_gpu_ctx *Ctx, u32 ChunkSamples, u32 TraceBlock, s32 TraceMinWaves);
+ * traversal workgroup size / register target / loop form (0, -1 and -1 keep
+ * them). */
+az_status az_gpu_set_render_tuning(az_gpu_ctx *Ctx, u32 ChunkSamples, u32 TraceBlock, s32 TraceMinWaves,
+                                   s32 TraceLoopForm);
 
 /* Device memory helpers, ordered on the context stream. */
 az_status az_gpu_malloc(az_gpu_ctx *Ctx, void **d_Ptr, u64 Bytes);

@@ -19,17 +19,17 @@ def test_trace_autotune_and_every_variant_identical(gpu):
     d = gpu.alloc(rays.nbytes)
     d.upload(rays)
     entries, best = gpu.autotune_trace(d, rays.shape[0], repeats=2)
-    print("trace candidates:", [(e["ring"], e["block"], e["waves"], round(e["ms"], 4)) for e in entries],
+    print("trace candidates:", [(e["ring"], e["block"], e["waves"], e["loop"], round(e["ms"], 4)) for e in entries],
           "best", best, flush=True)
     assert len(entries) >= 2 and 0 <= best < len(entries)
     assert all(e["ms"] > 0 for e in entries)
     assert gpu.stats().BlockSize == entries[best]["block"]
     for e in entries:
-        gpu.set_launch(block=e["block"], ring=e["ring"], waves=e["waves"])
+        gpu.set_launch(block=e["block"], ring=e["ring"], waves=e["waves"], loop=e["loop"])
         got = gpu.trace(rays)
         assert got.tobytes() == want.tobytes(), e
     b = entries[best]
-    gpu.set_launch(block=b["block"], ring=b["ring"], waves=b["waves"])
+    gpu.set_launch(block=b["block"], ring=b["ring"], waves=b["waves"], loop=b["loop"])
     d.free()
 
 
@@ -37,7 +37,7 @@ def test_render_autotune_and_variants_match_oracle(gpu):
     sc, cam = configs.cornell_scene(48, 48, spp=4, max_depth=6)
     gpu.upload_scene(sc)
     entries, best = gpu.autotune_render(cam, seed=3)
-    print("render candidates:", [(e["block"], e["waves"], e["chunk"], round(e["ms"], 3)) for e in entries],
+    print("render candidates:", [(e["block"], e["waves"], e["loop"], e["chunk"], round(e["ms"], 3)) for e in entries],
           "best", best, flush=True)
     assert len(entries) >= 4 and 0 <= best < len(entries)
     want = oracle.render(sc, cam, seed=3)
@@ -45,18 +45,18 @@ def test_render_autotune_and_variants_match_oracle(gpu):
     scale = float(np.mean(np.abs(wr)))
     seen = set()
     for e in entries:
-        key = (e["block"], e["waves"])
+        key = (e["block"], e["waves"], e["loop"])
         if key in seen:
             continue
         seen.add(key)
         # A small chunk forces several chunks per render.
-        gpu.set_render_tuning(chunk=4096, block=e["block"], waves=e["waves"])
+        gpu.set_render_tuning(chunk=4096, block=e["block"], waves=e["waves"], loop=e["loop"])
         got = gpu.render(cam, seed=3)
         assert np.array_equal(got["TotalWeight"], want["TotalWeight"])
         g = got["Color"] / got["TotalWeight"][:, None]
         rmse = float(np.sqrt(np.mean((g - wr) ** 2))) / scale
         assert rmse < 1e-3, (e, rmse)
-    gpu.set_render_tuning(chunk=2 << 20, block=256, waves=6)
+    gpu.set_render_tuning(chunk=2 << 20, block=256, waves=6, loop=1)
 
 
 def test_set_render_tuning_rejects_unknown_variant(gpu):
