@@ -1,7 +1,7 @@
 """Render throughput of the wavefront path tracer on one GPU:
 
     python -m azimuth_amd.render_bench --workload cornell|interior|motion
-        [--spp N] [--scale S] [--guide] [--autotune] [--cpu-seconds T]
+        [--spp N] [--scale S] [--guide] [--autotune | --tuning B,W,L,C] [--cpu-seconds T]
 
 Prints one JSON line: Msamples/s (width * height * spp_eff / render time,
 spp_eff = floor(sqrt(spp))^2 as the reference renders), the render time from
@@ -48,6 +48,7 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--guide", action="store_true", help="render with the learned path guide")
     ap.add_argument("--autotune", action="store_true", help="autotune chunk size and traversal variant first")
+    ap.add_argument("--tuning", default="", help="fixed tuning instead: block,waves,loop,chunk (e.g. 128,6,1,4194304)")
     ap.add_argument("--cpu-seconds", type=float, default=0.0, help="time the CPU oracle on a strip of rows")
     a = ap.parse_args(argv)
     sc, cam, filt = build(a.workload, a.scale, a.spp)
@@ -58,7 +59,11 @@ def main(argv=None):
     ctx.upload_scene(sc)
     upload_s = time.time() - t0
     tuned = None
-    if a.autotune:
+    if a.tuning:
+        b, w, l, c = (int(x) for x in a.tuning.split(","))
+        ctx.set_render_tuning(chunk=c, block=b, waves=w, loop=l)
+        tuned = {"block": b, "waves": w, "chunk": c, "loop": "while-while" if l else "if-if", "candidates": 0}
+    elif a.autotune:
         entries, best = ctx.autotune_render(cam, filt=filt, seed=a.seed)
         tuned = {"block": entries[best]["block"], "waves": entries[best]["waves"], "chunk": entries[best]["chunk"],
                  "loop": "while-while" if entries[best]["loop"] else "if-if", "candidates": len(entries)}
