@@ -221,11 +221,12 @@ def procedural_envmap(width=256, height=128, sun_peak=5000.0, sun_sigma_deg=0.5,
     return img.astype(np.float32)
 
 
-def feature_scene(width=48, height=48, spp=4, max_depth=5, seed=3, env=True):
+def feature_scene(width=48, height=48, spp=4, max_depth=5, seed=3, env=True, importance=True, aperture=0.0,
+                  focus=0.0):
     from .scene import directional_light, point_light
     g = np.random.default_rng(seed)
-    b = SceneBuilder(max_depth=max_depth, path_trace=True, importance_sampling=True, ambient=(0.02, 0.02, 0.03),
-                     background=(0.1, 0.1, 0.2))
+    b = SceneBuilder(max_depth=max_depth, path_trace=True, importance_sampling=importance,
+                     ambient=(0.02, 0.02, 0.03), background=(0.1, 0.1, 0.2))
     mats = []
     for bt in range(7):
         mats.append(b.add_material(make_material(kd=g.uniform(0.2, 0.8, 3), ks=g.uniform(0.1, 0.5, 3),
@@ -275,5 +276,6 @@ def feature_scene(width=48, height=48, spp=4, max_depth=5, seed=3, env=True):
         te, keep_env = image_texture(envimg, filt=N.TextureFilter_Nearest)
         b.set_environment_map(te)
     sc = b.build(meta={"name": "features"})
-    cam = look_at_camera((0.0, 3.0, 6.5), (0.0, 0.5, 0.0), (0, 1, 0), width, height, 45.0, spp=spp)
+    cam = look_at_camera((0.0, 3.0, 6.5), (0.0, 0.5, 0.0), (0, 1, 0), width, height, 45.0, spp=spp,
+                         aperture=aperture, focus=focus)
     return sc, cam

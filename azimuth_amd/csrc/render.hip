@@ -1604,7 +1604,9 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             AZ_HIPR(C, launch_extend(C->renderTuning, C->dg, qa, qin, count, (PathHit *)B.hits.ptr, S.epsIntersection,
                                      (int)cp.cull, st));
             GuideIO GI{nullptr, nullptr, recCnt, recSegCap, alpha};
-            if (guided) {
+            const bool guideLevel = o.GuideMaxDepth == 0 || depth <= o.GuideMaxDepth;
+            if (guided && training && !guideLevel) GI.recs = (GuideSample *)B.recs.ptr;
+            if (guided && guideLevel) {
                 // Query every hit, then one batched MFMA inference.
                 AZ_HIPR(C, B.queries.alloc((size_t)count * sizeof(GuideQuery)));
                 AZ_HIPR(C, B.lobes.alloc((size_t)count * 40 * sizeof(float)));

@@ -34,9 +34,22 @@ def resolve(px):
     return px[:, :3] / np.maximum(px[:, 3:4], 1e-300)
 
 
-def timed(ctx, cam, filt, seed, spp, guide, env=N.AZ_ENV_COSINE):
-    px = ctx.render_f64(cam, filt=filt, seed=seed, spp=spp, guide=guide, env_sampling=env)
+def timed(ctx, cam, filt, seed, spp, guide, env=N.AZ_ENV_COSINE, max_depth=0):
+    px = ctx.render_f64(cam, filt=filt, seed=seed, spp=spp, guide=guide, env_sampling=env, guide_max_depth=max_depth)
     return resolve(px), ctx.last_render_s
+
+
+def equal_time(ctx, cam, filt, seed, spp, t_plain, t_same, max_depth):
+    """The largest square spp whose guided render fits in t_plain (2%)."""
+    s = int(math.isqrt(max(1, int(spp * t_plain / t_same))))
+    img, t = timed(ctx, cam, filt, seed, s * s, True, max_depth=max_depth)
+    while t > t_plain * 1.02 and s > 1:
+        s -= 1
+        img, t = timed(ctx, cam, filt, seed, s * s, True, max_depth=max_depth)
+    up, t_up = timed(ctx, cam, filt, seed, (s + 1) * (s + 1), True, max_depth=max_depth)
+    if t_up <= t_plain * 1.02:
+        return (s + 1) * (s + 1), up, t_up
+    return s * s, img, t
 
 
 def reference(ctx, cam, filt, seed, spp):
@@ -76,19 +89,15 @@ def main(argv=None):
     g_same, t_g_same = timed(ctx, cam, filt, a.seed, a.spp, True)
     mse_g_same = float(np.mean((g_same - ref) ** 2))
     print(f"guided {a.spp} spp {t_g_same:.2f}s mse {mse_g_same:.4e}", file=sys.stderr, flush=True)
-    s = int(math.isqrt(max(1, int(a.spp * t_plain / t_g_same))))
-    spp_eq = max(1, s * s)
-    g_eq, t_g_eq = timed(ctx, cam, filt, a.seed + 7, spp_eq, True)
-    while t_g_eq > t_plain * 1.02 and s > 1:
-        s -= 1
-        spp_eq = s * s
-        g_eq, t_g_eq = timed(ctx, cam, filt, a.seed + 7, spp_eq, True)
-    # The next square up, if it also fits in the plain render's time.
-    g_up, t_up = timed(ctx, cam, filt, a.seed + 7, (s + 1) * (s + 1), True)
-    if t_up <= t_plain * 1.02:
-        spp_eq, g_eq, t_g_eq = (s + 1) * (s + 1), g_up, t_up
+    spp_eq, g_eq, t_g_eq = equal_time(ctx, cam, filt, a.seed + 7, a.spp, t_plain, t_g_same, 0)
     mse_g_eq = float(np.mean((g_eq - ref) ** 2))
     print(f"guided {spp_eq} spp {t_g_eq:.2f}s mse {mse_g_eq:.4e}", file=sys.stderr, flush=True)
+    # The guide up to path depth 3 only (deeper vertices: the reference's
+    # cosine sampling alone), a cheaper setting of the same estimator.
+    d3_same, t_d3_same = timed(ctx, cam, filt, a.seed, a.spp, True, max_depth=3)
+    spp_d3, d3_eq, t_d3 = equal_time(ctx, cam, filt, a.seed + 7, a.spp, t_plain, t_d3_same, 3)
+    mse_d3 = float(np.mean((d3_eq - ref) ** 2))
+    print(f"guided to depth 3: {spp_d3} spp {t_d3:.2f}s mse {mse_d3:.4e}", file=sys.stderr, flush=True)
     # Beside the learned guide: the environment map's luminance distribution
     # (no learning; samples the sun directly), alone and as the guide's base.
     lum, t_lum = timed(ctx, cam, filt, a.seed, a.spp, False, N.AZ_ENV_LUMINANCE)
@@ -107,6 +116,8 @@ def main(argv=None):
         "mse_ratio_equal_time": round(mse_plain / mse_g_eq, 3),
         "mse_ratio_time_normalized": round((mse_plain * t_plain) / (mse_g_eq * t_g_eq), 3),
         "mse_ratio_same_spp": round(mse_plain / mse_g_same, 3),
+        "guided_to_depth3_equal_time": {"spp": spp_d3, "seconds": round(t_d3, 3), "mse": mse_d3,
+                                        "mse_ratio": round(mse_plain / mse_d3, 3)},
         "luminance_env_sampling": {"spp": a.spp, "seconds": round(t_lum, 3), "mse": mse_lum},
         "guided_over_luminance": {"spp": a.spp, "seconds": round(t_glum, 3), "mse": mse_glum},
     }

@@ -171,7 +171,7 @@ AZ_ENV_COSINE, AZ_ENV_LUMINANCE = 0, 1
 class az_render_opts(C.Structure):
     _fields_ = [("Seed", u64), ("SampleCountOverride", u32), ("RowBegin", u32), ("RowEnd", u32),
                 ("AccumulateF64", u32), ("GuideEnable", u32), ("GuideFraction", f32), ("GuideTrainSpp", u32),
-                ("GuideStepsPerChunk", u32), ("EnvSampling", u32)]
+                ("GuideStepsPerChunk", u32), ("EnvSampling", u32), ("GuideMaxDepth", u32)]
 
 
 class az_stats(C.Structure):

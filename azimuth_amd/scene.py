@@ -323,9 +323,10 @@ class GpuContext:
         return ms.value
 
     def render(self, camera, filt=N.FilterType_Box, cull=False, seed=0, spp=0, row_begin=0, row_end=0,
-               guide=False, guide_fraction=0.5, env_sampling=N.AZ_ENV_COSINE):
+               guide=False, guide_fraction=0.5, env_sampling=N.AZ_ENV_COSINE, guide_max_depth=0):
         o = N.az_render_opts()
         o.EnvSampling = int(env_sampling)
+        o.GuideMaxDepth = int(guide_max_depth)
         o.Seed = int(seed)
         o.SampleCountOverride = int(spp)
         o.RowBegin, o.RowEnd = int(row_begin), int(row_end)
@@ -365,7 +366,7 @@ class GpuContext:
                 "az_gpu_set_render_tuning", self.h)
 
     def render_f64(self, camera, filt=N.FilterType_Box, seed=0, spp=0, guide=False, guide_fraction=0.5,
-                   env_sampling=N.AZ_ENV_COSINE):
+                   env_sampling=N.AZ_ENV_COSINE, guide_max_depth=0):
         """Device render accumulated in double precision; returns (W*H, 4)
         float64 (colour sums, weight sums).  The wall time of the render call
         (guide training included) is left in last_render_s."""
@@ -377,6 +378,7 @@ class GpuContext:
         o.SampleCountOverride = int(spp)
         o.AccumulateF64 = 1
         o.EnvSampling = int(env_sampling)
+        o.GuideMaxDepth = int(guide_max_depth)
         o.GuideEnable = 1 if guide else 0
         o.GuideFraction = float(guide_fraction)
         self.synchronize()

@@ -75,6 +75,8 @@ typedef struct {
                                cosine distribution about the normal) or AZ_ENV_LUMINANCE (the
                                environment map's luminance x sin(theta) distribution; unbiased,
                                different random directions) */
+    u32 GuideMaxDepth;      /* path depth up to which the guide is used (0 = every depth); deeper
+                               vertices use the base distribution alone (still unbiased) */
 } az_render_opts;
 
 #define AZ_ENV_COSINE    0u
