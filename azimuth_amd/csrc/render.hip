@@ -471,6 +471,20 @@ __device__ inline V3 bump(const DevScene &S, const DevTexture &T, V3 N, V3 P, fl
     return normalize(N);
 }
 
+// Shading class of a hit for the in-block sort of k_shade: misses, emitters,
+// then material flags (mirror, dielectric) and BRDF model, which select the
+// branches the vertex code takes.
+constexpr uint32_t kShadeClasses = 32;
+
+__device__ inline uint32_t shade_class(const DevScene &S, const PathHit &h) {
+    if (h.prim == kEmpty) return 0u;
+    const DevItemShade &sh = S.itemShade[h.item];
+    if (sh.isEmitter) return 1u;
+    const DevMaterial &M = S.materials[sh.material];
+    const uint32_t bt = (uint32_t)min(max(M.brdfType, 0), 6);
+    return 2u + (M.flags & 3u) + 4u * bt;
+}
+
 // Hit point, shading normal and uv of a closest hit, as RayEntityIntersection
 // leaves them in ray_hit (main.cpp:64-146, 147-243, 244-281, 323-382).
 __device__ inline void reconstruct(const DevScene &S, const DeviceGeometry &G, const PathRay &r, const PathHit &h,
@@ -535,8 +549,31 @@ __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeo
                                                         const PathHit *__restrict__ hits, uint32_t n,
                                                         float *__restrict__ acc, ShadeOut O, GuideIO GI) {
     __shared__ uint32_t soff[kSegs + 1];
+    __shared__ uint32_t cls[kShadeClasses];
+    __shared__ uint16_t order[kRenderBlock];
     const uint32_t *off = stage_offsets(qin, soff);
-    uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
+    // Rays of the block sorted by shading class (a counting sort in LDS), so
+    // that the lanes of a wave take the same branches of the vertex code.
+    {
+        const uint32_t i0 = blockIdx.x * kRenderBlock + threadIdx.x;
+        const uint32_t key = i0 < n ? shade_class(S, hits[i0]) : kShadeClasses - 1u;
+        if (threadIdx.x < kShadeClasses) cls[threadIdx.x] = 0u;
+        __syncthreads();
+        atomicAdd(&cls[key], 1u);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t run = 0;
+            for (uint32_t k = 0; k < kShadeClasses; ++k) {
+                const uint32_t c = cls[k];
+                cls[k] = run;
+                run += c;
+            }
+        }
+        __syncthreads();
+        order[atomicAdd(&cls[key], 1u)] = (uint16_t)threadIdx.x;
+        __syncthreads();
+    }
+    const uint32_t i = blockIdx.x * kRenderBlock + order[threadIdx.x];
     if (i >= n) return;
     const uint32_t seg = blockIdx.x % O.nseg;
     const PathRay r = q[seg_slot(off, qin.nseg, qin.segCap, i)];
