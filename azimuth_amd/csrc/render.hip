@@ -89,10 +89,12 @@ __device__ inline void add3(float *a, V3 c) {
 }
 
 // ---- generate: camera rays (main.cpp:1135-1178) ----------------------------
-__global__ __launch_bounds__(kRenderBlock) void k_generate(DevScene S, CameraSetup cam, ChunkParams cp,
-                                                           PathRay *__restrict__ q, float *__restrict__ acc,
-                                                           uint32_t n) {
-    uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
+// Kernel bodies take their block index: the same code runs as a one-scene
+// kernel (block = blockIdx.x) and as part of a batched launch over several
+// scenes, whose blocks each belong to one scene (block = index within it).
+__device__ inline void generate_body(const DevScene &S, const CameraSetup &cam, const ChunkParams &cp,
+                                     PathRay *__restrict__ q, float *__restrict__ acc, uint32_t n, uint32_t block) {
+    uint32_t i = block * kRenderBlock + threadIdx.x;
     if (i >= n) return;
     uint32_t pixLocal = i / cp.spp, sl = i % cp.spp;
     uint32_t row = cp.row0 + pixLocal / cam.width, col = pixLocal % cam.width;
@@ -129,6 +131,12 @@ __global__ __launch_bounds__(kRenderBlock) void k_generate(DevScene S, CameraSet
     acc[3 * i + 0] = 0.0f;
     acc[3 * i + 1] = 0.0f;
     acc[3 * i + 2] = 0.0f;
+}
+
+__global__ __launch_bounds__(kRenderBlock) void k_generate(DevScene S, CameraSetup cam, ChunkParams cp,
+                                                           PathRay *__restrict__ q, float *__restrict__ acc,
+                                                           uint32_t n) {
+    generate_body(S, cam, cp, q, acc, n, blockIdx.x);
 }
 
 // ---- extend: closest hit of every queued ray ---------------------------------
@@ -169,17 +177,23 @@ struct ExtendPool {
     }
 };
 
-template <int BLOCK, int WAVES, bool WW>
-__global__ __launch_bounds__(BLOCK, WAVES) void k_extend(DeviceGeometry G, const PathRay *__restrict__ q, SegIn qin,
-                                                        uint32_t n, PathHit *__restrict__ hits, float eps, int cull) {
+template <int BLOCK, bool WW>
+__device__ inline void extend_body(const DeviceGeometry &G, const PathRay *__restrict__ q, SegIn qin, uint32_t n,
+                                   PathHit *__restrict__ hits, float eps, int cull, uint32_t block) {
     __shared__ uint32_t smem[kRenderRing * BLOCK];
     __shared__ uint32_t soff[kSegs + 1];
     const uint32_t *off = stage_offsets(qin, soff);
-    const uint32_t begin = (blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)) * 64u * kRaysPerLane;
+    const uint32_t begin = (block * (BLOCK / 64) + (threadIdx.x >> 6)) * 64u * kRaysPerLane;
     const uint32_t end = min(begin + 64u * kRaysPerLane, n);
     ExtendPool pool{q, off, qin, hits};
     TraceCounters c{0u, 0u, 0u};
     trace_pool<0, false, kRenderRing, BLOCK, WW>(G, pool, begin, end, eps, cull != 0, smem + threadIdx.x, c);
+}
+
+template <int BLOCK, int WAVES, bool WW>
+__global__ __launch_bounds__(BLOCK, WAVES) void k_extend(DeviceGeometry G, const PathRay *__restrict__ q, SegIn qin,
+                                                        uint32_t n, PathHit *__restrict__ hits, float eps, int cull) {
+    extend_body<BLOCK, WW>(G, q, qin, n, hits, eps, cull, blockIdx.x);
 }
 
 // ---- connect: shadow rays ------------------------------------------------------
@@ -216,18 +230,24 @@ struct ConnectPool {
     }
 };
 
-template <int BLOCK, int WAVES, bool WW>
-__global__ __launch_bounds__(BLOCK, WAVES) void k_connect(DeviceGeometry G, const ShadowRay *__restrict__ sq, SegIn sin,
-                                                         uint32_t n, float *__restrict__ acc,
-                                                         GuideSample *__restrict__ recs) {
+template <int BLOCK, bool WW>
+__device__ inline void connect_body(const DeviceGeometry &G, const ShadowRay *__restrict__ sq, SegIn sin, uint32_t n,
+                                    float *__restrict__ acc, GuideSample *__restrict__ recs, uint32_t block) {
     __shared__ uint32_t smem[kRenderRing * BLOCK];
     __shared__ uint32_t soff[kSegs + 1];
     const uint32_t *off = stage_offsets(sin, soff);
-    const uint32_t begin = (blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6)) * 64u * kRaysPerLane;
+    const uint32_t begin = (block * (BLOCK / 64) + (threadIdx.x >> 6)) * 64u * kRaysPerLane;
     const uint32_t end = min(begin + 64u * kRaysPerLane, n);
     ConnectPool pool{sq, off, sin, &G, acc, recs};
     TraceCounters c{0u, 0u, 0u};
     trace_pool<2, false, kRenderRing, BLOCK, WW>(G, pool, begin, end, 0.0f, false, smem + threadIdx.x, c);
+}
+
+template <int BLOCK, int WAVES, bool WW>
+__global__ __launch_bounds__(BLOCK, WAVES) void k_connect(DeviceGeometry G, const ShadowRay *__restrict__ sq, SegIn sin,
+                                                         uint32_t n, float *__restrict__ acc,
+                                                         GuideSample *__restrict__ recs) {
+    connect_body<BLOCK, WW>(G, sq, sin, n, acc, recs, blockIdx.x);
 }
 
 static hipError_t launch_extend(const RenderTuning &T, const DeviceGeometry &G, const PathRay *q, SegIn qin, uint32_t n,
@@ -544,10 +564,10 @@ __device__ inline void reconstruct(const DevScene &S, const DeviceGeometry &G, c
 
 }
 
-__global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeometry G, CameraSetup cam, ChunkParams cp,
-                                                        const PathRay *__restrict__ q, SegIn qin,
-                                                        const PathHit *__restrict__ hits, uint32_t n,
-                                                        float *__restrict__ acc, ShadeOut O, GuideIO GI) {
+__device__ inline void shade_body(const DevScene &S, const DeviceGeometry &G, const CameraSetup &cam,
+                                  const ChunkParams &cp, const PathRay *__restrict__ q, SegIn qin,
+                                  const PathHit *__restrict__ hits, uint32_t n, float *__restrict__ acc,
+                                  const ShadeOut &O, const GuideIO &GI, uint32_t block) {
     __shared__ uint32_t soff[kSegs + 1];
     __shared__ uint32_t cls[kShadeClasses];
     __shared__ uint16_t order[kRenderBlock];
@@ -555,7 +575,7 @@ __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeo
     // Rays of the block sorted by shading class (a counting sort in LDS), so
     // that the lanes of a wave take the same branches of the vertex code.
     {
-        const uint32_t i0 = blockIdx.x * kRenderBlock + threadIdx.x;
+        const uint32_t i0 = block * kRenderBlock + threadIdx.x;
         const uint32_t key = i0 < n ? shade_class(S, hits[i0]) : kShadeClasses - 1u;
         if (threadIdx.x < kShadeClasses) cls[threadIdx.x] = 0u;
         __syncthreads();
@@ -573,9 +593,9 @@ __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeo
         order[atomicAdd(&cls[key], 1u)] = (uint16_t)threadIdx.x;
         __syncthreads();
     }
-    const uint32_t i = blockIdx.x * kRenderBlock + order[threadIdx.x];
+    const uint32_t i = block * kRenderBlock + order[threadIdx.x];
     if (i >= n) return;
-    const uint32_t seg = blockIdx.x % O.nseg;
+    const uint32_t seg = block % O.nseg;
     const PathRay r = q[seg_slot(off, qin.nseg, qin.segCap, i)];
     const PathHit h = hits[i];
     const uint32_t depth = r.flags & 0xFFu;
@@ -770,7 +790,7 @@ __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeo
         if (GI.recs) {
             // Training sample: direction, weight = luminance(L_env) cos / pdf;
             // connect zeroes the weight when the direction is blocked.
-            const uint32_t rseg = blockIdx.x % kSegs;
+            const uint32_t rseg = block % kSegs;
             uint32_t idx = wave_append(GI.recCounters + rseg * kSegStride, want);
             if (want && idx < GI.recSegCap) {
                 idx += rseg * GI.recSegCap;
@@ -886,6 +906,13 @@ __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeo
     }
 }
 
+__global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeometry G, CameraSetup cam, ChunkParams cp,
+                                                        const PathRay *__restrict__ q, SegIn qin,
+                                                        const PathHit *__restrict__ hits, uint32_t n,
+                                                        float *__restrict__ acc, ShadeOut O, GuideIO GI) {
+    shade_body(S, G, cam, cp, q, qin, hits, n, acc, O, GI, blockIdx.x);
+}
+
 // ---- learned guide: queries and training batches ------------------------------------------
 __global__ __launch_bounds__(kRenderBlock) void k_guide_query(DevScene S, DeviceGeometry G, const PathRay *__restrict__ q,
                                                               SegIn qin, const PathHit *__restrict__ hits, uint32_t n,
@@ -941,9 +968,9 @@ struct SegScan {
     uint32_t *total;                 // total[q] for queue q
 };
 
-__global__ __launch_bounds__(kSegs) void k_seg_scan(SegScan P) {
+__device__ inline void seg_scan_body(const SegScan &P, uint32_t q) {
     __shared__ uint32_t part[kSegs / 64];
-    const uint32_t q = blockIdx.x, t = threadIdx.x, nseg = P.nseg[q];
+    const uint32_t t = threadIdx.x, nseg = P.nseg[q];
     uint32_t c = 0u;
     if (t < nseg) {
         c = min(P.counters[q][t * kSegStride], P.segCap[q]);
@@ -968,6 +995,8 @@ __global__ __launch_bounds__(kSegs) void k_seg_scan(SegScan P) {
     }
 }
 
+__global__ __launch_bounds__(kSegs) void k_seg_scan(SegScan P) { seg_scan_body(P, blockIdx.x); }
+
 // Clears n words (a fill kernel rather than a memset node, so that the same
 // code can be captured in a graph).
 __global__ void k_clear_u32(uint32_t *p, uint32_t n) {
@@ -977,9 +1006,9 @@ __global__ void k_clear_u32(uint32_t *p, uint32_t n) {
 
 // ---- splat (main.cpp:1209-1254) -------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(kRenderBlock) void k_splat_box(CameraSetup cam, ChunkParams cp, const float *__restrict__ acc,
-                                                            T *__restrict__ px) {
-    uint32_t p = blockIdx.x * kRenderBlock + threadIdx.x;
+__device__ inline void splat_box_body(const CameraSetup &cam, const ChunkParams &cp, const float *__restrict__ acc,
+                                      T *__restrict__ px, uint32_t block) {
+    uint32_t p = block * kRenderBlock + threadIdx.x;
     if (p >= cp.rows * cam.width) return;
     uint32_t pixel = cp.row0 * cam.width + p;
     T cr = px[4 * pixel], cg = px[4 * pixel + 1], cb = px[4 * pixel + 2], cw = px[4 * pixel + 3];
@@ -996,13 +1025,18 @@ __global__ __launch_bounds__(kRenderBlock) void k_splat_box(CameraSetup cam, Chu
     px[4 * pixel + 3] = cw;
 }
 
+template <typename T>
+__global__ __launch_bounds__(kRenderBlock) void k_splat_box(CameraSetup cam, ChunkParams cp, const float *__restrict__ acc,
+                                                            T *__restrict__ px) {
+    splat_box_body<T>(cam, cp, acc, px, blockIdx.x);
+}
+
 // Gaussian: each destination pixel gathers the samples of the (up to) 3x3
 // source pixels of this chunk whose footprint covers it.
 template <typename T>
-__global__ __launch_bounds__(kRenderBlock) void k_splat_gauss(CameraSetup cam, ChunkParams cp,
-                                                              const float *__restrict__ acc, T *__restrict__ px,
-                                                              uint32_t dstRow0, uint32_t dstRows) {
-    uint32_t p = blockIdx.x * kRenderBlock + threadIdx.x;
+__device__ inline void splat_gauss_body(const CameraSetup &cam, const ChunkParams &cp, const float *__restrict__ acc,
+                                        T *__restrict__ px, uint32_t dstRow0, uint32_t dstRows, uint32_t block) {
+    uint32_t p = block * kRenderBlock + threadIdx.x;
     if (p >= dstRows * cam.width) return;
     int32_t gy = (int32_t)(dstRow0 + p / cam.width), gx = (int32_t)(p % cam.width);
     uint32_t pixel = (uint32_t)gy * cam.width + (uint32_t)gx;
@@ -1034,6 +1068,144 @@ __global__ __launch_bounds__(kRenderBlock) void k_splat_gauss(CameraSetup cam, C
     px[4 * pixel + 1] = cg;
     px[4 * pixel + 2] = cb;
     px[4 * pixel + 3] = cw;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kRenderBlock) void k_splat_gauss(CameraSetup cam, ChunkParams cp,
+                                                              const float *__restrict__ acc, T *__restrict__ px,
+                                                              uint32_t dstRow0, uint32_t dstRows) {
+    splat_gauss_body<T>(cam, cp, acc, px, dstRow0, dstRows, blockIdx.x);
+}
+
+// ---- batched launches: several scenes per launch ---------------------------------------
+// The blocks of a batched launch are split into consecutive ranges, one per
+// scene (BatchMap.start), so each block works on one scene; the per-scene
+// arguments live in device memory, read with scalar loads.
+constexpr uint32_t kMaxBatch = 32;
+
+struct BatchMap {
+    uint32_t count;
+    uint32_t start[kMaxBatch + 1];
+};
+
+__device__ inline uint32_t batch_scene(const BatchMap &M, uint32_t b) {
+    uint32_t s = 0;
+    while (s + 1 < M.count && b >= M.start[s + 1]) ++s;
+    return s;
+}
+
+struct GenArgs {
+    DevScene S;
+    CameraSetup cam;
+    ChunkParams cp;
+    PathRay *q;
+    float *acc;
+    uint32_t n;
+};
+
+struct ExtArgs {
+    DeviceGeometry G;
+    const PathRay *q;
+    SegIn qin;
+    uint32_t n;
+    PathHit *hits;
+    float eps;
+    int cull;
+};
+
+struct ShadeArgs {
+    DevScene S;
+    DeviceGeometry G;
+    CameraSetup cam;
+    ChunkParams cp;
+    const PathRay *q;
+    SegIn qin;
+    const PathHit *hits;
+    uint32_t n;
+    float *acc;
+    ShadeOut O;
+    GuideIO GI;
+};
+
+struct ConnArgs {
+    DeviceGeometry G;
+    const ShadowRay *sq;
+    SegIn sin;
+    uint32_t n;
+    float *acc;
+    GuideSample *recs;
+};
+
+struct SplatArgs {
+    CameraSetup cam;
+    ChunkParams cp;
+    const float *acc;
+    void *px;
+    uint32_t dstRow0, dstRows;
+};
+
+__global__ __launch_bounds__(kRenderBlock) void kb_generate(const GenArgs *__restrict__ A, BatchMap M) {
+    const uint32_t s = batch_scene(M, blockIdx.x);
+    const GenArgs &a = A[s];
+    generate_body(a.S, a.cam, a.cp, a.q, a.acc, a.n, blockIdx.x - M.start[s]);
+}
+
+template <int BLOCK, int WAVES, bool WW>
+__global__ __launch_bounds__(BLOCK, WAVES) void kb_extend(const ExtArgs *__restrict__ A, BatchMap M) {
+    const uint32_t s = batch_scene(M, blockIdx.x);
+    const ExtArgs &a = A[s];
+    extend_body<BLOCK, WW>(a.G, a.q, a.qin, a.n, a.hits, a.eps, a.cull, blockIdx.x - M.start[s]);
+}
+
+__global__ __launch_bounds__(kRenderBlock, 4) void kb_shade(const ShadeArgs *__restrict__ A, BatchMap M) {
+    const uint32_t s = batch_scene(M, blockIdx.x);
+    const ShadeArgs &a = A[s];
+    shade_body(a.S, a.G, a.cam, a.cp, a.q, a.qin, a.hits, a.n, a.acc, a.O, a.GI, blockIdx.x - M.start[s]);
+}
+
+__global__ __launch_bounds__(kSegs) void kb_seg_scan(const SegScan *__restrict__ P) {
+    seg_scan_body(P[blockIdx.x >> 1], blockIdx.x & 1u);
+}
+
+template <int BLOCK, int WAVES, bool WW>
+__global__ __launch_bounds__(BLOCK, WAVES) void kb_connect(const ConnArgs *__restrict__ A, BatchMap M) {
+    const uint32_t s = batch_scene(M, blockIdx.x);
+    const ConnArgs &a = A[s];
+    connect_body<BLOCK, WW>(a.G, a.sq, a.sin, a.n, a.acc, a.recs, blockIdx.x - M.start[s]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kRenderBlock) void kb_splat(const SplatArgs *__restrict__ A, BatchMap M) {
+    const uint32_t s = batch_scene(M, blockIdx.x);
+    const SplatArgs &a = A[s];
+    if (a.cp.gaussian)
+        splat_gauss_body<T>(a.cam, a.cp, a.acc, (T *)a.px, a.dstRow0, a.dstRows, blockIdx.x - M.start[s]);
+    else
+        splat_box_body<T>(a.cam, a.cp, a.acc, (T *)a.px, blockIdx.x - M.start[s]);
+}
+
+static hipError_t launch_extend_batch(const RenderTuning &T, const ExtArgs *A, const BatchMap &M, uint32_t blocks,
+                                      hipStream_t st) {
+#define AZ_CASE(B, W, F)                                                                                  \
+    if (T.traceBlock == B && T.traceWaves == W && T.traceLoop == F) {                                   \
+        hipLaunchKernelGGL((kb_extend<B, W, F != 0>), dim3(blocks), dim3(B), 0, st, A, M);             \
+        return hipGetLastError();                                                                        \
+    }
+    AZ_RENDER_TRACE_VARIANTS(AZ_CASE)
+#undef AZ_CASE
+    return hipErrorInvalidValue;
+}
+
+static hipError_t launch_connect_batch(const RenderTuning &T, const ConnArgs *A, const BatchMap &M, uint32_t blocks,
+                                       hipStream_t st) {
+#define AZ_CASE(B, W, F)                                                                                  \
+    if (T.traceBlock == B && T.traceWaves == W && T.traceLoop == F) {                                   \
+        hipLaunchKernelGGL((kb_connect<B, W, F != 0>), dim3(blocks), dim3(B), 0, st, A, M);            \
+        return hipGetLastError();                                                                        \
+    }
+    AZ_RENDER_TRACE_VARIANTS(AZ_CASE)
+#undef AZ_CASE
+    return hipErrorInvalidValue;
 }
 
 }  // namespace az
@@ -1159,6 +1331,7 @@ struct RenderBuffers {
     DeviceBuffer q0, q1, hits, shadow, acc, counters, pixels;
     DeviceBuffer queries, lobes, recs, batch;   // learned guide
     DeviceBuffer envCdf;                        // environment luminance distribution
+    DeviceBuffer batchArgs, batchTotals;        // batched renders: per-scene arguments, queue totals
 };
 
 // Row and column CDFs of the environment map's luminance over its (u, v)
@@ -1748,7 +1921,343 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
     return AZ_OK;
 }
 
+// ---- batched renders ---------------------------------------------------------------------
+// Several scenes, one per context, rendered together: each chunk's stages
+// (generate, and per level extend, shade, segment scan, connect, then
+// splat) are single launches over the queues of every scene still active,
+// with the blocks split into per-scene ranges.  Each scene keeps its own
+// queues and counters (its context's buffers); the arguments of a launch
+// go to device memory in one copy per stage group.
+namespace {
+
+struct BatchScene {
+    az_gpu_ctx *C = nullptr;
+    RenderBuffers *B = nullptr;
+    CameraSetup cam{};
+    ChunkParams cp{};
+    void *px = nullptr;
+    u32 row0 = 0, rows = 0, nPix = 0, sppTotal = 0, chunkSpp = 1, s0 = 0, spp = 0, shadowPerRay = 0;
+    bool envLum = false;
+    DeviceBuffer *bufA = nullptr, *bufB = nullptr;
+    int offA = 0;
+    u32 count = 0, depth = 0;
+    SegIn qin{};
+    uint32_t *childCnt = nullptr, *shadowCnt = nullptr, *offShadow = nullptr;
+    uint32_t *offQ[2] = {nullptr, nullptr};
+    u32 nseg = 0, childSegCap = 0, shadowSegCap = 0;
+};
+
+// Host staging for one group of arguments; the device copy goes to its own
+// region of the batch-argument buffer.  The callers synchronize the stream
+// before a staging buffer is refilled.
+struct ArgPack {
+    std::vector<uint8_t> host;
+    template <typename T>
+    size_t add(const T *items, size_t n) {
+        size_t off = (host.size() + 255) & ~(size_t)255;
+        host.resize(off + n * sizeof(T));
+        memcpy(host.data() + off, items, n * sizeof(T));
+        return off;
+    }
+};
+
+constexpr size_t kBatchRegion = (size_t)1 << 20;   // bytes per argument region (3 regions)
+
+}  // namespace
+
+static az_status renderBatchImpl(az_gpu_ctx *const *ctxs, const camera *const *cams, u32 count, filter_type Filter,
+                                 b32 Cull, const az_render_opts *Opts, void *const *pxs) {
+    az_gpu_ctx *C0 = ctxs[0];
+    RenderBuffers &B0 = buffersOf(C0);
+    az_render_opts o;
+    memset(&o, 0, sizeof(o));
+    if (Opts) o = *Opts;
+    if (o.GuideEnable) {
+        C0->lastError = "the learned guide is not available in batched renders";
+        return AZ_ERR_UNSUPPORTED;
+    }
+    if (o.EnvSampling > AZ_ENV_LUMINANCE) return AZ_ERR_INVALID_ARGUMENT;
+    const bool f64 = o.AccumulateF64 != 0;
+    hipStream_t st = C0->stream;
+    const RenderTuning tune = C0->renderTuning;
+    std::vector<BatchScene> sc(count);
+    const size_t cntWords = (size_t)kSegs * kSegStride, offWords = kSegs + 1;
+    const size_t ctlWords = 3 * cntWords + 16 + 4 * offWords;
+    for (u32 k = 0; k < count; ++k) {
+        BatchScene &b = sc[k];
+        b.C = ctxs[k];
+        if (!b.C || !cams[k] || !pxs[k]) return AZ_ERR_INVALID_ARGUMENT;
+        if (!b.C->hasScene || !b.C->render.ready) return AZ_ERR_NO_SCENE;
+        if (b.C->device != C0->device) {
+            C0->lastError = "batched contexts must share one device";
+            return AZ_ERR_INVALID_ARGUMENT;
+        }
+        if (b.C->stream != st) AZ_HIPR(C0, hipStreamSynchronize(b.C->stream));
+        b.B = &buffersOf(b.C);
+        b.px = pxs[k];
+        b.cam = setupCamera(cams[k], o.SampleCountOverride);
+        if (b.cam.width == 0 || b.cam.height == 0) return AZ_ERR_INVALID_ARGUMENT;
+        b.row0 = o.RowBegin;
+        const u32 row1 = o.RowEnd ? std::min(o.RowEnd, b.cam.height) : b.cam.height;
+        b.rows = row1 > b.row0 ? row1 - b.row0 : 0u;
+        b.nPix = b.rows * b.cam.width;
+        b.sppTotal = b.rows ? b.cam.sppSqrt * b.cam.sppSqrt : 0u;
+        const DevScene &S = b.C->devScene;
+        b.envLum = o.EnvSampling == AZ_ENV_LUMINANCE && S.useEnv;
+        if (b.envLum && !S.envCdf) {
+            C0->lastError = "luminance sampling needs an image environment map";
+            return AZ_ERR_UNSUPPORTED;
+        }
+        b.shadowPerRay = S.lightCount + (S.useEnv ? 1u : 0u);
+        b.chunkSpp = std::max<u32>(1u, std::min<u32>(b.sppTotal, tune.chunkSamples / std::max<u32>(1u, b.nPix)));
+        AZ_HIPR(C0, b.B->counters.alloc(ctlWords * sizeof(uint32_t)));
+        uint32_t *ctl = (uint32_t *)b.B->counters.ptr;
+        b.childCnt = ctl;
+        b.shadowCnt = ctl + cntWords;
+        uint32_t *dTotal = ctl + 3 * cntWords;
+        b.offQ[0] = dTotal + 16;
+        b.offQ[1] = dTotal + 16 + offWords;
+        b.offShadow = dTotal + 16 + 2 * offWords;
+        hipLaunchKernelGGL(k_clear_u32, dim3((u32)((ctlWords + 255) / 256)), dim3(256), 0, st, ctl, (u32)ctlWords);
+        AZ_HIPR(C0, hipGetLastError());
+    }
+    AZ_HIPR(C0, B0.batchTotals.alloc((size_t)2 * count * sizeof(uint32_t)));
+    AZ_HIPR(C0, B0.batchArgs.alloc(3 * kBatchRegion));
+    uint8_t *region[3] = {(uint8_t *)B0.batchArgs.ptr, (uint8_t *)B0.batchArgs.ptr + kBatchRegion,
+                          (uint8_t *)B0.batchArgs.ptr + 2 * kBatchRegion};
+    uint32_t *totals = (uint32_t *)B0.batchTotals.ptr;
+    const size_t budget = (size_t)48 << 30;
+    ArgPack packs[3];
+    auto upload = [&](int r) -> hipError_t {
+        if (packs[r].host.size() > kBatchRegion) return hipErrorInvalidValue;
+        return hipMemcpyAsync(region[r], packs[r].host.data(), packs[r].host.size(), hipMemcpyHostToDevice, st);
+    };
+    for (;;) {
+        std::vector<u32> act;
+        for (u32 k = 0; k < count; ++k)
+            if (sc[k].s0 < sc[k].sppTotal) act.push_back(k);
+        if (act.empty()) break;
+        // Chunk set-up and camera rays.
+        std::vector<GenArgs> gen;
+        BatchMap gm{};
+        gm.count = (u32)act.size();
+        u32 gblocks = 0;
+        for (u32 j = 0; j < act.size(); ++j) {
+            BatchScene &b = sc[act[j]];
+            b.spp = std::min(b.chunkSpp, b.sppTotal - b.s0);
+            const u32 n = b.nPix * b.spp;
+            AZ_HIPR(C0, b.B->q0.alloc((size_t)n * sizeof(PathRay)));
+            AZ_HIPR(C0, b.B->acc.alloc((size_t)n * 3 * sizeof(float)));
+            memset(&b.cp, 0, sizeof(b.cp));
+            b.cp.seed = o.Seed;
+            b.cp.row0 = b.row0;
+            b.cp.rows = b.rows;
+            b.cp.s0 = b.s0;
+            b.cp.spp = b.spp;
+            b.cp.capacity = n;
+            b.cp.cull = Cull ? 1u : 0u;
+            b.cp.gaussian = Filter == FilterType_Gaussian ? 1u : 0u;
+            b.cp.envLuminance = b.envLum ? 1u : 0u;
+            b.bufA = &b.B->q0;
+            b.bufB = &b.B->q1;
+            b.offA = 0;
+            b.count = n;
+            b.depth = 0;
+            b.qin = SegIn{nullptr, 1u, n};
+            GenArgs g;
+            g.S = b.C->devScene;
+            g.cam = b.cam;
+            g.cp = b.cp;
+            g.q = (PathRay *)b.bufA->ptr;
+            g.acc = (float *)b.B->acc.ptr;
+            g.n = n;
+            gen.push_back(g);
+            gm.start[j] = gblocks;
+            gblocks += (n + kRenderBlock - 1) / kRenderBlock;
+        }
+        gm.start[act.size()] = gblocks;
+        packs[2].host.clear();
+        const size_t genOff = packs[2].add(gen.data(), gen.size());
+        AZ_HIPR(C0, upload(2));
+        hipLaunchKernelGGL(kb_generate, dim3(gblocks), dim3(kRenderBlock), 0, st,
+                           (const GenArgs *)(region[2] + genOff), gm);
+        AZ_HIPR(C0, hipGetLastError());
+        // Levels.
+        for (;;) {
+            std::vector<u32> lv;
+            for (u32 k : act)
+                if (sc[k].count > 0 && sc[k].depth <= sc[k].C->devScene.maxDepth) lv.push_back(k);
+            if (lv.empty()) break;
+            std::vector<ExtArgs> ext;
+            std::vector<ShadeArgs> sh;
+            std::vector<SegScan> scans;
+            BatchMap em{}, smap{};
+            em.count = smap.count = (u32)lv.size();
+            u32 eblocks = 0, sblocks = 0;
+            const u32 perExt = (u32)tune.traceBlock * kRaysPerLane;
+            for (u32 j = 0; j < lv.size(); ++j) {
+                BatchScene &b = sc[lv[j]];
+                const u32 blocks = (b.count + kRenderBlock - 1) / kRenderBlock;
+                b.nseg = std::min<u32>(kSegs, blocks);
+                const u32 perSeg = (blocks + b.nseg - 1) / b.nseg * kRenderBlock;
+                b.childSegCap = 4u * perSeg;
+                b.shadowSegCap = std::max<u32>(1u, b.shadowPerRay * perSeg);
+                const size_t cap = (size_t)b.nseg * b.childSegCap, shadowCap = (size_t)b.nseg * b.shadowSegCap;
+                const size_t need = cap * sizeof(PathRay) + (size_t)b.count * (sizeof(PathRay) + sizeof(PathHit)) +
+                                    shadowCap * sizeof(ShadowRay);
+                if (need > budget || cap > 0xFFFFFFFFull || shadowCap > 0xFFFFFFFFull) {
+                    C0->lastError = "batched render: path queue exceeds the memory budget";
+                    return AZ_ERR_OUT_OF_MEMORY;
+                }
+                AZ_HIPR(C0, b.bufB->alloc(cap * sizeof(PathRay)));
+                AZ_HIPR(C0, b.B->hits.alloc((size_t)b.count * sizeof(PathHit)));
+                AZ_HIPR(C0, b.B->shadow.alloc(shadowCap * sizeof(ShadowRay)));
+                ExtArgs e;
+                e.G = b.C->dg;
+                e.q = (const PathRay *)b.bufA->ptr;
+                e.qin = b.qin;
+                e.n = b.count;
+                e.hits = (PathHit *)b.B->hits.ptr;
+                e.eps = b.C->devScene.epsIntersection;
+                e.cull = (int)b.cp.cull;
+                ext.push_back(e);
+                em.start[j] = eblocks;
+                eblocks += (b.count + perExt - 1) / perExt;
+                ShadeArgs a;
+                a.S = b.C->devScene;
+                a.G = b.C->dg;
+                a.cam = b.cam;
+                a.cp = b.cp;
+                a.q = (const PathRay *)b.bufA->ptr;
+                a.qin = b.qin;
+                a.hits = (const PathHit *)b.B->hits.ptr;
+                a.n = b.count;
+                a.acc = (float *)b.B->acc.ptr;
+                a.O = ShadeOut{(PathRay *)b.bufB->ptr, (ShadowRay *)b.B->shadow.ptr, b.childCnt, b.shadowCnt,
+                               b.nseg, b.childSegCap, b.shadowSegCap};
+                a.GI = GuideIO{nullptr, nullptr, b.childCnt, 0u, 0.5f};
+                sh.push_back(a);
+                smap.start[j] = sblocks;
+                sblocks += blocks;
+                SegScan q;
+                q.counters[0] = b.childCnt;
+                q.counters[1] = b.shadowCnt;
+                q.offsets[0] = b.offQ[1 - b.offA];
+                q.offsets[1] = b.offShadow;
+                q.segCap[0] = b.childSegCap;
+                q.segCap[1] = b.shadowSegCap;
+                q.nseg[0] = q.nseg[1] = b.nseg;
+                q.total = totals + 2 * j;
+                scans.push_back(q);
+            }
+            em.start[lv.size()] = eblocks;
+            smap.start[lv.size()] = sblocks;
+            packs[0].host.clear();
+            const size_t extOff = packs[0].add(ext.data(), ext.size());
+            const size_t shOff = packs[0].add(sh.data(), sh.size());
+            const size_t scOff = packs[0].add(scans.data(), scans.size());
+            AZ_HIPR(C0, upload(0));
+            AZ_HIPR(C0, launch_extend_batch(tune, (const ExtArgs *)(region[0] + extOff), em, eblocks, st));
+            hipLaunchKernelGGL(kb_shade, dim3(sblocks), dim3(kRenderBlock), 0, st,
+                               (const ShadeArgs *)(region[0] + shOff), smap);
+            AZ_HIPR(C0, hipGetLastError());
+            hipLaunchKernelGGL(kb_seg_scan, dim3(2 * (u32)lv.size()), dim3(kSegs), 0, st,
+                               (const SegScan *)(region[0] + scOff));
+            AZ_HIPR(C0, hipGetLastError());
+            std::vector<uint32_t> hc(2 * lv.size());
+            AZ_HIPR(C0, hipMemcpyAsync(hc.data(), totals, hc.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            AZ_HIPR(C0, hipStreamSynchronize(st));
+            std::vector<ConnArgs> con;
+            BatchMap cm{};
+            u32 cblocks = 0;
+            for (u32 j = 0; j < lv.size(); ++j) {
+                BatchScene &b = sc[lv[j]];
+                const u32 ns = hc[2 * j + 1];
+                if (ns) {
+                    ConnArgs c;
+                    c.G = b.C->dg;
+                    c.sq = (const ShadowRay *)b.B->shadow.ptr;
+                    c.sin = SegIn{b.offShadow, b.nseg, b.shadowSegCap};
+                    c.n = ns;
+                    c.acc = (float *)b.B->acc.ptr;
+                    c.recs = nullptr;
+                    cm.start[con.size()] = cblocks;
+                    cblocks += (ns + perExt - 1) / perExt;
+                    con.push_back(c);
+                }
+                b.count = hc[2 * j];
+                b.offA = 1 - b.offA;
+                b.qin = SegIn{b.offQ[b.offA], b.nseg, b.childSegCap};
+                std::swap(b.bufA, b.bufB);
+                b.depth++;
+            }
+            if (!con.empty()) {
+                cm.count = (u32)con.size();
+                cm.start[con.size()] = cblocks;
+                packs[1].host.clear();
+                const size_t cOff = packs[1].add(con.data(), con.size());
+                AZ_HIPR(C0, upload(1));
+                AZ_HIPR(C0, launch_connect_batch(tune, (const ConnArgs *)(region[1] + cOff), cm, cblocks, st));
+            }
+        }
+        // Splat every active scene's chunk into its pixels.
+        std::vector<SplatArgs> spl;
+        BatchMap pm{};
+        pm.count = (u32)act.size();
+        u32 pblocks = 0;
+        for (u32 j = 0; j < act.size(); ++j) {
+            BatchScene &b = sc[act[j]];
+            SplatArgs a;
+            a.cam = b.cam;
+            a.cp = b.cp;
+            a.acc = (const float *)b.B->acc.ptr;
+            a.px = b.px;
+            u32 pix = b.nPix;
+            a.dstRow0 = b.row0;
+            a.dstRows = b.rows;
+            if (b.cp.gaussian) {
+                const u32 row1 = b.row0 + b.rows;
+                a.dstRow0 = b.row0 > (u32)b.cam.kernel ? b.row0 - (u32)b.cam.kernel : 0u;
+                a.dstRows = std::min(b.cam.height, row1 + (u32)b.cam.kernel) - a.dstRow0;
+                pix = a.dstRows * b.cam.width;
+            }
+            spl.push_back(a);
+            pm.start[j] = pblocks;
+            pblocks += (pix + kRenderBlock - 1) / kRenderBlock;
+            b.s0 += b.spp;
+        }
+        pm.start[act.size()] = pblocks;
+        AZ_HIPR(C0, hipStreamSynchronize(st));   // the generate arguments' staging is reused
+        packs[2].host.clear();
+        const size_t pOff = packs[2].add(spl.data(), spl.size());
+        AZ_HIPR(C0, upload(2));
+        if (f64)
+            hipLaunchKernelGGL(kb_splat<double>, dim3(pblocks), dim3(kRenderBlock), 0, st,
+                               (const SplatArgs *)(region[2] + pOff), pm);
+        else
+            hipLaunchKernelGGL(kb_splat<float>, dim3(pblocks), dim3(kRenderBlock), 0, st,
+                               (const SplatArgs *)(region[2] + pOff), pm);
+        AZ_HIPR(C0, hipGetLastError());
+        AZ_HIPR(C0, hipStreamSynchronize(st));   // the splat arguments' staging is reused next chunk
+    }
+    return AZ_OK;
+}
+
 extern "C" {
+
+az_status az_gpu_render_batch(az_gpu_ctx *const *Ctxs, const camera *const *Cameras, u32 Count, filter_type Filter,
+                              b32 Cull, const az_render_opts *Opts, void *const *d_Pixels) {
+    if (!Ctxs || !Cameras || !d_Pixels || Count == 0 || Count > kMaxBatch || !Ctxs[0]) return AZ_ERR_INVALID_ARGUMENT;
+    az_gpu_ctx *C0 = Ctxs[0];
+    AZ_HIPR(C0, hipSetDevice(C0->device));
+    AZ_HIPR(C0, hipEventRecord(C0->ev0, C0->stream));
+    az_status st = renderBatchImpl(Ctxs, Cameras, Count, Filter, Cull, Opts, d_Pixels);
+    AZ_HIPR(C0, hipEventRecord(C0->ev1, C0->stream));
+    C0->lastStart = C0->ev0;
+    C0->lastStop = C0->ev1;
+    C0->pendingTiming = true;
+    return st;
+}
 
 az_status az_gpu_render_device(az_gpu_ctx *C, const camera *Cam, filter_type Filter, b32 Cull,
                                const az_render_opts *Opts, void *d_px) {

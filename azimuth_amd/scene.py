@@ -391,6 +391,40 @@ class GpuContext:
         d.free()
         return out
 
+    @staticmethod
+    def render_batch(contexts, cameras, filt=N.FilterType_Box, cull=False, seed=0, spp=0, f64=False):
+        """Renders cameras[k] against the scene of contexts[k], all in one
+        batched render on contexts[0]'s stream; returns one (W*H, 4) array per
+        scene (float32 pixel records, or float64 sums with f64) and the wall
+        time of the render call."""
+        assert len(contexts) == len(cameras) and contexts
+        lib = contexts[0].lib
+        dt = np.float64 if f64 else np.float32
+        bufs = []
+        for ctx, cam in zip(contexts, cameras):
+            nb = cam.Resolution.x * cam.Resolution.y * 4 * np.dtype(dt).itemsize
+            d = contexts[0].alloc(nb)
+            N.check(lib.az_gpu_memset(contexts[0].h, d.ptr, 0, nb), "az_gpu_memset", contexts[0].h)
+            bufs.append(d)
+        o = N.az_render_opts()
+        o.Seed = int(seed)
+        o.SampleCountOverride = int(spp)
+        o.AccumulateF64 = 1 if f64 else 0
+        hs = (C.c_void_p * len(contexts))(*[c.h.value for c in contexts])
+        cs = (C.c_void_p * len(cameras))(*[C.addressof(cam) for cam in cameras])
+        ps = (C.c_void_p * len(bufs))(*[b.ptr for b in bufs])
+        contexts[0].synchronize()
+        t0 = time.perf_counter()
+        N.check(lib.az_gpu_render_batch(hs, cs, len(contexts), int(filt), 1 if cull else 0, C.byref(o), ps),
+                "az_gpu_render_batch", contexts[0].h)
+        contexts[0].synchronize()
+        seconds = time.perf_counter() - t0
+        out = []
+        for b, cam in zip(bufs, cameras):
+            out.append(b.download(dt, cam.Resolution.x * cam.Resolution.y * 4).reshape(-1, 4))
+            b.free()
+        return out, seconds
+
     def close(self):
         if self.h:
             self.lib.az_gpu_destroy(self.h)

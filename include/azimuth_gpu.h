@@ -133,6 +133,19 @@ az_status az_gpu_render(az_gpu_ctx *Ctx, const camera *Camera, filter_type Filte
 az_status az_gpu_render_device(az_gpu_ctx *Ctx, const camera *Camera, filter_type Filter,
                                b32 BackfaceCulling, const az_render_opts *Opts, void *d_Pixels);
 
+/* Batched render: Count scenes, one per context, rendered together.  Each
+ * chunk's stages (camera rays; per path level the closest hits, the shading
+ * and the shadow rays; the pixel splat) are single launches over the queues
+ * of every scene, each workgroup working on one scene.  Ctxs[k] holds scene k
+ * (az_gpu_upload_scene) and renders Cameras[k] into d_Pixels[k] (device
+ * memory, Width*Height entries zeroed by the caller; doubles when
+ * Opts->AccumulateF64).  All contexts must be on one device; the work is
+ * ordered on Ctxs[0]'s stream and uses its render tuning.  Count <= 32.  The
+ * learned guide is not available in batches (AZ_ERR_UNSUPPORTED).  No
+ * reference counterpart: the reference renders one camera per call. */
+az_status az_gpu_render_batch(az_gpu_ctx *const *Ctxs, const camera *const *Cameras, u32 Count, filter_type Filter,
+                              b32 BackfaceCulling, const az_render_opts *Opts, void *const *d_Pixels);
+
 az_status az_gpu_get_stats(az_gpu_ctx *Ctx, az_stats *Stats);
 /* Enables per-ray node/triangle counting in trace calls (slower). */
 az_status az_gpu_set_counting(az_gpu_ctx *Ctx, b32 Enable);
