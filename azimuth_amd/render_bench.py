@@ -2,6 +2,7 @@
 
     python -m azimuth_amd.render_bench --workload cornell|interior|motion
         [--spp N] [--scale S] [--guide] [--autotune | --tuning B,W,L,C] [--cpu-seconds T]
+    python -m azimuth_amd.render_bench --workload motion --scenes 16 [--autotune]
 
 Prints one JSON line: Msamples/s (width * height * spp_eff / render time,
 spp_eff = floor(sqrt(spp))^2 as the reference renders), the render time from
@@ -50,7 +51,11 @@ def main(argv=None):
     ap.add_argument("--autotune", action="store_true", help="autotune chunk size and traversal variant first")
     ap.add_argument("--tuning", default="", help="fixed tuning instead: block,waves,loop,chunk (e.g. 128,6,1,4194304)")
     ap.add_argument("--cpu-seconds", type=float, default=0.0, help="time the CPU oracle on a strip of rows")
+    ap.add_argument("--scenes", type=int, default=1,
+                    help="motion workload only: render this many scenes (seeds 0..K-1) in one batched render")
     a = ap.parse_args(argv)
+    if a.scenes > 1:
+        return batch_main(a)
     sc, cam, filt = build(a.workload, a.scale, a.spp)
     W, H = cam.Resolution.x, cam.Resolution.y
     spp_eff = int(math.isqrt(cam.SampleCount)) ** 2
@@ -100,6 +105,53 @@ def main(argv=None):
         out["speedup_vs_cpu"] = round(out["msamples_per_s"] / rate, 1)
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def batch_main(a):
+    """K motion-blur scenes from seeds 0..K-1 (each 2,000 moving spheres and a
+    bumped icosphere), one context each, rendered by one batched render;
+    Msamples/s over all of them, with the per-scene renders one after another
+    timed beside it."""
+    if a.workload != "motion":
+        raise SystemExit("--scenes is for the motion workload")
+    W, H = int(1280 * a.scale), int(720 * a.scale)
+    ctxs, cams = [], []
+    t0 = time.time()
+    for k in range(a.scenes):
+        sc, cam = workloads.motion_scene(W, H, spp=a.spp or 128, max_depth=10, seed=k)
+        ctx = GpuContext(0)
+        ctx.upload_scene(sc)
+        ctxs.append(ctx)
+        cams.append(cam)
+        print(f"scene {k} built and uploaded ({time.time() - t0:.0f}s)", file=sys.stderr, flush=True)
+    tuned = None
+    if a.autotune:
+        entries, best = ctxs[0].autotune_render(cams[0], seed=a.seed)
+        e = entries[best]
+        tuned = {"block": e["block"], "waves": e["waves"], "chunk": e["chunk"],
+                 "loop": "while-while" if e["loop"] else "if-if", "candidates": len(entries)}
+    spp_eff = int(math.isqrt(cams[0].SampleCount)) ** 2
+    GpuContext.render_batch(ctxs, cams, seed=a.seed, spp=1)          # warm-up
+    _, t_batch = GpuContext.render_batch(ctxs, cams, seed=a.seed)
+    print(f"batched render {t_batch:.2f}s", file=sys.stderr, flush=True)
+    t_seq = 0.0
+    for ctx, cam in zip(ctxs, cams):
+        if tuned:
+            ctx.set_render_tuning(chunk=tuned["chunk"], block=tuned["block"], waves=tuned["waves"],
+                                  loop=1 if tuned["loop"] == "while-while" else 0)
+        ctx.render(cam, seed=a.seed, spp=1)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        ctx.render(cam, seed=a.seed)
+        t_seq += time.perf_counter() - t1
+    total = a.scenes * W * H * spp_eff
+    print(json.dumps({"workload": f"motion x{a.scenes} (seeds 0..{a.scenes - 1})", "width": W, "height": H,
+                      "spp": spp_eff, "scenes": a.scenes, "batched_s": round(t_batch, 3),
+                      "batched_msamples_per_s": round(total / t_batch / 1e6, 3),
+                      "sequential_s": round(t_seq, 3), "sequential_msamples_per_s": round(total / t_seq / 1e6, 3),
+                      "autotune": tuned}), flush=True)
+    for c in ctxs:
+        c.close()
 
 
 if __name__ == "__main__":
