@@ -121,6 +121,8 @@ az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
     uint32_t primBase = 0;
     BuildParams bp;
     bp.maxLeafItems = C->maxLeafItems;
+    bp.nodeCost = C->sahNodeCost;
+    bp.itemCost = C->sahItemCost;
     for (u32 m = 0; m < S->MeshCount; ++m) {
         const mesh &M = S->Meshes[m];
         std::vector<const entity *> &tl = info.meshTriEntities[m];
@@ -413,6 +415,15 @@ az_status az_gpu_set_launch(az_gpu_ctx *C, u32 block, u32 ring, s32 waves, s32 l
     C->launch.waves = w;
     C->launch.loop = f;
     C->stats.BlockSize = (u32)b;
+    return AZ_OK;
+}
+
+az_status az_gpu_set_build(az_gpu_ctx *C, u32 maxLeafItems, f32 nodeCost, f32 itemCost) {
+    if (!C || maxLeafItems < 1 || maxLeafItems > kMaxLeafItems || !(nodeCost > 0.0f) || !(itemCost > 0.0f))
+        return AZ_ERR_INVALID_ARGUMENT;
+    C->maxLeafItems = maxLeafItems;
+    C->sahNodeCost = nodeCost;
+    C->sahItemCost = itemCost;
     return AZ_OK;
 }
 

@@ -235,6 +235,7 @@ EXPORTS = {
     "az_gpu_set_launch": (s32, [_P, u32, u32, s32, s32]),
     "az_gpu_autotune_trace": (s32, [_P, _P, u32, u32, _P]),
     "az_gpu_render_batch": (s32, [_P, _P, u32, s32, u32, _P, _P]),
+    "az_gpu_set_build": (s32, [_P, u32, f32, f32]),
     "az_gpu_autotune_render": (s32, [_P, _P, s32, u32, _P, _P]),
     "az_gpu_set_render_tuning": (s32, [_P, u32, u32, s32, s32]),
     "az_gpu_malloc": (s32, [_P, C.POINTER(_P), u64]),

@@ -294,6 +294,11 @@ class GpuContext:
     def set_counting(self, on):
         N.check(self.lib.az_gpu_set_counting(self.h, 1 if on else 0), "az_gpu_set_counting", self.h)
 
+    def set_build(self, leaf=4, node_cost=1.0, item_cost=1.0):
+        """BVH build parameters of the next upload_scene."""
+        N.check(self.lib.az_gpu_set_build(self.h, int(leaf), float(node_cost), float(item_cost)),
+                "az_gpu_set_build", self.h)
+
     def set_launch(self, block=0, ring=0, waves=-1, loop=-1):
         N.check(self.lib.az_gpu_set_launch(self.h, int(block), int(ring), int(waves), int(loop)),
                 "az_gpu_set_launch", self.h)

@@ -159,6 +159,11 @@ az_status az_gpu_set_counting(az_gpu_ctx *Ctx, b32 Enable);
  * AZ_ERR_INVALID_ARGUMENT. */
 az_status az_gpu_set_launch(az_gpu_ctx *Ctx, u32 BlockSize, u32 LdsStackEntries, s32 MinWaves, s32 LoopForm);
 
+/* BVH build parameters of the next az_gpu_upload_scene: the most primitives
+ * a mesh BVH leaf holds (1..16, default 4) and the binned-SAH costs of
+ * visiting a 4-wide node and of testing a primitive (default 1, 1). */
+az_status az_gpu_set_build(az_gpu_ctx *Ctx, u32 MaxLeafItems, f32 NodeCost, f32 ItemCost);
+
 /* Autotuning.  Each candidate launch configuration is timed with HIP events
  * on the context stream; the fastest becomes the context's setting (and
  * stays until the next tuning call or az_gpu_set_launch /
