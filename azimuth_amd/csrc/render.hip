@@ -92,7 +92,7 @@ __device__ inline void add3(float *a, V3 c) {
 // Kernel bodies take their block index: the same code runs as a one-scene
 // kernel (block = blockIdx.x) and as part of a batched launch over several
 // scenes, whose blocks each belong to one scene (block = index within it).
-__device__ inline void generate_body(const DevScene &S, const CameraSetup &cam, const ChunkParams &cp,
+__device__ __forceinline__ void generate_body(const DevScene &S, const CameraSetup &cam, const ChunkParams &cp,
                                      PathRay *__restrict__ q, float *__restrict__ acc, uint32_t n, uint32_t block) {
     uint32_t i = block * kRenderBlock + threadIdx.x;
     if (i >= n) return;
@@ -178,7 +178,7 @@ struct ExtendPool {
 };
 
 template <int BLOCK, bool WW>
-__device__ inline void extend_body(const DeviceGeometry &G, const PathRay *__restrict__ q, SegIn qin, uint32_t n,
+__device__ __forceinline__ void extend_body(const DeviceGeometry &G, const PathRay *__restrict__ q, SegIn qin, uint32_t n,
                                    PathHit *__restrict__ hits, float eps, int cull, uint32_t block) {
     __shared__ uint32_t smem[kRenderRing * BLOCK];
     __shared__ uint32_t soff[kSegs + 1];
@@ -231,7 +231,7 @@ struct ConnectPool {
 };
 
 template <int BLOCK, bool WW>
-__device__ inline void connect_body(const DeviceGeometry &G, const ShadowRay *__restrict__ sq, SegIn sin, uint32_t n,
+__device__ __forceinline__ void connect_body(const DeviceGeometry &G, const ShadowRay *__restrict__ sq, SegIn sin, uint32_t n,
                                     float *__restrict__ acc, GuideSample *__restrict__ recs, uint32_t block) {
     __shared__ uint32_t smem[kRenderRing * BLOCK];
     __shared__ uint32_t soff[kSegs + 1];
@@ -564,7 +564,7 @@ __device__ inline void reconstruct(const DevScene &S, const DeviceGeometry &G, c
 
 }
 
-__device__ inline void shade_body(const DevScene &S, const DeviceGeometry &G, const CameraSetup &cam,
+__device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeometry &G, const CameraSetup &cam,
                                   const ChunkParams &cp, const PathRay *__restrict__ q, SegIn qin,
                                   const PathHit *__restrict__ hits, uint32_t n, float *__restrict__ acc,
                                   const ShadeOut &O, const GuideIO &GI, uint32_t block) {
@@ -968,7 +968,7 @@ struct SegScan {
     uint32_t *total;                 // total[q] for queue q
 };
 
-__device__ inline void seg_scan_body(const SegScan &P, uint32_t q) {
+__device__ __forceinline__ void seg_scan_body(const SegScan &P, uint32_t q) {
     __shared__ uint32_t part[kSegs / 64];
     const uint32_t t = threadIdx.x, nseg = P.nseg[q];
     uint32_t c = 0u;
@@ -1006,7 +1006,7 @@ __global__ void k_clear_u32(uint32_t *p, uint32_t n) {
 
 // ---- splat (main.cpp:1209-1254) -------------------------------------------------------
 template <typename T>
-__device__ inline void splat_box_body(const CameraSetup &cam, const ChunkParams &cp, const float *__restrict__ acc,
+__device__ __forceinline__ void splat_box_body(const CameraSetup &cam, const ChunkParams &cp, const float *__restrict__ acc,
                                       T *__restrict__ px, uint32_t block) {
     uint32_t p = block * kRenderBlock + threadIdx.x;
     if (p >= cp.rows * cam.width) return;
@@ -1034,7 +1034,7 @@ __global__ __launch_bounds__(kRenderBlock) void k_splat_box(CameraSetup cam, Chu
 // Gaussian: each destination pixel gathers the samples of the (up to) 3x3
 // source pixels of this chunk whose footprint covers it.
 template <typename T>
-__device__ inline void splat_gauss_body(const CameraSetup &cam, const ChunkParams &cp, const float *__restrict__ acc,
+__device__ __forceinline__ void splat_gauss_body(const CameraSetup &cam, const ChunkParams &cp, const float *__restrict__ acc,
                                         T *__restrict__ px, uint32_t dstRow0, uint32_t dstRows, uint32_t block) {
     uint32_t p = block * kRenderBlock + threadIdx.x;
     if (p >= dstRows * cam.width) return;

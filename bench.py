@@ -10,7 +10,8 @@ value is closest-hit Mrays/s over both batches.
 
 Before timing, the per-scene autotuner picks the traversal variant (LDS
 stack entries, workgroup size, register target) on both batches.
-Also reported: the live roofline of the traversal kernel (algorithmic bytes
+Also reported: the path tracer's throughput on the architectural interior
+(render_interior), the live roofline of the traversal kernel (algorithmic bytes
 per launch from kernel-counted node visits and triangle tests, over the
 launch's HIP-event duration), any-hit rates, and the oracle's reference
 traversal timed on the host cores as a CPU baseline.
@@ -46,6 +47,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile", action="store_true", help="short run for profilers: no CPU baseline, no any-hit")
+    ap.add_argument("--no-render", action="store_true", help="skip the interior render line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; sharding across GPUs is not "
@@ -177,6 +179,8 @@ def main():
     cpu = None
     if not (args.no_cpu_baseline or args.profile):
         cpu = cpu_baseline(sc, coh, inc, args.cpu_seconds)
+    if not (args.profile or args.no_render):
+        extra["render_interior"] = render_interior(ctx)
 
     line = {
         "metric": "Mrays/s closest-hit; equal-time MSE vs 4096spp ref, guided vs plain (1 GPU)",
@@ -213,6 +217,25 @@ def main():
     line.update({k: (round(v, 4) if isinstance(v, float) else v) for k, v in extra.items()})
     print(json.dumps(line), flush=True)
     ctx.close()
+
+
+def render_interior(ctx):
+    """Beside the traversal line: the full path tracer on the architectural
+    interior (299,722 triangles, 8 spot lights, HDR sky with a sun, 1920x1080,
+    64 spp, depth 6, Gaussian filter), autotuned, one timed render after a
+    warm-up render (HIP events around the render call)."""
+    from azimuth_amd import native as N, workloads
+    sc, cam = workloads.interior_scene(1920, 1080, spp=64, max_depth=6)
+    ctx.upload_scene(sc)
+    entries, best = ctx.autotune_render(cam, filt=N.FilterType_Gaussian, seed=1)
+    ctx.render(cam, filt=N.FilterType_Gaussian, seed=1)
+    ctx.timer_start()
+    ctx.render(cam, filt=N.FilterType_Gaussian, seed=1)
+    ms = ctx.timer_stop()
+    e = entries[best]
+    return {"msamples_per_s": round(1920 * 1080 * 64 / (ms * 1e-3) / 1e6, 2), "render_ms": round(ms, 2),
+            "tuning": f'{e["block"]} lanes, {e["waves"]} waves, {"while-while" if e["loop"] else "if-if"}, '
+                      f'{e["chunk"]} samples per chunk'}
 
 
 def cpu_baseline(sc, coh, inc, budget_s):
