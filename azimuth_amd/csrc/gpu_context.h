@@ -18,9 +18,13 @@ namespace az {
 struct DeviceBuffer {
     void *ptr = nullptr;
     size_t bytes = 0;
+    // Grows to n plus a quarter, so that queues whose level sizes vary from
+    // chunk to chunk settle after a few growths (each one a device-wide
+    // synchronising free and a fresh allocation).
     hipError_t alloc(size_t n) {
         if (n <= bytes && ptr) return hipSuccess;
         release();
+        n += n / 4;
         hipError_t e = hipMalloc(&ptr, n ? n : 1);
         if (e == hipSuccess) bytes = n;
         else ptr = nullptr;

@@ -25,9 +25,14 @@ int trace_variants(TraceLaunch *out, int cap);
 // Launch parameters of the renderer: camera samples per wavefront chunk (the
 // queue tile) and the workgroup size / register target of its traversal
 // kernels (extend, connect).  Chosen per scene by az_gpu_autotune_render.
+// A chunk of 16M samples gives a level's first traversal launches about ten
+// rounds of resident waves (6,144 at 6 waves per SIMD), so the partly filled
+// last round costs little.  Its queues take 3 GB of device memory per child
+// a shading point can emit (1 to 4) plus 2 GB per light and for the
+// environment.
 struct RenderTuning {
-    uint32_t chunkSamples = 2u << 20;
-    int traceBlock = 256;
+    uint32_t chunkSamples = 16u << 20;
+    int traceBlock = 128;
     int traceWaves = 6;
     int traceLoop = 1;   // while-while
 };

@@ -1031,43 +1031,77 @@ __global__ __launch_bounds__(kRenderBlock) void k_splat_box(CameraSetup cam, Chu
     splat_box_body<T>(cam, cp, acc, px, blockIdx.x);
 }
 
-// Gaussian: each destination pixel gathers the samples of the (up to) 3x3
-// source pixels of this chunk whose footprint covers it.
+// Gaussian, in scatter form over tiles: a block's 256 threads are a 16x16
+// tile of source pixels; each reads its samples of the chunk once and sums
+// them, weighted, for each of its 3x3 destination neighbours into LDS; then
+// each of the tile's inner 14x14 pixels adds the nine partial sums aimed at
+// it.  Tiles step by 14 pixels, so every destination is an inner pixel of
+// exactly one tile.  The filter radius (1.5: one pixel each way) is fixed.
+constexpr uint32_t kSplatSrc = 16, kSplatDst = kSplatSrc - 2;
+static_assert(kSplatSrc * kSplatSrc == kRenderBlock, "one source pixel per thread");
+
+__host__ __device__ inline uint32_t splat_gauss_blocks(uint32_t width, uint32_t dstRows) {
+    return ((width + kSplatDst - 1) / kSplatDst) * ((dstRows + kSplatDst - 1) / kSplatDst);
+}
+
 template <typename T>
 __device__ __forceinline__ void splat_gauss_body(const CameraSetup &cam, const ChunkParams &cp, const float *__restrict__ acc,
                                         T *__restrict__ px, uint32_t dstRow0, uint32_t dstRows, uint32_t block) {
-    uint32_t p = block * kRenderBlock + threadIdx.x;
-    if (p >= dstRows * cam.width) return;
-    int32_t gy = (int32_t)(dstRow0 + p / cam.width), gx = (int32_t)(p % cam.width);
-    uint32_t pixel = (uint32_t)gy * cam.width + (uint32_t)gx;
-    T cr = px[4 * pixel], cg = px[4 * pixel + 1], cb = px[4 * pixel + 2], cw = px[4 * pixel + 3];
-    const int32_t K = cam.kernel;
+    __shared__ T part[kRenderBlock][9][4];
+    const uint32_t tilesX = (cam.width + kSplatDst - 1) / kSplatDst;
+    const int32_t lx = (int32_t)(threadIdx.x % kSplatSrc), ly = (int32_t)(threadIdx.x / kSplatSrc);
+    const int32_t col = (int32_t)((block % tilesX) * kSplatDst) - 1 + lx;
+    const int32_t row = (int32_t)(dstRow0 + (block / tilesX) * kSplatDst) - 1 + ly;
     const int32_t r0 = (int32_t)cp.row0, r1 = (int32_t)(cp.row0 + cp.rows) - 1;
-    for (int32_t row = gy - K; row <= gy + K; ++row) {
-        if (row < r0 || row > r1) continue;
-        for (int32_t col = gx - K; col <= gx + K; ++col) {
-            if (col < 0 || col >= (int32_t)cam.width) continue;
-            uint32_t pl = (uint32_t)(row - r0) * cam.width + (uint32_t)col;
-            for (uint32_t k = 0; k < cp.spp; ++k) {
-                uint32_t s = cp.s0 + k;
-                uint32_t sx = s % cam.sppSqrt, sy = s / cam.sppSqrt;
-                float scx = col + (0.5f * cam.invSppSqrt + sx * cam.invSppSqrt);
-                float scy = row + (0.5f * cam.invSppSqrt + sy * cam.invSppSqrt);
-                float dx = scx - (gx + 0.5f), dy = scy - (gy + 0.5f);
-                float d2 = sq(dx) + sq(dy);
-                float g = cam.oneOverTwoPiSigma2 * Exp(cam.negOneOverTwoSigma2 * d2);
-                const float *c = acc + 3 * ((size_t)pl * cp.spp + k);
-                cr += (T)(c[0] * g);
-                cg += (T)(c[1] * g);
-                cb += (T)(c[2] * g);
-                cw += (T)g;
-            }
+    T sum[9][4];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) sum[j][0] = sum[j][1] = sum[j][2] = sum[j][3] = (T)0.0f;
+    if (row >= r0 && row <= r1 && col >= 0 && col < (int32_t)cam.width) {
+        const float *c = acc + 3 * ((size_t)((uint32_t)(row - r0) * cam.width + (uint32_t)col) * cp.spp);
+        for (uint32_t k = 0; k < cp.spp; ++k) {
+            const uint32_t s = cp.s0 + k;
+            const uint32_t sx = s % cam.sppSqrt, sy = s / cam.sppSqrt;
+            const float scx = col + (0.5f * cam.invSppSqrt + sx * cam.invSppSqrt);
+            const float scy = row + (0.5f * cam.invSppSqrt + sy * cam.invSppSqrt);
+            const float cr = c[3 * k], cg = c[3 * k + 1], cb = c[3 * k + 2];
+#pragma unroll
+            for (int oy = -1; oy <= 1; ++oy)
+#pragma unroll
+                for (int ox = -1; ox <= 1; ++ox) {
+                    const float dx = scx - ((col + ox) + 0.5f), dy = scy - ((row + oy) + 0.5f);
+                    const float g = cam.oneOverTwoPiSigma2 * Exp(cam.negOneOverTwoSigma2 * (sq(dx) + sq(dy)));
+                    T *a = sum[(oy + 1) * 3 + (ox + 1)];
+                    a[0] += (T)(cr * g);
+                    a[1] += (T)(cg * g);
+                    a[2] += (T)(cb * g);
+                    a[3] += (T)g;
+                }
         }
     }
-    px[4 * pixel] = cr;
-    px[4 * pixel + 1] = cg;
-    px[4 * pixel + 2] = cb;
-    px[4 * pixel + 3] = cw;
+#pragma unroll
+    for (int j = 0; j < 9; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) part[threadIdx.x][j][q] = sum[j][q];
+    __syncthreads();
+    if (lx < 1 || ly < 1 || lx > (int32_t)kSplatDst || ly > (int32_t)kSplatDst) return;
+    if (col >= (int32_t)cam.width || row >= (int32_t)(dstRow0 + dstRows)) return;
+    const uint32_t pixel = (uint32_t)row * cam.width + (uint32_t)col;
+    T o[4] = {px[4 * pixel], px[4 * pixel + 1], px[4 * pixel + 2], px[4 * pixel + 3]};
+    // Source (col - ox, row - oy) sent its offset-(ox, oy) sum here.
+#pragma unroll
+    for (int oy = -1; oy <= 1; ++oy)
+#pragma unroll
+        for (int ox = -1; ox <= 1; ++ox) {
+            const T *a = part[(ly - oy) * (int32_t)kSplatSrc + (lx - ox)][(oy + 1) * 3 + (ox + 1)];
+            o[0] += a[0];
+            o[1] += a[1];
+            o[2] += a[2];
+            o[3] += a[3];
+        }
+    px[4 * pixel] = o[0];
+    px[4 * pixel + 1] = o[1];
+    px[4 * pixel + 2] = o[2];
+    px[4 * pixel + 3] = o[3];
 }
 
 template <typename T>
@@ -1434,6 +1468,13 @@ az_status az_upload_shading(az_gpu_ctx *C, const scene *S, const UploadInfo &inf
         if (bitsOf(m.MirrorReflectance.x) | bitsOf(m.MirrorReflectance.y) | bitsOf(m.MirrorReflectance.z))
             d.flags |= kMatMirror;
     }
+    // Children per shading point: the hemisphere scatter when path tracing,
+    // a mirror ray, and a dielectric's refraction and reflection.
+    u32 matChildren = 0;
+    for (const DevMaterial &d : mats)
+        matChildren = std::max<u32>(matChildren, ((d.flags & kMatMirror) ? 1u : 0u) +
+                                                     ((d.flags & kMatTransparent) ? 2u : 0u));
+    D.maxChildren = std::max<u32>(1u, (S->PathTrace ? 1u : 0u) + matChildren);
     // Textures (index 0 unused: entities use 1-based indices).
     std::vector<uint8_t> texels;
     std::vector<uint32_t> perms;
@@ -1688,6 +1729,55 @@ static CameraSetup setupCamera(const camera *Cam, u32 sppOverride) {
     return c;
 }
 
+// Queue geometry of one level of `count` rays: shade block b appends to
+// segment b % nseg, and a segment's slice holds what its blocks can emit (per
+// ray the scene's most children -- scatter, mirror, refraction, reflection --
+// and one shadow ray per light plus the environment).
+struct LevelCaps {
+    u32 blocks, nseg, childSegCap, shadowSegCap;
+    size_t cap, shadowCap;
+};
+
+static LevelCaps level_caps(u32 count, u32 maxChildren, u32 shadowPerRay) {
+    LevelCaps L;
+    L.blocks = (count + kRenderBlock - 1) / kRenderBlock;
+    L.nseg = std::max<u32>(1u, std::min<u32>(kSegs, L.blocks));
+    const u32 perSeg = (L.blocks + L.nseg - 1) / L.nseg * kRenderBlock;
+    L.childSegCap = maxChildren * perSeg;
+    L.shadowSegCap = std::max<u32>(1u, shadowPerRay * perSeg);
+    L.cap = (size_t)L.nseg * L.childSegCap;
+    L.shadowCap = (size_t)L.nseg * L.shadowSegCap;
+    return L;
+}
+
+// True (and the error cleared) when e is a failed allocation, which a render
+// answers by halving its chunk.
+static bool out_of_memory(hipError_t e) {
+    if (e != hipErrorOutOfMemory) return false;
+    (void)hipGetLastError();
+    return true;
+}
+
+// Sizes the path, hit, shadow and accumulator buffers for chunks of up to n
+// camera samples, with room for levels of up to 1.5 n rays where a shading
+// point can emit more than one child, so that a render seldom allocates (and
+// frees) anything past its first chunk.  Skipped when the reservation would
+// pass `budget` bytes; a failed allocation is left to the render's own
+// allocations (which halve the chunk).
+static hipError_t reserve_chunk(RenderBuffers &B, u32 n, u32 maxChildren, u32 shadowPerRay, size_t budget) {
+    const u32 m = maxChildren > 1 ? n + n / 2 : n;
+    const LevelCaps L = level_caps(m, maxChildren, shadowPerRay);
+    const size_t need = 2 * L.cap * sizeof(PathRay) + (size_t)m * (sizeof(PathHit) + 3 * sizeof(float)) +
+                        L.shadowCap * sizeof(ShadowRay);
+    if (need > budget) return hipSuccess;
+    hipError_t e = B.q0.alloc(L.cap * sizeof(PathRay));
+    if (e == hipSuccess) e = B.q1.alloc(L.cap * sizeof(PathRay));
+    if (e == hipSuccess) e = B.acc.alloc((size_t)n * 3 * sizeof(float));
+    if (e == hipSuccess) e = B.hits.alloc((size_t)m * sizeof(PathHit));
+    if (e == hipSuccess) e = B.shadow.alloc(L.shadowCap * sizeof(ShadowRay));
+    return out_of_memory(e) ? hipSuccess : e;
+}
+
 static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter, b32 Cull, const az_render_opts *Opts,
                             void *d_px, bool f64) {
     RenderBuffers &B = buffersOf(C);
@@ -1712,12 +1802,16 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
         return AZ_ERR_UNSUPPORTED;
     }
     const u32 trainSpp = guided ? (o.GuideTrainSpp ? o.GuideTrainSpp : 32u) : 0u;
-    // Training chunks hold at most trainSpp / 8 samples per pixel (and at
-    // most the chunk size); by default the Adam steps are spread over them
-    // so that training takes about kGuideSteps steps whatever the image size.
+    // Training chunks hold about 2M camera samples (GuideChunkSpp samples
+    // per pixel when given), at most trainSpp / 8 samples per pixel and at
+    // most the chunk size: the guide is updated after each, so the first
+    // chunk is the only one rendered with the untrained network.  By default
+    // the Adam steps are spread over the training chunks so that training
+    // takes about kGuideSteps steps whatever the image size.
     const u32 chunkSamples = std::max<u32>(1u, C->renderTuning.chunkSamples);
     const u32 chunkSpp0 = std::max<u32>(1u, std::min<u32>(sppTotal, chunkSamples / std::max<u32>(1u, nPix)));
-    const u32 trainChunkSpp = std::max<u32>(1u, std::min<u32>(chunkSpp0, trainSpp / 8u));
+    const u32 trainWant = o.GuideChunkSpp ? o.GuideChunkSpp : std::max<u32>(1u, (2u << 20) / std::max<u32>(1u, nPix));
+    const u32 trainChunkSpp = std::max<u32>(1u, std::min<u32>(std::min(chunkSpp0, trainWant), trainSpp / 8u));
     const u32 trainChunks = trainSpp ? (std::min(trainSpp, sppTotal) + trainChunkSpp - 1) / trainChunkSpp : 1u;
     const u32 kGuideSteps = 256;
     const u32 stepsPerChunk =
@@ -1735,7 +1829,7 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
         AZ_HIPR(C, B.recs.alloc((size_t)recCap * sizeof(GuideSample)));
         AZ_HIPR(C, B.batch.alloc((size_t)65536 * sizeof(GuideSample)));
     }
-    // Samples per pixel per chunk: the tuned chunk size (2M camera samples
+    // Samples per pixel per chunk: the tuned chunk size (16M camera samples
     // by default); while the guide trains, at least 8 chunks so that it is
     // updated often.
     u32 chunkSpp = chunkSpp0;
@@ -1761,13 +1855,19 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
     // Device memory the queues of one chunk may take before the chunk is
     // split into fewer samples per pixel.
     const size_t budget = (size_t)48 << 30;
+    AZ_HIPR(C, reserve_chunk(B, nPix * chunkSpp0, S.maxChildren, shadowPerRay, budget));
     while (s0 < sppTotal) {
         const bool training = guided && s0 < trainSpp;
         u32 spp = std::min(chunkSpp, sppTotal - s0);
-        if (training) spp = std::min(spp, std::max<u32>(1u, std::min(trainSpp - s0, trainSpp / 8u)));
+        if (training) spp = std::min(spp, std::max<u32>(1u, std::min(trainSpp - s0, trainChunkSpp)));
         u32 n = nPix * spp;
-        AZ_HIPR(C, B.q0.alloc((size_t)n * sizeof(PathRay)));
-        AZ_HIPR(C, B.acc.alloc((size_t)n * 3 * sizeof(float)));
+        hipError_t ae = B.q0.alloc((size_t)n * sizeof(PathRay));
+        if (ae == hipSuccess) ae = B.acc.alloc((size_t)n * 3 * sizeof(float));
+        if (out_of_memory(ae) && chunkSpp > 1) {
+            chunkSpp = std::max<u32>(1u, chunkSpp / 2);
+            continue;
+        }
+        AZ_HIPR(C, ae);
         ChunkParams cp;
         memset(&cp, 0, sizeof(cp));
         cp.seed = o.Seed;
@@ -1790,15 +1890,9 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
         u32 count = n;
         bool overflow = false;
         for (u32 depth = 0; depth <= S.maxDepth && count > 0; ++depth) {
-            // Block b of shade appends to segment b % nseg; a segment's slice
-            // holds what its blocks can emit: at most four children
-            // (scatter, mirror, refraction, reflection) and one shadow ray
-            // per light plus the environment per ray.
-            const u32 blocks = (count + kRenderBlock - 1) / kRenderBlock;
-            const u32 nseg = std::min<u32>(kSegs, blocks);
-            const u32 perSeg = (blocks + nseg - 1) / nseg * kRenderBlock;
-            const u32 childSegCap = 4u * perSeg, shadowSegCap = std::max<u32>(1u, shadowPerRay * perSeg);
-            const size_t cap = (size_t)nseg * childSegCap, shadowCap = (size_t)nseg * shadowSegCap;
+            const LevelCaps L = level_caps(count, S.maxChildren, shadowPerRay);
+            const u32 blocks = L.blocks, nseg = L.nseg, childSegCap = L.childSegCap, shadowSegCap = L.shadowSegCap;
+            const size_t cap = L.cap, shadowCap = L.shadowCap;
             size_t need = cap * sizeof(PathRay) + (size_t)count * (sizeof(PathRay) + sizeof(PathHit)) +
                           shadowCap * sizeof(ShadowRay) +
                           (guided ? (size_t)count * (sizeof(GuideQuery) + 40 * sizeof(float)) : 0);
@@ -1806,9 +1900,14 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
                 overflow = true;
                 break;
             }
-            AZ_HIPR(C, bufB->alloc(cap * sizeof(PathRay)));
-            AZ_HIPR(C, B.hits.alloc((size_t)count * sizeof(PathHit)));
-            AZ_HIPR(C, B.shadow.alloc(shadowCap * sizeof(ShadowRay)));
+            ae = bufB->alloc(cap * sizeof(PathRay));
+            if (ae == hipSuccess) ae = B.hits.alloc((size_t)count * sizeof(PathHit));
+            if (ae == hipSuccess) ae = B.shadow.alloc(shadowCap * sizeof(ShadowRay));
+            if (out_of_memory(ae)) {
+                overflow = true;
+                break;
+            }
+            AZ_HIPR(C, ae);
             PathRay *qa = (PathRay *)bufA->ptr, *qb = (PathRay *)bufB->ptr;
             dim3 grid(blocks);
             AZ_HIPR(C, launch_extend(C->renderTuning, C->dg, qa, qin, count, (PathHit *)B.hits.ptr, S.epsIntersection,
@@ -1861,7 +1960,7 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
         }
         if (overflow) {
             if (chunkSpp == 1) {
-                C->lastError = "path queue exceeds the memory budget with one sample per chunk";
+                C->lastError = "path queue exceeds the memory budget (or free device memory) with one sample per chunk";
                 return AZ_ERR_OUT_OF_MEMORY;
             }
             chunkSpp = std::max<u32>(1u, chunkSpp / 2);
@@ -1874,8 +1973,7 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
         } else {
             u32 d0 = row0 > (u32)cam.kernel ? row0 - (u32)cam.kernel : 0u;
             u32 d1 = std::min(cam.height, row1 + (u32)cam.kernel);
-            u32 dn = (d1 - d0) * cam.width;
-            dim3 g((dn + kRenderBlock - 1) / kRenderBlock);
+            dim3 g(splat_gauss_blocks(cam.width, d1 - d0));
             if (f64)
                 hipLaunchKernelGGL(k_splat_gauss<double>, g, dim3(kRenderBlock), 0, st, cam, cp, acc, (double *)d_px, d0,
                                    d1 - d0);
@@ -2009,7 +2107,11 @@ static az_status renderBatchImpl(az_gpu_ctx *const *ctxs, const camera *const *c
             return AZ_ERR_UNSUPPORTED;
         }
         b.shadowPerRay = S.lightCount + (S.useEnv ? 1u : 0u);
-        b.chunkSpp = std::max<u32>(1u, std::min<u32>(b.sppTotal, tune.chunkSamples / std::max<u32>(1u, b.nPix)));
+        // The scenes share one chunk's worth of samples (at least a sample
+        // per pixel each), so a batch takes about the memory of one render.
+        const u32 share = tune.chunkSamples / count;
+        b.chunkSpp = std::max<u32>(1u, std::min<u32>(b.sppTotal, share / std::max<u32>(1u, b.nPix)));
+        AZ_HIPR(C0, reserve_chunk(*b.B, b.nPix * b.chunkSpp, S.maxChildren, b.shadowPerRay, (size_t)48 << 30));
         AZ_HIPR(C0, b.B->counters.alloc(ctlWords * sizeof(uint32_t)));
         uint32_t *ctl = (uint32_t *)b.B->counters.ptr;
         b.childCnt = ctl;
@@ -2097,12 +2199,12 @@ static az_status renderBatchImpl(az_gpu_ctx *const *ctxs, const camera *const *c
             const u32 perExt = (u32)tune.traceBlock * kRaysPerLane;
             for (u32 j = 0; j < lv.size(); ++j) {
                 BatchScene &b = sc[lv[j]];
-                const u32 blocks = (b.count + kRenderBlock - 1) / kRenderBlock;
-                b.nseg = std::min<u32>(kSegs, blocks);
-                const u32 perSeg = (blocks + b.nseg - 1) / b.nseg * kRenderBlock;
-                b.childSegCap = 4u * perSeg;
-                b.shadowSegCap = std::max<u32>(1u, b.shadowPerRay * perSeg);
-                const size_t cap = (size_t)b.nseg * b.childSegCap, shadowCap = (size_t)b.nseg * b.shadowSegCap;
+                const LevelCaps L = level_caps(b.count, b.C->devScene.maxChildren, b.shadowPerRay);
+                const u32 blocks = L.blocks;
+                b.nseg = L.nseg;
+                b.childSegCap = L.childSegCap;
+                b.shadowSegCap = L.shadowSegCap;
+                const size_t cap = L.cap, shadowCap = L.shadowCap;
                 const size_t need = cap * sizeof(PathRay) + (size_t)b.count * (sizeof(PathRay) + sizeof(PathHit)) +
                                     shadowCap * sizeof(ShadowRay);
                 if (need > budget || cap > 0xFFFFFFFFull || shadowCap > 0xFFFFFFFFull) {
@@ -2212,18 +2314,18 @@ static az_status renderBatchImpl(az_gpu_ctx *const *ctxs, const camera *const *c
             a.cp = b.cp;
             a.acc = (const float *)b.B->acc.ptr;
             a.px = b.px;
-            u32 pix = b.nPix;
+            u32 blocks = (b.nPix + kRenderBlock - 1) / kRenderBlock;
             a.dstRow0 = b.row0;
             a.dstRows = b.rows;
             if (b.cp.gaussian) {
                 const u32 row1 = b.row0 + b.rows;
                 a.dstRow0 = b.row0 > (u32)b.cam.kernel ? b.row0 - (u32)b.cam.kernel : 0u;
                 a.dstRows = std::min(b.cam.height, row1 + (u32)b.cam.kernel) - a.dstRow0;
-                pix = a.dstRows * b.cam.width;
+                blocks = splat_gauss_blocks(b.cam.width, a.dstRows);
             }
             spl.push_back(a);
             pm.start[j] = pblocks;
-            pblocks += (pix + kRenderBlock - 1) / kRenderBlock;
+            pblocks += blocks;
             b.s0 += b.spp;
         }
         pm.start[act.size()] = pblocks;
@@ -2302,6 +2404,19 @@ az_status az_gpu_set_render_tuning(az_gpu_ctx *C, u32 chunkSamples, u32 traceBlo
     return AZ_OK;
 }
 
+az_status az_gpu_trim(az_gpu_ctx *C) {
+    if (!C) return AZ_ERR_INVALID_ARGUMENT;
+    AZ_HIPR(C, hipSetDevice(C->device));
+    AZ_HIPR(C, hipStreamSynchronize(C->stream));
+    RenderBuffers &B = buffersOf(C);
+    DeviceBuffer *bufs[] = {&B.q0,      &B.q1,    &B.hits,  &B.shadow,    &B.acc,        &B.pixels,
+                            &B.queries, &B.lobes, &B.recs,  &B.batch,     &B.batchArgs,  &B.batchTotals};
+    for (DeviceBuffer *b : bufs) b->release();
+    if (C->guide) az_guide_destroy(C->guide);
+    C->guide = nullptr;
+    return AZ_OK;
+}
+
 az_status az_gpu_autotune_render(az_gpu_ctx *C, const camera *Cam, filter_type Filter, b32 Cull,
                                  const az_render_opts *Opts, az_tune_result *out) {
     if (!C || !Cam) return AZ_ERR_INVALID_ARGUMENT;
@@ -2313,9 +2428,10 @@ az_status az_gpu_autotune_render(az_gpu_ctx *C, const camera *Cam, filter_type F
     // Enough samples per pixel that the largest candidate chunk is filled
     // twice (up to the camera's own count), so chunking is part of what is
     // timed.
+    const uint32_t chunks[3] = {8u << 20, 16u << 20, 32u << 20};
     const u32 nPix = Cam->Resolution.x * Cam->Resolution.y;
     const u32 camSpp = o.SampleCountOverride ? o.SampleCountOverride : Cam->SampleCount;
-    u32 spp = std::max<u32>(1u, std::min<u32>(camSpp, (8u << 20) / std::max<u32>(1u, nPix)));
+    u32 spp = std::max<u32>(1u, std::min<u32>(camSpp, 2u * chunks[2] / std::max<u32>(1u, nPix)));
     u32 sq = (u32)std::sqrt((double)spp);
     o.SampleCountOverride = std::max<u32>(1u, sq * sq);
     o.GuideEnable = 0;
@@ -2326,7 +2442,6 @@ az_status az_gpu_autotune_render(az_gpu_ctx *C, const camera *Cam, filter_type F
     AZ_HIPR(C, hipMemsetAsync(B.pixels.ptr, 0, bytes, C->stream));
     RenderTuning variants[16];
     const int nv = render_trace_variants(variants, 16);
-    const uint32_t chunks[3] = {1u << 20, 2u << 20, 4u << 20};
     const RenderTuning keep = C->renderTuning;
     // Warm-up with the current setting: kernels loaded, queues allocated.
     az_status st = renderImpl(C, Cam, Filter, Cull, &o, B.pixels.ptr, false);

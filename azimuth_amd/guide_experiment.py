@@ -2,7 +2,7 @@
 architectural interior (environment with a sun lobe, 8 spots):
 
     python -m azimuth_amd.guide_experiment [--scale S] [--ref-spp 4096]
-        [--spp 64] [--seed 1]
+        [--spp 64] [--seed 1] [--seeds 4]
 
 The reference is a plain render with --ref-spp samples per pixel accumulated
 in double precision (chunks of 256 spp with distinct seeds).  The plain tracer renders --spp samples per pixel (time
@@ -14,7 +14,9 @@ training included), after a device synchronize.  Because both estimators are unb
 the report also gives the time-normalized ratio (MSE_plain * T_plain) /
 (MSE_guided * T_guided).  For comparison it also renders with the
 environment map's luminance distribution, alone and as the guide's base
-distribution.  One JSON line is printed.
+distribution.  Every MSE is the mean over --seeds renders with distinct seeds
+(the per-image MSE of this scene varies by tens of percent from seed to
+seed), every time the mean of their times.  One JSON line is printed.
 """
 from __future__ import annotations
 
@@ -42,14 +44,23 @@ def timed(ctx, cam, filt, seed, spp, guide, env=N.AZ_ENV_COSINE, max_depth=0):
 def equal_time(ctx, cam, filt, seed, spp, t_plain, t_same, max_depth):
     """The largest square spp whose guided render fits in t_plain (2%)."""
     s = int(math.isqrt(max(1, int(spp * t_plain / t_same))))
-    img, t = timed(ctx, cam, filt, seed, s * s, True, max_depth=max_depth)
+    _, t = timed(ctx, cam, filt, seed, s * s, True, max_depth=max_depth)
     while t > t_plain * 1.02 and s > 1:
         s -= 1
-        img, t = timed(ctx, cam, filt, seed, s * s, True, max_depth=max_depth)
-    up, t_up = timed(ctx, cam, filt, seed, (s + 1) * (s + 1), True, max_depth=max_depth)
-    if t_up <= t_plain * 1.02:
-        return (s + 1) * (s + 1), up, t_up
-    return s * s, img, t
+        _, t = timed(ctx, cam, filt, seed, s * s, True, max_depth=max_depth)
+    _, t_up = timed(ctx, cam, filt, seed, (s + 1) * (s + 1), True, max_depth=max_depth)
+    return (s + 1) * (s + 1) if t_up <= t_plain * 1.02 else s * s
+
+
+def measure(ctx, cam, filt, ref, seeds, spp, guide, env=N.AZ_ENV_COSINE, max_depth=0):
+    """Mean MSE against ref and mean time over renders with the given seeds;
+    also the per-seed MSEs."""
+    mses, ts = [], []
+    for s in seeds:
+        img, t = timed(ctx, cam, filt, s, spp, guide, env, max_depth)
+        mses.append(float(np.mean((img - ref) ** 2)))
+        ts.append(t)
+    return float(np.mean(mses)), float(np.mean(ts)), mses
 
 
 def reference(ctx, cam, filt, seed, spp):
@@ -70,7 +81,9 @@ def main(argv=None):
     ap.add_argument("--ref-spp", type=int, default=4096)
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--seeds", type=int, default=4)
     a = ap.parse_args(argv)
+    seeds = [a.seed + 11 * k for k in range(a.seeds)]
     W, H = int(1920 * a.scale), int(1080 * a.scale)
     sc, cam = workloads.interior_scene(W, H, spp=a.spp, max_depth=6)
     filt = N.FilterType_Gaussian
@@ -81,38 +94,36 @@ def main(argv=None):
     ref = reference(ctx, cam, filt, 1000 * (a.seed + 1), a.ref_spp)
     t_ref = time.time() - t0
     print(f"reference {a.ref_spp} spp in {t_ref:.1f}s", file=sys.stderr, flush=True)
-    plain, t_plain = timed(ctx, cam, filt, a.seed, a.spp, False)
-    mse_plain = float(np.mean((plain - ref) ** 2))
+    timed(ctx, cam, filt, a.seed, a.spp, False)                        # warm-up at full size
+    mse_plain, t_plain, per_plain = measure(ctx, cam, filt, ref, seeds, a.spp, False)
     print(f"plain {a.spp} spp {t_plain:.2f}s mse {mse_plain:.4e}", file=sys.stderr, flush=True)
     # Guided at the same spp gives its cost per sample; then the largest
     # square spp that fits in the plain time.
-    g_same, t_g_same = timed(ctx, cam, filt, a.seed, a.spp, True)
-    mse_g_same = float(np.mean((g_same - ref) ** 2))
+    mse_g_same, t_g_same, _ = measure(ctx, cam, filt, ref, seeds, a.spp, True)
     print(f"guided {a.spp} spp {t_g_same:.2f}s mse {mse_g_same:.4e}", file=sys.stderr, flush=True)
-    spp_eq, g_eq, t_g_eq = equal_time(ctx, cam, filt, a.seed + 7, a.spp, t_plain, t_g_same, 0)
-    mse_g_eq = float(np.mean((g_eq - ref) ** 2))
+    spp_eq = equal_time(ctx, cam, filt, a.seed + 7, a.spp, t_plain, t_g_same, 0)
+    mse_g_eq, t_g_eq, per_g_eq = measure(ctx, cam, filt, ref, seeds, spp_eq, True)
     print(f"guided {spp_eq} spp {t_g_eq:.2f}s mse {mse_g_eq:.4e}", file=sys.stderr, flush=True)
     # The guide up to path depth 3 only (deeper vertices: the reference's
     # cosine sampling alone), a cheaper setting of the same estimator.
-    d3_same, t_d3_same = timed(ctx, cam, filt, a.seed, a.spp, True, max_depth=3)
-    spp_d3, d3_eq, t_d3 = equal_time(ctx, cam, filt, a.seed + 7, a.spp, t_plain, t_d3_same, 3)
-    mse_d3 = float(np.mean((d3_eq - ref) ** 2))
+    _, t_d3_same, _ = measure(ctx, cam, filt, ref, seeds[:1], a.spp, True, max_depth=3)
+    spp_d3 = equal_time(ctx, cam, filt, a.seed + 7, a.spp, t_plain, t_d3_same, 3)
+    mse_d3, t_d3, _ = measure(ctx, cam, filt, ref, seeds, spp_d3, True, max_depth=3)
     print(f"guided to depth 3: {spp_d3} spp {t_d3:.2f}s mse {mse_d3:.4e}", file=sys.stderr, flush=True)
     # Beside the learned guide: the environment map's luminance distribution
     # (no learning; samples the sun directly), alone and as the guide's base.
-    lum, t_lum = timed(ctx, cam, filt, a.seed, a.spp, False, N.AZ_ENV_LUMINANCE)
-    mse_lum = float(np.mean((lum - ref) ** 2))
-    glum, t_glum = timed(ctx, cam, filt, a.seed, a.spp, True, N.AZ_ENV_LUMINANCE)
-    mse_glum = float(np.mean((glum - ref) ** 2))
+    mse_lum, t_lum, _ = measure(ctx, cam, filt, ref, seeds, a.spp, False, N.AZ_ENV_LUMINANCE)
+    mse_glum, t_glum, _ = measure(ctx, cam, filt, ref, seeds, a.spp, True, N.AZ_ENV_LUMINANCE)
     print(f"luminance {a.spp} spp {t_lum:.2f}s mse {mse_lum:.4e}; guide+luminance {t_glum:.2f}s mse {mse_glum:.4e}",
           file=sys.stderr, flush=True)
     out = {
         "experiment": "equal-time MSE vs reference, guided vs plain",
         "workload": f"architectural interior {W}x{H} ({sc.triangle_count} triangles), Gaussian filter, depth 6",
         "reference": {"spp": a.ref_spp, "accumulation": "f64", "seconds": round(t_ref, 2)},
-        "plain": {"spp": a.spp, "seconds": round(t_plain, 3), "mse": mse_plain},
+        "seeds": seeds,
+        "plain": {"spp": a.spp, "seconds": round(t_plain, 3), "mse": mse_plain, "mse_per_seed": per_plain},
         "guided_same_spp": {"spp": a.spp, "seconds": round(t_g_same, 3), "mse": mse_g_same},
-        "guided_equal_time": {"spp": spp_eq, "seconds": round(t_g_eq, 3), "mse": mse_g_eq},
+        "guided_equal_time": {"spp": spp_eq, "seconds": round(t_g_eq, 3), "mse": mse_g_eq, "mse_per_seed": per_g_eq},
         "mse_ratio_equal_time": round(mse_plain / mse_g_eq, 3),
         "mse_ratio_time_normalized": round((mse_plain * t_plain) / (mse_g_eq * t_g_eq), 3),
         "mse_ratio_same_spp": round(mse_plain / mse_g_same, 3),

@@ -171,7 +171,8 @@ AZ_ENV_COSINE, AZ_ENV_LUMINANCE = 0, 1
 class az_render_opts(C.Structure):
     _fields_ = [("Seed", u64), ("SampleCountOverride", u32), ("RowBegin", u32), ("RowEnd", u32),
                 ("AccumulateF64", u32), ("GuideEnable", u32), ("GuideFraction", f32), ("GuideTrainSpp", u32),
-                ("GuideStepsPerChunk", u32), ("EnvSampling", u32), ("GuideMaxDepth", u32)]
+                ("GuideStepsPerChunk", u32), ("EnvSampling", u32), ("GuideMaxDepth", u32), ("GuideChunkSpp", u32),
+                ("Reserved", u32)]
 
 
 class az_stats(C.Structure):
@@ -238,6 +239,7 @@ EXPORTS = {
     "az_gpu_set_build": (s32, [_P, u32, f32, f32]),
     "az_gpu_autotune_render": (s32, [_P, _P, s32, u32, _P, _P]),
     "az_gpu_set_render_tuning": (s32, [_P, u32, u32, s32, s32]),
+    "az_gpu_trim": (s32, [_P]),
     "az_gpu_malloc": (s32, [_P, C.POINTER(_P), u64]),
     "az_gpu_free": (s32, [_P, _P]),
     "az_gpu_upload": (s32, [_P, _P, _P, u64]),

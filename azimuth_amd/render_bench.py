@@ -131,19 +131,25 @@ def batch_main(a):
         tuned = {"block": e["block"], "waves": e["waves"], "chunk": e["chunk"],
                  "loop": "while-while" if e["loop"] else "if-if", "candidates": len(entries)}
     spp_eff = int(math.isqrt(cams[0].SampleCount)) ** 2
-    GpuContext.render_batch(ctxs, cams, seed=a.seed, spp=1)          # warm-up
+    if tuned:
+        for ctx in ctxs:
+            ctx.set_render_tuning(chunk=tuned["chunk"], block=tuned["block"], waves=tuned["waves"],
+                                  loop=1 if tuned["loop"] == "while-while" else 0)
+    # Both paths are warmed up by the same full render, untimed, so that
+    # their queues have their steady-state sizes when the timed render starts.
+    GpuContext.render_batch(ctxs, cams, seed=a.seed)
     _, t_batch = GpuContext.render_batch(ctxs, cams, seed=a.seed)
     print(f"batched render {t_batch:.2f}s", file=sys.stderr, flush=True)
     t_seq = 0.0
+    for c in ctxs:
+        c.trim()
     for ctx, cam in zip(ctxs, cams):
-        if tuned:
-            ctx.set_render_tuning(chunk=tuned["chunk"], block=tuned["block"], waves=tuned["waves"],
-                                  loop=1 if tuned["loop"] == "while-while" else 0)
-        ctx.render(cam, seed=a.seed, spp=1)
+        ctx.render(cam, seed=a.seed)
         ctx.synchronize()
         t1 = time.perf_counter()
         ctx.render(cam, seed=a.seed)
         t_seq += time.perf_counter() - t1
+        ctx.trim()                  # one context's full-size queues at a time
     total = a.scenes * W * H * spp_eff
     print(json.dumps({"workload": f"motion x{a.scenes} (seeds 0..{a.scenes - 1})", "width": W, "height": H,
                       "spp": spp_eff, "scenes": a.scenes, "batched_s": round(t_batch, 3),

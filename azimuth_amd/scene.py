@@ -328,10 +328,13 @@ class GpuContext:
         return ms.value
 
     def render(self, camera, filt=N.FilterType_Box, cull=False, seed=0, spp=0, row_begin=0, row_end=0,
-               guide=False, guide_fraction=0.5, env_sampling=N.AZ_ENV_COSINE, guide_max_depth=0):
+               guide=False, guide_fraction=0.5, env_sampling=N.AZ_ENV_COSINE, guide_max_depth=0,
+               guide_train_spp=0, guide_chunk_spp=0):
         o = N.az_render_opts()
         o.EnvSampling = int(env_sampling)
         o.GuideMaxDepth = int(guide_max_depth)
+        o.GuideTrainSpp = int(guide_train_spp)
+        o.GuideChunkSpp = int(guide_chunk_spp)
         o.Seed = int(seed)
         o.SampleCountOverride = int(spp)
         o.RowBegin, o.RowEnd = int(row_begin), int(row_end)
@@ -357,8 +360,10 @@ class GpuContext:
         return self._tune_list(res)
 
     def autotune_render(self, camera, filt=N.FilterType_Box, cull=False, seed=0):
-        """Times a 1-spp render for every (chunk size, traversal variant) and
-        keeps the fastest; returns (entries, best index)."""
+        """Times a render (enough samples per pixel to fill the largest
+        candidate chunk twice, at most the camera's count) for every (chunk
+        size, traversal variant) and keeps the fastest; returns (entries,
+        best index)."""
         o = N.az_render_opts()
         o.Seed = int(seed)
         res = N.az_tune_result()
@@ -370,8 +375,12 @@ class GpuContext:
         N.check(self.lib.az_gpu_set_render_tuning(self.h, int(chunk), int(block), int(waves), int(loop)),
                 "az_gpu_set_render_tuning", self.h)
 
+    def trim(self):
+        """Frees the context's render queues (the scene stays uploaded)."""
+        N.check(self.lib.az_gpu_trim(self.h), "az_gpu_trim", self.h)
+
     def render_f64(self, camera, filt=N.FilterType_Box, seed=0, spp=0, guide=False, guide_fraction=0.5,
-                   env_sampling=N.AZ_ENV_COSINE, guide_max_depth=0):
+                   env_sampling=N.AZ_ENV_COSINE, guide_max_depth=0, guide_train_spp=0, guide_chunk_spp=0):
         """Device render accumulated in double precision; returns (W*H, 4)
         float64 (colour sums, weight sums).  The wall time of the render call
         (guide training included) is left in last_render_s."""
@@ -384,6 +393,8 @@ class GpuContext:
         o.AccumulateF64 = 1
         o.EnvSampling = int(env_sampling)
         o.GuideMaxDepth = int(guide_max_depth)
+        o.GuideTrainSpp = int(guide_train_spp)
+        o.GuideChunkSpp = int(guide_chunk_spp)
         o.GuideEnable = 1 if guide else 0
         o.GuideFraction = float(guide_fraction)
         self.synchronize()

@@ -1,6 +1,9 @@
 /*
  * azimuth_gpu.h -- C ABI of the MI355X (gfx950) render hot path.
  *
+ * Reference citations (file:line) name the reference's src/ files without
+ * their azimuth_ prefix: main.cpp is src/azimuth_main.cpp.
+ *
  * The reference has no plugin or FFI layer: its render path is entered once
  * per camera through
  *     static b32 RenderScene(scene *Scene, camera *Camera,
@@ -77,6 +80,10 @@ typedef struct {
                                different random directions) */
     u32 GuideMaxDepth;      /* path depth up to which the guide is used (0 = every depth); deeper
                                vertices use the base distribution alone (still unbiased) */
+    u32 GuideChunkSpp;      /* samples per pixel per training chunk (0: about 2M camera samples
+                               per chunk, at most GuideTrainSpp / 8); the guide is updated after
+                               each chunk */
+    u32 Reserved;           /* 0 */
 } az_render_opts;
 
 #define AZ_ENV_COSINE    0u
@@ -140,7 +147,8 @@ az_status az_gpu_render_device(az_gpu_ctx *Ctx, const camera *Camera, filter_typ
  * (az_gpu_upload_scene) and renders Cameras[k] into d_Pixels[k] (device
  * memory, Width*Height entries zeroed by the caller; doubles when
  * Opts->AccumulateF64).  All contexts must be on one device; the work is
- * ordered on Ctxs[0]'s stream and uses its render tuning.  Count <= 32.  The
+ * ordered on Ctxs[0]'s stream and uses its render tuning, the scenes sharing
+ * its chunk (at least one sample per pixel per scene).  Count <= 32.  The
  * learned guide is not available in batches (AZ_ERR_UNSUPPORTED).  No
  * reference counterpart: the reference renders one camera per call. */
 az_status az_gpu_render_batch(az_gpu_ctx *const *Ctxs, const camera *const *Cameras, u32 Count, filter_type Filter,
@@ -191,15 +199,24 @@ az_status az_gpu_autotune_trace(az_gpu_ctx *Ctx, const az_ray *d_Rays, u32 N, u3
 
 /* Renderer: times a render of Camera (Opts' rows and seed; guide off) at
  * enough samples per pixel to fill the largest candidate chunk twice (at
- * most the camera's count) for every (chunk size, traversal variant)
- * candidate; the pixels written are discarded. */
+ * most the camera's count) for every (chunk size 8M, 16M, 32M samples,
+ * traversal variant) candidate; the pixels written are discarded. */
 az_status az_gpu_autotune_render(az_gpu_ctx *Ctx, const camera *Camera, filter_type Filter, b32 BackfaceCulling,
                                  const az_render_opts *Opts, az_tune_result *Out);
-/* Sets the renderer's chunk size (camera samples; 0 keeps it) and its
- * traversal workgroup size / register target / loop form (0, -1 and -1 keep
- * them). */
+/* Sets the renderer's chunk size (camera samples; 0 keeps it; default 16M,
+ * whose queues take 3 GB of device memory per child a shading point can emit
+ * -- scatter, mirror, refraction, reflection -- plus 2 GB per light) and
+ * its traversal workgroup size / register target / loop form (0, -1 and -1
+ * keep them; default 128, 6, while-while). */
 az_status az_gpu_set_render_tuning(az_gpu_ctx *Ctx, u32 ChunkSamples, u32 TraceBlock, s32 TraceMinWaves,
                                    s32 TraceLoopForm);
+
+/* Waits for the context's work, then frees its render queues, pixel staging
+ * and learned-guide buffers (the next render allocates them again); the
+ * scene stays uploaded.  For callers that keep many contexts on one device.
+ * No reference counterpart: the reference frees its pixel buffer at the end
+ * of RenderScene (main.cpp:1899). */
+az_status az_gpu_trim(az_gpu_ctx *Ctx);
 
 /* Device memory helpers, ordered on the context stream. */
 az_status az_gpu_malloc(az_gpu_ctx *Ctx, void **d_Ptr, u64 Bytes);
