@@ -82,10 +82,26 @@ __device__ inline uint32_t wave_append(uint32_t *counter, bool want) {
     return base + below;
 }
 
-__device__ inline void add3(float *a, V3 c) {
-    atomicAdd(a + 0, c.x);
-    atomicAdd(a + 1, c.y);
-    atomicAdd(a + 2, c.z);
+// Per-sample colour accumulators are 64-bit fixed-point sums in units of
+// 2^-32.  The contributions of a sample's path-tree vertices and shadow
+// rays arrive from different lanes in no fixed order; integer addition makes
+// the sums, and so the image, independent of that order (a render is
+// reproducible bit for bit).  A contribution is rounded to the nearest unit
+// (2.3e-10) and saturated at +-2^30 (NaN counts as 0).
+typedef unsigned long long AccWord;
+constexpr double kAccUnit = 4294967296.0;   // units per 1.0
+
+__device__ inline AccWord acc_fixed(float v) {
+    const double x = fmin(fmax((double)v * kAccUnit, -0x1p62), 0x1p62);   // NaN: fmin/fmax return the other operand
+    return (AccWord)(long long)rint(x);
+}
+
+__device__ inline float acc_value(AccWord w) { return (float)((double)(long long)w / kAccUnit); }
+
+__device__ inline void add3(AccWord *a, V3 c) {
+    atomicAdd(a + 0, acc_fixed(c.x));
+    atomicAdd(a + 1, acc_fixed(c.y));
+    atomicAdd(a + 2, acc_fixed(c.z));
 }
 
 // ---- generate: camera rays (main.cpp:1135-1178) ----------------------------
@@ -93,7 +109,7 @@ __device__ inline void add3(float *a, V3 c) {
 // kernel (block = blockIdx.x) and as part of a batched launch over several
 // scenes, whose blocks each belong to one scene (block = index within it).
 __device__ __forceinline__ void generate_body(const DevScene &S, const CameraSetup &cam, const ChunkParams &cp,
-                                     PathRay *__restrict__ q, float *__restrict__ acc, uint32_t n, uint32_t block) {
+                                     PathRay *__restrict__ q, AccWord *__restrict__ acc, uint32_t n, uint32_t block) {
     uint32_t i = block * kRenderBlock + threadIdx.x;
     if (i >= n) return;
     uint32_t pixLocal = i / cp.spp, sl = i % cp.spp;
@@ -128,13 +144,13 @@ __device__ __forceinline__ void generate_body(const DevScene &S, const CameraSet
     r.cx = r.cy = r.cz = 0.0f;
     r.flags = 0u;
     q[i] = r;
-    acc[3 * i + 0] = 0.0f;
-    acc[3 * i + 1] = 0.0f;
-    acc[3 * i + 2] = 0.0f;
+    acc[3 * i + 0] = 0ull;
+    acc[3 * i + 1] = 0ull;
+    acc[3 * i + 2] = 0ull;
 }
 
 __global__ __launch_bounds__(kRenderBlock) void k_generate(DevScene S, CameraSetup cam, ChunkParams cp,
-                                                           PathRay *__restrict__ q, float *__restrict__ acc,
+                                                           PathRay *__restrict__ q, AccWord *__restrict__ acc,
                                                            uint32_t n) {
     generate_body(S, cam, cp, q, acc, n, blockIdx.x);
 }
@@ -206,7 +222,7 @@ struct ConnectPool {
     const uint32_t *off;
     SegIn in;
     const DeviceGeometry *G;
-    float *acc;
+    AccWord *acc;
     GuideSample *recs;
     __device__ inline const ShadowRay &at(uint32_t id) const { return sq[seg_slot(off, in.nseg, in.segCap, id)]; }
     __device__ inline bool fetch(uint32_t id, Ray3 &r, float &tMax) {
@@ -232,7 +248,7 @@ struct ConnectPool {
 
 template <int BLOCK, bool WW>
 __device__ __forceinline__ void connect_body(const DeviceGeometry &G, const ShadowRay *__restrict__ sq, SegIn sin, uint32_t n,
-                                    float *__restrict__ acc, GuideSample *__restrict__ recs, uint32_t block) {
+                                    AccWord *__restrict__ acc, GuideSample *__restrict__ recs, uint32_t block) {
     __shared__ uint32_t smem[kRenderRing * BLOCK];
     __shared__ uint32_t soff[kSegs + 1];
     const uint32_t *off = stage_offsets(sin, soff);
@@ -245,7 +261,7 @@ __device__ __forceinline__ void connect_body(const DeviceGeometry &G, const Shad
 
 template <int BLOCK, int WAVES, bool WW>
 __global__ __launch_bounds__(BLOCK, WAVES) void k_connect(DeviceGeometry G, const ShadowRay *__restrict__ sq, SegIn sin,
-                                                         uint32_t n, float *__restrict__ acc,
+                                                         uint32_t n, AccWord *__restrict__ acc,
                                                          GuideSample *__restrict__ recs) {
     connect_body<BLOCK, WW>(G, sq, sin, n, acc, recs, blockIdx.x);
 }
@@ -264,7 +280,7 @@ static hipError_t launch_extend(const RenderTuning &T, const DeviceGeometry &G, 
 }
 
 static hipError_t launch_connect(const RenderTuning &T, const DeviceGeometry &G, const ShadowRay *sq, SegIn sin,
-                                 uint32_t n, float *acc, GuideSample *recs, hipStream_t st) {
+                                 uint32_t n, AccWord *acc, GuideSample *recs, hipStream_t st) {
 #define AZ_CASE(B, W, F)                                                                                  \
     if (T.traceBlock == B && T.traceWaves == W && T.traceLoop == F) {                                   \
         hipLaunchKernelGGL((k_connect<B, W, F != 0>), dim3((n + B * kRaysPerLane - 1) / (B * kRaysPerLane)), \
@@ -566,7 +582,7 @@ __device__ inline void reconstruct(const DevScene &S, const DeviceGeometry &G, c
 
 __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeometry &G, const CameraSetup &cam,
                                   const ChunkParams &cp, const PathRay *__restrict__ q, SegIn qin,
-                                  const PathHit *__restrict__ hits, uint32_t n, float *__restrict__ acc,
+                                  const PathHit *__restrict__ hits, uint32_t n, AccWord *__restrict__ acc,
                                   const ShadeOut &O, const GuideIO &GI, uint32_t block) {
     __shared__ uint32_t soff[kSegs + 1];
     __shared__ uint32_t cls[kShadeClasses];
@@ -603,7 +619,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
     const uint32_t pixLocal = r.sample / cp.spp;
     const uint32_t pixel = cp.row0 * cam.width + pixLocal;
     const uint32_t s = cp.s0 + r.sample % cp.spp;
-    float *a = acc + 3 * r.sample;
+    AccWord *a = acc + 3 * r.sample;
     V3 w = v3(r.wx, r.wy, r.wz);
     V3 D = v3(r.dx, r.dy, r.dz);
 
@@ -909,7 +925,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
 __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeometry G, CameraSetup cam, ChunkParams cp,
                                                         const PathRay *__restrict__ q, SegIn qin,
                                                         const PathHit *__restrict__ hits, uint32_t n,
-                                                        float *__restrict__ acc, ShadeOut O, GuideIO GI) {
+                                                        AccWord *__restrict__ acc, ShadeOut O, GuideIO GI) {
     shade_body(S, G, cam, cp, q, qin, hits, n, acc, O, GI, blockIdx.x);
 }
 
@@ -1006,17 +1022,17 @@ __global__ void k_clear_u32(uint32_t *p, uint32_t n) {
 
 // ---- splat (main.cpp:1209-1254) -------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ void splat_box_body(const CameraSetup &cam, const ChunkParams &cp, const float *__restrict__ acc,
+__device__ __forceinline__ void splat_box_body(const CameraSetup &cam, const ChunkParams &cp, const AccWord *__restrict__ acc,
                                       T *__restrict__ px, uint32_t block) {
     uint32_t p = block * kRenderBlock + threadIdx.x;
     if (p >= cp.rows * cam.width) return;
     uint32_t pixel = cp.row0 * cam.width + p;
     T cr = px[4 * pixel], cg = px[4 * pixel + 1], cb = px[4 * pixel + 2], cw = px[4 * pixel + 3];
     for (uint32_t k = 0; k < cp.spp; ++k) {
-        const float *c = acc + 3 * ((size_t)p * cp.spp + k);
-        cr += (T)c[0];
-        cg += (T)c[1];
-        cb += (T)c[2];
+        const AccWord *c = acc + 3 * ((size_t)p * cp.spp + k);
+        cr += (T)acc_value(c[0]);
+        cg += (T)acc_value(c[1]);
+        cb += (T)acc_value(c[2]);
         cw += (T)1.0f;
     }
     px[4 * pixel] = cr;
@@ -1026,7 +1042,7 @@ __device__ __forceinline__ void splat_box_body(const CameraSetup &cam, const Chu
 }
 
 template <typename T>
-__global__ __launch_bounds__(kRenderBlock) void k_splat_box(CameraSetup cam, ChunkParams cp, const float *__restrict__ acc,
+__global__ __launch_bounds__(kRenderBlock) void k_splat_box(CameraSetup cam, ChunkParams cp, const AccWord *__restrict__ acc,
                                                             T *__restrict__ px) {
     splat_box_body<T>(cam, cp, acc, px, blockIdx.x);
 }
@@ -1045,7 +1061,7 @@ __host__ __device__ inline uint32_t splat_gauss_blocks(uint32_t width, uint32_t 
 }
 
 template <typename T>
-__device__ __forceinline__ void splat_gauss_body(const CameraSetup &cam, const ChunkParams &cp, const float *__restrict__ acc,
+__device__ __forceinline__ void splat_gauss_body(const CameraSetup &cam, const ChunkParams &cp, const AccWord *__restrict__ acc,
                                         T *__restrict__ px, uint32_t dstRow0, uint32_t dstRows, uint32_t block) {
     __shared__ T part[kRenderBlock][9][4];
     const uint32_t tilesX = (cam.width + kSplatDst - 1) / kSplatDst;
@@ -1057,13 +1073,13 @@ __device__ __forceinline__ void splat_gauss_body(const CameraSetup &cam, const C
 #pragma unroll
     for (int j = 0; j < 9; ++j) sum[j][0] = sum[j][1] = sum[j][2] = sum[j][3] = (T)0.0f;
     if (row >= r0 && row <= r1 && col >= 0 && col < (int32_t)cam.width) {
-        const float *c = acc + 3 * ((size_t)((uint32_t)(row - r0) * cam.width + (uint32_t)col) * cp.spp);
+        const AccWord *c = acc + 3 * ((size_t)((uint32_t)(row - r0) * cam.width + (uint32_t)col) * cp.spp);
         for (uint32_t k = 0; k < cp.spp; ++k) {
             const uint32_t s = cp.s0 + k;
             const uint32_t sx = s % cam.sppSqrt, sy = s / cam.sppSqrt;
             const float scx = col + (0.5f * cam.invSppSqrt + sx * cam.invSppSqrt);
             const float scy = row + (0.5f * cam.invSppSqrt + sy * cam.invSppSqrt);
-            const float cr = c[3 * k], cg = c[3 * k + 1], cb = c[3 * k + 2];
+            const float cr = acc_value(c[3 * k]), cg = acc_value(c[3 * k + 1]), cb = acc_value(c[3 * k + 2]);
 #pragma unroll
             for (int oy = -1; oy <= 1; ++oy)
 #pragma unroll
@@ -1106,7 +1122,7 @@ __device__ __forceinline__ void splat_gauss_body(const CameraSetup &cam, const C
 
 template <typename T>
 __global__ __launch_bounds__(kRenderBlock) void k_splat_gauss(CameraSetup cam, ChunkParams cp,
-                                                              const float *__restrict__ acc, T *__restrict__ px,
+                                                              const AccWord *__restrict__ acc, T *__restrict__ px,
                                                               uint32_t dstRow0, uint32_t dstRows) {
     splat_gauss_body<T>(cam, cp, acc, px, dstRow0, dstRows, blockIdx.x);
 }
@@ -1133,7 +1149,7 @@ struct GenArgs {
     CameraSetup cam;
     ChunkParams cp;
     PathRay *q;
-    float *acc;
+    AccWord *acc;
     uint32_t n;
 };
 
@@ -1156,7 +1172,7 @@ struct ShadeArgs {
     SegIn qin;
     const PathHit *hits;
     uint32_t n;
-    float *acc;
+    AccWord *acc;
     ShadeOut O;
     GuideIO GI;
 };
@@ -1166,14 +1182,14 @@ struct ConnArgs {
     const ShadowRay *sq;
     SegIn sin;
     uint32_t n;
-    float *acc;
+    AccWord *acc;
     GuideSample *recs;
 };
 
 struct SplatArgs {
     CameraSetup cam;
     ChunkParams cp;
-    const float *acc;
+    const AccWord *acc;
     void *px;
     uint32_t dstRow0, dstRows;
 };
@@ -1767,12 +1783,12 @@ static bool out_of_memory(hipError_t e) {
 static hipError_t reserve_chunk(RenderBuffers &B, u32 n, u32 maxChildren, u32 shadowPerRay, size_t budget) {
     const u32 m = maxChildren > 1 ? n + n / 2 : n;
     const LevelCaps L = level_caps(m, maxChildren, shadowPerRay);
-    const size_t need = 2 * L.cap * sizeof(PathRay) + (size_t)m * (sizeof(PathHit) + 3 * sizeof(float)) +
+    const size_t need = 2 * L.cap * sizeof(PathRay) + (size_t)m * (sizeof(PathHit) + 3 * sizeof(AccWord)) +
                         L.shadowCap * sizeof(ShadowRay);
     if (need > budget) return hipSuccess;
     hipError_t e = B.q0.alloc(L.cap * sizeof(PathRay));
     if (e == hipSuccess) e = B.q1.alloc(L.cap * sizeof(PathRay));
-    if (e == hipSuccess) e = B.acc.alloc((size_t)n * 3 * sizeof(float));
+    if (e == hipSuccess) e = B.acc.alloc((size_t)n * 3 * sizeof(AccWord));
     if (e == hipSuccess) e = B.hits.alloc((size_t)m * sizeof(PathHit));
     if (e == hipSuccess) e = B.shadow.alloc(L.shadowCap * sizeof(ShadowRay));
     return out_of_memory(e) ? hipSuccess : e;
@@ -1862,7 +1878,7 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
         if (training) spp = std::min(spp, std::max<u32>(1u, std::min(trainSpp - s0, trainChunkSpp)));
         u32 n = nPix * spp;
         hipError_t ae = B.q0.alloc((size_t)n * sizeof(PathRay));
-        if (ae == hipSuccess) ae = B.acc.alloc((size_t)n * 3 * sizeof(float));
+        if (ae == hipSuccess) ae = B.acc.alloc((size_t)n * 3 * sizeof(AccWord));
         if (out_of_memory(ae) && chunkSpp > 1) {
             chunkSpp = std::max<u32>(1u, chunkSpp / 2);
             continue;
@@ -1882,7 +1898,7 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
         cp.envLuminance = envLum ? 1u : 0u;
         DeviceBuffer *bufA = &B.q0, *bufB = &B.q1;
         int offA = 0;
-        float *acc = (float *)B.acc.ptr;
+        AccWord *acc = (AccWord *)B.acc.ptr;
         hipLaunchKernelGGL(k_generate, dim3((n + kRenderBlock - 1) / kRenderBlock), dim3(kRenderBlock), 0, st, S, cam,
                            cp, (PathRay *)bufA->ptr, acc, n);
         AZ_HIPR(C, hipGetLastError());
@@ -2149,7 +2165,7 @@ static az_status renderBatchImpl(az_gpu_ctx *const *ctxs, const camera *const *c
             b.spp = std::min(b.chunkSpp, b.sppTotal - b.s0);
             const u32 n = b.nPix * b.spp;
             AZ_HIPR(C0, b.B->q0.alloc((size_t)n * sizeof(PathRay)));
-            AZ_HIPR(C0, b.B->acc.alloc((size_t)n * 3 * sizeof(float)));
+            AZ_HIPR(C0, b.B->acc.alloc((size_t)n * 3 * sizeof(AccWord)));
             memset(&b.cp, 0, sizeof(b.cp));
             b.cp.seed = o.Seed;
             b.cp.row0 = b.row0;
@@ -2171,7 +2187,7 @@ static az_status renderBatchImpl(az_gpu_ctx *const *ctxs, const camera *const *c
             g.cam = b.cam;
             g.cp = b.cp;
             g.q = (PathRay *)b.bufA->ptr;
-            g.acc = (float *)b.B->acc.ptr;
+            g.acc = (AccWord *)b.B->acc.ptr;
             g.n = n;
             gen.push_back(g);
             gm.start[j] = gblocks;
@@ -2234,7 +2250,7 @@ static az_status renderBatchImpl(az_gpu_ctx *const *ctxs, const camera *const *c
                 a.qin = b.qin;
                 a.hits = (const PathHit *)b.B->hits.ptr;
                 a.n = b.count;
-                a.acc = (float *)b.B->acc.ptr;
+                a.acc = (AccWord *)b.B->acc.ptr;
                 a.O = ShadeOut{(PathRay *)b.bufB->ptr, (ShadowRay *)b.B->shadow.ptr, b.childCnt, b.shadowCnt,
                                b.nseg, b.childSegCap, b.shadowSegCap};
                 a.GI = GuideIO{nullptr, nullptr, b.childCnt, 0u, 0.5f};
@@ -2281,7 +2297,7 @@ static az_status renderBatchImpl(az_gpu_ctx *const *ctxs, const camera *const *c
                     c.sq = (const ShadowRay *)b.B->shadow.ptr;
                     c.sin = SegIn{b.offShadow, b.nseg, b.shadowSegCap};
                     c.n = ns;
-                    c.acc = (float *)b.B->acc.ptr;
+                    c.acc = (AccWord *)b.B->acc.ptr;
                     c.recs = nullptr;
                     cm.start[con.size()] = cblocks;
                     cblocks += (ns + perExt - 1) / perExt;
@@ -2312,7 +2328,7 @@ static az_status renderBatchImpl(az_gpu_ctx *const *ctxs, const camera *const *c
             SplatArgs a;
             a.cam = b.cam;
             a.cp = b.cp;
-            a.acc = (const float *)b.B->acc.ptr;
+            a.acc = (const AccWord *)b.B->acc.ptr;
             a.px = b.px;
             u32 blocks = (b.nPix + kRenderBlock - 1) / kRenderBlock;
             a.dstRow0 = b.row0;
