@@ -1,0 +1,27 @@
+"""The device renderer against the closed forms of analytic_pins.py: the
+Gaussian pixel filter's weights, Fresnel and Beer through a glass slab, and
+the pinhole camera's pixel-centre rays."""
+import pytest
+
+import analytic_pins as P
+
+pytestmark = pytest.mark.gpu
+
+
+def make_render(gpu):
+    def render(scene, camera, filt):
+        gpu.upload_scene(scene)
+        return gpu.render(camera, filt=filt, seed=1)
+    return render
+
+
+def test_gaussian_filter_weights(gpu):
+    P.check_gaussian(make_render(gpu))
+
+
+def test_dielectric_slab_fresnel_and_beer(gpu):
+    P.check_slab(make_render(gpu))
+
+
+def test_pinhole_camera_pixel_centres(gpu):
+    P.check_camera(make_render(gpu))

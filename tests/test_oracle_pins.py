@@ -1,0 +1,20 @@
+"""Closed forms that pin the oracle's pixel filter, dielectric and camera
+(the analytic scenes and expectations are in analytic_pins.py)."""
+import oracle
+import analytic_pins as P
+
+
+def render(scene, camera, filt):
+    return oracle.render(scene, camera, filt=filt, seed=1)
+
+
+def test_gaussian_filter_weights():
+    P.check_gaussian(render)
+
+
+def test_dielectric_slab_fresnel_and_beer():
+    P.check_slab(render)
+
+
+def test_pinhole_camera_pixel_centres():
+    P.check_camera(render)
