@@ -17,6 +17,12 @@ derived from the reference's definitions, not from either implementation:
   f + (2 (i + 0.5) / W - 1) tan(fovy / 2) (W / H) r + (1 - 2 (j + 0.5) / H) tan(fovy / 2) u
   for the orthonormal frame f (forward), r = f x up, u = r x f; row 0 is the
   top of the image.
+- Sphere light (SampleEntityLight, main.cpp:392-459, and
+  CalculateLightContribution, main.cpp:571-603): directions uniform in the
+  subtended cone, InvPDF 2 pi (1 - cos theta_max), light distance 1.
+- Hemisphere scatter (main.cpp:1029-1083): cosine-weighted (InvPDF pi / cos)
+  with importance sampling, else uniform (InvPDF 2 pi), weighted by the BRDF
+  and cos.
 """
 import math
 
@@ -191,3 +197,97 @@ def check_camera(render):
     for (i, j) in CAMERA_PIXELS:
         want[j, i] = True
     assert np.array_equal(lit, want), np.argwhere(lit ^ want)
+
+
+# ---------------------------------------------------------------- sphere light (cone sampling)
+
+FLOOR_KD = (0.6, 0.5, 0.4)
+
+
+def _floor(b, material):
+    v = np.array([(-50, 0, -50), (50, 0, -50), (50, 0, 50), (-50, 0, 50)], np.float32)
+    b.add_mesh(v, np.array([[0, 2, 1], [0, 3, 2]], np.uint32), material=material)
+
+
+SPHERE_LIGHT = dict(center=(0.0, 2.0, 0.0), radius=0.2, radiance=(5.0, 4.0, 3.0))
+
+
+def sphere_light_scene():
+    """A Kd/pi floor under an emitting sphere (an entity light), next-event
+    estimation only."""
+    b = SceneBuilder(max_depth=0)
+    floor = b.add_material(make_material(kd=FLOOR_KD))
+    dark = b.add_material(make_material(kd=(0, 0, 0)))
+    _floor(b, floor)
+    b.add_sphere(SPHERE_LIGHT["center"], SPHERE_LIGHT["radius"], material=dark, emitter=True,
+                 radiance=SPHERE_LIGHT["radiance"])
+    return b.build()
+
+
+def check_sphere_light(render):
+    """SampleEntityLight draws directions uniformly in the cone that the
+    sphere subtends (InvPDF 2 pi (1 - cos theta_max)), with light distance 1
+    in CalculateLightContribution: the mean of Kd/pi L cos InvPDF over the
+    cone is Kd L sin^2(theta_max) (n . w), the irradiance of the sphere."""
+    W = H = 3
+    cam = configs.look_at_camera((0, 1, 0), (0, 0, 0), (0, 0, 1), W, H, 2.0, spp=64)
+    px = render(sphere_light_scene(), cam, N.FilterType_Box)
+    got = px["Color"] / px["TotalWeight"][:, None]
+    h = math.tan(math.radians(1.0))
+    c = np.asarray(SPHERE_LIGHT["center"])
+    for j in range(H):
+        for i in range(W):
+            # Pixel centre ray from (0, 1, 0) down to the floor.
+            f, r, u = camera_frame((0, 1, 0), (0, 0, 0), (0, 0, 1))
+            d = f + (2.0 * (i + 0.5) / W - 1.0) * h * r + (1.0 - 2.0 * (j + 0.5) / H) * h * u
+            p = np.array([0.0, 1.0, 0.0]) + d * (1.0 / -d[1])
+            dist = np.linalg.norm(c - p)
+            want = np.asarray(FLOOR_KD) * np.asarray(SPHERE_LIGHT["radiance"]) * \
+                (SPHERE_LIGHT["radius"] / dist) ** 2 * ((c - p)[1] / dist)
+            np.testing.assert_allclose(got[j * W + i], want, rtol=3e-3, err_msg=f"pixel {i},{j}")
+
+
+# ---------------------------------------------------------------- hemisphere scatter
+
+CEILING_RADIANCE = (3.0, 2.0, 1.0)
+
+
+def scatter_scene(path_trace, importance):
+    """A Kd/pi floor under an emitting 2 x 2 quad at height 1 (facing down),
+    depth 1 so that a scatter ray ends at the quad or in the sky."""
+    b = SceneBuilder(max_depth=1, path_trace=path_trace, importance_sampling=importance)
+    floor = b.add_material(make_material(kd=FLOOR_KD))
+    dark = b.add_material(make_material(kd=(0, 0, 0)))
+    _floor(b, floor)
+    q = np.array([(-1, 1, -1), (1, 1, -1), (1, 1, 1), (-1, 1, 1)], np.float32)
+    b.add_mesh(q, np.array([[0, 1, 2], [0, 2, 3]], np.uint32), material=dark, emitter=True,
+               radiance=CEILING_RADIANCE)
+    return b.build()
+
+
+def view_factor_square(half=1.0, height=1.0, n=800):
+    """(1/pi) int cos cos' / d^2 dA from the point below the centre of a
+    square of side 2 half at the given height (midpoint rule)."""
+    g = (np.arange(n) + 0.5) / n * 2.0 * half - half
+    x, z = np.meshgrid(g, g)
+    d2 = x * x + z * z + height * height
+    return float(np.sum(height * height / d2 ** 2) * (2.0 * half / n) ** 2 / math.pi)
+
+
+def check_scatter(render):
+    """The hemisphere scatter (cosine-weighted with importance sampling, else
+    uniform; main.cpp:1029-1083) reaches the quad with weight Kd for every
+    cosine sample that hits it, 2 Kd cos for a uniform one: either way its
+    mean is Kd L F, F the quad's view factor.  The scatter part is isolated
+    by subtracting the same render without path tracing (the next-event
+    draws come first in a vertex, so they are the same in both)."""
+    W = H = 8
+    cam = configs.look_at_camera((0, 0.5, 0), (0, 0, 0), (0, 0, 1), W, H, 2.0, spp=1024)
+    want = np.asarray(FLOOR_KD) * np.asarray(CEILING_RADIANCE) * view_factor_square()
+    for importance in (True, False):
+        on = render(scatter_scene(True, importance), cam, N.FilterType_Box)
+        off = render(scatter_scene(False, importance), cam, N.FilterType_Box)
+        part = (on["Color"] - off["Color"]) / on["TotalWeight"][:, None]
+        # 65,536 scatter samples: Bernoulli(F ~ 0.55) noise of 0.35 % (cosine),
+        # about 0.5 % (uniform).
+        np.testing.assert_allclose(part.mean(0), want, rtol=0.02, err_msg=f"importance={importance}")

@@ -25,3 +25,11 @@ def test_dielectric_slab_fresnel_and_beer(gpu):
 
 def test_pinhole_camera_pixel_centres(gpu):
     P.check_camera(make_render(gpu))
+
+
+def test_sphere_light_cone_sampling(gpu):
+    P.check_sphere_light(make_render(gpu))
+
+
+def test_hemisphere_scatter_samplers(gpu):
+    P.check_scatter(make_render(gpu))

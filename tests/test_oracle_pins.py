@@ -18,3 +18,11 @@ def test_dielectric_slab_fresnel_and_beer():
 
 def test_pinhole_camera_pixel_centres():
     P.check_camera(render)
+
+
+def test_sphere_light_cone_sampling():
+    P.check_sphere_light(render)
+
+
+def test_hemisphere_scatter_samplers():
+    P.check_scatter(render)
