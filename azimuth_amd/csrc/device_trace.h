@@ -150,15 +150,31 @@ struct NodeHits {
     uint32_t ref[4];
 };
 
+// Byte offsets, within a node, of the near-plane float4 of each axis for a
+// ray with inverse direction i: the node holds lo x, hi x, lo y, hi y, lo z,
+// hi z (16 bytes each), and the far plane of an axis sits at near ^ 16.  A
+// ray's signs do not change during a walk, so the near/far choice is made
+// once per ray (and instance) as an address instead of per node as selects.
+struct NearPlanes {
+    uint32_t x, y, z;
+};
+
+__device__ inline NearPlanes near_planes(float ix, float iy, float iz) {
+    return NearPlanes{ix >= 0.0f ? 0u : 16u, iy >= 0.0f ? 32u : 48u, iz >= 0.0f ? 64u : 80u};
+}
+
+// Node byte offsets are 32-bit: the upload keeps BVHs below 2^25 nodes.
 __device__ inline int node_test(const Node4 *__restrict__ nodes, uint32_t idx, float ox, float oy,
-                                float oz, float ix, float iy, float iz, float tclip, NodeHits &h) {
-    const float4 *p = reinterpret_cast<const float4 *>(nodes + idx);
-    float4 lx = p[0], hx = p[1], ly = p[2], hy = p[3], lz = p[4], hz = p[5];
-    uint4 ch = reinterpret_cast<const uint4 *>(p)[6];
-    const bool sx = ix >= 0.0f, sy = iy >= 0.0f, sz = iz >= 0.0f;
-    float4 nx = sx ? lx : hx, fx = sx ? hx : lx;
-    float4 ny = sy ? ly : hy, fy = sy ? hy : ly;
-    float4 nz = sz ? lz : hz, fz = sz ? hz : lz;
+                                float oz, float ix, float iy, float iz, NearPlanes np, float tclip, NodeHits &h) {
+    const char *base = reinterpret_cast<const char *>(nodes);
+    const uint32_t o = idx * (uint32_t)sizeof(Node4);
+    const float4 nx = *reinterpret_cast<const float4 *>(base + (o + np.x));
+    const float4 fx = *reinterpret_cast<const float4 *>(base + (o + (np.x ^ 16u)));
+    const float4 ny = *reinterpret_cast<const float4 *>(base + (o + np.y));
+    const float4 fy = *reinterpret_cast<const float4 *>(base + (o + (np.y ^ 16u)));
+    const float4 nz = *reinterpret_cast<const float4 *>(base + (o + np.z));
+    const float4 fz = *reinterpret_cast<const float4 *>(base + (o + (np.z ^ 16u)));
+    const uint4 ch = *reinterpret_cast<const uint4 *>(base + (o + 96u));
     float tn[4], tf[4];
 #define AZ_SLAB(k, c)                                                                   \
     {                                                                                   \
@@ -175,9 +191,12 @@ __device__ inline int node_test(const Node4 *__restrict__ nodes, uint32_t idx, f
 #undef AZ_SLAB
     uint32_t cr[4] = {ch.x, ch.y, ch.z, ch.w};
     int n = 0;
+    // An empty slot's box (lo = +inf, hi = -inf, as the builder writes it)
+    // gives tn = +inf and tf = -inf whatever the ray's signs, so it never
+    // passes this test and needs no check of its reference.
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        bool hit = tn[k] <= tf[k] && cr[k] != kEmpty;
+        bool hit = tn[k] <= tf[k];
         h.tn[k] = hit ? tn[k] : INFINITY;
         h.ref[k] = cr[k];
         n += hit ? 1 : 0;
@@ -260,6 +279,7 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
     HitRec out{};
     // Current walk ray: world ray in the TLAS, instance-local ray in a BLAS.
     float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, ix = 0.f, iy = 0.f, iz = 0.f;
+    NearPlanes np{0u, 32u, 64u};
     // Instance state while inside a BLAS.
     bool inBlas = false;
     uint32_t curItem = 0;
@@ -300,6 +320,7 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
                 ix = 1.0f / dx;
                 iy = 1.0f / dy;
                 iz = 1.0f / dz;
+                np = near_planes(ix, iy, iz);
                 inBlas = false;
                 localBest = kFltMax;
                 localPrim = kEmpty;
@@ -320,7 +341,7 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
             if (COUNT) cnt.nodes++;
             float tclip = inBlas ? localBest : bestW;
             NodeHits h;
-            int n = node_test(G.nodes, cur, ox, oy, oz, ix, iy, iz, tclip, h);
+            int n = node_test(G.nodes, cur, ox, oy, oz, ix, iy, iz, np, tclip, h);
             if (n > 0) {
                 cur = h.ref[0];
                 if (n > 3) stack.push(sp, h.ref[3]);
@@ -363,11 +384,12 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
             ix = 1.0f / dx;
             iy = 1.0f / dy;
             iz = 1.0f / dz;
+            np = near_planes(ix, iy, iz);
         } else if (!WW && !ref_is_leaf(cur)) {
             if (COUNT) cnt.nodes++;
             float tclip = inBlas ? localBest : bestW;
             NodeHits h;
-            int n = node_test(G.nodes, cur, ox, oy, oz, ix, iy, iz, tclip, h);
+            int n = node_test(G.nodes, cur, ox, oy, oz, ix, iy, iz, np, tclip, h);
             if (n > 0) {
                 cur = h.ref[0];
                 if (n > 3) stack.push(sp, h.ref[3]);
@@ -405,6 +427,7 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
                 ix = 1.0f / dx;
                 iy = 1.0f / dy;
                 iz = 1.0f / dz;
+                np = near_planes(ix, iy, iz);
                 inBlas = true;
                 curItem = itemIdx;
                 // Local distances scale with |M^-1 d|; the bound is widened so

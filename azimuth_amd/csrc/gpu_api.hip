@@ -243,6 +243,11 @@ az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
     info.stackNeeded = ts.maxStack + 1 + info.maxBlasStack;
 
     // ---- upload ---------------------------------------------------------------
+    // Traversal addresses nodes with 32-bit byte offsets.
+    if (nodes.size() >= ((size_t)1 << 25)) {
+        C->lastError = "BVH exceeds 2^25 nodes (4 GB of nodes)";
+        return AZ_ERR_UNSUPPORTED;
+    }
     C->geom.free();
     AZ_HIP(C, C->geom.nodes.alloc(nodes.size() * sizeof(Node4)));
     AZ_HIP(C, C->geom.tris.alloc(std::max<size_t>(1, tris.size()) * sizeof(TriPacked)));
