@@ -311,6 +311,8 @@ az_status az_gpu_create(az_gpu_ctx **Out, int dev) {
         delete c;
         return AZ_ERR_DEVICE;
     }
+    if (hipDeviceGetAttribute(&c->cuCount, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) c->cuCount = 256;
+    c->cuCount = std::max(1, c->cuCount);
     memset(&c->stats, 0, sizeof(c->stats));
     *Out = c;
     return AZ_OK;
@@ -365,7 +367,7 @@ static az_status traceCommon(az_gpu_ctx *C, const az_ray *rays, u32 n, az_hit *h
     }
     AZ_HIP(C, hipEventRecord(e0, C->stream));
     AZ_HIP(C, launch_trace(C->launch, any, C->counting, C->dg, rays, n, hits, C->eps.Intersection, cull ? 1 : 0, cnt,
-                           C->stream));
+                           C->cuCount, C->stream));
     AZ_HIP(C, hipEventRecord(e1, C->stream));
     C->lastStart = e0;   // LastKernelMs reads the latest pair
     C->lastStop = e1;
@@ -451,11 +453,11 @@ az_status az_gpu_autotune_trace(az_gpu_ctx *C, const az_ray *rays, u32 n, u32 re
     memset(&res, 0, sizeof(res));
     for (int i = 0; i < k && st == AZ_OK; ++i) {
         hipError_t e = launch_trace(cands[i], false, false, C->dg, rays, n, (az_hit *)hits.ptr, C->eps.Intersection, 0,
-                                    nullptr, C->stream);
+                                    nullptr, C->cuCount, C->stream);
         if (e == hipSuccess) e = hipEventRecord(a, C->stream);
         for (u32 r = 0; r < repeats && e == hipSuccess; ++r)
             e = launch_trace(cands[i], false, false, C->dg, rays, n, (az_hit *)hits.ptr, C->eps.Intersection, 0,
-                             nullptr, C->stream);
+                             nullptr, C->cuCount, C->stream);
         if (e == hipSuccess) e = hipEventRecord(b, C->stream);
         if (e == hipSuccess) e = hipEventSynchronize(b);
         float ms = 0.0f;

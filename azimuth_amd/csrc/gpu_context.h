@@ -70,6 +70,7 @@ az_status az_guide_train_on_stream(az_guide *G, const void *d_Samples, u32 N, hi
 
 struct az_gpu_ctx {
     int device = 0;
+    int cuCount = 256;               // compute units of the device
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, tev0 = nullptr, tev1 = nullptr;
     hipEvent_t lastStart = nullptr, lastStop = nullptr;   // not owned

@@ -37,11 +37,23 @@ struct RenderTuning {
     int traceLoop = 1;   // while-while
 };
 
+// Rays per wave of a traversal launch of n rays: 64 x kRaysPerLane (each
+// lane walks several, refilled from the wave's range) when n fills at least
+// two rounds of the device's resident waves, else halved down to 64 (one per
+// lane), so that small launches still reach every SIMD.  `waves` is the
+// register target in waves per SIMD (0: the compiler's choice, taken as 5).
+inline uint32_t rays_per_wave(uint32_t n, int cuCount, int waves) {
+    const uint64_t resident = (uint64_t)cuCount * 4u * (uint64_t)(waves > 0 ? waves : 5);
+    uint32_t per = 64u * kRaysPerLane;
+    while (per > 64u && (uint64_t)n < 2u * resident * per) per >>= 1;
+    return per;
+}
+
 bool render_trace_variant_exists(int block, int waves, int loop);
 int render_trace_variants(RenderTuning *out, int cap);   // chunkSamples left at its default
 
 hipError_t launch_trace(const TraceLaunch &L, bool any, bool count, const DeviceGeometry &G, const az_ray *rays,
-                        uint32_t n, az_hit *hits, float eps, int cull, unsigned long long *counters,
+                        uint32_t n, az_hit *hits, float eps, int cull, unsigned long long *counters, int cuCount,
                         hipStream_t s);
 
 hipError_t launch_copy16(const void *src, void *dst, size_t bytes, hipStream_t s);

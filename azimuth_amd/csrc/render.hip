@@ -195,12 +195,12 @@ struct ExtendPool {
 
 template <int BLOCK, bool WW>
 __device__ __forceinline__ void extend_body(const DeviceGeometry &G, const PathRay *__restrict__ q, SegIn qin, uint32_t n,
-                                   PathHit *__restrict__ hits, float eps, int cull, uint32_t block) {
+                                   PathHit *__restrict__ hits, float eps, int cull, uint32_t perWave, uint32_t block) {
     __shared__ uint32_t smem[kRenderRing * BLOCK];
     __shared__ uint32_t soff[kSegs + 1];
     const uint32_t *off = stage_offsets(qin, soff);
-    const uint32_t begin = (block * (BLOCK / 64) + (threadIdx.x >> 6)) * 64u * kRaysPerLane;
-    const uint32_t end = min(begin + 64u * kRaysPerLane, n);
+    const uint32_t begin = (block * (BLOCK / 64) + (threadIdx.x >> 6)) * perWave;
+    const uint32_t end = min(begin + perWave, n);
     ExtendPool pool{q, off, qin, hits};
     TraceCounters c{0u, 0u, 0u};
     trace_pool<0, false, kRenderRing, BLOCK, WW>(G, pool, begin, end, eps, cull != 0, smem + threadIdx.x, c);
@@ -208,8 +208,9 @@ __device__ __forceinline__ void extend_body(const DeviceGeometry &G, const PathR
 
 template <int BLOCK, int WAVES, bool WW>
 __global__ __launch_bounds__(BLOCK, WAVES) void k_extend(DeviceGeometry G, const PathRay *__restrict__ q, SegIn qin,
-                                                        uint32_t n, PathHit *__restrict__ hits, float eps, int cull) {
-    extend_body<BLOCK, WW>(G, q, qin, n, hits, eps, cull, blockIdx.x);
+                                                        uint32_t n, PathHit *__restrict__ hits, float eps, int cull,
+                                                        uint32_t perWave) {
+    extend_body<BLOCK, WW>(G, q, qin, n, hits, eps, cull, perWave, blockIdx.x);
 }
 
 // ---- connect: shadow rays ------------------------------------------------------
@@ -248,12 +249,13 @@ struct ConnectPool {
 
 template <int BLOCK, bool WW>
 __device__ __forceinline__ void connect_body(const DeviceGeometry &G, const ShadowRay *__restrict__ sq, SegIn sin, uint32_t n,
-                                    AccWord *__restrict__ acc, GuideSample *__restrict__ recs, uint32_t block) {
+                                    AccWord *__restrict__ acc, GuideSample *__restrict__ recs, uint32_t perWave,
+                                    uint32_t block) {
     __shared__ uint32_t smem[kRenderRing * BLOCK];
     __shared__ uint32_t soff[kSegs + 1];
     const uint32_t *off = stage_offsets(sin, soff);
-    const uint32_t begin = (block * (BLOCK / 64) + (threadIdx.x >> 6)) * 64u * kRaysPerLane;
-    const uint32_t end = min(begin + 64u * kRaysPerLane, n);
+    const uint32_t begin = (block * (BLOCK / 64) + (threadIdx.x >> 6)) * perWave;
+    const uint32_t end = min(begin + perWave, n);
     ConnectPool pool{sq, off, sin, &G, acc, recs};
     TraceCounters c{0u, 0u, 0u};
     trace_pool<2, false, kRenderRing, BLOCK, WW>(G, pool, begin, end, 0.0f, false, smem + threadIdx.x, c);
@@ -262,16 +264,18 @@ __device__ __forceinline__ void connect_body(const DeviceGeometry &G, const Shad
 template <int BLOCK, int WAVES, bool WW>
 __global__ __launch_bounds__(BLOCK, WAVES) void k_connect(DeviceGeometry G, const ShadowRay *__restrict__ sq, SegIn sin,
                                                          uint32_t n, AccWord *__restrict__ acc,
-                                                         GuideSample *__restrict__ recs) {
-    connect_body<BLOCK, WW>(G, sq, sin, n, acc, recs, blockIdx.x);
+                                                         GuideSample *__restrict__ recs, uint32_t perWave) {
+    connect_body<BLOCK, WW>(G, sq, sin, n, acc, recs, perWave, blockIdx.x);
 }
 
-static hipError_t launch_extend(const RenderTuning &T, const DeviceGeometry &G, const PathRay *q, SegIn qin, uint32_t n,
-                                PathHit *hits, float eps, int cull, hipStream_t st) {
+static hipError_t launch_extend(const RenderTuning &T, int cuCount, const DeviceGeometry &G, const PathRay *q,
+                                SegIn qin, uint32_t n, PathHit *hits, float eps, int cull, hipStream_t st) {
+    const uint32_t per = rays_per_wave(n, cuCount, T.traceWaves);
 #define AZ_CASE(B, W, F)                                                                                 \
     if (T.traceBlock == B && T.traceWaves == W && T.traceLoop == F) {                                  \
-        hipLaunchKernelGGL((k_extend<B, W, F != 0>), dim3((n + B * kRaysPerLane - 1) / (B * kRaysPerLane)), \
-                           dim3(B), 0, st, G, q, qin, n, hits, eps, cull);                              \
+        const uint32_t perBlock = (B / 64) * per;                                                       \
+        hipLaunchKernelGGL((k_extend<B, W, F != 0>), dim3((n + perBlock - 1) / perBlock), dim3(B), 0, st, G, q, qin, \
+                           n, hits, eps, cull, per);                                                    \
         return hipGetLastError();                                                                       \
     }
     AZ_RENDER_TRACE_VARIANTS(AZ_CASE)
@@ -279,12 +283,14 @@ static hipError_t launch_extend(const RenderTuning &T, const DeviceGeometry &G, 
     return hipErrorInvalidValue;
 }
 
-static hipError_t launch_connect(const RenderTuning &T, const DeviceGeometry &G, const ShadowRay *sq, SegIn sin,
-                                 uint32_t n, AccWord *acc, GuideSample *recs, hipStream_t st) {
+static hipError_t launch_connect(const RenderTuning &T, int cuCount, const DeviceGeometry &G, const ShadowRay *sq,
+                                 SegIn sin, uint32_t n, AccWord *acc, GuideSample *recs, hipStream_t st) {
+    const uint32_t per = rays_per_wave(n, cuCount, T.traceWaves);
 #define AZ_CASE(B, W, F)                                                                                  \
     if (T.traceBlock == B && T.traceWaves == W && T.traceLoop == F) {                                   \
-        hipLaunchKernelGGL((k_connect<B, W, F != 0>), dim3((n + B * kRaysPerLane - 1) / (B * kRaysPerLane)), \
-                           dim3(B), 0, st, G, sq, sin, n, acc, recs);                                    \
+        const uint32_t perBlock = (B / 64) * per;                                                        \
+        hipLaunchKernelGGL((k_connect<B, W, F != 0>), dim3((n + perBlock - 1) / perBlock), dim3(B), 0, st, G, sq,   \
+                           sin, n, acc, recs, per);                                                      \
         return hipGetLastError();                                                                        \
     }
     AZ_RENDER_TRACE_VARIANTS(AZ_CASE)
@@ -1204,7 +1210,7 @@ template <int BLOCK, int WAVES, bool WW>
 __global__ __launch_bounds__(BLOCK, WAVES) void kb_extend(const ExtArgs *__restrict__ A, BatchMap M) {
     const uint32_t s = batch_scene(M, blockIdx.x);
     const ExtArgs &a = A[s];
-    extend_body<BLOCK, WW>(a.G, a.q, a.qin, a.n, a.hits, a.eps, a.cull, blockIdx.x - M.start[s]);
+    extend_body<BLOCK, WW>(a.G, a.q, a.qin, a.n, a.hits, a.eps, a.cull, 64u * kRaysPerLane, blockIdx.x - M.start[s]);
 }
 
 __global__ __launch_bounds__(kRenderBlock, 4) void kb_shade(const ShadeArgs *__restrict__ A, BatchMap M) {
@@ -1221,7 +1227,7 @@ template <int BLOCK, int WAVES, bool WW>
 __global__ __launch_bounds__(BLOCK, WAVES) void kb_connect(const ConnArgs *__restrict__ A, BatchMap M) {
     const uint32_t s = batch_scene(M, blockIdx.x);
     const ConnArgs &a = A[s];
-    connect_body<BLOCK, WW>(a.G, a.sq, a.sin, a.n, a.acc, a.recs, blockIdx.x - M.start[s]);
+    connect_body<BLOCK, WW>(a.G, a.sq, a.sin, a.n, a.acc, a.recs, 64u * kRaysPerLane, blockIdx.x - M.start[s]);
 }
 
 template <typename T>
@@ -1926,8 +1932,8 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             AZ_HIPR(C, ae);
             PathRay *qa = (PathRay *)bufA->ptr, *qb = (PathRay *)bufB->ptr;
             dim3 grid(blocks);
-            AZ_HIPR(C, launch_extend(C->renderTuning, C->dg, qa, qin, count, (PathHit *)B.hits.ptr, S.epsIntersection,
-                                     (int)cp.cull, st));
+            AZ_HIPR(C, launch_extend(C->renderTuning, C->cuCount, C->dg, qa, qin, count, (PathHit *)B.hits.ptr,
+                                     S.epsIntersection, (int)cp.cull, st));
             GuideIO GI{nullptr, nullptr, recCnt, recSegCap, alpha};
             const bool guideLevel = o.GuideMaxDepth == 0 || depth <= o.GuideMaxDepth;
             if (guided && training && !guideLevel) GI.recs = (GuideSample *)B.recs.ptr;
@@ -1966,8 +1972,8 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             AZ_HIPR(C, hipStreamSynchronize(st));
             if (hc[1]) {
                 SegIn sin{offShadow, nseg, shadowSegCap};
-                AZ_HIPR(C, launch_connect(C->renderTuning, C->dg, (const ShadowRay *)B.shadow.ptr, sin, hc[1], acc,
-                                          (GuideSample *)B.recs.ptr, st));
+                AZ_HIPR(C, launch_connect(C->renderTuning, C->cuCount, C->dg, (const ShadowRay *)B.shadow.ptr, sin,
+                                          hc[1], acc, (GuideSample *)B.recs.ptr, st));
             }
             count = hc[0];
             offA = 1 - offA;

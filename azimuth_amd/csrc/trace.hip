@@ -52,17 +52,17 @@ __device__ inline void add_counters(unsigned long long *counters, const TraceCou
 
 // WAVES > 0 asks the compiler to fit WAVES waves per SIMD (register budget
 // 512/WAVES VGPRs); 0 leaves the allocation to the compiler.  Each wave
-// walks kRaysPerLane x 64 consecutive rays with refill (trace_pool); WW
-// selects the while-while loop form.
+// walks perWave consecutive rays with refill (trace_pool); WW selects the
+// while-while loop form.
 template <bool ANY, bool COUNT, int RING, int BLOCK, int WAVES, bool WW>
 __global__ __launch_bounds__(BLOCK, (WAVES > 0 ? WAVES : 1)) void k_trace(DeviceGeometry G, const az_ray *__restrict__ rays, uint32_t n,
                                                 az_hit *__restrict__ hits, float eps, int cull,
-                                                unsigned long long *__restrict__ counters) {
+                                                unsigned long long *__restrict__ counters, uint32_t perWave) {
     __shared__ uint32_t smem[RING * BLOCK];
     const uint32_t wave = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
-    const uint32_t begin = wave * 64u * kRaysPerLane;
+    const uint32_t begin = wave * perWave;
     if (begin >= n) return;                          // whole waves only; no barriers follow
-    const uint32_t end = min(begin + 64u * kRaysPerLane, n);
+    const uint32_t end = min(begin + perWave, n);
     ArrayPool pool{rays, hits};
     TraceCounters c{0u, 0u, 0u};
     trace_pool<ANY ? 1 : 0, COUNT, RING, BLOCK, WW>(G, pool, begin, end, ANY ? 0.0f : eps,
@@ -72,11 +72,12 @@ __global__ __launch_bounds__(BLOCK, (WAVES > 0 ? WAVES : 1)) void k_trace(Device
 
 template <bool ANY, bool COUNT, int RING, int BLOCK, int WAVES, bool WW>
 static hipError_t launch_variant(const DeviceGeometry &G, const az_ray *rays, uint32_t n, az_hit *hits, float eps,
-                                 int cull, unsigned long long *counters, hipStream_t s) {
-    const uint32_t perBlock = BLOCK * kRaysPerLane;
+                                 int cull, unsigned long long *counters, int cuCount, hipStream_t s) {
+    const uint32_t per = rays_per_wave(n, cuCount, WAVES);
+    const uint32_t perBlock = (BLOCK / 64) * per;
     dim3 grid((n + perBlock - 1) / perBlock);
     hipLaunchKernelGGL((k_trace<ANY, COUNT, RING, BLOCK, WAVES, WW>), grid, dim3(BLOCK), 0, s, G, rays, n, hits, eps,
-                       cull, counters);
+                       cull, counters, per);
     return hipGetLastError();
 }
 
@@ -96,25 +97,26 @@ static hipError_t launch_variant(const DeviceGeometry &G, const az_ray *rays, ui
 
 template <bool ANY, bool COUNT>
 static hipError_t launch_ring(const TraceLaunch &L, const DeviceGeometry &G, const az_ray *rays, uint32_t n,
-                              az_hit *hits, float eps, int cull, unsigned long long *counters, hipStream_t s) {
+                              az_hit *hits, float eps, int cull, unsigned long long *counters, int cuCount,
+                              hipStream_t s) {
 #define AZ_CASE(R, B, W, F)                                                                     \
     if (L.ring == R && L.block == B && L.waves == W && L.loop == F)                             \
-        return launch_variant<ANY, COUNT, R, B, W, F != 0>(G, rays, n, hits, eps, cull, counters, s);
+        return launch_variant<ANY, COUNT, R, B, W, F != 0>(G, rays, n, hits, eps, cull, counters, cuCount, s);
     AZ_TRACE_VARIANTS(AZ_CASE)
 #undef AZ_CASE
     return hipErrorInvalidValue;
 }
 
 hipError_t launch_trace(const TraceLaunch &L, bool any, bool count, const DeviceGeometry &G, const az_ray *rays,
-                        uint32_t n, az_hit *hits, float eps, int cull, unsigned long long *counters,
+                        uint32_t n, az_hit *hits, float eps, int cull, unsigned long long *counters, int cuCount,
                         hipStream_t s) {
     if (n == 0) return hipSuccess;
     if (any) {
-        return count ? launch_ring<true, true>(L, G, rays, n, hits, eps, cull, counters, s)
-                     : launch_ring<true, false>(L, G, rays, n, hits, eps, cull, counters, s);
+        return count ? launch_ring<true, true>(L, G, rays, n, hits, eps, cull, counters, cuCount, s)
+                     : launch_ring<true, false>(L, G, rays, n, hits, eps, cull, counters, cuCount, s);
     }
-    return count ? launch_ring<false, true>(L, G, rays, n, hits, eps, cull, counters, s)
-                 : launch_ring<false, false>(L, G, rays, n, hits, eps, cull, counters, s);
+    return count ? launch_ring<false, true>(L, G, rays, n, hits, eps, cull, counters, cuCount, s)
+                 : launch_ring<false, false>(L, G, rays, n, hits, eps, cull, counters, cuCount, s);
 }
 
 bool trace_variant_exists(int ring, int block, int waves, int loop) {
