@@ -285,11 +285,14 @@ az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
     C->stats.NodeBytes = sizeof(Node4);
     C->stats.TriangleBytes = sizeof(TriPacked);
     C->stats.BvhWidth = 4;
-    // Stack capacity of the kernels: the LDS ring plus the scratch spill.
+    // Stack capacity of the kernels: the LDS ring plus the scratch spill
+    // (the renderer's kernels and the default traversal use a 16-entry ring;
+    // a shallower tuned ring falls back to the default here).
     if (info.stackNeeded > (uint32_t)(16 + kSpill)) {
         C->lastError = "BVH needs a deeper traversal stack than the kernels hold";
         return AZ_ERR_UNSUPPORTED;
     }
+    if ((uint32_t)(C->launch.ring + kSpill) < info.stackNeeded) C->launch = TraceLaunch{};
     C->stats.StackDepth = info.stackNeeded;
     C->stats.BlockSize = (u32)C->launch.block;
     return AZ_OK;
@@ -417,6 +420,10 @@ az_status az_gpu_set_launch(az_gpu_ctx *C, u32 block, u32 ring, s32 waves, s32 l
     int w = waves >= 0 ? (int)waves : C->launch.waves;
     int f = loop >= 0 ? (int)loop : C->launch.loop;
     if (!trace_variant_exists(r, b, w, f)) return AZ_ERR_INVALID_ARGUMENT;
+    if (C->hasScene && (uint32_t)(r + kSpill) < C->stats.StackDepth) {
+        C->lastError = "LDS stack ring too shallow for the uploaded BVH";
+        return AZ_ERR_INVALID_ARGUMENT;
+    }
     C->launch.block = b;
     C->launch.ring = r;
     C->launch.waves = w;
@@ -448,10 +455,13 @@ az_status az_gpu_autotune_trace(az_gpu_ctx *C, const az_ray *rays, u32 n, u32 re
     az_status st = AZ_OK;
     if (hipEventCreate(&b) != hipSuccess) st = AZ_ERR_DEVICE;
     double best = 1e300;
-    int bestIdx = 0;
+    int bestIdx = -1;
     az_tune_result res;
     memset(&res, 0, sizeof(res));
     for (int i = 0; i < k && st == AZ_OK; ++i) {
+        // A ring whose stack (ring plus scratch spill) is too shallow for
+        // this scene's BVH is not a candidate.
+        if ((uint32_t)(cands[i].ring + kSpill) < C->stats.StackDepth) continue;
         hipError_t e = launch_trace(cands[i], false, false, C->dg, rays, n, (az_hit *)hits.ptr, C->eps.Intersection, 0,
                                     nullptr, C->cuCount, C->stream);
         if (e == hipSuccess) e = hipEventRecord(a, C->stream);
@@ -468,6 +478,7 @@ az_status az_gpu_autotune_trace(az_gpu_ctx *C, const az_ray *rays, u32 n, u32 re
             break;
         }
         const double per = (double)ms / repeats;
+        if (res.Count >= 32) continue;
         az_tune_entry &t = res.Entries[res.Count++];
         t.Block = (u32)cands[i].block;
         t.LdsStackEntries = (u32)cands[i].ring;
@@ -478,14 +489,15 @@ az_status az_gpu_autotune_trace(az_gpu_ctx *C, const az_ray *rays, u32 n, u32 re
         if (per < best) {
             best = per;
             bestIdx = i;
+            res.Best = res.Count - 1;
         }
     }
     (void)hipEventDestroy(a);
     if (b) (void)hipEventDestroy(b);
     if (st != AZ_OK) return st;
+    if (bestIdx < 0) return AZ_ERR_UNSUPPORTED;
     C->launch = cands[bestIdx];
     C->stats.BlockSize = (u32)cands[bestIdx].block;
-    res.Best = (u32)bestIdx;
     if (out) *out = res;
     return AZ_OK;
 }

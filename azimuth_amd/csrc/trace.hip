@@ -81,7 +81,8 @@ static hipError_t launch_variant(const DeviceGeometry &G, const az_ray *rays, ui
     return hipGetLastError();
 }
 
-// The compiled (ring, block, waves, loop form) variants.
+// The compiled (ring, block, waves, loop form) variants: LDS stack rings of
+// 8, 16 and 32 entries per lane (older entries spill to scratch).
 #define AZ_TRACE_VARIANTS(X) \
     X(16, 64, 0, 0)          \
     X(16, 64, 6, 0)          \
@@ -93,7 +94,11 @@ static hipError_t launch_variant(const DeviceGeometry &G, const az_ray *rays, ui
     X(16, 128, 0, 1)         \
     X(16, 128, 6, 1)         \
     X(16, 256, 0, 1)         \
-    X(16, 256, 6, 1)
+    X(16, 256, 6, 1)         \
+    X(8, 128, 6, 0)          \
+    X(32, 128, 6, 0)         \
+    X(8, 128, 6, 1)          \
+    X(32, 128, 6, 1)
 
 template <bool ANY, bool COUNT>
 static hipError_t launch_ring(const TraceLaunch &L, const DeviceGeometry &G, const az_ray *rays, uint32_t n,

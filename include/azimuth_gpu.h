@@ -159,12 +159,15 @@ az_status az_gpu_get_stats(az_gpu_ctx *Ctx, az_stats *Stats);
 az_status az_gpu_set_counting(az_gpu_ctx *Ctx, b32 Enable);
 
 /* Traversal launch parameters, as az_gpu_autotune_trace chooses them: workgroup
- * size (64, 128, 256), LDS stack entries per lane (16), the waves per SIMD
- * the register allocation targets (0 = compiler's choice, 6) and the loop
- * form (0: one node or leaf per step; 1: while-while, all of a lane's node
- * steps before its next leaf).  0 (-1 for MinWaves and LoopForm) keeps the
- * current value; combinations that were not compiled return
- * AZ_ERR_INVALID_ARGUMENT. */
+ * size (64, 128, 256), LDS stack entries per lane (8, 16, 32: the top of
+ * each lane's stack; older entries spill to scratch), the waves per SIMD the
+ * register allocation targets (0 = compiler's choice, 6) and the loop form
+ * (0: one node or leaf per step; 1: while-while, all of a lane's node steps
+ * before its next leaf).  0 (-1 for MinWaves and LoopForm) keeps the current
+ * value; combinations that were not compiled, or whose stack (ring plus
+ * spill) is too shallow for the uploaded BVH, return AZ_ERR_INVALID_ARGUMENT.
+ * An upload whose BVH needs more than the current ring holds resets the
+ * launch to the default. */
 az_status az_gpu_set_launch(az_gpu_ctx *Ctx, u32 BlockSize, u32 LdsStackEntries, s32 MinWaves, s32 LoopForm);
 
 /* BVH build parameters of the next az_gpu_upload_scene: the most primitives
