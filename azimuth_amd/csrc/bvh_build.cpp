@@ -75,6 +75,7 @@ uint32_t BuildWideBvh(const std::vector<BuildBox> &boxes, const BuildParams &P, 
                       std::vector<Node4> &nodes, std::vector<uint32_t> &order, BuildStats *stats) {
     const uint32_t n = (uint32_t)boxes.size();
     const uint32_t maxLeaf = std::max(1u, std::min(P.maxLeafItems, kMaxLeafItems));
+    const uint32_t width = std::max(2u, std::min(P.width, 4u));
     order.resize(n);
     for (uint32_t i = 0; i < n; ++i) order[i] = i;
     std::vector<Aabb> ib(n);
@@ -195,8 +196,9 @@ uint32_t BuildWideBvh(const std::vector<BuildBox> &boxes, const BuildParams &P, 
     BuildStats st;
     const uint32_t rootIndex = (uint32_t)nodes.size();
     struct Frame { uint32_t b2node, slotNode, slot, depth; };
-    // Each 4-wide node is created from one binary node; children are gathered
-    // by repeatedly opening the internal child with the largest area.
+    // Each node is created from one binary node; its (up to `width`)
+    // children are gathered by repeatedly opening the internal child with the
+    // largest area.
     std::vector<uint32_t> maxStackOf;   // per created node, filled bottom-up
     std::vector<uint32_t> parentOf, childCountOf;
     struct Pend { uint32_t b2node, parent, slot, depth; };
@@ -249,7 +251,7 @@ uint32_t BuildWideBvh(const std::vector<BuildBox> &boxes, const BuildParams &P, 
         } else {
             kids[k++] = src.left;
             kids[k++] = src.right;
-            while (k < 4) {
+            while (k < width) {
                 int best = -1;
                 float bestArea = -1.0f;
                 for (uint32_t c = 0; c < k; ++c) {

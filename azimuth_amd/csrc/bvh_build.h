@@ -15,6 +15,7 @@ struct BuildBox {
 
 struct BuildParams {
     uint32_t maxLeafItems = 4;     // leaves never hold more items than this
+    uint32_t width = 4;            // children per node (2..4; unused slots stay empty)
     uint32_t bins = 32;            // SAH bins per axis
     float nodeCost = 1.0f;         // SAH cost of visiting one 4-wide node
     float itemCost = 1.0f;         // SAH cost of testing one item

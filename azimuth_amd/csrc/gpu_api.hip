@@ -121,6 +121,7 @@ az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
     uint32_t primBase = 0;
     BuildParams bp;
     bp.maxLeafItems = C->maxLeafItems;
+    bp.width = C->bvhWidth;
     bp.nodeCost = C->sahNodeCost;
     bp.itemCost = C->sahItemCost;
     for (u32 m = 0; m < S->MeshCount; ++m) {
@@ -220,6 +221,7 @@ az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
     }
     BuildParams tp;
     tp.maxLeafItems = 1;
+    tp.width = C->bvhWidth;
     std::vector<uint32_t> torder;
     BuildStats ts;
     uint32_t tlasRoot = BuildWideBvh(tboxes, tp, 0, nodes, torder, &ts);
@@ -284,7 +286,7 @@ az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
     }
     C->stats.NodeBytes = sizeof(Node4);
     C->stats.TriangleBytes = sizeof(TriPacked);
-    C->stats.BvhWidth = 4;
+    C->stats.BvhWidth = C->bvhWidth;
     // Stack capacity of the kernels: the LDS ring plus the scratch spill
     // (the renderer's kernels and the default traversal use a 16-entry ring;
     // a shallower tuned ring falls back to the default here).
@@ -441,6 +443,79 @@ az_status az_gpu_set_build(az_gpu_ctx *C, u32 maxLeafItems, f32 nodeCost, f32 it
     return AZ_OK;
 }
 
+az_status az_gpu_set_bvh_width(az_gpu_ctx *C, u32 width) {
+    if (!C || width < 2 || width > 4) return AZ_ERR_INVALID_ARGUMENT;
+    C->bvhWidth = width;
+    return AZ_OK;
+}
+
+az_status az_gpu_autotune_build(az_gpu_ctx *C, const scene *S, const epsilon_values *E, const az_ray *rays, u32 n,
+                                u32 repeats, az_tune_result *out) {
+    if (!C || !S || !E || !rays || n == 0) return AZ_ERR_INVALID_ARGUMENT;
+    AZ_HIP(C, hipSetDevice(C->device));
+    repeats = std::max<u32>(1u, repeats);
+    const u32 widths[3] = {2u, 3u, 4u}, leaves[3] = {2u, 4u, 8u};
+    const u32 keepWidth = C->bvhWidth, keepLeaf = C->maxLeafItems;
+    DeviceBuffer hits;
+    AZ_HIP(C, hits.alloc((size_t)n * sizeof(az_hit)));
+    hipEvent_t a = nullptr, b = nullptr;
+    AZ_HIP(C, hipEventCreate(&a));
+    az_status st = AZ_OK;
+    if (hipEventCreate(&b) != hipSuccess) st = AZ_ERR_DEVICE;
+    az_tune_result res;
+    memset(&res, 0, sizeof(res));
+    double best = 1e300;
+    u32 bestW = keepWidth, bestL = keepLeaf;
+    for (u32 wi = 0; wi < 3 && st == AZ_OK; ++wi)
+        for (u32 li = 0; li < 3 && st == AZ_OK; ++li) {
+            C->bvhWidth = widths[wi];
+            C->maxLeafItems = leaves[li];
+            st = az_gpu_upload_scene(C, S, E);
+            if (st != AZ_OK) break;
+            const TraceLaunch L = C->launch;
+            hipError_t e = launch_trace(L, false, false, C->dg, rays, n, (az_hit *)hits.ptr, C->eps.Intersection, 0,
+                                        nullptr, C->cuCount, C->stream);
+            if (e == hipSuccess) e = hipEventRecord(a, C->stream);
+            for (u32 r = 0; r < repeats && e == hipSuccess; ++r)
+                e = launch_trace(L, false, false, C->dg, rays, n, (az_hit *)hits.ptr, C->eps.Intersection, 0, nullptr,
+                                 C->cuCount, C->stream);
+            if (e == hipSuccess) e = hipEventRecord(b, C->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(b);
+            float ms = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+            if (e != hipSuccess) {
+                C->lastError = std::string("build autotune: ") + hipGetErrorString(e);
+                st = AZ_ERR_DEVICE;
+                break;
+            }
+            az_tune_entry &t = res.Entries[res.Count++];
+            t.Block = (u32)L.block;
+            t.LdsStackEntries = (u32)L.ring;
+            t.MinWaves = L.waves;
+            t.LoopForm = (u32)L.loop;
+            t.BvhWidth = widths[wi];
+            t.MaxLeafItems = leaves[li];
+            t.Milliseconds = (double)ms / repeats;
+            if (t.Milliseconds < best) {
+                best = t.Milliseconds;
+                bestW = widths[wi];
+                bestL = leaves[li];
+                res.Best = res.Count - 1;
+            }
+        }
+    (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    // Leave the scene uploaded with the fastest build (or, after a failure,
+    // with the settings the caller had).
+    C->bvhWidth = st == AZ_OK ? bestW : keepWidth;
+    C->maxLeafItems = st == AZ_OK ? bestL : keepLeaf;
+    az_status up = az_gpu_upload_scene(C, S, E);
+    if (st != AZ_OK) return st;
+    if (up != AZ_OK) return up;
+    if (out) *out = res;
+    return AZ_OK;
+}
+
 az_status az_gpu_autotune_trace(az_gpu_ctx *C, const az_ray *rays, u32 n, u32 repeats, az_tune_result *out) {
     if (!C || !rays || n == 0) return AZ_ERR_INVALID_ARGUMENT;
     if (!C->hasScene) return AZ_ERR_NO_SCENE;
@@ -485,6 +560,8 @@ az_status az_gpu_autotune_trace(az_gpu_ctx *C, const az_ray *rays, u32 n, u32 re
         t.MinWaves = cands[i].waves;
         t.ChunkSamples = 0;
         t.LoopForm = (u32)cands[i].loop;
+        t.BvhWidth = C->bvhWidth;
+        t.MaxLeafItems = C->maxLeafItems;
         t.Milliseconds = per;
         if (per < best) {
             best = per;

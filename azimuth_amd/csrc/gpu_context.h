@@ -78,7 +78,8 @@ struct az_gpu_ctx {
     bool hasScene = false;
     bool counting = false;
     bool pendingTiming = false;
-    uint32_t maxLeafItems = 4;       // BVH build of the next upload: leaf size and SAH costs
+    uint32_t maxLeafItems = 4;       // BVH build of the next upload: leaf size, node width, SAH costs
+    uint32_t bvhWidth = 4;
     float sahNodeCost = 1.0f, sahItemCost = 1.0f;
     epsilon_values eps{0.0f, 0.000275f};
     az::GeometryBuffers geom;

@@ -184,7 +184,8 @@ class az_stats(C.Structure):
 
 class az_tune_entry(C.Structure):
     _fields_ = [("Block", u32), ("LdsStackEntries", u32), ("MinWaves", s32), ("ChunkSamples", u32),
-                ("LoopForm", u32), ("Reserved", u32), ("Milliseconds", f64)]
+                ("LoopForm", u32), ("BvhWidth", u32), ("MaxLeafItems", u32), ("Reserved", u32),
+                ("Milliseconds", f64)]
 
 
 class az_tune_result(C.Structure):
@@ -240,6 +241,8 @@ EXPORTS = {
     "az_gpu_autotune_render": (s32, [_P, _P, s32, u32, _P, _P]),
     "az_gpu_set_render_tuning": (s32, [_P, u32, u32, s32, s32]),
     "az_gpu_trim": (s32, [_P]),
+    "az_gpu_set_bvh_width": (s32, [_P, u32]),
+    "az_gpu_autotune_build": (s32, [_P, _P, _P, _P, u32, u32, _P]),
     "az_gpu_malloc": (s32, [_P, C.POINTER(_P), u64]),
     "az_gpu_free": (s32, [_P, _P]),
     "az_gpu_upload": (s32, [_P, _P, _P, u64]),

@@ -299,6 +299,20 @@ class GpuContext:
         N.check(self.lib.az_gpu_set_build(self.h, int(leaf), float(node_cost), float(item_cost)),
                 "az_gpu_set_build", self.h)
 
+    def set_bvh_width(self, width=4):
+        """Children per BVH node (2..4) of the next upload_scene."""
+        N.check(self.lib.az_gpu_set_bvh_width(self.h, int(width)), "az_gpu_set_bvh_width", self.h)
+
+    def autotune_build(self, scene, d_rays, n, repeats=3, eps_intersection=0.0, eps_shadow=0.000275):
+        """Uploads the scene once per (node width, leaf size) candidate, times
+        the current closest-hit launch on the device rays and keeps the
+        fastest build; returns (entries, best index)."""
+        e = N.epsilon_values(float(eps_intersection), float(eps_shadow))
+        res = N.az_tune_result()
+        N.check(self.lib.az_gpu_autotune_build(self.h, scene.ptr, C.byref(e), d_rays.ptr, int(n), int(repeats),
+                                               C.byref(res)), "az_gpu_autotune_build", self.h)
+        return self._tune_list(res)
+
     def set_launch(self, block=0, ring=0, waves=-1, loop=-1):
         N.check(self.lib.az_gpu_set_launch(self.h, int(block), int(ring), int(waves), int(loop)),
                 "az_gpu_set_launch", self.h)
@@ -349,7 +363,8 @@ class GpuContext:
     @staticmethod
     def _tune_list(res):
         return [{"block": e.Block, "ring": e.LdsStackEntries, "waves": e.MinWaves, "chunk": e.ChunkSamples,
-                 "loop": e.LoopForm, "ms": e.Milliseconds} for e in res.Entries[:res.Count]], int(res.Best)
+                 "loop": e.LoopForm, "width": e.BvhWidth, "leaf": e.MaxLeafItems, "ms": e.Milliseconds}
+                for e in res.Entries[:res.Count]], int(res.Best)
 
     def autotune_trace(self, d_rays, n, repeats=3):
         """Times every traversal variant on the device rays and keeps the

@@ -48,6 +48,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile", action="store_true", help="short run for profilers: no CPU baseline, no any-hit")
     ap.add_argument("--no-render", action="store_true", help="skip the interior render line")
+    ap.add_argument("--tune-build", action="store_true",
+                    help="also autotune the BVH build (node width 2-4 x leaf size 2/4/8, one upload each)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on exactly one GPU; sharding across GPUs is not "
@@ -82,12 +84,21 @@ def main():
         d_tune = ctx.alloc(sample.nbytes)
         d_tune.upload(sample)
         entries, best = ctx.autotune_trace(d_tune, sample.shape[0], repeats=3)
-        d_tune.free()
         tuned = {"candidates": len(entries), "ring": entries[best]["ring"], "block": entries[best]["block"],
                  "waves": entries[best]["waves"], "loop": "while-while" if entries[best]["loop"] else "if-if",
                  "ms_per_1m_rays": {f'{e["ring"]}/{e["block"]}/{e["waves"]}/{e["loop"]}':
                                     round(e["ms"] * 1e6 / sample.shape[0], 4) for e in entries}}
         log(f"autotune: ring {tuned['ring']} block {tuned['block']} waves {tuned['waves']} loop {tuned['loop']}")
+        if args.tune_build:
+            # One upload (host BVH build) per candidate, timed with the tuned
+            # launch; the scene stays uploaded with the fastest build.
+            t0 = time.time()
+            bent, bbest = ctx.autotune_build(sc, d_tune, sample.shape[0], repeats=2)
+            tuned["build"] = {"width": bent[bbest]["width"], "leaf": bent[bbest]["leaf"],
+                              "ms_per_1m_rays": {f'w{e["width"]}/l{e["leaf"]}': round(e["ms"] * 1e6 / sample.shape[0], 4)
+                                                 for e in bent}}
+            log(f"build autotune ({time.time() - t0:.0f}s): width {bent[bbest]['width']} leaf {bent[bbest]['leaf']}")
+        d_tune.free()
 
     # Kernel-counted work per batch (an untimed counting launch).
     ctx.set_counting(True)

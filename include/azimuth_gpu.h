@@ -174,6 +174,10 @@ az_status az_gpu_set_launch(az_gpu_ctx *Ctx, u32 BlockSize, u32 LdsStackEntries,
  * a mesh BVH leaf holds (1..16, default 4) and the binned-SAH costs of
  * visiting a 4-wide node and of testing a primitive (default 1, 1). */
 az_status az_gpu_set_build(az_gpu_ctx *Ctx, u32 MaxLeafItems, f32 NodeCost, f32 ItemCost);
+/* Children per BVH node of the next upload, 2..4 (default 4): every node is
+ * one 128-byte line with four child slots; narrower trees leave slots
+ * empty and are deeper. */
+az_status az_gpu_set_bvh_width(az_gpu_ctx *Ctx, u32 Width);
 
 /* Autotuning.  Each candidate launch configuration is timed with HIP events
  * on the context stream; the fastest becomes the context's setting (and
@@ -185,6 +189,8 @@ typedef struct {
     s32 MinWaves;         /* register target in waves per SIMD (0: compiler's choice) */
     u32 ChunkSamples;     /* camera samples per wavefront chunk (render tuning) */
     u32 LoopForm;         /* 0: one node or leaf per step; 1: while-while */
+    u32 BvhWidth;         /* children per BVH node (build tuning; else the current build) */
+    u32 MaxLeafItems;     /* primitives per mesh BVH leaf (likewise) */
     u32 Reserved;
     f64 Milliseconds;     /* measured time of the candidate */
 } az_tune_entry;
@@ -194,6 +200,13 @@ typedef struct {
     u32 Best;             /* index of the chosen entry */
     az_tune_entry Entries[32];
 } az_tune_result;
+
+/* BVH build: uploads Scene once per (node width 2, 3, 4) x (leaf size 2, 4,
+ * 8) candidate, times the context's closest-hit launch on d_Rays (N rays,
+ * Repeats launches after one warm-up) and leaves the scene uploaded with the
+ * fastest build (its width and leaf size become the context's settings). */
+az_status az_gpu_autotune_build(az_gpu_ctx *Ctx, const scene *Scene, const epsilon_values *Epsilons,
+                                const az_ray *d_Rays, u32 N, u32 Repeats, az_tune_result *Out);
 
 /* Traversal: times closest-hit batches of d_Rays (N rays, Repeats launches
  * per candidate after one warm-up) for every compiled (LDS stack entries,

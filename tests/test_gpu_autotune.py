@@ -74,3 +74,39 @@ def test_trim_frees_queues_and_render_repeats(gpu):
 def test_set_render_tuning_rejects_unknown_variant(gpu):
     with pytest.raises(N.AzError):
         gpu.set_render_tuning(block=96, waves=6)
+
+
+def test_bvh_width_and_leaf_size_builds_identical(gpu):
+    """Every BVH build the build autotuner can choose (node width 2-4, leaf
+    size 2/4/8) gives the oracle's closest hits bit for bit, on a mesh and on
+    a two-level scene with moving spheres."""
+    from azimuth_amd import workloads
+    sc = configs.icosphere_scene(5, seed=7)
+    rays = np.concatenate([configs.coherent_rays(48, 48), configs.incoherent_rays(4096, sc.meta["bbox"], seed=3)])
+    want = oracle.trace(sc, rays)
+    motion, cam = workloads.motion_scene(8, 8, spp=1, max_depth=2, spheres=60, ico_subdiv=3)
+    mrays = configs.incoherent_rays(4096, (np.full(3, -10.0), np.full(3, 10.0)), seed=5)   # the spheres' box
+    mwant = oracle.trace(motion, mrays)
+    assert np.sum(mwant["Prim"] != 0xFFFFFFFF) > 100
+    try:
+        for width in (2, 3, 4):
+            for leaf in (2, 4, 8):
+                gpu.set_bvh_width(width)
+                gpu.set_build(leaf=leaf)
+                gpu.upload_scene(sc)
+                assert gpu.stats().BvhWidth == width
+                assert gpu.trace(rays).tobytes() == want.tobytes(), (width, leaf)
+                gpu.upload_scene(motion)
+                assert gpu.trace(mrays).tobytes() == mwant.tobytes(), ("motion", width, leaf)
+        d = gpu.alloc(rays.nbytes)
+        d.upload(rays)
+        entries, best = gpu.autotune_build(sc, d, rays.shape[0], repeats=1)
+        print("build candidates:", [(e["width"], e["leaf"], round(e["ms"], 4)) for e in entries], "best", best,
+              flush=True)
+        assert len(entries) == 9 and 0 <= best < 9
+        assert gpu.stats().BvhWidth == entries[best]["width"]
+        assert gpu.trace(rays).tobytes() == want.tobytes()
+        d.free()
+    finally:
+        gpu.set_bvh_width(4)
+        gpu.set_build(leaf=4)
