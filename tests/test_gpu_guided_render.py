@@ -36,6 +36,15 @@ def test_guided_interior_unbiased_and_lower_variance(gpu):
     assert np.all(np.isfinite(guided))
     assert abs(m_g - m_ref) < 5 * se + 1e-3 * m_ref
     assert mse_g < mse_p
+    # The training schedule is a caller's choice (16 spp in 2-spp chunks
+    # here); the estimator stays unbiased whatever the network learned.
+    g2 = resolve(gpu.render_f64(cam, filt=filt, seed=2, spp=64, guide=True, guide_train_spp=16, guide_chunk_spp=2))
+    mse_2 = float(np.mean((g2 - ref) ** 2))
+    se2 = float(np.sqrt(mse_2 / g2.size)) * 4 + float(np.sqrt(mse_p / plain.size))
+    print(f"guided (train 16 spp, 2-spp chunks) mean {g2.mean():.5f} mse {mse_2:.4e}", flush=True)
+    assert np.all(np.isfinite(g2))
+    assert abs(float(g2.mean()) - m_ref) < 5 * se2 + 1e-3 * m_ref
+    assert mse_2 < mse_p
 
 
 def test_luminance_environment_sampling_unbiased_and_lower_variance(gpu):
@@ -68,5 +77,4 @@ def test_luminance_sampling_without_environment_changes_nothing(gpu):
     gpu.upload_scene(sc)
     a = gpu.render(cam, seed=2)
     b = gpu.render(cam, seed=2, env_sampling=N.AZ_ENV_LUMINANCE)
-    assert np.array_equal(a["TotalWeight"], b["TotalWeight"])
-    np.testing.assert_allclose(a["Color"], b["Color"], rtol=1e-5, atol=1e-6)
+    assert a.tobytes() == b.tobytes()
