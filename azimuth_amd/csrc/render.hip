@@ -1839,7 +1839,11 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
     const u32 stepsPerChunk =
         o.GuideStepsPerChunk ? o.GuideStepsPerChunk : std::max<u32>(1u, (kGuideSteps + trainChunks - 1) / trainChunks);
     const float alpha = o.GuideFraction > 0.0f && o.GuideFraction < 1.0f ? o.GuideFraction : 0.5f;
-    const u32 recCap = 8u << 20;
+    // Training records: room for one per path vertex of a training chunk (a
+    // segment that fills drops its later records, which would be the deep
+    // levels'), between 8M and 64M records (48 B each).
+    const uint64_t recWant = (uint64_t)nPix * trainChunkSpp * (S.maxDepth + 1u);
+    const u32 recCap = (u32)std::min<uint64_t>(64u << 20, std::max<uint64_t>(8u << 20, (recWant + kSegs - 1) / kSegs * kSegs));
     if (guided) {
         if (C->guide) az_guide_destroy(C->guide);
         C->guide = nullptr;
