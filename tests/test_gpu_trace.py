@@ -105,6 +105,29 @@ def test_axis_aligned_and_degenerate_rays(gpu, ico5):
     compare_closest(got, want, "axis-aligned")
 
 
+def test_ragged_batches_zero_and_far_rays(gpu, ico5):
+    """Batch sizes that leave partial waves and small launches (one ray, one
+    short of a wave, one past it, ...), with zero-length directions (the
+    reference normalizes them to zero: no hit) and rays from far away."""
+    gpu.upload_scene(ico5)
+    g = np.random.default_rng(11)
+    for n in (1, 63, 65, 257, 4099):
+        r = np.zeros(n, dtype=N.RAY_DTYPE)
+        r["Origin"] = g.uniform(-1.5, 1.5, (n, 3)).astype(np.float32)
+        d = g.normal(size=(n, 3))
+        d[::7] = 0.0                                         # zero-length directions
+        r["Direction"] = (d / np.maximum(np.linalg.norm(d, axis=1, keepdims=True), 1e-30)).astype(np.float32)
+        far = np.arange(n) % 5 == 3                          # from 1e4 away, aimed at the mesh
+        r["Origin"][far] = (-1e4 * r["Direction"][far]).astype(np.float32)
+        r["TMax"] = np.inf
+        got = gpu.trace(r)
+        want = oracle.trace(ico5, r)
+        compare_closest(got, want, f"ragged n={n}")
+        assert np.all(got["Prim"][::7][np.all(r["Direction"][::7] == 0, axis=1)] == N.AZ_MISS)
+        ga, wa = gpu.trace(r, any_hit=True), oracle.trace(ico5, r, any_hit=True)
+        assert np.array_equal(ga["Prim"] != N.AZ_MISS, wa["Prim"] != N.AZ_MISS), f"any-hit n={n}"
+
+
 def test_empty_batch(gpu, ico5):
     gpu.upload_scene(ico5)
     r = np.zeros(0, dtype=N.RAY_DTYPE)
