@@ -145,6 +145,7 @@ struct ChunkParams {
     uint32_t cull;
     uint32_t gaussian;
     uint32_t envLuminance;           // environment NEE from the luminance distribution
+    uint32_t brdfSampling;           // hemisphere scatter from the BRDF lobe mixture
 };
 
 struct RenderState {

@@ -835,25 +835,34 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
     }
     const bool childOk = depth + 1u <= S.maxDepth;
     const V3 zero = v3(0.0f, 0.0f, 0.0f);
-    // Hemisphere scatter (main.cpp:1024-1083).
+    // Hemisphere scatter (main.cpp:1024-1083), or, as an option, from the
+    // BRDF lobe mixture.
     if (S.pathTrace) {
-        float theta = S.importance ? Asin(sqrtf(rng.uniform())) : Acos(rng.uniform());
-        float ct = cosf(theta), st = sinf(theta);
-        float phi = 2.0f * kPiF * rng.uniform();
-        float cp_ = cosf(phi), sp_ = sinf(phi);
-        V3 U, Vv;
-        orthonormal_basis(N, U, Vv);
-        V3 Sd = normalize(U * st * cp_ + Vv * st * sp_ + N * ct);
+        V3 Sd;
         float invPdf;
-        if (S.importance) {
-            float cs = dot(Sd, N);
-            if (cs == 0.0f) cs = 0.000001f;
-            invPdf = kPiF / cs;
+        if (cp.brdfSampling) {
+            const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
+            float pdf;
+            Sd = sample_brdf_mixture(M, kd, N, -D, u0, u1, u2, pdf);
+            invPdf = pdf > 0.0f ? 1.0f / pdf : 0.0f;
         } else {
-            invPdf = kPiF * 2.0f;
+            float theta = S.importance ? Asin(sqrtf(rng.uniform())) : Acos(rng.uniform());
+            float ct = cosf(theta), st = sinf(theta);
+            float phi = 2.0f * kPiF * rng.uniform();
+            float cp_ = cosf(phi), sp_ = sinf(phi);
+            V3 U, Vv;
+            orthonormal_basis(N, U, Vv);
+            Sd = normalize(U * st * cp_ + Vv * st * sp_ + N * ct);
+            if (S.importance) {
+                float cs = dot(Sd, N);
+                if (cs == 0.0f) cs = 0.000001f;
+                invPdf = kPiF / cs;
+            } else {
+                invPdf = kPiF * 2.0f;
+            }
         }
         V3 cw;
-        bool want = childOk && scatter_weight(M, kd, Sd, D, N, invPdf, cw);
+        bool want = childOk && invPdf > 0.0f && scatter_weight(M, kd, Sd, D, N, invPdf, cw);
         if (want) cw = had(w, cw);
         emit_child(O, seg, want, make_child(P + (Sd * eps), Sd, r.time, child_node(r.node, 0u), cw, r.sample, zero,
                                        depth + 1u, inside));
@@ -1817,7 +1826,7 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
     // The learned guide: a fresh network per render, trained on the
     // environment samples of the first trainSpp samples per pixel.
     const bool guided = o.GuideEnable != 0 && S.useEnv;
-    if (o.EnvSampling > AZ_ENV_LUMINANCE) return AZ_ERR_INVALID_ARGUMENT;
+    if (o.EnvSampling > AZ_ENV_LUMINANCE || o.ScatterSampling > AZ_SCATTER_BRDF) return AZ_ERR_INVALID_ARGUMENT;
     const bool envLum = o.EnvSampling == AZ_ENV_LUMINANCE && S.useEnv;
     if (envLum && !S.envCdf) {
         C->lastError = "luminance sampling needs an image environment map";
@@ -1906,6 +1915,7 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
         cp.cull = Cull ? 1u : 0u;
         cp.gaussian = Filter == FilterType_Gaussian ? 1u : 0u;
         cp.envLuminance = envLum ? 1u : 0u;
+        cp.brdfSampling = o.ScatterSampling == AZ_SCATTER_BRDF ? 1u : 0u;
         DeviceBuffer *bufA = &B.q0, *bufB = &B.q1;
         int offA = 0;
         AccWord *acc = (AccWord *)B.acc.ptr;
@@ -2100,7 +2110,7 @@ static az_status renderBatchImpl(az_gpu_ctx *const *ctxs, const camera *const *c
         C0->lastError = "the learned guide is not available in batched renders";
         return AZ_ERR_UNSUPPORTED;
     }
-    if (o.EnvSampling > AZ_ENV_LUMINANCE) return AZ_ERR_INVALID_ARGUMENT;
+    if (o.EnvSampling > AZ_ENV_LUMINANCE || o.ScatterSampling > AZ_SCATTER_BRDF) return AZ_ERR_INVALID_ARGUMENT;
     const bool f64 = o.AccumulateF64 != 0;
     hipStream_t st = C0->stream;
     const RenderTuning tune = C0->renderTuning;
@@ -2186,6 +2196,7 @@ static az_status renderBatchImpl(az_gpu_ctx *const *ctxs, const camera *const *c
             b.cp.cull = Cull ? 1u : 0u;
             b.cp.gaussian = Filter == FilterType_Gaussian ? 1u : 0u;
             b.cp.envLuminance = b.envLum ? 1u : 0u;
+            b.cp.brdfSampling = o.ScatterSampling == AZ_SCATTER_BRDF ? 1u : 0u;
             b.bufA = &b.B->q0;
             b.bufB = &b.B->q1;
             b.offA = 0;

@@ -290,4 +290,75 @@ __device__ inline bool scatter_weight(const DevMaterial &M, V3 kd, V3 S, V3 Vw, 
     return true;
 }
 
+// ---- BRDF lobe sampling of the hemisphere scatter -------------------------------
+// An option (the reference draws scatter directions from the cosine or the
+// uniform hemisphere only, main.cpp:1029-1083): a one-sample mixture of the
+// cosine lobe and the material's specular lobe, the reference's BRDF value
+// divided by the mixture's density, so the estimate stays unbiased.
+//
+// The specular lobe: 0 none (black specular or no exponent), 1 cos^e about
+// the viewer's mirror direction (the three Phong models: their lobe is
+// (L . reflect(V, N))^e), 2 cos^e(N . H) of the half vector H about the
+// normal (the three Blinn models and Torrance-Sparrow's Blinn D).
+__device__ inline int spec_lobe(const DevMaterial &M) {
+    if (!(M.exponent > 0.0f) || (M.ks[0] == 0.0f && M.ks[1] == 0.0f && M.ks[2] == 0.0f)) return 0;
+    return M.brdfType <= 2 ? 1 : 2;
+}
+
+// Probability of drawing from the specular lobe: its reflectance's share of
+// the luminance of both (Torrance-Sparrow's specular reflectance scaled by
+// its Schlick Fresnel term at the viewing angle), kept within [0.1, 0.9] so
+// that neither lobe is starved.
+__device__ inline float spec_prob(const DevMaterial &M, V3 kd, V3 N, V3 Vo) {
+    float ls = 0.2126f * M.ks[0] + 0.7152f * M.ks[1] + 0.0722f * M.ks[2];
+    if (M.brdfType >= 6) {
+        const float r0 = sq(M.brdfIor - 1.0f) / sq(M.brdfIor + 1.0f);
+        ls *= r0 + (1.0f - r0) * powf(1.0f - fminf(1.0f, fmaxf(0.0f, dot(N, Vo))), 5.0f);
+    }
+    const float ld = 0.2126f * kd.x + 0.7152f * kd.y + 0.0722f * kd.z;
+    return fminf(0.9f, fmaxf(0.1f, ls / fmaxf(ls + ld, 1e-20f)));
+}
+
+// Density (solid angle) of unit direction L under the specular lobe, Vo the
+// unit direction towards the viewer: (e + 1) / 2pi cos^e about the lobe's
+// axis; for the half-vector lobe divided by 4 (Vo . H), the Jacobian of
+// reflecting Vo about H.
+__device__ inline float spec_pdf(int lobe, float e, V3 L, V3 N, V3 Vo) {
+    if (lobe == 1) {
+        const float c = dot(L, reflect(Vo, N));
+        return c > 0.0f ? (e + 1.0f) / (2.0f * kPiF) * powf(c, e) : 0.0f;
+    }
+    const V3 H = normalize(Vo + L);
+    const float c = dot(N, H), vh = dot(Vo, H);
+    if (!(c > 0.0f) || !(vh > 0.0f)) return 0.0f;
+    return (e + 1.0f) / (2.0f * kPiF) * powf(c, e) / (4.0f * vh);
+}
+
+// A scatter direction from the mixture (1 - ps) cos/pi + ps specular, from
+// three uniforms (lobe choice, then the lobe's two), with the mixture's
+// density at it.  Directions below the surface can come from the specular
+// lobe; their BRDF term is zero (the caller emits no child), and the density
+// still counts them.
+__device__ inline V3 sample_brdf_mixture(const DevMaterial &M, V3 kd, V3 N, V3 Vo, float u0, float u1, float u2,
+                                         float &pdf) {
+    const int lobe = spec_lobe(M);
+    const float ps = lobe ? spec_prob(M, kd, N, Vo) : 0.0f;
+    const float e = M.exponent;
+    const float phi = 2.0f * kPiF * u2;
+    V3 L, U, V;
+    if (u0 >= ps) {
+        const float st = sqrtf(u1), ct = sqrtf(fmaxf(0.0f, 1.0f - u1));
+        orthonormal_basis(N, U, V);
+        L = normalize(U * (st * cosf(phi)) + V * (st * sinf(phi)) + N * ct);
+    } else {
+        const float ca = powf(u1, 1.0f / (e + 1.0f)), sa = sqrtf(fmaxf(0.0f, 1.0f - ca * ca));
+        const V3 A = lobe == 1 ? reflect(Vo, N) : N;
+        orthonormal_basis(A, U, V);
+        const V3 X = normalize(U * (sa * cosf(phi)) + V * (sa * sinf(phi)) + A * ca);
+        L = lobe == 1 ? X : normalize(reflect(Vo, X));
+    }
+    pdf = (1.0f - ps) * fmaxf(dot(N, L), 0.0f) / kPiF + (lobe ? ps * spec_pdf(lobe, e, L, N, Vo) : 0.0f);
+    return L;
+}
+
 }  // namespace az

@@ -83,11 +83,17 @@ typedef struct {
     u32 GuideChunkSpp;      /* samples per pixel per training chunk (0: about 2M camera samples
                                per chunk, at most GuideTrainSpp / 8); the guide is updated after
                                each chunk */
-    u32 Reserved;           /* 0 */
+    u32 ScatterSampling;    /* hemisphere scatter directions: AZ_SCATTER_REFERENCE (the reference's
+                               cosine or uniform hemisphere) or AZ_SCATTER_BRDF (a one-sample mixture
+                               of the cosine lobe and the material's specular lobe, weighted by the
+                               mixture pdf; unbiased, different random directions) */
 } az_render_opts;
 
 #define AZ_ENV_COSINE    0u
 #define AZ_ENV_LUMINANCE 1u
+
+#define AZ_SCATTER_REFERENCE 0u
+#define AZ_SCATTER_BRDF      1u
 
 typedef struct {
     f64 LastKernelMs;       /* device time of the last trace/render call's kernels */

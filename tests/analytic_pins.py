@@ -291,3 +291,52 @@ def check_scatter(render):
         # 65,536 scatter samples: Bernoulli(F ~ 0.55) noise of 0.35 % (cosine),
         # about 0.5 % (uniform).
         np.testing.assert_allclose(part.mean(0), want, rtol=0.02, err_msg=f"importance={importance}")
+
+
+# ---------------------------------------------------------------- glossy scatter (both samplers)
+
+GLOSSY_KD, GLOSSY_KS, GLOSSY_EXP = (0.2, 0.15, 0.1), (0.6, 0.6, 0.6), 30.0
+
+
+def glossy_scatter_scene(path_trace):
+    """scatter_scene with a normalized modified Blinn-Phong floor."""
+    b = SceneBuilder(max_depth=1, path_trace=path_trace, importance_sampling=True)
+    floor = b.add_material(make_material(kd=GLOSSY_KD, ks=GLOSSY_KS, exponent=GLOSSY_EXP,
+                                         brdf_type=N.BRDF_NormalizedModifiedBlinnPhong))
+    dark = b.add_material(make_material(kd=(0, 0, 0)))
+    _floor(b, floor)
+    q = np.array([(-1, 1, -1), (1, 1, -1), (1, 1, 1), (-1, 1, 1)], np.float32)
+    b.add_mesh(q, np.array([[0, 1, 2], [0, 2, 3]], np.uint32), material=dark, emitter=True,
+               radiance=CEILING_RADIANCE)
+    return b.build()
+
+
+def glossy_scatter_expected(n=600):
+    """int_quad f(L) cos(theta) L dOmega at the origin, viewer straight up:
+    f = Kd / pi + Ks (e + 8) / (8 pi) (N . H)^e (main.cpp:532-548)."""
+    g = (np.arange(n) + 0.5) / n * 2.0 - 1.0
+    x, z = np.meshgrid(g, g)
+    d = np.stack([x, np.ones_like(x), z], -1)
+    dist = np.linalg.norm(d, axis=-1)
+    L = d / dist[..., None]
+    H = L + np.array([0.0, 1.0, 0.0])
+    H /= np.linalg.norm(H, axis=-1)[..., None]
+    spec = (GLOSSY_EXP + 8.0) / (8.0 * math.pi) * H[..., 1] ** GLOSSY_EXP
+    cos = L[..., 1]
+    w = cos * cos / dist ** 2 * (2.0 / n) ** 2            # cos theta x dOmega (cos' dA / d^2)
+    kd, ks = np.asarray(GLOSSY_KD), np.asarray(GLOSSY_KS)
+    return np.asarray(CEILING_RADIANCE) * (kd * float(np.sum(w)) / math.pi + ks * float(np.sum(w * spec)))
+
+
+def check_glossy_scatter(render_opts):
+    """render_opts(scene, camera, sampling) renders with the given scatter
+    sampler.  Each sampler's scatter part, isolated as in check_scatter,
+    converges to the quadrature."""
+    W = H = 8
+    cam = configs.look_at_camera((0, 0.5, 0), (0, 0, 0), (0, 0, 1), W, H, 2.0, spp=1024)
+    want = glossy_scatter_expected()
+    off = render_opts(glossy_scatter_scene(False), cam, N.AZ_SCATTER_REFERENCE)
+    for sampling in (N.AZ_SCATTER_REFERENCE, N.AZ_SCATTER_BRDF):
+        on = render_opts(glossy_scatter_scene(True), cam, sampling)
+        part = (on["Color"] - off["Color"]) / on["TotalWeight"][:, None]
+        np.testing.assert_allclose(part.mean(0), want, rtol=0.03, err_msg=f"sampling={sampling}")

@@ -4,6 +4,7 @@ the pinhole camera's pixel-centre rays."""
 import pytest
 
 import analytic_pins as P
+from azimuth_amd import native as N
 
 pytestmark = pytest.mark.gpu
 
@@ -33,3 +34,12 @@ def test_sphere_light_cone_sampling(gpu):
 
 def test_hemisphere_scatter_samplers(gpu):
     P.check_scatter(make_render(gpu))
+
+
+def test_glossy_scatter_both_samplers(gpu):
+    """The BRDF lobe sampler (an option) and the reference's cosine sampler
+    both converge to the quadrature of the reference BRDF over the emitter."""
+    def render_opts(scene, camera, sampling):
+        gpu.upload_scene(scene)
+        return gpu.render(camera, filt=N.FilterType_Box, seed=1, scatter_sampling=sampling)
+    P.check_glossy_scatter(render_opts)
