@@ -305,17 +305,30 @@ __device__ inline int spec_lobe(const DevMaterial &M) {
     return M.brdfType <= 2 ? 1 : 2;
 }
 
-// Probability of drawing from the specular lobe: its reflectance's share of
-// the luminance of both (Torrance-Sparrow's specular reflectance scaled by
-// its Schlick Fresnel term at the viewing angle), kept within [0.1, 0.9] so
-// that neither lobe is starved.
+// Probability of drawing from the specular lobe: its share of the two
+// lobes' approximate albedos (luminance of the reflectance times the lobe's
+// cosine-weighted integral: 1 for the normalized models' lobes and
+// Torrance-Sparrow's diffuse; pi for the unnormalized diffuse (Kd);
+// 2 pi / (e + 2) or 8 pi / (e + 8) for the unnormalized Phong and Blinn
+// lobes; Torrance-Sparrow's specular scaled by its Schlick term at the
+// viewing angle), kept within [0.1, 0.9] so that neither lobe is starved.
+// Only the noise depends on this choice, not the expected value.
 __device__ inline float spec_prob(const DevMaterial &M, V3 kd, V3 N, V3 Vo) {
     float ls = 0.2126f * M.ks[0] + 0.7152f * M.ks[1] + 0.0722f * M.ks[2];
-    if (M.brdfType >= 6) {
+    float ld = 0.2126f * kd.x + 0.7152f * kd.y + 0.0722f * kd.z;
+    const float e = M.exponent;
+    switch (M.brdfType) {
+    case 0:
+    case 1: ls *= 2.0f * kPiF / (e + 2.0f); ld *= kPiF; break;
+    case 3:
+    case 4: ls *= 8.0f * kPiF / (e + 8.0f); ld *= kPiF; break;
+    case 2:
+    case 5: break;
+    default: {
         const float r0 = sq(M.brdfIor - 1.0f) / sq(M.brdfIor + 1.0f);
         ls *= r0 + (1.0f - r0) * powf(1.0f - fminf(1.0f, fmaxf(0.0f, dot(N, Vo))), 5.0f);
+    } break;
     }
-    const float ld = 0.2126f * kd.x + 0.7152f * kd.y + 0.0722f * kd.z;
     return fminf(0.9f, fmaxf(0.1f, ls / fmaxf(ls + ld, 1e-20f)));
 }
 

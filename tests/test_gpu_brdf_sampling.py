@@ -28,8 +28,9 @@ def resolve(px):
     return px[:, :3] / np.maximum(px[:, 3:4], 1e-300)
 
 
-@pytest.mark.parametrize("brdf_type", [N.BRDF_NormalizedModifiedBlinnPhong, N.BRDF_NormalizedModifiedPhong,
-                                       N.BRDF_TorranceSparrow])
+@pytest.mark.parametrize("brdf_type", [N.BRDF_Phong, N.BRDF_ModifiedPhong, N.BRDF_NormalizedModifiedPhong,
+                                       N.BRDF_BlinnPhong, N.BRDF_ModifiedBlinnPhong,
+                                       N.BRDF_NormalizedModifiedBlinnPhong, N.BRDF_TorranceSparrow])
 def test_brdf_sampling_unbiased_and_lower_variance(gpu, brdf_type):
     sc = glossy_scene(brdf_type)
     cam = configs.look_at_camera((0, 1, 1), (0, 0, 0), (0, 1, 0), 48, 48, 40.0, spp=64)
