@@ -123,8 +123,8 @@ struct alignas(16) ShadowRay {
     uint32_t sample;
     uint32_t mode;                   // kShadowAny or kShadowEntity
     uint32_t entityId;               // kShadowEntity: the light's UniqueEntityID
-    uint32_t rec;                    // guide training record of this ray, or kEmpty
-    uint32_t pad1;
+    uint32_t rec;                    // environment guide training record of this ray, or kEmpty
+    uint32_t srec;                   // scatter guide record whose path this ray's vertex is on, or kEmpty
 };
 
 struct CameraSetup {                 // main.cpp:1265-1347

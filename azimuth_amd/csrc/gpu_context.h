@@ -91,13 +91,15 @@ struct az_gpu_ctx {
     az::RenderState render;
     az::DevScene devScene{};
     void *renderBuffers = nullptr;   // owned; released by az_release_render_buffers
-    az_guide *guide = nullptr;       // owned; the learned guide of guided renders
+    az_guide *guide = nullptr;       // owned; the learned guide of guided renders (environment NEE)
+    az_guide *scatterGuide = nullptr;   // owned; the learned guide of the hemisphere scatter
     az_stats stats{};
     bool recordLaunches = false;
     std::vector<hipEvent_t> launchEvents;   // pairs: start, stop
     size_t launchUsed = 0;                  // pairs in use since the last collect
     ~az_gpu_ctx() {
         if (guide) az_guide_destroy(guide);
+        if (scatterGuide) az_guide_destroy(scatterGuide);
         az_release_render_buffers(renderBuffers);
         for (hipEvent_t e : launchEvents) (void)hipEventDestroy(e);
         if (ev0) (void)hipEventDestroy(ev0);

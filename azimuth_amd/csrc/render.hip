@@ -104,6 +104,25 @@ __device__ inline void add3(AccWord *a, V3 c) {
     atomicAdd(a + 2, acc_fixed(c.z));
 }
 
+// Luminance weights of the guide's training targets.
+__device__ inline float guide_lum(V3 c) { return 0.27f * c.x + 0.67f * c.y + 0.06f * c.z; }
+
+// A scatter guide training record's place in its sample's path tree: the
+// luminance that its scatter ray's subtree delivered to the sample (fixed
+// point, as the colour accumulators), the record of the nearest scatter
+// vertex above it (kEmpty at the first), and the depth of its vertex.
+struct ScatterLink {
+    AccWord sum;
+    uint32_t parent;
+    uint32_t depth;
+};
+
+// A contribution c to the sample, delivered along the subtree of scatter
+// record rec (kEmpty: none).
+__device__ inline void credit(ScatterLink *links, uint32_t rec, V3 c) {
+    if (rec != kEmpty) atomicAdd(&links[rec].sum, acc_fixed(guide_lum(c)));
+}
+
 // ---- generate: camera rays (main.cpp:1135-1178) ----------------------------
 // Kernel bodies take their block index: the same code runs as a one-scene
 // kernel (block = blockIdx.x) and as part of a batched launch over several
@@ -225,6 +244,7 @@ struct ConnectPool {
     const DeviceGeometry *G;
     AccWord *acc;
     GuideSample *recs;
+    ScatterLink *links;
     __device__ inline const ShadowRay &at(uint32_t id) const { return sq[seg_slot(off, in.nseg, in.segCap, id)]; }
     __device__ inline bool fetch(uint32_t id, Ray3 &r, float &tMax) {
         const ShadowRay &s = at(id);
@@ -241,7 +261,10 @@ struct ConnectPool {
     __device__ inline void finish(uint32_t id, bool found, const HitRec &h) {
         const ShadowRay &s = at(id);
         const bool reaches = s.mode == kShadowAny ? !found : (!found || G->items[h.item].entityId == s.entityId);
-        if (reaches) add3(acc + 3 * s.sample, v3(s.cx, s.cy, s.cz));
+        if (reaches) {
+            add3(acc + 3 * s.sample, v3(s.cx, s.cy, s.cz));
+            credit(links, s.srec, v3(s.cx, s.cy, s.cz));
+        }
         // A guide training sample whose direction is blocked saw no radiance.
         if (s.rec != kEmpty && !reaches) recs[s.rec].wgt = 0.0f;
     }
@@ -249,14 +272,14 @@ struct ConnectPool {
 
 template <int BLOCK, bool WW>
 __device__ __forceinline__ void connect_body(const DeviceGeometry &G, const ShadowRay *__restrict__ sq, SegIn sin, uint32_t n,
-                                    AccWord *__restrict__ acc, GuideSample *__restrict__ recs, uint32_t perWave,
-                                    uint32_t block) {
+                                    AccWord *__restrict__ acc, GuideSample *__restrict__ recs,
+                                    ScatterLink *__restrict__ links, uint32_t perWave, uint32_t block) {
     __shared__ uint32_t smem[kRenderRing * BLOCK];
     __shared__ uint32_t soff[kSegs + 1];
     const uint32_t *off = stage_offsets(sin, soff);
     const uint32_t begin = (block * (BLOCK / 64) + (threadIdx.x >> 6)) * perWave;
     const uint32_t end = min(begin + perWave, n);
-    ConnectPool pool{sq, off, sin, &G, acc, recs};
+    ConnectPool pool{sq, off, sin, &G, acc, recs, links};
     TraceCounters c{0u, 0u, 0u};
     trace_pool<2, false, kRenderRing, BLOCK, WW>(G, pool, begin, end, 0.0f, false, smem + threadIdx.x, c);
 }
@@ -264,8 +287,9 @@ __device__ __forceinline__ void connect_body(const DeviceGeometry &G, const Shad
 template <int BLOCK, int WAVES, bool WW>
 __global__ __launch_bounds__(BLOCK, WAVES) void k_connect(DeviceGeometry G, const ShadowRay *__restrict__ sq, SegIn sin,
                                                          uint32_t n, AccWord *__restrict__ acc,
-                                                         GuideSample *__restrict__ recs, uint32_t perWave) {
-    connect_body<BLOCK, WW>(G, sq, sin, n, acc, recs, perWave, blockIdx.x);
+                                                         GuideSample *__restrict__ recs,
+                                                         ScatterLink *__restrict__ links, uint32_t perWave) {
+    connect_body<BLOCK, WW>(G, sq, sin, n, acc, recs, links, perWave, blockIdx.x);
 }
 
 static hipError_t launch_extend(const RenderTuning &T, int cuCount, const DeviceGeometry &G, const PathRay *q,
@@ -284,13 +308,14 @@ static hipError_t launch_extend(const RenderTuning &T, int cuCount, const Device
 }
 
 static hipError_t launch_connect(const RenderTuning &T, int cuCount, const DeviceGeometry &G, const ShadowRay *sq,
-                                 SegIn sin, uint32_t n, AccWord *acc, GuideSample *recs, hipStream_t st) {
+                                 SegIn sin, uint32_t n, AccWord *acc, GuideSample *recs, ScatterLink *links,
+                                 hipStream_t st) {
     const uint32_t per = rays_per_wave(n, cuCount, T.traceWaves);
 #define AZ_CASE(B, W, F)                                                                                  \
     if (T.traceBlock == B && T.traceWaves == W && T.traceLoop == F) {                                   \
         const uint32_t perBlock = (B / 64) * per;                                                        \
         hipLaunchKernelGGL((k_connect<B, W, F != 0>), dim3((n + perBlock - 1) / perBlock), dim3(B), 0, st, G, sq,   \
-                           sin, n, acc, recs, per);                                                      \
+                           sin, n, acc, recs, links, per);                                               \
         return hipGetLastError();                                                                        \
     }
     AZ_RENDER_TRACE_VARIANTS(AZ_CASE)
@@ -329,6 +354,16 @@ struct GuideIO {
     uint32_t *recCounters;           // kSegs counters, kSegStride apart
     uint32_t recSegCap;
     float alpha;                     // probability of drawing from the guide
+    // Scatter guide: lobes per queued ray (null: plain scatter sampling);
+    // training records, their links and kSegs counters (null when not
+    // training); per slot of the current queue, the record whose subtree the
+    // ray is in (null at the first level).
+    const float *scatLobes;
+    GuideSample *srecs;
+    ScatterLink *links;
+    uint32_t *srecCounters;
+    uint32_t srecSegCap;
+    const uint32_t *rayRec;
 };
 
 // Guide query of a hit: position in the scene box, normal, roughness.
@@ -434,11 +469,16 @@ struct ShadeOut {
     ShadowRay *shadow;
     uint32_t *childCounters, *shadowCounters;     // kSegs counters each, kSegStride apart
     uint32_t nseg, childSegCap, shadowSegCap;
+    uint32_t *nextRec;               // per slot of next: the child's scatter guide record (null: not kept)
 };
 
-__device__ inline void emit_child(const ShadeOut &O, uint32_t seg, bool want, const PathRay &c) {
+__device__ inline void emit_child(const ShadeOut &O, uint32_t seg, bool want, const PathRay &c, uint32_t rec) {
     uint32_t idx = wave_append(O.childCounters + seg * kSegStride, want);
-    if (want && idx < O.childSegCap) O.next[(size_t)seg * O.childSegCap + idx] = c;
+    if (want && idx < O.childSegCap) {
+        const size_t slot = (size_t)seg * O.childSegCap + idx;
+        O.next[slot] = c;
+        if (O.nextRec) O.nextRec[slot] = rec;
+    }
 }
 
 __device__ inline void emit_shadow(const ShadeOut &O, uint32_t seg, bool want, const ShadowRay &s) {
@@ -464,7 +504,7 @@ __device__ inline ShadowRay make_shadow(V3 o, V3 d, float time, float tmax, V3 c
     s.mode = mode;
     s.entityId = entityId;
     s.rec = kEmpty;
-    s.pad1 = 0u;
+    s.srec = kEmpty;
     return s;
 }
 
@@ -586,6 +626,9 @@ __device__ inline void reconstruct(const DevScene &S, const DeviceGeometry &G, c
 
 }
 
+// GUIDED = false compiles the learned guides' code out (plain renders and
+// batches); with true it runs where GI's pointers are set.
+template <bool GUIDED>
 __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeometry &G, const CameraSetup &cam,
                                   const ChunkParams &cp, const PathRay *__restrict__ q, SegIn qin,
                                   const PathHit *__restrict__ hits, uint32_t n, AccWord *__restrict__ acc,
@@ -618,9 +661,11 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
     const uint32_t i = block * kRenderBlock + order[threadIdx.x];
     if (i >= n) return;
     const uint32_t seg = block % O.nseg;
-    const PathRay r = q[seg_slot(off, qin.nseg, qin.segCap, i)];
+    const uint32_t slot = seg_slot(off, qin.nseg, qin.segCap, i);
+    const PathRay r = q[slot];
     const PathHit h = hits[i];
     const uint32_t depth = r.flags & 0xFFu;
+    const uint32_t rin = GUIDED && GI.rayRec && depth > 0u ? GI.rayRec[slot] : kEmpty;
     const bool inside = (r.flags & 0x100u) != 0u;
     const uint32_t pixLocal = r.sample / cp.spp;
     const uint32_t pixel = cp.row0 * cam.width + pixLocal;
@@ -655,7 +700,9 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
 
     const DevItemShade sh = S.itemShade[h.item];
     if (sh.isEmitter) {                                   // main.cpp:984-985
-        add3(a, had(w, v3(sh.radiance)));
+        const V3 e = had(w, v3(sh.radiance));
+        add3(a, e);
+        credit(GI.links, rin, e);
         return;
     }
 
@@ -674,6 +721,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
         if (tex->blend == 0) {            // ReplaceAll: DC starts at 1, no ambient
             kd = ts;
             add3(a, had(w, ts));
+            credit(GI.links, rin, had(w, ts));
             ignoreAmbient = true;
         } else if (tex->blend == 1) {
             kd = ts;
@@ -758,7 +806,9 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
         // A contribution of exactly zero (outside a spot cone, a black
         // material) adds nothing whatever the occlusion: no shadow ray.
         want = want && (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f);
-        emit_shadow(O, seg, want, make_shadow(P + (Ld * eps), Ld, r.time, tmax, c, r.sample, mode, L.entityId));
+        ShadowRay sr = make_shadow(P + (Ld * eps), Ld, r.time, tmax, c, r.sample, mode, L.entityId);
+        sr.srec = rin;
+        emit_shadow(O, seg, want, sr);
     }
     // Environment NEE, cosine-distributed (main.cpp:719-778).  With the
     // guide, the direction comes from the guide with probability alpha and
@@ -767,7 +817,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
     // estimator unbiased: the cosine term covers every direction with
     // non-zero contribution.
     if (S.useEnv) {
-        const float *lb = GI.lobes ? GI.lobes + (size_t)i * 40 : nullptr;
+        const float *lb = GUIDED && GI.lobes ? GI.lobes + (size_t)i * 40 : nullptr;
         const bool lum = cp.envLuminance != 0u;
         V3 L;
         bool fromGuide = false;
@@ -809,7 +859,8 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
         if (want) c = had(w, c);
         want = want && (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f);
         ShadowRay sr = make_shadow(P + (L * eps), L, r.time, INFINITY, c, r.sample, kShadowAny, 0u);
-        if (GI.recs) {
+        sr.srec = rin;
+        if (GUIDED && GI.recs) {
             // Training sample: direction, weight = luminance(L_env) cos / pdf;
             // connect zeroes the weight when the direction is blocked.
             const uint32_t rseg = block % kSegs;
@@ -825,7 +876,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
                 g.dir[0] = L.x;
                 g.dir[1] = L.y;
                 g.dir[2] = L.z;
-                g.wgt = (0.27f * I.x + 0.67f * I.y + 0.06f * I.z) * fmaxf(ca, 0.0f) * invPdf;
+                g.wgt = guide_lum(I) * fmaxf(ca, 0.0f) * invPdf;
                 g.pad = 0.0f;
                 GI.recs[idx] = g;
                 sr.rec = idx;
@@ -836,15 +887,23 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
     const bool childOk = depth + 1u <= S.maxDepth;
     const V3 zero = v3(0.0f, 0.0f, 0.0f);
     // Hemisphere scatter (main.cpp:1024-1083), or, as an option, from the
-    // BRDF lobe mixture.
+    // BRDF lobe mixture.  With the scatter guide, the direction comes from
+    // the guide with probability alpha and from that distribution otherwise,
+    // weighted by the mixture pdf (one-sample MIS, balance heuristic), which
+    // keeps the estimator unbiased: the base distribution covers every
+    // direction with non-zero BRDF.
     if (S.pathTrace) {
+        const float *sl = GUIDED && GI.scatLobes ? GI.scatLobes + (size_t)i * 40 : nullptr;
+        bool fromGuide = false;
+        if (sl) fromGuide = rng.uniform() < GI.alpha;
         V3 Sd;
-        float invPdf;
-        if (cp.brdfSampling) {
+        float invPdf, pBase = 0.0f;
+        if (fromGuide) {
             const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
-            float pdf;
-            Sd = sample_brdf_mixture(M, kd, N, -D, u0, u1, u2, pdf);
-            invPdf = pdf > 0.0f ? 1.0f / pdf : 0.0f;
+            Sd = guide_sample(sl, u0, u1, u2);
+        } else if (cp.brdfSampling) {
+            const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
+            Sd = sample_brdf_mixture(M, kd, N, -D, u0, u1, u2, pBase);
         } else {
             float theta = S.importance ? Asin(sqrtf(rng.uniform())) : Acos(rng.uniform());
             float ct = cosf(theta), st = sinf(theta);
@@ -853,25 +912,67 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
             V3 U, Vv;
             orthonormal_basis(N, U, Vv);
             Sd = normalize(U * st * cp_ + Vv * st * sp_ + N * ct);
-            if (S.importance) {
-                float cs = dot(Sd, N);
-                if (cs == 0.0f) cs = 0.000001f;
-                invPdf = kPiF / cs;
-            } else {
-                invPdf = kPiF * 2.0f;
+            pBase = S.importance ? fmaxf(dot(Sd, N), 0.0f) / kPiF : 1.0f / (2.0f * kPiF);
+        }
+        if (sl) {
+            if (fromGuide) {
+                const float cs = dot(Sd, N);
+                pBase = cp.brdfSampling ? brdf_mixture_pdf(M, kd, N, -D, Sd)
+                                        : (cs > 0.0f ? (S.importance ? cs / kPiF : 1.0f / (2.0f * kPiF)) : 0.0f);
             }
+            const float pdf = GI.alpha * guide_pdf(sl, Sd) + (1.0f - GI.alpha) * pBase;
+            invPdf = pdf > 0.0f ? 1.0f / pdf : 0.0f;
+        } else if (cp.brdfSampling) {
+            invPdf = pBase > 0.0f ? 1.0f / pBase : 0.0f;
+        } else if (S.importance) {
+            float cs = dot(Sd, N);
+            if (cs == 0.0f) cs = 0.000001f;
+            invPdf = kPiF / cs;
+        } else {
+            invPdf = kPiF * 2.0f;
         }
         V3 cw;
         bool want = childOk && invPdf > 0.0f && scatter_weight(M, kd, Sd, D, N, invPdf, cw);
         if (want) cw = had(w, cw);
+        uint32_t crec = rin;
+        if (GUIDED && GI.srecs) {
+            // Training sample of the scatter guide: direction and
+            // cos / (pdf lum(child throughput)); the luminance that the
+            // child's subtree delivers to the sample, divided by the child
+            // throughput, estimates the incident radiance, and multiplies
+            // this weight when the chunk is done.
+            const uint32_t rseg = block % kSegs;
+            uint32_t idx = wave_append(GI.srecCounters + rseg * kSegStride, want);
+            if (want && idx < GI.srecSegCap) {
+                idx += rseg * GI.srecSegCap;
+                const float lw = guide_lum(cw);
+                GuideSample g;
+                guide_position(S, P, g.p);
+                g.n[0] = N.x;
+                g.n[1] = N.y;
+                g.n[2] = N.z;
+                g.rough = guide_roughness(M);
+                g.dir[0] = Sd.x;
+                g.dir[1] = Sd.y;
+                g.dir[2] = Sd.z;
+                g.wgt = lw > 0.0f ? fmaxf(dot(Sd, N), 0.0f) * invPdf / lw : 0.0f;
+                g.pad = 0.0f;
+                GI.srecs[idx] = g;
+                GI.links[idx] = ScatterLink{0ull, rin, depth};
+                crec = idx;
+            }
+        }
         emit_child(O, seg, want, make_child(P + (Sd * eps), Sd, r.time, child_node(r.node, 0u), cw, r.sample, zero,
-                                       depth + 1u, inside));
+                                       depth + 1u, inside), crec);
     }
     // Ambient (main.cpp:1088-1099).
     const bool transparent = (M.flags & kMatTransparent) != 0u;
     if ((!inside || !transparent) && !ignoreAmbient) {
         V3 am = had(w, had(v3(M.ka), v3(S.ambient)));
-        if (am.x != 0.0f || am.y != 0.0f || am.z != 0.0f) add3(a, am);
+        if (am.x != 0.0f || am.y != 0.0f || am.z != 0.0f) {
+            add3(a, am);
+            credit(GI.links, rin, am);
+        }
     }
     // Mirror (main.cpp:781-854).
     if (M.flags & kMatMirror) {
@@ -894,7 +995,8 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
             R = normalize(R);
         }
         emit_child(O, seg, childOk, make_child(P + (R * 0.000825f), R, r.time, child_node(r.node, 1u),
-                                          had(w, v3(M.km)), r.sample, v3(r.cx, r.cy, r.cz), depth + 1u, inside));
+                                          had(w, v3(M.km)), r.sample, v3(r.cx, r.cy, r.cz), depth + 1u, inside),
+                   rin);
     }
     // Dielectric (main.cpp:856-964): both refraction and reflection are traced.
     if (transparent) {
@@ -929,19 +1031,21 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
             rc = make_child(P + (T * 0.000100f), T, r.time, child_node(r.node, 2u), w * (1.0f - refl), r.sample, coef,
                             depth + 1u, !inside);
         }
-        emit_child(O, seg, childOk && refract, rc);
+        emit_child(O, seg, childOk && refract, rc, rin);
         V3 R = reflect(-D, Nn);
         V3 coef = inside ? v3(tr.x - 1.0f, tr.y - 1.0f, tr.z - 1.0f) : zero;
         emit_child(O, seg, childOk, make_child(P + (R * 0.000100f), R, r.time, child_node(r.node, 3u), w * refl, r.sample,
-                                          coef, depth + 1u, inside));
+                                          coef, depth + 1u, inside),
+                   rin);
     }
 }
 
+template <bool GUIDED>
 __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeometry G, CameraSetup cam, ChunkParams cp,
                                                         const PathRay *__restrict__ q, SegIn qin,
                                                         const PathHit *__restrict__ hits, uint32_t n,
                                                         AccWord *__restrict__ acc, ShadeOut O, GuideIO GI) {
-    shade_body(S, G, cam, cp, q, qin, hits, n, acc, O, GI, blockIdx.x);
+    shade_body<GUIDED>(S, G, cam, cp, q, qin, hits, n, acc, O, GI, blockIdx.x);
 }
 
 // ---- learned guide: queries and training batches ------------------------------------------
@@ -986,6 +1090,34 @@ __global__ __launch_bounds__(kRenderBlock) void k_guide_gather(const GuideSample
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     z ^= z >> 31;
     batch[i] = recs[seg_slot(off, rin.nseg, rin.segCap, (uint32_t)(z % count))];
+}
+
+// Scatter guide records of depth `depth` pass the luminance their subtree
+// delivered on to their parent record.  Launched for depth = maxDepth - 1
+// down to 1 after a chunk's paths are done, so that a record holds its whole
+// subtree's sum when it passes it on (its descendants are deeper; no record
+// of this depth is a parent of another).
+__global__ __launch_bounds__(kRenderBlock) void k_scatter_propagate(ScatterLink *__restrict__ links, SegIn rin,
+                                                                    uint32_t count, uint32_t depth) {
+    __shared__ uint32_t soff[kSegs + 1];
+    const uint32_t *off = stage_offsets(rin, soff);
+    const uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
+    if (i >= count) return;
+    const ScatterLink L = links[seg_slot(off, rin.nseg, rin.segCap, i)];
+    if (L.depth == depth && L.parent != kEmpty && L.sum != 0ull) atomicAdd(&links[L.parent].sum, L.sum);
+}
+
+// Training weight of each scatter record: its factor times the luminance its
+// subtree delivered.
+__global__ __launch_bounds__(kRenderBlock) void k_scatter_finalize(GuideSample *__restrict__ recs,
+                                                                   const ScatterLink *__restrict__ links, SegIn rin,
+                                                                   uint32_t count) {
+    __shared__ uint32_t soff[kSegs + 1];
+    const uint32_t *off = stage_offsets(rin, soff);
+    const uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t slot = seg_slot(off, rin.nseg, rin.segCap, i);
+    recs[slot].wgt *= acc_value(links[slot].sum);
 }
 
 // Segment counts -> offsets (one block per queue, one thread per segment):
@@ -1225,7 +1357,7 @@ __global__ __launch_bounds__(BLOCK, WAVES) void kb_extend(const ExtArgs *__restr
 __global__ __launch_bounds__(kRenderBlock, 4) void kb_shade(const ShadeArgs *__restrict__ A, BatchMap M) {
     const uint32_t s = batch_scene(M, blockIdx.x);
     const ShadeArgs &a = A[s];
-    shade_body(a.S, a.G, a.cam, a.cp, a.q, a.qin, a.hits, a.n, a.acc, a.O, a.GI, blockIdx.x - M.start[s]);
+    shade_body<false>(a.S, a.G, a.cam, a.cp, a.q, a.qin, a.hits, a.n, a.acc, a.O, a.GI, blockIdx.x - M.start[s]);
 }
 
 __global__ __launch_bounds__(kSegs) void kb_seg_scan(const SegScan *__restrict__ P) {
@@ -1236,7 +1368,7 @@ template <int BLOCK, int WAVES, bool WW>
 __global__ __launch_bounds__(BLOCK, WAVES) void kb_connect(const ConnArgs *__restrict__ A, BatchMap M) {
     const uint32_t s = batch_scene(M, blockIdx.x);
     const ConnArgs &a = A[s];
-    connect_body<BLOCK, WW>(a.G, a.sq, a.sin, a.n, a.acc, a.recs, 64u * kRaysPerLane, blockIdx.x - M.start[s]);
+    connect_body<BLOCK, WW>(a.G, a.sq, a.sin, a.n, a.acc, a.recs, nullptr, 64u * kRaysPerLane, blockIdx.x - M.start[s]);
 }
 
 template <typename T>
@@ -1395,6 +1527,7 @@ struct RenderBuffers {
     DeviceBuffer materials, textures, texels, perms, lights, lightTris, prims, itemShade, perlin, pcg;
     DeviceBuffer q0, q1, hits, shadow, acc, counters, pixels;
     DeviceBuffer queries, lobes, recs, batch;   // learned guide
+    DeviceBuffer scatLobes, srecs, links, qrec0, qrec1;   // scatter guide: lobes, records, per-slot record of the queues
     DeviceBuffer envCdf;                        // environment luminance distribution
     DeviceBuffer batchArgs, batchTotals;        // batched renders: per-scene arguments, queue totals
 };
@@ -1823,10 +1956,14 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
     const u32 nPix = rows * cam.width;
     const u32 sppTotal = cam.sppSqrt * cam.sppSqrt;
     DevScene &S = C->devScene;
-    // The learned guide: a fresh network per render, trained on the
-    // environment samples of the first trainSpp samples per pixel.
-    const bool guided = o.GuideEnable != 0 && S.useEnv;
-    if (o.EnvSampling > AZ_ENV_LUMINANCE || o.ScatterSampling > AZ_SCATTER_BRDF) return AZ_ERR_INVALID_ARGUMENT;
+    // The learned guide: a fresh network per render and use (environment
+    // NEE, scatter), trained on the samples of the first trainSpp samples per
+    // pixel.
+    if (o.EnvSampling > AZ_ENV_LUMINANCE || o.ScatterSampling > AZ_SCATTER_BRDF || o.GuideTarget > AZ_GUIDE_BOTH)
+        return AZ_ERR_INVALID_ARGUMENT;
+    const bool envGuide = o.GuideEnable != 0 && o.GuideTarget != AZ_GUIDE_SCATTER && S.useEnv;
+    const bool scatGuide = o.GuideEnable != 0 && o.GuideTarget != AZ_GUIDE_ENV && S.pathTrace;
+    const bool guided = envGuide || scatGuide;
     const bool envLum = o.EnvSampling == AZ_ENV_LUMINANCE && S.useEnv;
     if (envLum && !S.envCdf) {
         C->lastError = "luminance sampling needs an image environment map";
@@ -1848,40 +1985,52 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
     const u32 stepsPerChunk =
         o.GuideStepsPerChunk ? o.GuideStepsPerChunk : std::max<u32>(1u, (kGuideSteps + trainChunks - 1) / trainChunks);
     const float alpha = o.GuideFraction > 0.0f && o.GuideFraction < 1.0f ? o.GuideFraction : 0.5f;
-    // Training records: room for one per path vertex of a training chunk (a
-    // segment that fills drops its later records, which would be the deep
-    // levels'), between 8M and 64M records (48 B each).
+    // Training records, for each guide: room for one per path vertex of a
+    // training chunk (a segment that fills drops its later records, which
+    // would be the deep levels'), between 8M and 64M records (48 B each, and
+    // a 16-byte link for a scatter record).
     const uint64_t recWant = (uint64_t)nPix * trainChunkSpp * (S.maxDepth + 1u);
     const u32 recCap = (u32)std::min<uint64_t>(64u << 20, std::max<uint64_t>(8u << 20, (recWant + kSegs - 1) / kSegs * kSegs));
-    if (guided) {
-        if (C->guide) az_guide_destroy(C->guide);
-        C->guide = nullptr;
+    if (C->guide) az_guide_destroy(C->guide);
+    if (C->scatterGuide) az_guide_destroy(C->scatterGuide);
+    C->guide = C->scatterGuide = nullptr;
+    if (envGuide) {
         az_status gs = az_guide_create(&C->guide, C->device, o.Seed ^ 0x67756964ull);
         if (gs != AZ_OK) {
             C->lastError = "guide creation failed";
             return gs;
         }
         AZ_HIPR(C, B.recs.alloc((size_t)recCap * sizeof(GuideSample)));
-        AZ_HIPR(C, B.batch.alloc((size_t)65536 * sizeof(GuideSample)));
     }
+    if (scatGuide) {
+        az_status gs = az_guide_create(&C->scatterGuide, C->device, o.Seed ^ 0x7363617474657221ull);
+        if (gs != AZ_OK) {
+            C->lastError = "guide creation failed";
+            return gs;
+        }
+        AZ_HIPR(C, B.srecs.alloc((size_t)recCap * sizeof(GuideSample)));
+        AZ_HIPR(C, B.links.alloc((size_t)recCap * sizeof(ScatterLink)));
+    }
+    if (guided) AZ_HIPR(C, B.batch.alloc((size_t)65536 * sizeof(GuideSample)));
     // Samples per pixel per chunk: the tuned chunk size (16M camera samples
     // by default); while the guide trains, at least 8 chunks so that it is
     // updated often.
     u32 chunkSpp = chunkSpp0;
     const u32 shadowPerRay = S.lightCount + (S.useEnv ? 1u : 0u);
     hipStream_t st = C->stream;
-    // Counters and offsets of the segmented queues: child, shadow and
-    // training-record counters (kSegs each, one 128-byte line apart), the
-    // totals, and the offsets of the current and next path queues, of the
-    // shadow queue and of the training records.
+    // Counters and offsets of the segmented queues: child, shadow and the two
+    // guides' training-record counters (kSegs each, one 128-byte line
+    // apart), the totals, and the offsets of the current and next path
+    // queues, of the shadow queue and of the training records.
     const size_t cntWords = (size_t)kSegs * kSegStride, offWords = kSegs + 1;
-    const size_t ctlWords = 3 * cntWords + 16 + 4 * offWords;
+    const size_t ctlWords = 4 * cntWords + 16 + 5 * offWords;
     AZ_HIPR(C, B.counters.alloc(ctlWords * sizeof(uint32_t)));
     uint32_t *ctl = (uint32_t *)B.counters.ptr;
-    uint32_t *childCnt = ctl, *shadowCnt = ctl + cntWords, *recCnt = ctl + 2 * cntWords;
-    uint32_t *dTotal = ctl + 3 * cntWords;
+    uint32_t *childCnt = ctl, *shadowCnt = ctl + cntWords, *recCnt = ctl + 2 * cntWords, *srecCnt = ctl + 3 * cntWords;
+    uint32_t *dTotal = ctl + 4 * cntWords;
     uint32_t *offQ[2] = {dTotal + 16, dTotal + 16 + offWords};
     uint32_t *offShadow = dTotal + 16 + 2 * offWords, *offRec = dTotal + 16 + 3 * offWords;
+    uint32_t *offSrec = dTotal + 16 + 4 * offWords;
     hipLaunchKernelGGL(k_clear_u32, dim3((u32)((ctlWords + 255) / 256)), dim3(256), 0, st, ctl, (u32)ctlWords);
     AZ_HIPR(C, hipGetLastError());
     const u32 recSegCap = recCap / kSegs;
@@ -1917,6 +2066,8 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
         cp.envLuminance = envLum ? 1u : 0u;
         cp.brdfSampling = o.ScatterSampling == AZ_SCATTER_BRDF ? 1u : 0u;
         DeviceBuffer *bufA = &B.q0, *bufB = &B.q1;
+        DeviceBuffer *recA = &B.qrec0, *recB = &B.qrec1;   // scatter training: per-slot records of the queues
+        const bool scatTraining = scatGuide && training;
         int offA = 0;
         AccWord *acc = (AccWord *)B.acc.ptr;
         hipLaunchKernelGGL(k_generate, dim3((n + kRenderBlock - 1) / kRenderBlock), dim3(kRenderBlock), 0, st, S, cam,
@@ -1931,7 +2082,8 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             const size_t cap = L.cap, shadowCap = L.shadowCap;
             size_t need = cap * sizeof(PathRay) + (size_t)count * (sizeof(PathRay) + sizeof(PathHit)) +
                           shadowCap * sizeof(ShadowRay) +
-                          (guided ? (size_t)count * (sizeof(GuideQuery) + 40 * sizeof(float)) : 0);
+                          (guided ? (size_t)count * (sizeof(GuideQuery) + 80 * sizeof(float)) : 0) +
+                          (scatTraining ? cap * sizeof(uint32_t) : 0);
             if (need > budget || cap > 0xFFFFFFFFull || shadowCap > 0xFFFFFFFFull) {
                 overflow = true;
                 break;
@@ -1939,6 +2091,7 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             ae = bufB->alloc(cap * sizeof(PathRay));
             if (ae == hipSuccess) ae = B.hits.alloc((size_t)count * sizeof(PathHit));
             if (ae == hipSuccess) ae = B.shadow.alloc(shadowCap * sizeof(ShadowRay));
+            if (ae == hipSuccess && scatTraining) ae = recB->alloc(cap * sizeof(uint32_t));
             if (out_of_memory(ae)) {
                 overflow = true;
                 break;
@@ -1949,26 +2102,49 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             AZ_HIPR(C, launch_extend(C->renderTuning, C->cuCount, C->dg, qa, qin, count, (PathHit *)B.hits.ptr,
                                      S.epsIntersection, (int)cp.cull, st));
             GuideIO GI{nullptr, nullptr, recCnt, recSegCap, alpha};
+            ShadeOut O{qb, (ShadowRay *)B.shadow.ptr, childCnt, shadowCnt, nseg, childSegCap, shadowSegCap};
             const bool guideLevel = o.GuideMaxDepth == 0 || depth <= o.GuideMaxDepth;
-            if (guided && training && !guideLevel) GI.recs = (GuideSample *)B.recs.ptr;
+            if (envGuide && training) GI.recs = (GuideSample *)B.recs.ptr;
+            if (scatTraining) {
+                GI.srecs = (GuideSample *)B.srecs.ptr;
+                GI.links = (ScatterLink *)B.links.ptr;
+                GI.srecCounters = srecCnt;
+                GI.srecSegCap = recSegCap;
+                GI.rayRec = depth > 0 ? (const uint32_t *)recA->ptr : nullptr;
+                O.nextRec = (uint32_t *)recB->ptr;
+            }
             if (guided && guideLevel) {
-                // Query every hit, then one batched MFMA inference.
+                // Query every hit, then one batched MFMA inference per guide.
                 AZ_HIPR(C, B.queries.alloc((size_t)count * sizeof(GuideQuery)));
-                AZ_HIPR(C, B.lobes.alloc((size_t)count * 40 * sizeof(float)));
                 hipLaunchKernelGGL(k_guide_query, grid, dim3(kRenderBlock), 0, st, S, C->dg, qa, qin,
                                    (const PathHit *)B.hits.ptr, count, (GuideQuery *)B.queries.ptr);
                 AZ_HIPR(C, hipGetLastError());
-                az_status gs = az_guide_infer_on_stream(C->guide, B.queries.ptr, count, (float *)B.lobes.ptr, st);
-                if (gs != AZ_OK) {
-                    C->lastError = "guide inference failed";
-                    return gs;
+                if (envGuide) {
+                    AZ_HIPR(C, B.lobes.alloc((size_t)count * 40 * sizeof(float)));
+                    az_status gs = az_guide_infer_on_stream(C->guide, B.queries.ptr, count, (float *)B.lobes.ptr, st);
+                    if (gs != AZ_OK) {
+                        C->lastError = "guide inference failed";
+                        return gs;
+                    }
+                    GI.lobes = (const float *)B.lobes.ptr;
                 }
-                GI.lobes = (const float *)B.lobes.ptr;
-                if (training) GI.recs = (GuideSample *)B.recs.ptr;
+                if (scatGuide) {
+                    AZ_HIPR(C, B.scatLobes.alloc((size_t)count * 40 * sizeof(float)));
+                    az_status gs = az_guide_infer_on_stream(C->scatterGuide, B.queries.ptr, count,
+                                                            (float *)B.scatLobes.ptr, st);
+                    if (gs != AZ_OK) {
+                        C->lastError = "guide inference failed";
+                        return gs;
+                    }
+                    GI.scatLobes = (const float *)B.scatLobes.ptr;
+                }
             }
-            ShadeOut O{qb, (ShadowRay *)B.shadow.ptr, childCnt, shadowCnt, nseg, childSegCap, shadowSegCap};
-            hipLaunchKernelGGL(k_shade, grid, dim3(kRenderBlock), 0, st, S, C->dg, cam, cp, qa, qin,
-                               (const PathHit *)B.hits.ptr, count, acc, O, GI);
+            if (guided)
+                hipLaunchKernelGGL(k_shade<true>, grid, dim3(kRenderBlock), 0, st, S, C->dg, cam, cp, qa, qin,
+                                   (const PathHit *)B.hits.ptr, count, acc, O, GI);
+            else
+                hipLaunchKernelGGL(k_shade<false>, grid, dim3(kRenderBlock), 0, st, S, C->dg, cam, cp, qa, qin,
+                                   (const PathHit *)B.hits.ptr, count, acc, O, GI);
             AZ_HIPR(C, hipGetLastError());
             SegScan sc;
             sc.counters[0] = childCnt;
@@ -1987,12 +2163,13 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             if (hc[1]) {
                 SegIn sin{offShadow, nseg, shadowSegCap};
                 AZ_HIPR(C, launch_connect(C->renderTuning, C->cuCount, C->dg, (const ShadowRay *)B.shadow.ptr, sin,
-                                          hc[1], acc, (GuideSample *)B.recs.ptr, st));
+                                          hc[1], acc, (GuideSample *)B.recs.ptr, (ScatterLink *)B.links.ptr, st));
             }
             count = hc[0];
             offA = 1 - offA;
             qin = SegIn{offQ[offA], nseg, childSegCap};
             std::swap(bufA, bufB);
+            std::swap(recA, recB);
         }
         if (overflow) {
             if (chunkSpp == 1) {
@@ -2019,29 +2196,44 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
         }
         AZ_HIPR(C, hipGetLastError());
         if (training) {
-            // Adam steps on batches gathered from this chunk's samples.
+            // Adam steps on batches gathered from this chunk's samples: the
+            // environment records (queue 0) and the scatter records (queue 1),
+            // whose weights wait for their subtrees' sums.
             SegScan sc;
             memset(&sc, 0, sizeof(sc));
             sc.counters[0] = recCnt;
             sc.offsets[0] = offRec;
-            sc.segCap[0] = recSegCap;
-            sc.nseg[0] = kSegs;
+            sc.counters[1] = srecCnt;
+            sc.offsets[1] = offSrec;
+            sc.segCap[0] = sc.segCap[1] = recSegCap;
+            sc.nseg[0] = sc.nseg[1] = kSegs;
             sc.total = dTotal + 2;
-            hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(kSegs), 0, st, sc);
+            hipLaunchKernelGGL(k_seg_scan, dim3(2), dim3(kSegs), 0, st, sc);
             AZ_HIPR(C, hipGetLastError());
-            uint32_t R = 0;
-            AZ_HIPR(C, hipMemcpyAsync(&R, dTotal + 2, 4, hipMemcpyDeviceToHost, st));
+            uint32_t R[2] = {0u, 0u};
+            AZ_HIPR(C, hipMemcpyAsync(R, dTotal + 2, 8, hipMemcpyDeviceToHost, st));
             AZ_HIPR(C, hipStreamSynchronize(st));
-            if (R >= 1024) {
-                u32 bn = std::min<u32>(65536u, R) & ~15u;
-                SegIn rin{offRec, kSegs, recSegCap};
-                for (u32 k = 0; k < stepsPerChunk; ++k) {
-                    uint64_t seed = (o.Seed + 0x9E3779B97F4A7C15ull) * (uint64_t)(trainRound * 131 + k + 1);
-                    hipLaunchKernelGGL(k_guide_gather, dim3((bn + kRenderBlock - 1) / kRenderBlock), dim3(kRenderBlock),
-                                       0, st, (const GuideSample *)B.recs.ptr, rin, R, (GuideSample *)B.batch.ptr, bn,
-                                       seed);
+            for (int g = 0; g < 2; ++g) {
+                az_guide *net = g == 0 ? C->guide : C->scatterGuide;
+                const GuideSample *recs = (const GuideSample *)(g == 0 ? B.recs.ptr : B.srecs.ptr);
+                if (!net || R[g] < 1024) continue;
+                SegIn rin{g == 0 ? offRec : offSrec, kSegs, recSegCap};
+                const dim3 rg((R[g] + kRenderBlock - 1) / kRenderBlock);
+                if (g == 1) {
+                    for (u32 d = S.maxDepth; d-- > 1u;)
+                        hipLaunchKernelGGL(k_scatter_propagate, rg, dim3(kRenderBlock), 0, st, (ScatterLink *)B.links.ptr,
+                                           rin, R[g], d);
+                    hipLaunchKernelGGL(k_scatter_finalize, rg, dim3(kRenderBlock), 0, st, (GuideSample *)B.srecs.ptr,
+                                       (const ScatterLink *)B.links.ptr, rin, R[g]);
                     AZ_HIPR(C, hipGetLastError());
-                    az_status gs = az_guide_train_on_stream(C->guide, B.batch.ptr, bn, st);
+                }
+                u32 bn = std::min<u32>(65536u, R[g]) & ~15u;
+                for (u32 k = 0; k < stepsPerChunk; ++k) {
+                    uint64_t seed = (o.Seed + 0x9E3779B97F4A7C15ull * (uint64_t)(g + 1)) * (uint64_t)(trainRound * 131 + k + 1);
+                    hipLaunchKernelGGL(k_guide_gather, dim3((bn + kRenderBlock - 1) / kRenderBlock), dim3(kRenderBlock),
+                                       0, st, recs, rin, R[g], (GuideSample *)B.batch.ptr, bn, seed);
+                    AZ_HIPR(C, hipGetLastError());
+                    az_status gs = az_guide_train_on_stream(net, B.batch.ptr, bn, st);
                     if (gs != AZ_OK) {
                         C->lastError = "guide training failed";
                         return gs;
@@ -2446,11 +2638,13 @@ az_status az_gpu_trim(az_gpu_ctx *C) {
     AZ_HIPR(C, hipSetDevice(C->device));
     AZ_HIPR(C, hipStreamSynchronize(C->stream));
     RenderBuffers &B = buffersOf(C);
-    DeviceBuffer *bufs[] = {&B.q0,      &B.q1,    &B.hits,  &B.shadow,    &B.acc,        &B.pixels,
-                            &B.queries, &B.lobes, &B.recs,  &B.batch,     &B.batchArgs,  &B.batchTotals};
+    DeviceBuffer *bufs[] = {&B.q0,      &B.q1,        &B.hits,  &B.shadow,      &B.acc,   &B.pixels, &B.queries,
+                            &B.lobes,   &B.recs,      &B.batch, &B.batchArgs,   &B.batchTotals,     &B.scatLobes,
+                            &B.srecs,   &B.links,     &B.qrec0, &B.qrec1};
     for (DeviceBuffer *b : bufs) b->release();
     if (C->guide) az_guide_destroy(C->guide);
-    C->guide = nullptr;
+    if (C->scatterGuide) az_guide_destroy(C->scatterGuide);
+    C->guide = C->scatterGuide = nullptr;
     return AZ_OK;
 }
 

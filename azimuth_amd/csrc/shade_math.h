@@ -347,6 +347,13 @@ __device__ inline float spec_pdf(int lobe, float e, V3 L, V3 N, V3 Vo) {
     return (e + 1.0f) / (2.0f * kPiF) * powf(c, e) / (4.0f * vh);
 }
 
+// Density of unit direction L under the mixture (1 - ps) cos/pi + ps specular.
+__device__ inline float brdf_mixture_pdf(const DevMaterial &M, V3 kd, V3 N, V3 Vo, V3 L) {
+    const int lobe = spec_lobe(M);
+    const float ps = lobe ? spec_prob(M, kd, N, Vo) : 0.0f;
+    return (1.0f - ps) * fmaxf(dot(N, L), 0.0f) / kPiF + (lobe ? ps * spec_pdf(lobe, M.exponent, L, N, Vo) : 0.0f);
+}
+
 // A scatter direction from the mixture (1 - ps) cos/pi + ps specular, from
 // three uniforms (lobe choice, then the lobe's two), with the mixture's
 // density at it.  Directions below the surface can come from the specular

@@ -14,7 +14,8 @@ training included), after a device synchronize.  Because both estimators are unb
 the report also gives the time-normalized ratio (MSE_plain * T_plain) /
 (MSE_guided * T_guided).  For comparison it also renders with the
 environment map's luminance distribution, alone and as the guide's base
-distribution.  Every MSE is the mean over --seeds renders with distinct seeds
+distribution, and at the same spp with the scatter guide (hemisphere scatter
+directions) alone and beside the environment guide.  Every MSE is the mean over --seeds renders with distinct seeds
 (the per-image MSE of this scene varies by tens of percent from seed to
 seed), every time the mean of their times.  One JSON line is printed.
 """
@@ -36,8 +37,9 @@ def resolve(px):
     return px[:, :3] / np.maximum(px[:, 3:4], 1e-300)
 
 
-def timed(ctx, cam, filt, seed, spp, guide, env=N.AZ_ENV_COSINE, max_depth=0):
-    px = ctx.render_f64(cam, filt=filt, seed=seed, spp=spp, guide=guide, env_sampling=env, guide_max_depth=max_depth)
+def timed(ctx, cam, filt, seed, spp, guide, env=N.AZ_ENV_COSINE, max_depth=0, target=N.AZ_GUIDE_ENV):
+    px = ctx.render_f64(cam, filt=filt, seed=seed, spp=spp, guide=guide, env_sampling=env, guide_max_depth=max_depth,
+                        guide_target=target)
     return resolve(px), ctx.last_render_s
 
 
@@ -52,12 +54,12 @@ def equal_time(ctx, cam, filt, seed, spp, t_plain, t_same, max_depth):
     return (s + 1) * (s + 1) if t_up <= t_plain * 1.02 else s * s
 
 
-def measure(ctx, cam, filt, ref, seeds, spp, guide, env=N.AZ_ENV_COSINE, max_depth=0):
+def measure(ctx, cam, filt, ref, seeds, spp, guide, env=N.AZ_ENV_COSINE, max_depth=0, target=N.AZ_GUIDE_ENV):
     """Mean MSE against ref and mean time over renders with the given seeds;
     also the per-seed MSEs."""
     mses, ts = [], []
     for s in seeds:
-        img, t = timed(ctx, cam, filt, s, spp, guide, env, max_depth)
+        img, t = timed(ctx, cam, filt, s, spp, guide, env, max_depth, target)
         mses.append(float(np.mean((img - ref) ** 2)))
         ts.append(t)
     return float(np.mean(mses)), float(np.mean(ts)), mses
@@ -116,6 +118,12 @@ def main(argv=None):
     mse_glum, t_glum, _ = measure(ctx, cam, filt, ref, seeds, a.spp, True, N.AZ_ENV_LUMINANCE)
     print(f"luminance {a.spp} spp {t_lum:.2f}s mse {mse_lum:.4e}; guide+luminance {t_glum:.2f}s mse {mse_glum:.4e}",
           file=sys.stderr, flush=True)
+    # The scatter guide (hemisphere scatter directions mixed with the
+    # reference's cosine sampling), alone and beside the environment guide.
+    mse_sc, t_sc, _ = measure(ctx, cam, filt, ref, seeds, a.spp, True, target=N.AZ_GUIDE_SCATTER)
+    mse_both, t_both, _ = measure(ctx, cam, filt, ref, seeds, a.spp, True, target=N.AZ_GUIDE_BOTH)
+    print(f"scatter guide {a.spp} spp {t_sc:.2f}s mse {mse_sc:.4e}; both guides {t_both:.2f}s mse {mse_both:.4e}",
+          file=sys.stderr, flush=True)
     out = {
         "experiment": "equal-time MSE vs reference, guided vs plain",
         "workload": f"architectural interior {W}x{H} ({sc.triangle_count} triangles), Gaussian filter, depth 6",
@@ -131,6 +139,8 @@ def main(argv=None):
                                         "mse_ratio": round(mse_plain / mse_d3, 3)},
         "luminance_env_sampling": {"spp": a.spp, "seconds": round(t_lum, 3), "mse": mse_lum},
         "guided_over_luminance": {"spp": a.spp, "seconds": round(t_glum, 3), "mse": mse_glum},
+        "scatter_guide_same_spp": {"spp": a.spp, "seconds": round(t_sc, 3), "mse": mse_sc},
+        "both_guides_same_spp": {"spp": a.spp, "seconds": round(t_both, 3), "mse": mse_both},
     }
     print(json.dumps(out), flush=True)
     ctx.close()

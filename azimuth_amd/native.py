@@ -167,13 +167,14 @@ class az_scene_params(C.Structure):
 
 AZ_ENV_COSINE, AZ_ENV_LUMINANCE = 0, 1
 AZ_SCATTER_REFERENCE, AZ_SCATTER_BRDF = 0, 1
+AZ_GUIDE_ENV, AZ_GUIDE_SCATTER, AZ_GUIDE_BOTH = 0, 1, 2
 
 
 class az_render_opts(C.Structure):
     _fields_ = [("Seed", u64), ("SampleCountOverride", u32), ("RowBegin", u32), ("RowEnd", u32),
                 ("AccumulateF64", u32), ("GuideEnable", u32), ("GuideFraction", f32), ("GuideTrainSpp", u32),
                 ("GuideStepsPerChunk", u32), ("EnvSampling", u32), ("GuideMaxDepth", u32), ("GuideChunkSpp", u32),
-                ("ScatterSampling", u32)]
+                ("ScatterSampling", u32), ("GuideTarget", u32)]
 
 
 class az_stats(C.Structure):

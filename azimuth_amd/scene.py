@@ -343,9 +343,11 @@ class GpuContext:
 
     def render(self, camera, filt=N.FilterType_Box, cull=False, seed=0, spp=0, row_begin=0, row_end=0,
                guide=False, guide_fraction=0.5, env_sampling=N.AZ_ENV_COSINE, guide_max_depth=0,
-               guide_train_spp=0, guide_chunk_spp=0, scatter_sampling=N.AZ_SCATTER_REFERENCE):
+               guide_train_spp=0, guide_chunk_spp=0, scatter_sampling=N.AZ_SCATTER_REFERENCE,
+               guide_target=N.AZ_GUIDE_ENV):
         o = N.az_render_opts()
         o.ScatterSampling = int(scatter_sampling)
+        o.GuideTarget = int(guide_target)
         o.EnvSampling = int(env_sampling)
         o.GuideMaxDepth = int(guide_max_depth)
         o.GuideTrainSpp = int(guide_train_spp)
@@ -397,7 +399,7 @@ class GpuContext:
 
     def render_f64(self, camera, filt=N.FilterType_Box, seed=0, spp=0, guide=False, guide_fraction=0.5,
                    env_sampling=N.AZ_ENV_COSINE, guide_max_depth=0, guide_train_spp=0, guide_chunk_spp=0,
-                   scatter_sampling=N.AZ_SCATTER_REFERENCE):
+                   scatter_sampling=N.AZ_SCATTER_REFERENCE, guide_target=N.AZ_GUIDE_ENV):
         """Device render accumulated in double precision; returns (W*H, 4)
         float64 (colour sums, weight sums).  The wall time of the render call
         (guide training included) is left in last_render_s."""
@@ -409,6 +411,7 @@ class GpuContext:
         o.SampleCountOverride = int(spp)
         o.AccumulateF64 = 1
         o.ScatterSampling = int(scatter_sampling)
+        o.GuideTarget = int(guide_target)
         o.EnvSampling = int(env_sampling)
         o.GuideMaxDepth = int(guide_max_depth)
         o.GuideTrainSpp = int(guide_train_spp)

@@ -67,9 +67,9 @@ typedef struct {
     u32 SampleCountOverride;/* 0: camera->SampleCount; else used instead (still truncated to a square) */
     u32 RowBegin, RowEnd;   /* scanline range to render; RowEnd 0 = all rows */
     u32 AccumulateF64;      /* 1: accumulate in double (long reference renders) */
-    u32 GuideEnable;        /* 1: sample environment directions from the learned guide mixed
-                               with the reference's cosine sampling (one-sample MIS); the
-                               guide starts from seeded weights and trains online */
+    u32 GuideEnable;        /* 1: draw directions from a learned guide mixed with the base
+                               distribution (one-sample MIS; GuideTarget says which directions);
+                               the guide starts from seeded weights and trains online */
     f32 GuideFraction;      /* probability of drawing the direction from the guide (0 = 0.5) */
     u32 GuideTrainSpp;      /* samples per pixel rendered while training (0 = 32) */
     u32 GuideStepsPerChunk; /* Adam steps after each training chunk (0: spread 256 steps over the
@@ -87,6 +87,11 @@ typedef struct {
                                cosine or uniform hemisphere) or AZ_SCATTER_BRDF (a one-sample mixture
                                of the cosine lobe and the material's specular lobe, weighted by the
                                mixture pdf; unbiased, different random directions) */
+    u32 GuideTarget;        /* what the guide proposes (with GuideEnable): AZ_GUIDE_ENV (environment
+                               next-event directions, mixed with EnvSampling's distribution),
+                               AZ_GUIDE_SCATTER (hemisphere scatter directions, mixed with
+                               ScatterSampling's distribution; trained on the radiance each scatter
+                               ray's path collects) or AZ_GUIDE_BOTH (one network for each) */
 } az_render_opts;
 
 #define AZ_ENV_COSINE    0u
@@ -94,6 +99,10 @@ typedef struct {
 
 #define AZ_SCATTER_REFERENCE 0u
 #define AZ_SCATTER_BRDF      1u
+
+#define AZ_GUIDE_ENV     0u
+#define AZ_GUIDE_SCATTER 1u
+#define AZ_GUIDE_BOTH    2u
 
 typedef struct {
     f64 LastKernelMs;       /* device time of the last trace/render call's kernels */
