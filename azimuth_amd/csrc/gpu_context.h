@@ -88,6 +88,7 @@ struct az_gpu_ctx {
     az::TraceLaunch launch;
     az::RenderTuning renderTuning;
     az::DeviceBuffer counters;
+    az::DeviceBuffer outputScratch;  // az_gpu_output_pixels: luminance keys, logs, histogram
     az::RenderState render;
     az::DevScene devScene{};
     void *renderBuffers = nullptr;   // owned; released by az_release_render_buffers

@@ -2642,6 +2642,7 @@ az_status az_gpu_trim(az_gpu_ctx *C) {
                             &B.lobes,   &B.recs,      &B.batch, &B.batchArgs,   &B.batchTotals,     &B.scatLobes,
                             &B.srecs,   &B.links,     &B.qrec0, &B.qrec1};
     for (DeviceBuffer *b : bufs) b->release();
+    C->outputScratch.release();
     if (C->guide) az_guide_destroy(C->guide);
     if (C->scatterGuide) az_guide_destroy(C->scatterGuide);
     C->guide = C->scatterGuide = nullptr;

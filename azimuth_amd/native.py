@@ -65,6 +65,7 @@ LightType_PointLight, LightType_AreaLight, LightType_DirectionalLight, LightType
     LightType_EntityLight = range(5)
 ShadingMode_Flat, ShadingMode_Smooth = 0, 1
 FilterType_Box, FilterType_Gaussian = 0, 1
+TonemapOperator_None, TonemapOperator_Photographic = 0, 1
 CameraType_Detailed, CameraType_Simple = 0, 1
 TextureType_2D, TextureType_PerlinNoise = 0, 1
 PixelFormat_RGB_U8, PixelFormat_RGB_F32 = 0, 1
@@ -235,6 +236,7 @@ EXPORTS = {
     "az_gpu_render": (s32, [_P, C.POINTER(camera), s32, u32, C.POINTER(az_render_opts), _P]),
     "az_gpu_render_device": (s32, [_P, C.POINTER(camera), s32, u32, C.POINTER(az_render_opts), _P]),
     "az_gpu_get_stats": (s32, [_P, C.POINTER(az_stats)]),
+    "az_gpu_output_pixels": (s32, [_P, _P, u32, u32, C.POINTER(tonemap), f32, u32, _P]),
     "az_gpu_set_counting": (s32, [_P, u32]),
     "az_gpu_set_launch": (s32, [_P, u32, u32, s32, s32]),
     "az_gpu_autotune_trace": (s32, [_P, _P, u32, u32, _P]),

@@ -393,6 +393,27 @@ class GpuContext:
         N.check(self.lib.az_gpu_set_render_tuning(self.h, int(chunk), int(block), int(waves), int(loop)),
                 "az_gpu_set_render_tuning", self.h)
 
+    def output_pixels_device(self, d_pixels, width, height, d_out, tonemap=None, inv_gamma=1.0, srgb=False):
+        """OutputPixelsToFile's colour path on the device: d_pixels holds
+        width*height pixels, d_out receives 3 bytes per pixel."""
+        N.check(self.lib.az_gpu_output_pixels(self.h, d_pixels.ptr, int(width), int(height),
+                                              C.byref(tonemap) if tonemap is not None else None, float(inv_gamma),
+                                              1 if srgb else 0, d_out.ptr), "az_gpu_output_pixels", self.h)
+
+    def output_pixels(self, pixels, width, height, tonemap=None, inv_gamma=1.0, srgb=False):
+        """Host convenience: uploads a PIXEL_DTYPE array, runs the output
+        stage, returns (height, width, 3) uint8."""
+        px = np.ascontiguousarray(pixels, dtype=N.PIXEL_DTYPE)
+        n = int(width) * int(height)
+        d_px, d_out = self.alloc(max(1, n) * 16), self.alloc(max(1, n) * 3)
+        try:
+            d_px.upload(px)
+            self.output_pixels_device(d_px, width, height, d_out, tonemap, inv_gamma, srgb)
+            return d_out.download(np.uint8, n * 3).reshape(int(height), int(width), 3)
+        finally:
+            d_px.free()
+            d_out.free()
+
     def trim(self):
         """Frees the context's render queues (the scene stays uploaded)."""
         N.check(self.lib.az_gpu_trim(self.h), "az_gpu_trim", self.h)

@@ -169,6 +169,23 @@ az_status az_gpu_render_device(az_gpu_ctx *Ctx, const camera *Camera, filter_typ
 az_status az_gpu_render_batch(az_gpu_ctx *const *Ctxs, const camera *const *Cameras, u32 Count, filter_type Filter,
                               b32 BackfaceCulling, const az_render_opts *Opts, void *const *d_Pixels);
 
+/* The colour path of OutputPixelsToFile (main.cpp:1620-1797) for the 8-bit
+ * formats (PNG, BMP, TGA, JPG), ending in their byte conversion
+ * (main.cpp:1461-1477): each colour divided by its TotalWeight; with
+ * Tonemap->Operator == TonemapOperator_Photographic the photographic
+ * operator (log-average luminance summed in pixel order as the reference
+ * does, white at the BurnoutPercentage percentile of the luminances,
+ * Saturation); gamma (sRGB with exponent InvGamma when sRGB, else the power
+ * InvGamma); d_Out[3 i + c] = (u8)(Clamp01(C) * 255).  Replaces the pixel
+ * loops of OutputPixelsToFile; the image writer (stb) stays with the caller.
+ * d_Pixels (Width*Height pixel, TotalWeight > 0 where tonemapped) and d_Out
+ * (3 bytes per pixel) are device memory; d_Pixels is read only (the
+ * reference normalizes it in place).  Tonemap may be NULL (no tonemapping);
+ * a BurnoutPercentage outside [0, 100] returns AZ_ERR_INVALID_ARGUMENT.
+ * Images of fewer than 2^31 pixels. */
+az_status az_gpu_output_pixels(az_gpu_ctx *Ctx, const pixel *d_Pixels, u32 Width, u32 Height,
+                               const tonemap *Tonemap, f32 InvGamma, b32 sRGB, u8 *d_Out);
+
 az_status az_gpu_get_stats(az_gpu_ctx *Ctx, az_stats *Stats);
 /* Enables per-ray node/triangle counting in trace calls (slower). */
 az_status az_gpu_set_counting(az_gpu_ctx *Ctx, b32 Enable);

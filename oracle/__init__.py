@@ -37,6 +37,8 @@ def load():
         lib.azo_perlin_sample.argtypes = [C.POINTER(N.texture), C.c_float, C.c_float, C.c_float]
         lib.azo_perlin_sample.restype = C.c_float
         lib.azo_evaluate_brdf.argtypes = [C.POINTER(N.brdf), P, P, P, C.c_float, P, P]
+        lib.azo_output_pixels.argtypes = [P, C.c_uint32, C.c_uint32, C.POINTER(N.tonemap), C.c_float, C.c_uint32, P]
+        lib.azo_output_pixels.restype = C.c_int
         _lib = lib
     return _lib
 
@@ -84,3 +86,17 @@ def render(scene, camera, filt=N.FilterType_Box, cull=False, seed=0, spp=0, row_
 
 def rng_draw(seed, pixel, sample, node, dim):
     return load().azo_rng_draw(seed, pixel, sample, node, dim)
+
+
+def output_pixels(pixels, width, height, tonemap=None, inv_gamma=1.0, srgb=False):
+    """OutputPixelsToFile's colour path for the 8-bit formats on the CPU;
+    pixels is a PIXEL_DTYPE array of width*height entries, tonemap an
+    N.tonemap or None.  Returns (height, width, 3) uint8."""
+    lib = load()
+    px = np.ascontiguousarray(pixels, dtype=N.PIXEL_DTYPE)
+    out = np.zeros((height, width, 3), dtype=np.uint8)
+    rc = lib.azo_output_pixels(px.ctypes.data, int(width), int(height), C.byref(tonemap) if tonemap is not None else None,
+                               float(inv_gamma), 1 if srgb else 0, out.ctypes.data)
+    if rc != 0:
+        raise ValueError("burnout percentile outside the image")
+    return out

@@ -19,6 +19,7 @@
 
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -1199,3 +1200,79 @@ void azo_evaluate_brdf(const brdf *B, const f32 *L, const f32 *N, const f32 *Vw,
 }
 
 }  // extern "C"
+
+// ---- output stage -----------------------------------------------------------------------
+// OutputPixelsToFile (main.cpp:1620-1797) for the 8-bit formats, then the
+// byte conversion of OutputPixelsToPNG/BMP/TGA/JPG (main.cpp:1461-1477).
+
+// Clamp(Val, 0, 1) = MaxOf(0, MinOf(Val, 1)) (math.cpp:11-26, math.h:6-7):
+// NaN compares false in MinOf and comes out as 1.
+static f32 clamp01Ref(f32 v) {
+    const f32 m = v < 1.0f ? v : 1.0f;
+    return 0.0f > m ? 0.0f : m;
+}
+
+// Luminance (math.cpp:1866-1879, the enabled #if branch).
+static f32 luminanceRef(f32 r, f32 g, f32 b) { return r * 0.27f + g * 0.67f + b * 0.06f; }
+
+// CompareF32 (util.h:71-81): the sign of the float difference.
+static int compareF32Ref(const void *a, const void *b) {
+    const f32 d = *(const f32 *)a - *(const f32 *)b;
+    return d > 0.0f ? 1 : (d < 0.0f ? -1 : 0);
+}
+
+extern "C" int azo_output_pixels(const pixel *Pixels, u32 Width, u32 Height, const tonemap *Tonemap, f32 InvGamma,
+                                 b32 SRGB, u8 *Out) {
+    const u32 n = Width * Height;
+    if (n == 0) return 0;
+    const bool tm = Tonemap && Tonemap->Operator != TonemapOperator_None;
+    std::vector<f32> col((size_t)n * 3), lum(tm ? n : 0);
+    // Filtering: colour over weight, luminance and the sequential f32 sum
+    // of log(L + 1e-5) (main.cpp:1640-1661; LogBaseE = double log,
+    // math.cpp:172-179).
+    f32 logAvg = 0.0f;
+    for (u32 i = 0; i < n; ++i) {
+        const f32 w = Pixels[i].TotalWeight;
+        const f32 r = Pixels[i].Color.x / w, g = Pixels[i].Color.y / w, b = Pixels[i].Color.z / w;
+        col[3 * i + 0] = r;
+        col[3 * i + 1] = g;
+        col[3 * i + 2] = b;
+        if (tm) {
+            const f32 L = luminanceRef(r, g, b);
+            lum[i] = L;
+            logAvg += (f32)log((double)(L + 0.00001f));
+        }
+    }
+    // Photographic operator (main.cpp:1686-1731; Exp = double exp,
+    // math.cpp:145-152; RoundAsU32 = (u32)(x + 0.5f), math.cpp:100-107).
+    if (tm) {
+        logAvg = (f32)exp((double)(logAvg / (f32)n));
+        const f32 factor = Tonemap->KeyValue / logAvg;
+        const f32 at = (1.0f - Tonemap->BurnoutPercentage * 0.01f) * (n - 1) + 0.5f;
+        if (!(at >= 0.0f) || !(at < (f32)n)) return -1;   // the reference would index outside the array
+        const u32 li = (u32)at;
+        qsort(lum.data(), n, sizeof(f32), compareF32Ref);
+        f32 lw = lum[li] * factor;
+        lw = lw * lw;
+        const f32 sat = Tonemap->Saturation;
+        for (u32 i = 0; i < n; ++i) {
+            const f32 r = col[3 * i + 0], g = col[3 * i + 1], b = col[3 * i + 2];
+            const f32 L = luminanceRef(r, g, b);
+            if (L != 0.0f) {
+                const f32 Ls = L * factor;
+                const f32 Ld = (Ls * (1.0f + (Ls / lw))) / (1.0f + Ls);
+                col[3 * i + 0] = clamp01Ref(powf(r / L, sat) * Ld);
+                col[3 * i + 1] = clamp01Ref(powf(g / L, sat) * Ld);
+                col[3 * i + 2] = clamp01Ref(powf(b / L, sat) * Ld);
+            }
+        }
+    }
+    // Gamma (main.cpp:1734-1778) and bytes (main.cpp:1468-1474).
+    for (size_t k = 0; k < (size_t)n * 3; ++k) {
+        f32 c = col[k];
+        if (SRGB) c = c > 0.0031308f ? 1.055f * powf(c, InvGamma) - 0.055f : c * 12.92f;
+        else c = powf(c, InvGamma);
+        Out[k] = (u8)(clamp01Ref(c) * 255.0f);
+    }
+    return 0;
+}

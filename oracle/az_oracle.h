@@ -62,6 +62,14 @@ f32  azo_perlin_sample(const texture *Texture, f32 X, f32 Y, f32 Z);
 void azo_evaluate_brdf(const brdf *Brdf, const f32 *LightDir, const f32 *HitN,
                        const f32 *ViewDir, f32 CosThetaIncident, f32 *Diffuse, f32 *Specular);
 
+/* OutputPixelsToFile's colour path for the 8-bit formats: colour over
+ * weight, the photographic operator when Tonemap->Operator is not None
+ * (Tonemap may be NULL), gamma (sRGB or power InvGamma), then
+ * (u8)(Clamp01(C) * 255) into Out (3 bytes per pixel).  Returns -1 (Out
+ * untouched) when the burnout percentile indexes outside the image. */
+int azo_output_pixels(const pixel *Pixels, u32 Width, u32 Height, const tonemap *Tonemap, f32 InvGamma,
+                      b32 SRGB, u8 *Out);
+
 #ifdef __cplusplus
 }
 #endif

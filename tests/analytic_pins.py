@@ -340,3 +340,74 @@ def check_glossy_scatter(render_opts):
         on = render_opts(glossy_scatter_scene(True), cam, sampling)
         part = (on["Color"] - off["Color"]) / on["TotalWeight"][:, None]
         np.testing.assert_allclose(part.mean(0), want, rtol=0.03, err_msg=f"sampling={sampling}")
+
+
+# ---- output stage (OutputPixelsToFile for the 8-bit formats) -------------------------------
+# fn(pixels, width, height, tonemap, inv_gamma, srgb) -> (height, width, 3) uint8.
+
+def _pixels(colors, weights=None):
+    import numpy as np
+    from azimuth_amd import native as N
+    colors = np.asarray(colors, np.float32).reshape(-1, 3)
+    px = np.zeros(colors.shape[0], dtype=N.PIXEL_DTYPE)
+    px["Color"] = colors
+    px["TotalWeight"] = 1.0 if weights is None else np.asarray(weights, np.float32)
+    return px
+
+
+def _tonemap(key, burnout, saturation):
+    from azimuth_amd import native as N
+    t = N.tonemap()
+    t.KeyValue, t.BurnoutPercentage, t.Saturation = key, burnout, saturation
+    t.Operator = N.TonemapOperator_Photographic
+    return t
+
+
+def check_output_bytes_and_srgb(fn):
+    """PNG conversion (u8)(Clamp01(c) * 255) truncates, clamps, and maps NaN
+    to 255 (MinOf(NaN, 1) is 1); the weight divides the colour; the sRGB
+    curve (IEC 61966-2-1: 12.92 c below 0.0031308, else 1.055 c^(1/2.4) -
+    0.055) at known points."""
+    import numpy as np
+    cols = [(0.5, 1.0, 2.0), (-1.0, 0.0, float("nan")), (1.5, 3.0, 0.75)]
+    out = fn(_pixels(cols, [1.0, 1.0, 3.0]), 3, 1, None, 1.0, False)
+    assert out.reshape(-1).tolist() == [127, 255, 255, 0, 0, 255, 127, 255, 63]
+    # sRGB: linear 0.5 -> 0.735357 (187.5), 0.2 -> 0.484529 (123.6),
+    # 0.003 (below the knee) -> 0.03876 (9.88); 1 -> 1.055f - 0.055f, which
+    # binary32 rounds to 0.99999994 (254.99998): 254.
+    assert np.float32(1.055) - np.float32(0.055) < np.float32(1.0)
+    out = fn(_pixels([(0.5, 0.2, 0.003), (1.0, 0.0, 0.0031308)]), 2, 1, None, 1.0 / 2.4, True)
+    assert out.reshape(-1).tolist()[:4] == [187, 123, 9, 254]
+    assert out.reshape(-1)[4] == 0 and abs(int(out.reshape(-1)[5]) - 10) <= 1
+    # A power gamma of 1/2.2: 0.5 -> 0.72974 (186.1).
+    out = fn(_pixels([(0.5, 0.5, 0.5)]), 1, 1, None, 1.0 / 2.2, False)
+    assert out.reshape(-1).tolist() == [186, 186, 186]
+
+
+def check_photographic_operator(fn):
+    """Reinhard et al. 2002 with a burnout white point, on a two-level grey
+    image (half the pixels luminance a, half b): log-average sqrt((a + e)
+    (b + e)) with e = 1e-5, scale key / log-average, white = the luminance at
+    the (1 - burnout) percentile; Ld = Ls (1 + Ls / Lw^2) / (1 + Ls)."""
+    import math
+    import numpy as np
+    a, b, key, n = 0.05, 0.8, 0.18, 64
+    cols = [(a, a, a)] * (n // 2) + [(b, b, b)] * (n // 2)
+    lum = lambda c: c * 0.27 + c * 0.67 + c * 0.06
+    la, lb = lum(a), lum(b)
+    scale = key / math.sqrt((la + 1e-5) * (lb + 1e-5))
+    for burnout, white in ((0.0, lb), (60.0, la)):
+        out = fn(_pixels(cols), n, 1, _tonemap(key, burnout, 1.0), 1.0, False).reshape(n, 3).astype(int)
+        lw2 = (white * scale) ** 2
+        for px, L in ((0, la), (n - 1, lb)):
+            ls = L * scale
+            want = min(1.0, ls * (1 + ls / lw2) / (1 + ls)) * 255
+            assert np.all(np.abs(out[px] - want) <= 1.0), (burnout, px, out[px], want)
+    # Saturation on a uniform colour image: the white is the colour's own
+    # luminance, Ld = 1, and each channel is (c / L)^s.
+    c = (0.5, 0.25, 0.125)
+    L = 0.5 * 0.27 + 0.25 * 0.67 + 0.125 * 0.06
+    for s in (0.5, 1.0):
+        out = fn(_pixels([c] * 16), 4, 4, _tonemap(key, 1.0, s), 1.0, False).reshape(16, 3).astype(int)
+        want = [min(1.0, (ch / L) ** s) * 255 for ch in c]
+        assert np.all(np.abs(out - np.array(want)) <= 1.0), (s, out[0], want)

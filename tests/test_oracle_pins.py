@@ -26,3 +26,15 @@ def test_sphere_light_cone_sampling():
 
 def test_hemisphere_scatter_samplers():
     P.check_scatter(render)
+
+
+def output(px, w, h, tm, inv_gamma, srgb):
+    return oracle.output_pixels(px, w, h, tm, inv_gamma, srgb)
+
+
+def test_output_bytes_and_srgb():
+    P.check_output_bytes_and_srgb(output)
+
+
+def test_photographic_operator():
+    P.check_photographic_operator(output)
