@@ -4,8 +4,10 @@
 //
 //   k_out_prepare : colour / weight, luminance, log(L + 1e-5), and an
 //                   order-preserving integer key of L
-//   k_out_logsum  : the reference's sum of the logs, in its order (one
-//                   sequential f32 chain), then the scale key / log-average
+//   k_out_chunk_sums, k_out_chunk_scan, k_out_chunk_fold, k_out_logsum :
+//                   the reference's sum of the logs, bit for bit as its
+//                   sequential f32 loop computes it, then the scale
+//                   key / log-average
 //   k_out_hist, k_out_pick (x3) : radix select of the luminance at the burnout
 //                   percentile (11 + 11 + 10 key bits), the order statistic
 //                   that the reference reads from its qsort'ed array
@@ -68,28 +70,167 @@ __global__ __launch_bounds__(kOutBlock) void k_out_prepare(const float4 *__restr
     logs[i] = (float)log((double)(L + 0.00001f));
 }
 
-// One wave: the sum in the reference's order (main.cpp:1643-1661), 64 logs
-// per coalesced load, added one after another from lane registers; then
-// log-average = exp(sum / n) in double (Exp, math.cpp:145-152) and the
-// scale.  Also starts the radix select and clears its histogram.
-__global__ __launch_bounds__(64) void k_out_logsum(const float *__restrict__ logs, uint32_t n, float key,
-                                                   uint32_t rank, OutCtl *__restrict__ ctl,
-                                                   uint32_t *__restrict__ hist) {
-    const uint32_t lane = threadIdx.x;
-    for (uint32_t b = lane; b < kRadixBins; b += 64u) hist[b] = 0u;
-    float s = 0.0f;
-    uint32_t base = 0u;
-    float x = lane < n ? logs[lane] : 0.0f;
-    while (base + 64u <= n) {
-        const uint32_t nx = base + 64u + lane;
-        const float next = nx < n ? logs[nx] : 0.0f;
-#pragma unroll
-        for (int j = 0; j < 64; ++j) s += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j));
-        x = next;
-        base += 64u;
+// ---- the reference's sequential f32 sum, in parallel -------------------------------------
+// The reference adds the n logs one after another into an f32 (main.cpp:
+// 1643-1661).  While the running sum s stays inside one binade
+// 2^e <= |s| < 2^(e+1) (ulp u = 2^(e-23)), each step fl(s + x) equals
+// s + u q with q = x / u rounded to the nearest integer, an exact tie going
+// to the q that leaves the integer mantissa (s / u + q) even.  A chunk of
+// logs therefore moves s / u by an integer that depends on e and, at ties,
+// on the parity of s / u: one lane per chunk computes it for both parities
+// (with the lowest and highest partial offsets) at the binade its start is
+// expected in, from a double prefix of the chunk sums.  A single wave then
+// walks the chunks: where s is in the expected binade and every partial sum
+// stays strictly inside it, s jumps by the chunk's offset (each
+// intermediate value is then exactly the reference's); any other chunk (s
+// near zero, crossing a power of two, a non-finite log) is added element by
+// element.  The result is the reference's sum bit for bit.
+constexpr uint32_t kChunk = 1024;
+constexpr int kNoBinade = -100000;
+
+struct ChunkFold {
+    long long q[2];                  // offset of s / u over the chunk, by start parity
+    long long lo[2], hi[2];          // lowest and highest partial offset
+    int ok[2];                       // 0: a log is not finite or too large for the grid
+    int binade;                      // e the fold assumes, or kNoBinade
+    int pad;
+};
+
+// Approximate chunk sums in double (their order does not matter: they only
+// predict the binade).
+__global__ __launch_bounds__(kOutBlock) void k_out_chunk_sums(const float *__restrict__ logs, uint32_t n,
+                                                              double *__restrict__ sums) {
+    __shared__ double part[kOutBlock / 64];
+    const uint32_t base = blockIdx.x * kChunk;
+    double v = 0.0;
+    for (uint32_t j = threadIdx.x; j < kChunk && base + j < n; j += kOutBlock) v += (double)logs[base + j];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+    if ((threadIdx.x & 63u) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) sums[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// Exclusive prefix of the chunk sums (one block, in tiles) -> the binade each
+// chunk is expected to start in.
+__global__ __launch_bounds__(kOutBlock) void k_out_chunk_scan(const double *__restrict__ sums, uint32_t nch,
+                                                              ChunkFold *__restrict__ folds) {
+    __shared__ double tile[kOutBlock];
+    double carry = 0.0;
+    for (uint32_t t0 = 0; t0 < nch; t0 += kOutBlock) {
+        const uint32_t k = t0 + threadIdx.x;
+        tile[threadIdx.x] = k < nch ? sums[k] : 0.0;
+        __syncthreads();
+        for (uint32_t o = 1; o < kOutBlock; o <<= 1) {
+            const double add = threadIdx.x >= o ? tile[threadIdx.x - o] : 0.0;
+            __syncthreads();
+            tile[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (k < nch) {
+            const float s = (float)(carry + (threadIdx.x ? tile[threadIdx.x - 1] : 0.0));
+            int e = kNoBinade;
+            if (s != 0.0f && isfinite(s) && fabsf(s) >= 1.17549435e-38f) {
+                int ex;
+                frexpf(s, &ex);
+                e = ex - 1;
+            }
+            folds[k].binade = e;
+        }
+        carry += tile[kOutBlock - 1];
+        __syncthreads();
     }
-    for (uint32_t j = 0; base + j < n; ++j) s += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), (int)j));
-    if (lane == 0) {
+}
+
+// One lane per chunk: the integer offsets of the chunk at its binade.
+__global__ __launch_bounds__(kOutBlock) void k_out_chunk_fold(const float *__restrict__ logs, uint32_t n, uint32_t nch,
+                                                              ChunkFold *__restrict__ folds) {
+    const uint32_t k = blockIdx.x * kOutBlock + threadIdx.x;
+    if (k >= nch) return;
+    ChunkFold f = folds[k];
+    const int e = f.binade;
+    const uint32_t base = k * kChunk, end = min(n, base + kChunk);
+    long long P[2] = {0, 0}, lo[2] = {0, 0}, hi[2] = {0, 0};
+    bool ok = e != kNoBinade;
+    bool first = true;
+    for (uint32_t j = base; ok && j < end; ++j) {
+        const double y = ldexp((double)logs[j], 23 - e);    // exact: a power-of-two scale
+        if (!(fabs(y) <= 1099511627776.0)) {                  // not finite, or beyond 2^40
+            ok = false;
+            break;
+        }
+        const double fl = floor(y), fr = y - fl;
+        const long long qf = (long long)fl;
+        for (int p = 0; p < 2; ++p) {
+            long long q = qf;
+            if (fr > 0.5) q = qf + 1;
+            else if (fr == 0.5) q = (((long long)p + P[p] + qf) & 1ll) ? qf + 1 : qf;
+            P[p] += q;
+            lo[p] = first ? P[p] : min(lo[p], P[p]);
+            hi[p] = first ? P[p] : max(hi[p], P[p]);
+        }
+        first = false;
+    }
+    for (int p = 0; p < 2; ++p) {
+        f.q[p] = P[p];
+        f.lo[p] = lo[p];
+        f.hi[p] = hi[p];
+        f.ok[p] = ok ? 1 : 0;
+    }
+    folds[k] = f;
+}
+
+// One wave: walks the chunks (fold where it is exact, else the sequential
+// chain: 64 logs per coalesced load, added one after another from lane
+// registers); then log-average = exp(sum / n) in double (Exp,
+// math.cpp:145-152) and the scale.  Also starts the radix select and clears
+// its histogram.
+__global__ __launch_bounds__(64) void k_out_logsum(const float *__restrict__ logs, uint32_t n,
+                                                   const ChunkFold *__restrict__ folds, float key, uint32_t rank,
+                                                   OutCtl *__restrict__ ctl, uint32_t *__restrict__ hist,
+                                                   float *__restrict__ sumOut) {
+    const uint32_t lane = threadIdx.x;
+    if (hist)
+        for (uint32_t b = lane; b < kRadixBins; b += 64u) hist[b] = 0u;
+    const uint32_t nch = (n + kChunk - 1) / kChunk;
+    const long long lo24 = 1ll << 23, hi24 = 1ll << 24;
+    float s = 0.0f;
+    for (uint32_t k = 0; k < nch; ++k) {
+        bool done = false;
+        if (s != 0.0f && isfinite(s) && fabsf(s) >= 1.17549435e-38f) {
+            int ex;
+            frexpf(s, &ex);
+            const int e = ex - 1;
+            const ChunkFold &f = folds[k];
+            if (e == f.binade) {
+                const long long S = (long long)ldexp((double)s, 23 - e);   // the integer mantissa, signed
+                const int p = (int)(S & 1ll);
+                if (f.ok[p]) {
+                    const bool inside = S > 0 ? (S + f.lo[p] > lo24 && S + f.hi[p] < hi24)
+                                              : (S + f.hi[p] < -lo24 && S + f.lo[p] > -hi24);
+                    if (inside) {
+                        s = (float)ldexp((double)(S + f.q[p]), e - 23);
+                        done = true;
+                    }
+                }
+            }
+        }
+        if (!done) {
+            const uint32_t base = k * kChunk, end = min(n, base + kChunk);
+            for (uint32_t b = base; b < end; b += 64u) {
+                const float x = b + lane < end ? logs[b + lane] : 0.0f;
+                const uint32_t m = min(64u, end - b);
+                if (m == 64u) {
+#pragma unroll
+                    for (int j = 0; j < 64; ++j) s += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j));
+                } else {
+                    for (uint32_t j = 0; j < m; ++j)
+                        s += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), (int)j));
+                }
+            }
+        }
+    }
+    if (lane == 0 && sumOut) *sumOut = s;
+    if (lane == 0 && ctl) {
         const float avg = (float)exp((double)(s / (float)n));
         ctl->logSum = s;
         ctl->scale = key / avg;
@@ -170,6 +311,19 @@ __global__ __launch_bounds__(kOutBlock) void k_out_map(const float4 *__restrict_
     }
 }
 
+// The kernels of the sequential sum over x[0..n): the sum to sumOut and, with
+// ctl, the scale and the start of the radix select.
+void launch_sequential_sum(const float *x, uint32_t n, ChunkFold *folds, double *sums, float key, uint32_t rank,
+                           OutCtl *ctl, uint32_t *hist, float *sumOut, hipStream_t st) {
+    const uint32_t nch = (n + kChunk - 1) / kChunk;
+    hipLaunchKernelGGL(k_out_chunk_sums, dim3(nch), dim3(kOutBlock), 0, st, x, n, sums);
+    hipLaunchKernelGGL(k_out_chunk_scan, dim3(1), dim3(kOutBlock), 0, st, (const double *)sums, nch, folds);
+    hipLaunchKernelGGL(k_out_chunk_fold, dim3((nch + kOutBlock - 1) / kOutBlock), dim3(kOutBlock), 0, st, x, n, nch,
+                       folds);
+    hipLaunchKernelGGL(k_out_logsum, dim3(1), dim3(64), 0, st, x, n, (const ChunkFold *)folds, key, rank, ctl, hist,
+                       sumOut);
+}
+
 }  // namespace
 }  // namespace az
 
@@ -213,16 +367,19 @@ az_status az_gpu_output_pixels(az_gpu_ctx *C, const pixel *d_Pixels, u32 Width, 
             return AZ_ERR_INVALID_ARGUMENT;
         }
         const uint32_t rank = (uint32_t)at;
-        const size_t need = sizeof(OutCtl) + kRadixBins * sizeof(uint32_t) + (size_t)n * 8;
+        const uint32_t nch = (n + kChunk - 1) / kChunk;
+        const size_t need = sizeof(OutCtl) + kRadixBins * sizeof(uint32_t) + (size_t)nch * (sizeof(ChunkFold) + 8) +
+                            (size_t)n * 8;
         AZ_HIPO(C, C->outputScratch.alloc(need));
         char *base = (char *)C->outputScratch.ptr;
         ctl = (OutCtl *)base;
-        uint32_t *hist = (uint32_t *)(base + sizeof(OutCtl));
+        ChunkFold *folds = (ChunkFold *)(base + sizeof(OutCtl));
+        double *sums = (double *)(folds + nch);
+        uint32_t *hist = (uint32_t *)(sums + nch);
         uint32_t *keys = hist + kRadixBins;
         float *logs = (float *)(keys + n);
         hipLaunchKernelGGL(k_out_prepare, grid, dim3(kOutBlock), 0, st, (const float4 *)d_Pixels, n, keys, logs);
-        hipLaunchKernelGGL(k_out_logsum, dim3(1), dim3(64), 0, st, (const float *)logs, n, Tonemap->KeyValue, rank, ctl,
-                           hist);
+        launch_sequential_sum(logs, n, folds, sums, Tonemap->KeyValue, rank, ctl, hist, nullptr, st);
         const uint32_t histBlocks = std::min<uint32_t>(grid.x, (uint32_t)C->cuCount * 4u);
         const uint32_t shifts[3] = {21u, 10u, 0u};
         const uint32_t masks[3] = {0u, 0xFFE00000u, 0xFFFFFC00u};
@@ -239,5 +396,26 @@ az_status az_gpu_output_pixels(az_gpu_ctx *C, const pixel *d_Pixels, u32 Width, 
     C->lastStart = C->ev0;
     C->lastStop = C->ev1;
     C->pendingTiming = true;
+    return AZ_OK;
+}
+
+az_status az_gpu_sum_f32_sequential(az_gpu_ctx *C, const f32 *d_X, u32 N, f32 *d_Sum) {
+    if (!C || !d_Sum || (N && !d_X)) return AZ_ERR_INVALID_ARGUMENT;
+    if (N >= 0x80000000u) {
+        C->lastError = "more than 2^31 - 1 values";
+        return AZ_ERR_INVALID_ARGUMENT;
+    }
+    AZ_HIPO(C, hipSetDevice(C->device));
+    hipStream_t st = C->stream;
+    if (N == 0) {
+        AZ_HIPO(C, hipMemsetAsync(d_Sum, 0, sizeof(f32), st));
+        return AZ_OK;
+    }
+    const uint32_t nch = (N + kChunk - 1) / kChunk;
+    AZ_HIPO(C, C->outputScratch.alloc((size_t)nch * (sizeof(ChunkFold) + 8)));
+    ChunkFold *folds = (ChunkFold *)C->outputScratch.ptr;
+    double *sums = (double *)(folds + nch);
+    launch_sequential_sum(d_X, N, folds, sums, 0.0f, 0u, nullptr, nullptr, d_Sum, st);
+    AZ_HIPO(C, hipGetLastError());
     return AZ_OK;
 }

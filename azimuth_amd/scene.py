@@ -414,6 +414,19 @@ class GpuContext:
             d_px.free()
             d_out.free()
 
+    def sum_f32_sequential(self, x):
+        """The float32 sum of x in sequence order, bit for bit."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        d_x, d_s = self.alloc(max(4, x.nbytes)), self.alloc(4)
+        try:
+            d_x.upload(x)
+            N.check(self.lib.az_gpu_sum_f32_sequential(self.h, d_x.ptr, int(x.size), d_s.ptr),
+                    "az_gpu_sum_f32_sequential", self.h)
+            return d_s.download(np.float32, 1)[0]
+        finally:
+            d_x.free()
+            d_s.free()
+
     def trim(self):
         """Frees the context's render queues (the scene stays uploaded)."""
         N.check(self.lib.az_gpu_trim(self.h), "az_gpu_trim", self.h)

@@ -186,6 +186,15 @@ az_status az_gpu_render_batch(az_gpu_ctx *const *Ctxs, const camera *const *Came
 az_status az_gpu_output_pixels(az_gpu_ctx *Ctx, const pixel *d_Pixels, u32 Width, u32 Height,
                                const tonemap *Tonemap, f32 InvGamma, b32 sRGB, u8 *d_Out);
 
+/* The f32 sum of d_X[0..N) bit for bit as the loop `s = 0; for i: s += x[i]`
+ * computes it in binary32 (the order OutputPixelsToFile sums its log
+ * luminances in, main.cpp:1643-1661), into *d_Sum (device memory); runs in
+ * parallel (chunks whose running sum stays in one binade are folded as
+ * integer offsets of the mantissa grid).  N < 2^31.  No reference
+ * counterpart as an entry point: the primitive behind az_gpu_output_pixels's
+ * log-average. */
+az_status az_gpu_sum_f32_sequential(az_gpu_ctx *Ctx, const f32 *d_X, u32 N, f32 *d_Sum);
+
 az_status az_gpu_get_stats(az_gpu_ctx *Ctx, az_stats *Stats);
 /* Enables per-ray node/triangle counting in trace calls (slower). */
 az_status az_gpu_set_counting(az_gpu_ctx *Ctx, b32 Enable);

@@ -90,3 +90,40 @@ def test_output_rejects_a_burnout_outside_the_image(gpu):
             gpu.output_pixels(px, 4, 4, tonemap(0.18, burnout, 1.0), 1.0, False)
     # An empty image is a no-op.
     assert gpu.output_pixels(px[:0], 0, 0, tonemap(), 1.0, False).size == 0
+
+
+def sequential_sum(x):
+    """numpy's float32 accumulate adds in sequence order, one rounding per step."""
+    x = np.asarray(x, np.float32)
+    return np.add.accumulate(x, dtype=np.float32)[-1] if x.size else np.float32(0.0)
+
+
+def sum_inputs(kind, n, rng):
+    if kind == "logs":
+        return np.log(rng.exponential(0.3, n).astype(np.float32) + np.float32(1e-5))
+    if kind == "dyadic":            # few mantissa bits: many exact ties
+        return (rng.integers(-40, 8, n) / 8.0).astype(np.float32)
+    if kind == "near_zero":         # the running sum keeps crossing zero
+        return (rng.integers(-3, 4, n) * 0.5).astype(np.float32)
+    if kind == "wide":              # magnitudes over many binades
+        return (rng.normal(0, 1, n) * np.exp2(rng.integers(-20, 20, n))).astype(np.float32)
+    if kind == "with_inf":
+        x = rng.normal(-1.0, 1.0, n).astype(np.float32)
+        x[n // 2] = np.inf
+        return x
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("kind", ["logs", "dyadic", "near_zero", "wide", "with_inf"])
+def test_sequential_sum_bit_exact(gpu, kind):
+    """The parallel fold reproduces the sequential float32 sum bit for bit,
+    at sizes from one value to a 4K image's worth, including partial chunks
+    and inputs full of rounding ties."""
+    rng = np.random.default_rng(5)
+    for n in (1, 2, 63, 64, 65, 1023, 1024, 1025, 4097, 100003, 2073600, 8294400):
+        x = sum_inputs(kind, n, rng)
+        want = sequential_sum(x)
+        got = gpu.sum_f32_sequential(x)
+        assert (got == want) or (np.isnan(got) and np.isnan(want)), (kind, n, got, want)
+        assert np.signbit(got) == np.signbit(want), (kind, n, got, want)
+    assert gpu.sum_f32_sequential(np.zeros(0, np.float32)) == 0.0
