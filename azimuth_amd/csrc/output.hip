@@ -88,12 +88,11 @@ __global__ __launch_bounds__(kOutBlock) void k_out_prepare(const float4 *__restr
 constexpr uint32_t kChunk = 1024;
 constexpr int kNoBinade = -100000;
 
-struct ChunkFold {
-    long long q[2];                  // offset of s / u over the chunk, by start parity
-    long long lo[2], hi[2];          // lowest and highest partial offset
-    int ok[2];                       // 0: a log is not finite or too large for the grid
+struct ChunkFold {                   // 32 bytes: eight words a lane each in the walk
+    int q[2];                        // offset of the integer mantissa over the chunk, by start parity
+    int lo[2], hi[2];                // lowest and highest partial offset
     int binade;                      // e the fold assumes, or kNoBinade
-    int pad;
+    uint32_t ok;                     // bit p: the fold for start parity p holds (values finite, offsets < 2^25)
 };
 
 // Approximate chunk sums in double (their order does not matter: they only
@@ -141,48 +140,98 @@ __global__ __launch_bounds__(kOutBlock) void k_out_chunk_scan(const double *__re
     }
 }
 
-// One lane per chunk: the integer offsets of the chunk at its binade.
+// A run of values as a map of the integer mantissa: for start parity p the
+// offset d[p] it adds and its lowest and highest partial offsets.  An empty
+// run is the identity (lo = +big, hi = -big).
+struct RunMap {
+    long long d[2], lo[2], hi[2];
+    bool ok;
+};
+
+// The map of run a followed by run b.
+__device__ inline RunMap compose(const RunMap &a, const RunMap &b) {
+    RunMap r;
+    for (int p = 0; p < 2; ++p) {
+        const int pb = (int)((p + a.d[p]) & 1ll);
+        r.d[p] = a.d[p] + b.d[pb];
+        r.lo[p] = min(a.lo[p], a.d[p] + b.lo[pb]);
+        r.hi[p] = max(a.hi[p], a.d[p] + b.hi[pb]);
+    }
+    r.ok = a.ok && b.ok;
+    return r;
+}
+
+// One wave per chunk: each lane maps its 16 values (exact in binary32: the
+// scale by 2^(23 - e) only moves the exponent, and q = round(y) is exact for
+// |y| <= 2^40), then an order-keeping tree of compositions gives the chunk's
+// map in lane 0.
 __global__ __launch_bounds__(kOutBlock) void k_out_chunk_fold(const float *__restrict__ logs, uint32_t n, uint32_t nch,
                                                               ChunkFold *__restrict__ folds) {
-    const uint32_t k = blockIdx.x * kOutBlock + threadIdx.x;
+    const uint32_t k = blockIdx.x * (kOutBlock / 64u) + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
     if (k >= nch) return;
-    ChunkFold f = folds[k];
-    const int e = f.binade;
+    const int e = folds[k].binade;
     const uint32_t base = k * kChunk, end = min(n, base + kChunk);
-    long long P[2] = {0, 0}, lo[2] = {0, 0}, hi[2] = {0, 0};
-    bool ok = e != kNoBinade;
-    bool first = true;
-    for (uint32_t j = base; ok && j < end; ++j) {
-        const double y = ldexp((double)logs[j], 23 - e);    // exact: a power-of-two scale
-        if (!(fabs(y) <= 1099511627776.0)) {                  // not finite, or beyond 2^40
-            ok = false;
+    constexpr long long kBig = 1ll << 60;
+    RunMap m;
+    for (int p = 0; p < 2; ++p) {
+        m.d[p] = 0;
+        m.lo[p] = kBig;
+        m.hi[p] = -kBig;
+    }
+    m.ok = e != kNoBinade;
+    const uint32_t j0 = base + 16u * lane;
+    for (uint32_t t = 0; t < 16u && m.ok && j0 + t < end; ++t) {
+        const float y = ldexpf(logs[j0 + t], 23 - e);
+        if (!(fabsf(y) <= 1099511627776.0f)) {             // not finite, or beyond 2^40
+            m.ok = false;
             break;
         }
-        const double fl = floor(y), fr = y - fl;
+        const float fl = floorf(y), fr = y - fl;
         const long long qf = (long long)fl;
         for (int p = 0; p < 2; ++p) {
             long long q = qf;
-            if (fr > 0.5) q = qf + 1;
-            else if (fr == 0.5) q = (((long long)p + P[p] + qf) & 1ll) ? qf + 1 : qf;
-            P[p] += q;
-            lo[p] = first ? P[p] : min(lo[p], P[p]);
-            hi[p] = first ? P[p] : max(hi[p], P[p]);
+            if (fr > 0.5f) q = qf + 1;
+            else if (fr == 0.5f) q = (((long long)p + m.d[p] + qf) & 1ll) ? qf + 1 : qf;
+            m.d[p] += q;
+            m.lo[p] = min(m.lo[p], m.d[p]);
+            m.hi[p] = max(m.hi[p], m.d[p]);
         }
-        first = false;
     }
-    for (int p = 0; p < 2; ++p) {
-        f.q[p] = P[p];
-        f.lo[p] = lo[p];
-        f.hi[p] = hi[p];
-        f.ok[p] = ok ? 1 : 0;
+    for (uint32_t s = 1; s < 64u; s <<= 1) {
+        RunMap o;
+        for (int p = 0; p < 2; ++p) {
+            o.d[p] = __shfl_down(m.d[p], s);
+            o.lo[p] = __shfl_down(m.lo[p], s);
+            o.hi[p] = __shfl_down(m.hi[p], s);
+        }
+        o.ok = __shfl_down((int)m.ok, s) != 0;
+        if ((lane & (2u * s - 1u)) == 0u && lane + s < 64u) m = compose(m, o);
     }
-    folds[k] = f;
+    if (lane == 0) {
+        ChunkFold f;
+        f.binade = e;
+        f.ok = 0u;
+        for (int p = 0; p < 2; ++p) {
+            const bool fits = m.ok && m.lo[p] > -(1ll << 25) && m.hi[p] < (1ll << 25) && m.d[p] > -(1ll << 25) &&
+                              m.d[p] < (1ll << 25);
+            f.q[p] = fits ? (int)m.d[p] : 0;
+            f.lo[p] = fits ? (int)m.lo[p] : 0;
+            f.hi[p] = fits ? (int)m.hi[p] : 0;
+            f.ok |= fits ? (1u << p) : 0u;
+        }
+        folds[k] = f;
+    }
 }
 
-// One wave: walks the chunks (fold where it is exact, else the sequential
-// chain: 64 logs per coalesced load, added one after another from lane
-// registers); then log-average = exp(sum / n) in double (Exp,
-// math.cpp:145-152) and the scale.  Also starts the radix select and clears
+// One wave walks the chunks, their folds staged in LDS 1024 at a time (one
+// wait on global memory per tile; the per-chunk reads are LDS broadcasts):
+// where s is a normal float in the fold's
+// binade and every partial mantissa stays strictly inside (2^23, 2^24), the
+// chunk is one integer step of the mantissa; any other chunk is added
+// element by element (64 logs per coalesced load, one after another from
+// lane registers).  Then log-average = exp(sum / n) in double (Exp,
+// math.cpp:145-152) and the scale; also starts the radix select and clears
 // its histogram.
 __global__ __launch_bounds__(64) void k_out_logsum(const float *__restrict__ logs, uint32_t n,
                                                    const ChunkFold *__restrict__ folds, float key, uint32_t rank,
@@ -192,42 +241,74 @@ __global__ __launch_bounds__(64) void k_out_logsum(const float *__restrict__ log
     if (hist)
         for (uint32_t b = lane; b < kRadixBins; b += 64u) hist[b] = 0u;
     const uint32_t nch = (n + kChunk - 1) / kChunk;
-    const long long lo24 = 1ll << 23, hi24 = 1ll << 24;
+    constexpr uint32_t kTile = 1024;
+    __shared__ ChunkFold tile[kTile];
     float s = 0.0f;
-    for (uint32_t k = 0; k < nch; ++k) {
-        bool done = false;
-        if (s != 0.0f && isfinite(s) && fabsf(s) >= 1.17549435e-38f) {
-            int ex;
-            frexpf(s, &ex);
-            const int e = ex - 1;
-            const ChunkFold &f = folds[k];
-            if (e == f.binade) {
-                const long long S = (long long)ldexp((double)s, 23 - e);   // the integer mantissa, signed
-                const int p = (int)(S & 1ll);
-                if (f.ok[p]) {
-                    const bool inside = S > 0 ? (S + f.lo[p] > lo24 && S + f.hi[p] < hi24)
-                                              : (S + f.hi[p] < -lo24 && S + f.lo[p] > -hi24);
+    for (uint32_t k0 = 0; k0 < nch; k0 += kTile) {
+        const uint32_t tn = min(kTile, nch - k0);
+        const uint4 *src = (const uint4 *)(folds + k0);
+        uint4 *dst = (uint4 *)tile;
+        for (uint32_t i = lane; i < 2u * tn; i += 64u) dst[i] = src[i];
+        __syncthreads();
+        // Each fold is read whole (two 16-byte LDS reads), one chunk ahead.
+        uint4 na = dst[0], nb = dst[1];
+        for (uint32_t k = k0; k < k0 + tn; ++k) {
+            const uint4 fa = na, fb = nb;   // q[0], q[1], lo[0], lo[1] | hi[0], hi[1], binade, ok
+            if (k + 1u < k0 + tn) {
+                na = dst[2u * (k + 1u - k0)];
+                nb = dst[2u * (k + 1u - k0) + 1u];
+            }
+            bool done = false;
+            const uint32_t bits = __float_as_uint(s);
+            const uint32_t E = (bits >> 23) & 0xFFu;
+            if (E != 0u && E != 0xFFu && (int)E - 127 == (int)fb.z) {
+                const int M = (int)((bits & 0x7FFFFFu) | 0x800000u);
+                const int S = (bits >> 31) ? -M : M;
+                const int p = S & 1;
+                if ((fb.w >> p) & 1u) {
+                    const int q = (int)(p ? fa.y : fa.x);
+                    const int lo = (int)(p ? fa.w : fa.z);
+                    const int hi = (int)(p ? fb.y : fb.x);
+                    const bool inside = S > 0 ? (S + lo > (1 << 23) && S + hi < (1 << 24))
+                                              : (S + hi < -(1 << 23) && S + lo > -(1 << 24));
                     if (inside) {
-                        s = (float)ldexp((double)(S + f.q[p]), e - 23);
+                        const int S2 = S + q;
+                        const uint32_t M2 = (uint32_t)(S2 < 0 ? -S2 : S2);
+                        s = __uint_as_float((bits & 0xFF800000u) | (M2 & 0x7FFFFFu));
                         done = true;
                     }
                 }
             }
-        }
-        if (!done) {
-            const uint32_t base = k * kChunk, end = min(n, base + kChunk);
-            for (uint32_t b = base; b < end; b += 64u) {
-                const float x = b + lane < end ? logs[b + lane] : 0.0f;
-                const uint32_t m = min(64u, end - b);
-                if (m == 64u) {
+            if (!done && k * kChunk + kChunk <= n) {
+                // A whole chunk: its 1024 logs in 16 registers a lane (one
+                // wait), then added in order.
+                const uint32_t base = k * kChunk;
+                float v[kChunk / 64];
 #pragma unroll
-                    for (int j = 0; j < 64; ++j) s += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j));
-                } else {
-                    for (uint32_t j = 0; j < m; ++j)
-                        s += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), (int)j));
+                for (uint32_t t = 0; t < kChunk / 64; ++t) v[t] = logs[base + 64u * t + lane];
+#pragma unroll
+                for (uint32_t t = 0; t < kChunk / 64; ++t)
+#pragma unroll
+                    for (int j = 0; j < 64; ++j) s += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[t]), j));
+                done = true;
+            }
+            if (!done) {
+                const uint32_t base = k * kChunk, end = min(n, base + kChunk);
+                for (uint32_t b = base; b < end; b += 64u) {
+                    const float x = b + lane < end ? logs[b + lane] : 0.0f;
+                    const uint32_t m = min(64u, end - b);
+                    if (m == 64u) {
+#pragma unroll
+                        for (int j = 0; j < 64; ++j)
+                            s += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), j));
+                    } else {
+                        for (uint32_t j = 0; j < m; ++j)
+                            s += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), (int)j));
+                    }
                 }
             }
         }
+        __syncthreads();
     }
     if (lane == 0 && sumOut) *sumOut = s;
     if (lane == 0 && ctl) {
@@ -260,24 +341,42 @@ __global__ __launch_bounds__(kOutBlock) void k_out_hist(const uint32_t *__restri
 
 // The bin that holds the wanted rank: its bits join the prefix, the rank
 // becomes the rank within it; the histogram is cleared for the next pass.
-// After the last pass the prefix is the key of the order statistic.
+// After the last pass the prefix is the key of the order statistic.  Each
+// thread sums 8 bins, a block scan finds the thread whose bins hold the
+// rank, and that thread walks its 8.
 __global__ __launch_bounds__(kOutBlock) void k_out_pick(uint32_t *__restrict__ hist, OutCtl *__restrict__ ctl,
                                                         uint32_t shift, uint32_t last) {
     __shared__ uint32_t h[kRadixBins];
-    for (uint32_t b = threadIdx.x; b < kRadixBins; b += kOutBlock) {
-        h[b] = hist[b];
+    __shared__ uint32_t part[kOutBlock];
+    constexpr uint32_t kPer = kRadixBins / kOutBlock;
+    const uint32_t t = threadIdx.x;
+    uint32_t local = 0u;
+    for (uint32_t i = 0; i < kPer; ++i) {
+        const uint32_t b = t * kPer + i;
+        const uint32_t c = hist[b];
         hist[b] = 0u;
+        h[b] = c;
+        local += c;
     }
+    part[t] = local;
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    uint32_t rank = ctl->rank, b = 0u;
-    while (b < kRadixBins - 1u && rank >= h[b]) {
-        rank -= h[b];
+    for (uint32_t o = 1; o < kOutBlock; o <<= 1) {
+        const uint32_t add = t >= o ? part[t - o] : 0u;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    const uint32_t rank = ctl->rank;
+    const uint32_t before = t ? part[t - 1] : 0u;
+    if (rank < before || rank >= part[t]) return;
+    uint32_t r = rank - before, b = t * kPer;
+    while (b < t * kPer + kPer - 1u && r >= h[b]) {
+        r -= h[b];
         ++b;
     }
     const uint32_t prefix = ctl->prefix | (b << shift);
     ctl->prefix = prefix;
-    ctl->rank = rank;
+    ctl->rank = r;
     if (last) {
         const float w = key_float(prefix) * ctl->scale;
         ctl->whiteSqr = w * w;
@@ -318,8 +417,8 @@ void launch_sequential_sum(const float *x, uint32_t n, ChunkFold *folds, double 
     const uint32_t nch = (n + kChunk - 1) / kChunk;
     hipLaunchKernelGGL(k_out_chunk_sums, dim3(nch), dim3(kOutBlock), 0, st, x, n, sums);
     hipLaunchKernelGGL(k_out_chunk_scan, dim3(1), dim3(kOutBlock), 0, st, (const double *)sums, nch, folds);
-    hipLaunchKernelGGL(k_out_chunk_fold, dim3((nch + kOutBlock - 1) / kOutBlock), dim3(kOutBlock), 0, st, x, n, nch,
-                       folds);
+    hipLaunchKernelGGL(k_out_chunk_fold, dim3((nch + kOutBlock / 64u - 1) / (kOutBlock / 64u)), dim3(kOutBlock), 0, st,
+                       x, n, nch, folds);
     hipLaunchKernelGGL(k_out_logsum, dim3(1), dim3(64), 0, st, x, n, (const ChunkFold *)folds, key, rank, ctl, hist,
                        sumOut);
 }
