@@ -44,13 +44,6 @@ struct Aabb {
     }
 };
 
-struct Node2 {
-    Aabb box;
-    uint32_t left = 0, right = 0;
-    uint32_t first = 0, count = 0;
-    bool leaf = false;
-};
-
 struct Bin {
     Aabb box;
     uint32_t count;
@@ -71,11 +64,16 @@ void PadBox(BuildBox &b) {
     }
 }
 
-uint32_t BuildWideBvh(const std::vector<BuildBox> &boxes, const BuildParams &P, uint32_t itemBase,
-                      std::vector<Node4> &nodes, std::vector<uint32_t> &order, BuildStats *stats) {
+static float binArea(const BinNode &b) {
+    float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
+    if (dx < 0.0f || dy < 0.0f || dz < 0.0f) return 0.0f;
+    return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+
+void BuildBinaryBvh(const std::vector<BuildBox> &boxes, const BuildParams &P, std::vector<BinNode> &b2,
+                    std::vector<uint32_t> &order) {
     const uint32_t n = (uint32_t)boxes.size();
     const uint32_t maxLeaf = std::max(1u, std::min(P.maxLeafItems, kMaxLeafItems));
-    const uint32_t width = std::max(2u, std::min(P.width, 4u));
     order.resize(n);
     for (uint32_t i = 0; i < n; ++i) order[i] = i;
     std::vector<Aabb> ib(n);
@@ -87,9 +85,8 @@ uint32_t BuildWideBvh(const std::vector<BuildBox> &boxes, const BuildParams &P, 
             cen[3 * (size_t)i + k] = 0.5f * (boxes[i].lo[k] + boxes[i].hi[k]);
         }
     }
-
+    b2.clear();
     // ---- binary build -------------------------------------------------------
-    std::vector<Node2> b2;
     b2.reserve(n > 0 ? 2 * (size_t)n : 1);
     struct Work { uint32_t node, begin, end; };
     std::vector<Work> stack;
@@ -102,7 +99,6 @@ uint32_t BuildWideBvh(const std::vector<BuildBox> &boxes, const BuildParams &P, 
     while (!stack.empty()) {
         Work w = stack.back();
         stack.pop_back();
-        Node2 &nd = b2[w.node];
         Aabb box, cb;
         box.reset();
         cb.reset();
@@ -110,7 +106,10 @@ uint32_t BuildWideBvh(const std::vector<BuildBox> &boxes, const BuildParams &P, 
             box.grow(ib[order[i]]);
             cb.growPoint(&cen[3 * (size_t)order[i]]);
         }
-        nd.box = box;
+        for (int k = 0; k < 3; ++k) {
+            b2[w.node].lo[k] = box.lo[k];
+            b2[w.node].hi[k] = box.hi[k];
+        }
         const uint32_t cnt = w.end - w.begin;
         float bestCost = FLT_MAX;
         int bestAxis = -1;
@@ -163,9 +162,9 @@ uint32_t BuildWideBvh(const std::vector<BuildBox> &boxes, const BuildParams &P, 
         float leafCost = P.itemCost * (float)cnt;
         bool makeLeaf = cnt <= 1 || (cnt <= maxLeaf && leafCost <= splitCost);
         if (makeLeaf) {
-            nd.leaf = true;
-            nd.first = w.begin;
-            nd.count = cnt;
+            b2[w.node].leaf = 1u;
+            b2[w.node].first = w.begin;
+            b2[w.node].count = cnt;
             continue;
         }
         uint32_t mid;
@@ -173,7 +172,7 @@ uint32_t BuildWideBvh(const std::vector<BuildBox> &boxes, const BuildParams &P, 
             float ext = cb.hi[bestAxis] - cb.lo[bestAxis];
             float scale = (float)nb / ext;
             uint32_t *b0 = order.data() + w.begin, *b1 = order.data() + w.end;
-            uint32_t *m = std::partition(b0, b1, [&](uint32_t it) {
+            uint32_t *m = std::stable_partition(b0, b1, [&](uint32_t it) {
                 int b = (int)((cen[3 * (size_t)it + bestAxis] - cb.lo[bestAxis]) * scale);
                 b = std::min(std::max(b, 0), (int)nb - 1);
                 return (uint32_t)b < bestSplit;
@@ -191,7 +190,11 @@ uint32_t BuildWideBvh(const std::vector<BuildBox> &boxes, const BuildParams &P, 
         stack.push_back({l + 1, mid, w.end});
         stack.push_back({l, w.begin, mid});
     }
+}
 
+uint32_t CollapseWideBvh(const std::vector<BinNode> &b2, uint32_t root, uint32_t n, const BuildParams &P,
+                         uint32_t itemBase, std::vector<Node4> &nodes, BuildStats *stats) {
+    const uint32_t width = std::max(2u, std::min(P.width, 4u));
     // ---- collapse to 4-wide -------------------------------------------------
     BuildStats st;
     const uint32_t rootIndex = (uint32_t)nodes.size();
@@ -222,7 +225,7 @@ uint32_t BuildWideBvh(const std::vector<BuildBox> &boxes, const BuildParams &P, 
     }
     // The root: when the whole tree is one leaf, the root node holds it as
     // its single child.
-    pend.push_back({0, kEmpty, 0, 1});
+    pend.push_back({root, kEmpty, 0, 1});
     std::vector<uint32_t> depthOf;
     depthOf.push_back(1);
     bool first = true;
@@ -245,7 +248,7 @@ uint32_t BuildWideBvh(const std::vector<BuildBox> &boxes, const BuildParams &P, 
         st.maxDepth = std::max(st.maxDepth, p.depth);
         uint32_t kids[4];
         uint32_t k = 0;
-        const Node2 &src = b2[p.b2node];
+        const BinNode &src = b2[p.b2node];
         if (src.leaf) {
             kids[k++] = p.b2node;
         } else {
@@ -255,9 +258,9 @@ uint32_t BuildWideBvh(const std::vector<BuildBox> &boxes, const BuildParams &P, 
                 int best = -1;
                 float bestArea = -1.0f;
                 for (uint32_t c = 0; c < k; ++c) {
-                    const Node2 &cn = b2[kids[c]];
+                    const BinNode &cn = b2[kids[c]];
                     if (cn.leaf) continue;
-                    float a = cn.box.area();
+                    float a = binArea(cn);
                     if (a > bestArea) {
                         bestArea = a;
                         best = (int)c;
@@ -271,14 +274,14 @@ uint32_t BuildWideBvh(const std::vector<BuildBox> &boxes, const BuildParams &P, 
         }
         childCountOf[local] = k;
         for (uint32_t c = 0; c < k; ++c) {
-            const Node2 &cn = b2[kids[c]];
+            const BinNode &cn = b2[kids[c]];
             Node4 &dst = nodes[me];
-            dst.lox[c] = cn.box.lo[0];
-            dst.loy[c] = cn.box.lo[1];
-            dst.loz[c] = cn.box.lo[2];
-            dst.hix[c] = cn.box.hi[0];
-            dst.hiy[c] = cn.box.hi[1];
-            dst.hiz[c] = cn.box.hi[2];
+            dst.lox[c] = cn.lo[0];
+            dst.loy[c] = cn.lo[1];
+            dst.loz[c] = cn.lo[2];
+            dst.hix[c] = cn.hi[0];
+            dst.hiy[c] = cn.hi[1];
+            dst.hiz[c] = cn.hi[2];
             if (cn.leaf) {
                 dst.child[c] = make_leaf(itemBase + cn.first, cn.count);
                 st.leaves++;
@@ -306,6 +309,13 @@ uint32_t BuildWideBvh(const std::vector<BuildBox> &boxes, const BuildParams &P, 
     (void)parentOf;
     if (stats) *stats = st;
     return rootIndex;
+}
+
+uint32_t BuildWideBvh(const std::vector<BuildBox> &boxes, const BuildParams &P, uint32_t itemBase,
+                      std::vector<Node4> &nodes, std::vector<uint32_t> &order, BuildStats *stats) {
+    std::vector<BinNode> b2;
+    BuildBinaryBvh(boxes, P, b2, order);
+    return CollapseWideBvh(b2, 0, (uint32_t)boxes.size(), P, itemBase, nodes, stats);
 }
 
 }  // namespace az

@@ -28,6 +28,33 @@ struct BuildStats {
     uint32_t maxStack = 0;         // stack entries a front-to-back walk can need
 };
 
+// A node of the binary binned-SAH tree: its box, its children (internal) or
+// its item range [first, first + count) of the order (leaf).
+struct BinNode {
+    float lo[3], hi[3];
+    uint32_t left = 0, right = 0;
+    uint32_t first = 0, count = 0;
+    uint32_t leaf = 0;
+};
+
+// The binary tree on the host: node 0 is the root; `order` receives the item
+// permutation.  Each split partitions its range stably (items keep their
+// relative order on each side), so the device builder (bvh_gpu.hip) makes
+// the same tree.
+void BuildBinaryBvh(const std::vector<BuildBox> &boxes, const BuildParams &params, std::vector<BinNode> &b2,
+                    std::vector<uint32_t> &order);
+
+// Collapses a binary tree (root `root`) into `width`-wide nodes appended to
+// `nodes`; leaf references point at itemBase + first.  Returns the root node
+// index.
+uint32_t CollapseWideBvh(const std::vector<BinNode> &b2, uint32_t root, uint32_t n, const BuildParams &params,
+                         uint32_t itemBase, std::vector<Node4> &nodes, BuildStats *stats);
+
+// Builds the binary tree on the device (hipSuccess, or the error of the
+// first failing call; all device memory is freed either way).
+int BuildBinaryBvhDevice(void *stream, const std::vector<BuildBox> &boxes, const BuildParams &params,
+                         std::vector<BinNode> &b2, std::vector<uint32_t> &order);
+
 // Builds a 4-wide BVH over `boxes` and appends its nodes to `nodes`.  `order`
 // receives the item permutation: leaf references index into it, so item
 // order[i] must be stored at position itemBase + i.  Returns the root node

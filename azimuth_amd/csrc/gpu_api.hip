@@ -108,6 +108,30 @@ BuildBox sphereWorldBox(const entity &e) {
 
 }  // namespace
 
+// The BVH build of one mesh or of the top level: the binary tree on the
+// device (AZ_BUILD_DEVICE, or AZ_BUILD_AUTO from kDeviceBuildMin items on) or
+// on the host, then the host's collapse into wide nodes.  Both give the same
+// tree.
+constexpr size_t kDeviceBuildMin = 65536;
+
+static az_status buildWide(az_gpu_ctx *C, const std::vector<BuildBox> &boxes, const BuildParams &bp, uint32_t itemBase,
+                           std::vector<Node4> &nodes, std::vector<uint32_t> &order, BuildStats *bs, uint32_t *root) {
+    const bool device = C->buildMode == AZ_BUILD_DEVICE ||
+                        (C->buildMode == AZ_BUILD_AUTO && boxes.size() >= kDeviceBuildMin);
+    if (!device) {
+        *root = BuildWideBvh(boxes, bp, itemBase, nodes, order, bs);
+        return AZ_OK;
+    }
+    std::vector<BinNode> b2;
+    const int e = BuildBinaryBvhDevice(C->stream, boxes, bp, b2, order);
+    if (e != (int)hipSuccess) {
+        C->lastError = std::string("device BVH build: ") + hipGetErrorString((hipError_t)e);
+        return AZ_ERR_DEVICE;
+    }
+    *root = CollapseWideBvh(b2, 0, (uint32_t)boxes.size(), bp, itemBase, nodes, bs);
+    return AZ_OK;
+}
+
 az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
     // ---- meshes: leaf-ordered triangles, one BLAS per mesh -----------------
     std::vector<Node4> nodes;
@@ -142,7 +166,9 @@ az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
         std::vector<uint32_t> order;
         BuildStats bs;
         uint32_t triBase = (uint32_t)tris.size();
-        uint32_t root = BuildWideBvh(boxes, bp, triBase, nodes, order, &bs);
+        uint32_t root = 0;
+        az_status bst = buildWide(C, boxes, bp, triBase, nodes, order, &bs, &root);
+        if (bst != AZ_OK) return bst;
         info.meshBlasRoot[m] = root;
         info.meshTriBase[m] = triBase;
         info.maxBlasStack = std::max(info.maxBlasStack, bs.maxStack);
@@ -224,7 +250,9 @@ az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
     tp.width = C->bvhWidth;
     std::vector<uint32_t> torder;
     BuildStats ts;
-    uint32_t tlasRoot = BuildWideBvh(tboxes, tp, 0, nodes, torder, &ts);
+    uint32_t tlasRoot = 0;
+    az_status tst = buildWide(C, tboxes, tp, 0, nodes, torder, &ts, &tlasRoot);
+    if (tst != AZ_OK) return tst;
     std::vector<TopItem> items(items0.size());
     std::vector<const entity *> topOrdered(items0.size());
     for (uint32_t i = 0; i < torder.size(); ++i) {
@@ -440,6 +468,12 @@ az_status az_gpu_set_build(az_gpu_ctx *C, u32 maxLeafItems, f32 nodeCost, f32 it
     C->maxLeafItems = maxLeafItems;
     C->sahNodeCost = nodeCost;
     C->sahItemCost = itemCost;
+    return AZ_OK;
+}
+
+az_status az_gpu_set_build_mode(az_gpu_ctx *C, u32 mode) {
+    if (!C || mode > AZ_BUILD_DEVICE) return AZ_ERR_INVALID_ARGUMENT;
+    C->buildMode = mode;
     return AZ_OK;
 }
 

@@ -427,6 +427,10 @@ class GpuContext:
             d_x.free()
             d_s.free()
 
+    def set_build_mode(self, mode):
+        """Where the next uploads build their binary BVHs (N.AZ_BUILD_*)."""
+        N.check(self.lib.az_gpu_set_build_mode(self.h, int(mode)), "az_gpu_set_build_mode", self.h)
+
     def trim(self):
         """Frees the context's render queues (the scene stays uploaded)."""
         N.check(self.lib.az_gpu_trim(self.h), "az_gpu_trim", self.h)

@@ -219,6 +219,16 @@ az_status az_gpu_set_build(az_gpu_ctx *Ctx, u32 MaxLeafItems, f32 NodeCost, f32 
  * one 128-byte line with four child slots; narrower trees leave slots
  * empty and are deeper. */
 az_status az_gpu_set_bvh_width(az_gpu_ctx *Ctx, u32 Width);
+/* Where the binned-SAH binary tree of each BVH of the next uploads is built:
+ * AZ_BUILD_AUTO (default: on the device for meshes and top levels of 65,536
+ * items or more, on the host below), AZ_BUILD_HOST or AZ_BUILD_DEVICE.  Both
+ * builders make the same tree (the device one repeats the host's float
+ * operations and stable partitions), which the host then collapses into the
+ * wide nodes.  Replaces ConstructBVH* (bvh.cpp:1-328) for the device BVH. */
+az_status az_gpu_set_build_mode(az_gpu_ctx *Ctx, u32 Mode);
+#define AZ_BUILD_AUTO   0u
+#define AZ_BUILD_HOST   1u
+#define AZ_BUILD_DEVICE 2u
 
 /* Autotuning.  Each candidate launch configuration is timed with HIP events
  * on the context stream; the fastest becomes the context's setting (and
