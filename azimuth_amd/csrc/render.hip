@@ -1076,11 +1076,72 @@ __global__ __launch_bounds__(kRenderBlock) void k_guide_query(DevScene S, Device
     out[i] = g;
 }
 
-// A training batch: n of the count recorded samples, picked by a hash of
-// (seed, slot) (splitmix64 finalizer).
-__global__ __launch_bounds__(kRenderBlock) void k_guide_gather(const GuideSample *__restrict__ recs, SegIn rin,
-                                                               uint32_t count, GuideSample *__restrict__ batch,
-                                                               uint32_t n, uint64_t seed) {
+// Training batches are drawn in proportion to the records' weights (a
+// record of weight w stands for w times the loss -log q of its direction),
+// each drawn sample then carrying weight 1: the same gradient in
+// expectation, with far less variance when a few records (directions
+// towards the sun) carry most of the weight; records of weight zero
+// (occluded, or whose path delivered nothing) are never drawn.  The prefix
+// of the weights in dense record order, in three passes (block scans, one
+// block over the block sums, offsets):
+__global__ __launch_bounds__(kRenderBlock) void k_rec_prefix_blocks(const GuideSample *__restrict__ recs, SegIn rin,
+                                                                    uint32_t count, float *__restrict__ cum,
+                                                                    float *__restrict__ sums) {
+    __shared__ uint32_t soff[kSegs + 1];
+    __shared__ float s[kRenderBlock];
+    const uint32_t *off = stage_offsets(rin, soff);
+    const uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
+    float w = 0.0f;
+    if (i < count) {
+        w = recs[seg_slot(off, rin.nseg, rin.segCap, i)].wgt;
+        w = w > 0.0f && w < INFINITY ? w : 0.0f;          // NaN, negative and infinite weights are not drawn
+    }
+    s[threadIdx.x] = w;
+    __syncthreads();
+    for (uint32_t o = 1; o < kRenderBlock; o <<= 1) {
+        const float a = threadIdx.x >= o ? s[threadIdx.x - o] : 0.0f;
+        __syncthreads();
+        s[threadIdx.x] += a;
+        __syncthreads();
+    }
+    if (i < count) cum[i] = s[threadIdx.x];
+    if (threadIdx.x == kRenderBlock - 1) sums[blockIdx.x] = s[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kRenderBlock) void k_rec_prefix_sums(float *__restrict__ sums, uint32_t nb) {
+    __shared__ float s[kRenderBlock];
+    float carry = 0.0f;
+    for (uint32_t b0 = 0; b0 < nb; b0 += kRenderBlock) {
+        const uint32_t i = b0 + threadIdx.x;
+        const float v = i < nb ? sums[i] : 0.0f;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (uint32_t o = 1; o < kRenderBlock; o <<= 1) {
+            const float a = threadIdx.x >= o ? s[threadIdx.x - o] : 0.0f;
+            __syncthreads();
+            s[threadIdx.x] += a;
+            __syncthreads();
+        }
+        if (i < nb) sums[i] = carry + s[threadIdx.x] - v;   // exclusive
+        carry += s[kRenderBlock - 1];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kRenderBlock) void k_rec_prefix_add(float *__restrict__ cum, uint32_t count,
+                                                                 const float *__restrict__ sums) {
+    const uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
+    if (i < count) cum[i] += sums[blockIdx.x];
+}
+
+// A training batch of n records drawn by weight from the count recorded
+// ones (hash of (seed, slot), splitmix64 finalizer, then the first record
+// whose prefix exceeds u times the total); all weights zero: drawn
+// uniformly with their weights.
+__global__ __launch_bounds__(kRenderBlock) void k_guide_gather_weighted(const GuideSample *__restrict__ recs, SegIn rin,
+                                                                        uint32_t count, const float *__restrict__ cum,
+                                                                        GuideSample *__restrict__ batch, uint32_t n,
+                                                                        uint64_t seed) {
     __shared__ uint32_t soff[kSegs + 1];
     const uint32_t *off = stage_offsets(rin, soff);
     uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
@@ -1089,7 +1150,21 @@ __global__ __launch_bounds__(kRenderBlock) void k_guide_gather(const GuideSample
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     z ^= z >> 31;
-    batch[i] = recs[seg_slot(off, rin.nseg, rin.segCap, (uint32_t)(z % count))];
+    const float total = cum[count - 1];
+    if (!(total > 0.0f)) {
+        batch[i] = recs[seg_slot(off, rin.nseg, rin.segCap, (uint32_t)(z % count))];
+        return;
+    }
+    const float target = (float)(z >> 40) * 0x1p-24f * total;
+    uint32_t lo = 0u, hi = count - 1u;                      // first j with cum[j] > target
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cum[mid] > target) hi = mid;
+        else lo = mid + 1u;
+    }
+    GuideSample g = recs[seg_slot(off, rin.nseg, rin.segCap, lo)];
+    g.wgt = 1.0f;
+    batch[i] = g;
 }
 
 // Scatter guide records of depth `depth` pass the luminance their subtree
@@ -1528,6 +1603,7 @@ struct RenderBuffers {
     DeviceBuffer q0, q1, hits, shadow, acc, counters, pixels;
     DeviceBuffer queries, lobes, recs, batch;   // learned guide
     DeviceBuffer scatLobes, srecs, links, qrec0, qrec1;   // scatter guide: lobes, records, per-slot record of the queues
+    DeviceBuffer recCum;                        // prefix of the training records' weights
     DeviceBuffer envCdf;                        // environment luminance distribution
     DeviceBuffer batchArgs, batchTotals;        // batched renders: per-scene arguments, queue totals
 };
@@ -2228,10 +2304,18 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
                     AZ_HIPR(C, hipGetLastError());
                 }
                 u32 bn = std::min<u32>(65536u, R[g]) & ~15u;
+                const u32 pb = (R[g] + kRenderBlock - 1) / kRenderBlock;
+                AZ_HIPR(C, B.recCum.alloc(((size_t)R[g] + pb) * sizeof(float)));
+                float *cum = (float *)B.recCum.ptr, *sums = cum + R[g];
+                hipLaunchKernelGGL(k_rec_prefix_blocks, dim3(pb), dim3(kRenderBlock), 0, st, recs, rin, R[g], cum, sums);
+                hipLaunchKernelGGL(k_rec_prefix_sums, dim3(1), dim3(kRenderBlock), 0, st, sums, pb);
+                hipLaunchKernelGGL(k_rec_prefix_add, dim3(pb), dim3(kRenderBlock), 0, st, cum, R[g], (const float *)sums);
+                AZ_HIPR(C, hipGetLastError());
                 for (u32 k = 0; k < stepsPerChunk; ++k) {
                     uint64_t seed = (o.Seed + 0x9E3779B97F4A7C15ull * (uint64_t)(g + 1)) * (uint64_t)(trainRound * 131 + k + 1);
-                    hipLaunchKernelGGL(k_guide_gather, dim3((bn + kRenderBlock - 1) / kRenderBlock), dim3(kRenderBlock),
-                                       0, st, recs, rin, R[g], (GuideSample *)B.batch.ptr, bn, seed);
+                    hipLaunchKernelGGL(k_guide_gather_weighted, dim3((bn + kRenderBlock - 1) / kRenderBlock),
+                                       dim3(kRenderBlock), 0, st, recs, rin, R[g], (const float *)cum,
+                                       (GuideSample *)B.batch.ptr, bn, seed);
                     AZ_HIPR(C, hipGetLastError());
                     az_status gs = az_guide_train_on_stream(net, B.batch.ptr, bn, st);
                     if (gs != AZ_OK) {
@@ -2640,7 +2724,7 @@ az_status az_gpu_trim(az_gpu_ctx *C) {
     RenderBuffers &B = buffersOf(C);
     DeviceBuffer *bufs[] = {&B.q0,      &B.q1,        &B.hits,  &B.shadow,      &B.acc,   &B.pixels, &B.queries,
                             &B.lobes,   &B.recs,      &B.batch, &B.batchArgs,   &B.batchTotals,     &B.scatLobes,
-                            &B.srecs,   &B.links,     &B.qrec0, &B.qrec1};
+                            &B.srecs,   &B.links,     &B.qrec0, &B.qrec1,         &B.recCum};
     for (DeviceBuffer *b : bufs) b->release();
     C->outputScratch.release();
     if (C->guide) az_guide_destroy(C->guide);
