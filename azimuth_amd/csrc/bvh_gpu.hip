@@ -8,7 +8,7 @@
 //              left-to-right sweeps with the same float operations; the
 //              first minimum over (axis, bin) wins, as in the host loop
 //   decision : the host's leaf test
-//   partition: stable (ballot ranks within each 256-item tile), through a
+//   partition: stable (ballot ranks within each workgroup-wide tile), through a
 //              scratch copy of the range
 // then a scan over the split flags numbers the children and lays out the
 // next level's ranges.  The host reads the tree back and collapses it into
@@ -56,23 +56,30 @@ __global__ __launch_bounds__(kBuildBlock) void k_bvh_init(const float *__restric
     order[i] = i;
 }
 
-// Block-wide min (or max) of one float per thread, in `red`.
+// Block-wide min (or max) of one float per thread, in `red` (one entry per
+// wave of the BLOCK-thread workgroup).
+template <int BLOCK>
 __device__ inline float block_min(float v, float *red) {
     for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_down(v, o));
     const uint32_t w = threadIdx.x >> 6;
     __syncthreads();
     if ((threadIdx.x & 63u) == 0) red[w] = v;
     __syncthreads();
-    return fminf(fminf(red[0], red[1]), fminf(red[2], red[3]));
+    float r = red[0];
+    for (int k = 1; k < BLOCK / 64; ++k) r = fminf(r, red[k]);
+    return r;
 }
 
+template <int BLOCK>
 __device__ inline float block_max(float v, float *red) {
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_down(v, o));
     const uint32_t w = threadIdx.x >> 6;
     __syncthreads();
     if ((threadIdx.x & 63u) == 0) red[w] = v;
     __syncthreads();
-    return fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    float r = red[0];
+    for (int k = 1; k < BLOCK / 64; ++k) r = fmaxf(r, red[k]);
+    return r;
 }
 
 struct LevelParams {
@@ -80,17 +87,21 @@ struct LevelParams {
     float nodeCost, itemCost;
 };
 
-__global__ __launch_bounds__(kBuildBlock) void k_bvh_level(const float *__restrict__ boxes, const float *__restrict__ cen,
+// BLOCK threads per range: 1024 while a level has few ranges (the top
+// levels, where one workgroup walks up to all items), 256 below.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_bvh_level(const float *__restrict__ boxes, const float *__restrict__ cen,
                                                            uint32_t *__restrict__ order, uint32_t *__restrict__ tmp,
                                                            const BuildRange *__restrict__ ranges,
                                                            DevBinNode *__restrict__ nodes, uint32_t *__restrict__ split,
                                                            uint32_t *__restrict__ mids, LevelParams P) {
-    __shared__ float red[4];
+    constexpr uint32_t W = BLOCK / 64;
+    __shared__ float red[W];
     __shared__ float box[6], cb[6];
     __shared__ uint32_t binLo[3][kBuildBins][3], binHi[3][kBuildBins][3], binCnt[3][kBuildBins];
     __shared__ float axCost[3];
     __shared__ uint32_t axSplit[3];
-    __shared__ uint32_t tileCount[4];
+    __shared__ uint32_t tileCount[W];
     __shared__ uint32_t decision[2];   // split (1/0), mid
     const BuildRange rg = ranges[blockIdx.x];
     const uint32_t t = threadIdx.x;
@@ -99,7 +110,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_bvh_level(const float *__restri
     // ---- bounds -------------------------------------------------------------
     float bl[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, bh[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
     float cl[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, ch[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-    for (uint32_t i = rg.begin + t; i < rg.end; i += kBuildBlock) {
+    for (uint32_t i = rg.begin + t; i < rg.end; i += BLOCK) {
         const uint32_t it = order[i];
         for (int k = 0; k < 3; ++k) {
             bl[k] = fminf(bl[k], boxes[6 * (size_t)it + k]);
@@ -110,10 +121,10 @@ __global__ __launch_bounds__(kBuildBlock) void k_bvh_level(const float *__restri
         }
     }
     for (int k = 0; k < 3; ++k) {
-        const float a = block_min(bl[k], red);
-        const float b = block_max(bh[k], red);
-        const float c = block_min(cl[k], red);
-        const float d = block_max(ch[k], red);
+        const float a = block_min<BLOCK>(bl[k], red);
+        const float b = block_max<BLOCK>(bh[k], red);
+        const float c = block_min<BLOCK>(cl[k], red);
+        const float d = block_max<BLOCK>(ch[k], red);
         if (t == 0) {
             box[k] = a;
             box[3 + k] = b;
@@ -125,7 +136,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_bvh_level(const float *__restri
 
     // ---- binning ------------------------------------------------------------
     if (cnt > 1) {
-        for (uint32_t b = t; b < 3 * kBuildBins; b += kBuildBlock) {
+        for (uint32_t b = t; b < 3 * kBuildBins; b += BLOCK) {
             const uint32_t ax = b / kBuildBins, bb = b % kBuildBins;
             for (int k = 0; k < 3; ++k) {
                 binLo[ax][bb][k] = fkey(FLT_MAX);
@@ -134,7 +145,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_bvh_level(const float *__restri
             binCnt[ax][bb] = 0u;
         }
         __syncthreads();
-        for (uint32_t i = rg.begin + t; i < rg.end; i += kBuildBlock) {
+        for (uint32_t i = rg.begin + t; i < rg.end; i += BLOCK) {
             const uint32_t it = order[i];
             for (int ax = 0; ax < 3; ++ax) {
                 const float ext = cb[3 + ax] - cb[ax];
@@ -239,16 +250,17 @@ __global__ __launch_bounds__(kBuildBlock) void k_bvh_level(const float *__restri
             return (uint32_t)b < bestSplit;
         };
         uint32_t myLeft = 0;
-        for (uint32_t i = rg.begin + t; i < rg.end; i += kBuildBlock) myLeft += goesLeft(order[i]) ? 1u : 0u;
+        for (uint32_t i = rg.begin + t; i < rg.end; i += BLOCK) myLeft += goesLeft(order[i]) ? 1u : 0u;
         for (int o = 32; o > 0; o >>= 1) myLeft += __shfl_down(myLeft, o);
         if ((t & 63u) == 0) tileCount[t >> 6] = myLeft;
         __syncthreads();
-        const uint32_t nLeft = tileCount[0] + tileCount[1] + tileCount[2] + tileCount[3];
+        uint32_t nLeft = 0;
+        for (uint32_t w = 0; w < W; ++w) nLeft += tileCount[w];
         __syncthreads();
         if (nLeft != 0u && nLeft != cnt) {
             mid = rg.begin + nLeft;
             uint32_t leftBase = 0, rightBase = 0;
-            for (uint32_t t0 = rg.begin; t0 < rg.end; t0 += kBuildBlock) {
+            for (uint32_t t0 = rg.begin; t0 < rg.end; t0 += BLOCK) {
                 const uint32_t i = t0 + t;
                 const bool valid = i < rg.end;
                 const uint32_t it = valid ? order[i] : 0u;
@@ -261,7 +273,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_bvh_level(const float *__restri
                 if ((t & 63u) == 0) tileCount[t >> 6] = (uint32_t)__popcll(ml);
                 __syncthreads();
                 uint32_t waveLeft = 0, tileLeft = 0;
-                for (uint32_t w = 0; w < 4u; ++w) {
+                for (uint32_t w = 0; w < W; ++w) {
                     if (w < (t >> 6)) waveLeft += tileCount[w];
                     tileLeft += tileCount[w];
                 }
@@ -273,12 +285,12 @@ __global__ __launch_bounds__(kBuildBlock) void k_bvh_level(const float *__restri
                     tmp[dst] = it;
                 }
                 leftBase += tileLeft;
-                rightBase += min(kBuildBlock, rg.end - t0) - tileLeft;
+                rightBase += min((uint32_t)BLOCK, rg.end - t0) - tileLeft;
                 __syncthreads();
             }
             __threadfence_block();
             __syncthreads();
-            for (uint32_t i = rg.begin + t; i < rg.end; i += kBuildBlock) order[i] = tmp[i];
+            for (uint32_t i = rg.begin + t; i < rg.end; i += BLOCK) order[i] = tmp[i];
         }
     }
     if (t == 0) {
@@ -400,9 +412,14 @@ int BuildBinaryBvhDevice(void *stream, const std::vector<BuildBox> &boxes, const
     BuildRange *cur = (BuildRange *)dR0.p, *nxt = (BuildRange *)dR1.p;
     uint32_t R = 1, nodeCount = 1;
     while (R > 0) {
-        hipLaunchKernelGGL(k_bvh_level, dim3(R), dim3(kBuildBlock), 0, st, (const float *)dBoxes.p,
-                           (const float *)dCen.p, (uint32_t *)dOrder.p, (uint32_t *)dTmp.p, (const BuildRange *)cur,
-                           (DevBinNode *)dNodes.p, (uint32_t *)dSplit.p, (uint32_t *)dMids.p, lp);
+        if (R < 1024u)
+            hipLaunchKernelGGL(k_bvh_level<1024>, dim3(R), dim3(1024), 0, st, (const float *)dBoxes.p,
+                               (const float *)dCen.p, (uint32_t *)dOrder.p, (uint32_t *)dTmp.p, (const BuildRange *)cur,
+                               (DevBinNode *)dNodes.p, (uint32_t *)dSplit.p, (uint32_t *)dMids.p, lp);
+        else
+            hipLaunchKernelGGL(k_bvh_level<256>, dim3(R), dim3(256), 0, st, (const float *)dBoxes.p,
+                               (const float *)dCen.p, (uint32_t *)dOrder.p, (uint32_t *)dTmp.p, (const BuildRange *)cur,
+                               (DevBinNode *)dNodes.p, (uint32_t *)dSplit.p, (uint32_t *)dMids.p, lp);
         const uint32_t rb = (R + kBuildBlock - 1) / kBuildBlock;
         hipLaunchKernelGGL(k_scan_blocks, dim3(rb), dim3(kBuildBlock), 0, st, (const uint32_t *)dSplit.p, R,
                            (uint32_t *)dRank.p, (uint32_t *)dSums.p);
