@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "azimuth_gpu.h"
+#include "host_parallel.h"
 #include "bvh_build.h"
 #include "device_scene.h"
 #include "gpu_context.h"
@@ -158,11 +159,11 @@ az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
         }
         info.meshPrimBase[m] = primBase;
         std::vector<BuildBox> boxes(tl.size());
-        for (size_t i = 0; i < tl.size(); ++i) {
+        parallel_for(tl.size(), [&](size_t i) {
             vec3 p[3] = {tl[i]->P0, tl[i]->P1, tl[i]->P2};
             boxes[i] = boxOfPoints(p, 3);
             PadBox(boxes[i]);
-        }
+        });
         std::vector<uint32_t> order;
         BuildStats bs;
         uint32_t triBase = (uint32_t)tris.size();
@@ -173,7 +174,9 @@ az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
         info.meshTriBase[m] = triBase;
         info.maxBlasStack = std::max(info.maxBlasStack, bs.maxStack);
         info.leafCount += bs.leaves;
-        for (uint32_t i = 0; i < order.size(); ++i) {
+        tris.resize((size_t)triBase + order.size());
+        faceN.resize(3 * ((size_t)triBase + order.size()));
+        parallel_for(order.size(), [&](size_t i) {
             const entity *e = tl[order[i]];
             TriPacked t;
             t.p0x = e->P0.x;
@@ -188,11 +191,11 @@ az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
             t.h = e->P0.z - e->P2.z;
             t.pad0 = 0.0f;
             t.pad1 = 0.0f;
-            tris.push_back(t);
-            faceN.push_back(e->FaceNormal.x);
-            faceN.push_back(e->FaceNormal.y);
-            faceN.push_back(e->FaceNormal.z);
-        }
+            tris[triBase + i] = t;
+            faceN[3 * (triBase + i) + 0] = e->FaceNormal.x;
+            faceN[3 * (triBase + i) + 1] = e->FaceNormal.y;
+            faceN[3 * (triBase + i) + 2] = e->FaceNormal.z;
+        });
         info.leafOrder.insert(info.leafOrder.end(), order.begin(), order.end());
         primBase += M.TriangleCount;
     }

@@ -27,6 +27,7 @@
 #include "gpu_context.h"
 #include "azimuth_guide.h"
 #include "guide.h"
+#include "host_parallel.h"
 #include "shade_math.h"
 
 namespace az {
@@ -1797,7 +1798,7 @@ az_status az_upload_shading(az_gpu_ctx *C, const scene *S, const UploadInfo &inf
     std::vector<DevPrimAttr> prims(std::max<u32>(1, info.triTotal));
     for (u32 m = 0; m < S->MeshCount; ++m) {
         const std::vector<const entity *> &tl = info.meshTriEntities[m];
-        for (size_t k = 0; k < tl.size(); ++k) {
+        parallel_for(tl.size(), [&](size_t k) {
             const entity *e = tl[k];
             DevPrimAttr &p = prims[info.meshPrimBase[m] + k];
             auto cp3 = [](float *o, const vec3 &v) {
@@ -1819,7 +1820,7 @@ az_status az_upload_shading(az_gpu_ctx *C, const scene *S, const UploadInfo &inf
             cp3(p.p1, e->P1);
             cp3(p.p2, e->P2);
             p.smooth = e->ShadingMode == ShadingMode_Smooth ? 1 : 0;
-        }
+        });
     }
     // Per top-level item.
     std::vector<DevItemShade> items(info.topEntities.size());
