@@ -145,21 +145,54 @@ __global__ __launch_bounds__(BLOCK) void k_bvh_level(const float *__restrict__ b
             binCnt[ax][bb] = 0u;
         }
         __syncthreads();
-        for (uint32_t i = rg.begin + t; i < rg.end; i += BLOCK) {
+        // Each thread bins a contiguous slice of the range and keeps, per
+        // axis, the box and count of its current bin, adding them to the
+        // shared bins only when the bin changes: neighbouring items tend to
+        // share bins, and a wave's atomics on one bin would serialize.
+        const uint32_t per = (cnt + BLOCK - 1) / BLOCK;
+        const uint32_t s0 = rg.begin + min(cnt, t * per), s1 = rg.begin + min(cnt, (t + 1) * per);
+        int curBin[3] = {-1, -1, -1};
+        uint32_t runLo[3][3], runHi[3][3], runCnt[3] = {0u, 0u, 0u};
+        auto flush = [&](int ax) {
+            const int b = curBin[ax];
+            if (b < 0) return;
+            for (int k = 0; k < 3; ++k) {
+                atomicMin(&binLo[ax][b][k], runLo[ax][k]);
+                atomicMax(&binHi[ax][b][k], runHi[ax][k]);
+            }
+            atomicAdd(&binCnt[ax][b], runCnt[ax]);
+        };
+        for (uint32_t i = s0; i < s1; ++i) {
             const uint32_t it = order[i];
+            uint32_t lo[3], hi[3];
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = fkey(boxes[6 * (size_t)it + k]);
+                hi[k] = fkey(boxes[6 * (size_t)it + 3 + k]);
+            }
             for (int ax = 0; ax < 3; ++ax) {
                 const float ext = cb[3 + ax] - cb[ax];
                 if (!(ext > 0.0f)) continue;
                 const float scale = (float)kBuildBins / ext;
                 int b = (int)((cen[3 * (size_t)it + ax] - cb[ax]) * scale);
                 b = min(max(b, 0), (int)kBuildBins - 1);
-                for (int k = 0; k < 3; ++k) {
-                    atomicMin(&binLo[ax][b][k], fkey(boxes[6 * (size_t)it + k]));
-                    atomicMax(&binHi[ax][b][k], fkey(boxes[6 * (size_t)it + 3 + k]));
+                if (b != curBin[ax]) {
+                    flush(ax);
+                    curBin[ax] = b;
+                    for (int k = 0; k < 3; ++k) {
+                        runLo[ax][k] = lo[k];
+                        runHi[ax][k] = hi[k];
+                    }
+                    runCnt[ax] = 1u;
+                } else {
+                    for (int k = 0; k < 3; ++k) {
+                        runLo[ax][k] = min(runLo[ax][k], lo[k]);
+                        runHi[ax][k] = max(runHi[ax][k], hi[k]);
+                    }
+                    ++runCnt[ax];
                 }
-                atomicAdd(&binCnt[ax][b], 1u);
             }
         }
+        for (int ax = 0; ax < 3; ++ax) flush(ax);
         __syncthreads();
         // The host's sweeps, one axis per thread (BuildBinaryBvh).
         if (t < 3) {
