@@ -4,9 +4,10 @@
 // Per level, one workgroup per range of items still to split:
 //   bounds   : the range's box and centroid box (exact: min and max)
 //   binning  : 32 bins per axis in LDS, as orderable integer min/max
-//   sweep    : one thread per axis repeats the host's right-to-left and
-//              left-to-right sweeps with the same float operations; the
-//              first minimum over (axis, bin) wins, as in the host loop
+//   sweep    : one lane per bin, scans over the lanes in place of the host's
+//              right-to-left and left-to-right sweeps, each split priced with
+//              the host's float operations; the first minimum over
+//              (axis, bin) wins, as in the host loop
 //   decision : the host's leaf test
 //   partition: stable (ballot ranks within each workgroup-wide tile), through a
 //              scratch copy of the range
@@ -25,6 +26,8 @@ namespace {
 
 constexpr uint32_t kBuildBlock = 256;
 constexpr uint32_t kBuildBins = 32;
+// Levels with this many ranges or more split each range with one wave.
+constexpr uint32_t kOneWaveLevel = 4096;
 
 struct BuildRange {
     uint32_t begin, end, node;
@@ -88,7 +91,8 @@ struct LevelParams {
 };
 
 // BLOCK threads per range: 1024 while a level has few ranges (the top
-// levels, where one workgroup walks up to all items), 256 below.
+// levels, where one workgroup walks up to all items), 256 below, and one
+// wave once the ranges are many and small.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_bvh_level(const float *__restrict__ boxes, const float *__restrict__ cen,
                                                            uint32_t *__restrict__ order, uint32_t *__restrict__ tmp,
@@ -194,49 +198,73 @@ __global__ __launch_bounds__(BLOCK) void k_bvh_level(const float *__restrict__ b
         }
         for (int ax = 0; ax < 3; ++ax) flush(ax);
         __syncthreads();
-        // The host's sweeps, one axis per thread (BuildBinaryBvh).
-        if (t < 3) {
-            const int ax = (int)t;
+        // The host's two sweeps (BuildBinaryBvh), one lane per bin: the
+        // running boxes and counts are scans over the lanes (min and max are
+        // exact in any order; lanes 32-63 hold empty bins), each lane prices
+        // the split above its bin with the host's float operations, and the
+        // cheapest split wins, the lowest bin on ties, as in the host's
+        // first-minimum loop.  A cost that is not below FLT_MAX never wins.
+        for (uint32_t ax = t >> 6; ax < 3u; ax += W) {
+            const uint32_t lane = t & 63u;
+            const float ext = cb[3 + ax] - cb[ax];
             float best = FLT_MAX;
             uint32_t bestSplit = 0;
-            const float ext = cb[3 + ax] - cb[ax];
             if (ext > 0.0f) {
-                float rlo[kBuildBins][3], rhi[kBuildBins][3];
-                uint32_t rc[kBuildBins];
-                float alo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, ahi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-                uint32_t c = 0;
-                for (int b = (int)kBuildBins - 1; b > 0; --b) {
-                    for (int k = 0; k < 3; ++k) {
-                        alo[k] = fminf(alo[k], fval(binLo[ax][b][k]));
-                        ahi[k] = fmaxf(ahi[k], fval(binHi[ax][b][k]));
-                        rlo[b][k] = alo[k];
-                        rhi[b][k] = ahi[k];
-                    }
-                    c += binCnt[ax][b];
-                    rc[b] = c;
-                }
+                float plo[3], phi[3], slo[3], shi[3];
+                uint32_t pc = 0u;
                 for (int k = 0; k < 3; ++k) {
-                    alo[k] = FLT_MAX;
-                    ahi[k] = -FLT_MAX;
+                    plo[k] = lane < kBuildBins ? fval(binLo[ax][lane][k]) : FLT_MAX;
+                    phi[k] = lane < kBuildBins ? fval(binHi[ax][lane][k]) : -FLT_MAX;
+                    slo[k] = plo[k];
+                    shi[k] = phi[k];
                 }
-                c = 0;
-                for (uint32_t b = 0; b + 1 < kBuildBins; ++b) {
+                if (lane < kBuildBins) pc = binCnt[ax][lane];
+                uint32_t sc = pc;
+                for (uint32_t o = 1; o < 64u; o <<= 1) {
+                    const bool up = lane >= o, dn = lane + o < 64u;
                     for (int k = 0; k < 3; ++k) {
-                        alo[k] = fminf(alo[k], fval(binLo[ax][b][k]));
-                        ahi[k] = fmaxf(ahi[k], fval(binHi[ax][b][k]));
+                        const float a0 = __shfl_up(plo[k], o), a1 = __shfl_up(phi[k], o);
+                        const float d0 = __shfl_down(slo[k], o), d1 = __shfl_down(shi[k], o);
+                        if (up) {
+                            plo[k] = fminf(plo[k], a0);
+                            phi[k] = fmaxf(phi[k], a1);
+                        }
+                        if (dn) {
+                            slo[k] = fminf(slo[k], d0);
+                            shi[k] = fmaxf(shi[k], d1);
+                        }
                     }
-                    c += binCnt[ax][b];
-                    const uint32_t r = rc[b + 1];
-                    if (c == 0 || r == 0) continue;
-                    const float cost = box_area(alo, ahi) * (float)c + box_area(rlo[b + 1], rhi[b + 1]) * (float)r;
-                    if (cost < best) {
+                    const uint32_t ac = __shfl_up(pc, o), dc = __shfl_down(sc, o);
+                    if (up) pc += ac;
+                    if (dn) sc += dc;
+                }
+                // Right side of the split above bin `lane`: the suffix from lane + 1.
+                float rlo[3], rhi[3];
+                for (int k = 0; k < 3; ++k) {
+                    rlo[k] = __shfl_down(slo[k], 1);
+                    rhi[k] = __shfl_down(shi[k], 1);
+                }
+                const uint32_t rc = __shfl_down(sc, 1);
+                if (lane + 1u < kBuildBins && pc != 0u && rc != 0u) {
+                    const float cost = box_area(plo, phi) * (float)pc + box_area(rlo, rhi) * (float)rc;
+                    if (cost < FLT_MAX) {
                         best = cost;
-                        bestSplit = b + 1;
+                        bestSplit = lane + 1u;
+                    }
+                }
+                for (uint32_t o = 32; o > 0; o >>= 1) {
+                    const float oc = __shfl_xor(best, o);
+                    const uint32_t os = __shfl_xor(bestSplit, o);
+                    if (oc < best || (oc == best && os < bestSplit)) {
+                        best = oc;
+                        bestSplit = os;
                     }
                 }
             }
-            axCost[ax] = best;
-            axSplit[ax] = bestSplit;
+            if (lane == 0u) {
+                axCost[ax] = best;
+                axSplit[ax] = bestSplit;
+            }
         }
         __syncthreads();
     }
@@ -449,8 +477,12 @@ int BuildBinaryBvhDevice(void *stream, const std::vector<BuildBox> &boxes, const
             hipLaunchKernelGGL(k_bvh_level<1024>, dim3(R), dim3(1024), 0, st, (const float *)dBoxes.p,
                                (const float *)dCen.p, (uint32_t *)dOrder.p, (uint32_t *)dTmp.p, (const BuildRange *)cur,
                                (DevBinNode *)dNodes.p, (uint32_t *)dSplit.p, (uint32_t *)dMids.p, lp);
-        else
+        else if (R < kOneWaveLevel)
             hipLaunchKernelGGL(k_bvh_level<256>, dim3(R), dim3(256), 0, st, (const float *)dBoxes.p,
+                               (const float *)dCen.p, (uint32_t *)dOrder.p, (uint32_t *)dTmp.p, (const BuildRange *)cur,
+                               (DevBinNode *)dNodes.p, (uint32_t *)dSplit.p, (uint32_t *)dMids.p, lp);
+        else
+            hipLaunchKernelGGL(k_bvh_level<64>, dim3(R), dim3(64), 0, st, (const float *)dBoxes.p,
                                (const float *)dCen.p, (uint32_t *)dOrder.p, (uint32_t *)dTmp.p, (const BuildRange *)cur,
                                (DevBinNode *)dNodes.p, (uint32_t *)dSplit.p, (uint32_t *)dMids.p, lp);
         const uint32_t rb = (R + kBuildBlock - 1) / kBuildBlock;
