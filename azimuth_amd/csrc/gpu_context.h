@@ -81,6 +81,7 @@ struct az_gpu_ctx {
     uint32_t maxLeafItems = 4;       // BVH build of the next upload: leaf size, node width, SAH costs
     uint32_t bvhWidth = 4;
     uint32_t buildMode = AZ_BUILD_AUTO;   // where the binary BVH of an upload is built
+    uint32_t levelMode = AZ_LEVELS_AUTO;  // how a render's bounce levels are launched
     float sahNodeCost = 1.0f, sahItemCost = 1.0f;
     epsilon_values eps{0.0f, 0.000275f};
     az::GeometryBuffers geom;

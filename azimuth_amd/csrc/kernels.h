@@ -42,7 +42,7 @@ struct RenderTuning {
 // two rounds of the device's resident waves, else halved down to 64 (one per
 // lane), so that small launches still reach every SIMD.  `waves` is the
 // register target in waves per SIMD (0: the compiler's choice, taken as 5).
-inline uint32_t rays_per_wave(uint32_t n, int cuCount, int waves) {
+__host__ __device__ inline uint32_t rays_per_wave(uint32_t n, int cuCount, int waves) {
     const uint64_t resident = (uint64_t)cuCount * 4u * (uint64_t)(waves > 0 ? waves : 5);
     uint32_t per = 64u * kRaysPerLane;
     while (per > 64u && (uint64_t)n < 2u * resident * per) per >>= 1;

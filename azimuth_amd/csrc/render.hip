@@ -226,10 +226,27 @@ __device__ __forceinline__ void extend_body(const DeviceGeometry &G, const PathR
     trace_pool<0, false, kRenderRing, BLOCK, WW>(G, pool, begin, end, eps, cull != 0, smem + threadIdx.x, c);
 }
 
+// Rays per wave for a level whose count is read on the device (nDev), over
+// a grid sized for the largest count at 64 rays per wave: the host's choice
+// for n (at least 64), or more if the grid's waves would not cover n.
+template <int BLOCK, int WAVES>
+__device__ inline uint32_t device_rays_per_wave(uint32_t n, int cuCount) {
+    const uint32_t waves = gridDim.x * (BLOCK / 64);
+    return max(rays_per_wave(n, cuCount, WAVES), (n + waves - 1) / waves);
+}
+
+// nDev, when set, holds the level's ray count (written by the previous
+// level's segment scan) and n the largest count, for which the grid is
+// sized; the waves past the count return at once.
 template <int BLOCK, int WAVES, bool WW>
 __global__ __launch_bounds__(BLOCK, WAVES) void k_extend(DeviceGeometry G, const PathRay *__restrict__ q, SegIn qin,
                                                         uint32_t n, PathHit *__restrict__ hits, float eps, int cull,
-                                                        uint32_t perWave) {
+                                                        uint32_t perWave, const uint32_t *__restrict__ nDev, int cuCount) {
+    if (nDev) {
+        n = min(*nDev, n);
+        perWave = device_rays_per_wave<BLOCK, WAVES>(n, cuCount);
+        if ((uint64_t)blockIdx.x * (BLOCK / 64) * perWave >= n) return;
+    }
     extend_body<BLOCK, WW>(G, q, qin, n, hits, eps, cull, perWave, blockIdx.x);
 }
 
@@ -289,18 +306,29 @@ template <int BLOCK, int WAVES, bool WW>
 __global__ __launch_bounds__(BLOCK, WAVES) void k_connect(DeviceGeometry G, const ShadowRay *__restrict__ sq, SegIn sin,
                                                          uint32_t n, AccWord *__restrict__ acc,
                                                          GuideSample *__restrict__ recs,
-                                                         ScatterLink *__restrict__ links, uint32_t perWave) {
+                                                         ScatterLink *__restrict__ links, uint32_t perWave,
+                                                         const uint32_t *__restrict__ nDev, int cuCount) {
+    if (nDev) {
+        n = min(*nDev, n);
+        perWave = device_rays_per_wave<BLOCK, WAVES>(n, cuCount);
+        if ((uint64_t)blockIdx.x * (BLOCK / 64) * perWave >= n) return;
+    }
     connect_body<BLOCK, WW>(G, sq, sin, n, acc, recs, links, perWave, blockIdx.x);
 }
 
+// n is the level's ray count, or with nDev set the largest count the level
+// can hold (the count itself is read from nDev on the device).
 static hipError_t launch_extend(const RenderTuning &T, int cuCount, const DeviceGeometry &G, const PathRay *q,
-                                SegIn qin, uint32_t n, PathHit *hits, float eps, int cull, hipStream_t st) {
-    const uint32_t per = rays_per_wave(n, cuCount, T.traceWaves);
+                                SegIn qin, uint32_t n, PathHit *hits, float eps, int cull, hipStream_t st,
+                                const uint32_t *nDev = nullptr) {
+    // With the count on the device, the grid has a wave per 64 rays of the
+    // largest count, so that every count gets the split it gets on the host.
+    const uint32_t per = nDev ? 64u : rays_per_wave(n, cuCount, T.traceWaves);
 #define AZ_CASE(B, W, F)                                                                                 \
     if (T.traceBlock == B && T.traceWaves == W && T.traceLoop == F) {                                  \
         const uint32_t perBlock = (B / 64) * per;                                                       \
         hipLaunchKernelGGL((k_extend<B, W, F != 0>), dim3((n + perBlock - 1) / perBlock), dim3(B), 0, st, G, q, qin, \
-                           n, hits, eps, cull, per);                                                    \
+                           n, hits, eps, cull, per, nDev, cuCount);                                     \
         return hipGetLastError();                                                                       \
     }
     AZ_RENDER_TRACE_VARIANTS(AZ_CASE)
@@ -310,13 +338,13 @@ static hipError_t launch_extend(const RenderTuning &T, int cuCount, const Device
 
 static hipError_t launch_connect(const RenderTuning &T, int cuCount, const DeviceGeometry &G, const ShadowRay *sq,
                                  SegIn sin, uint32_t n, AccWord *acc, GuideSample *recs, ScatterLink *links,
-                                 hipStream_t st) {
-    const uint32_t per = rays_per_wave(n, cuCount, T.traceWaves);
+                                 hipStream_t st, const uint32_t *nDev = nullptr) {
+    const uint32_t per = nDev ? 64u : rays_per_wave(n, cuCount, T.traceWaves);
 #define AZ_CASE(B, W, F)                                                                                  \
     if (T.traceBlock == B && T.traceWaves == W && T.traceLoop == F) {                                   \
         const uint32_t perBlock = (B / 64) * per;                                                        \
         hipLaunchKernelGGL((k_connect<B, W, F != 0>), dim3((n + perBlock - 1) / perBlock), dim3(B), 0, st, G, sq,   \
-                           sin, n, acc, recs, links, per);                                               \
+                           sin, n, acc, recs, links, per, nDev, cuCount);                                \
         return hipGetLastError();                                                                        \
     }
     AZ_RENDER_TRACE_VARIANTS(AZ_CASE)
@@ -1041,12 +1069,30 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
     }
 }
 
+// nDev and cpd, when set, hold the level's ray count and the chunk's
+// parameters in device memory (levels replayed from a graph); the grid is
+// then sized for the largest count.
 template <bool GUIDED>
 __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeometry G, CameraSetup cam, ChunkParams cp,
                                                         const PathRay *__restrict__ q, SegIn qin,
                                                         const PathHit *__restrict__ hits, uint32_t n,
-                                                        AccWord *__restrict__ acc, ShadeOut O, GuideIO GI) {
-    shade_body<GUIDED>(S, G, cam, cp, q, qin, hits, n, acc, O, GI, blockIdx.x);
+                                                        AccWord *__restrict__ acc, ShadeOut O, GuideIO GI,
+                                                        const uint32_t *__restrict__ nDev,
+                                                        const ChunkParams *__restrict__ cpd) {
+    if (nDev) {
+        n = min(*nDev, n);
+        if (blockIdx.x * kRenderBlock >= n) return;
+    }
+    shade_body<GUIDED>(S, G, cam, cpd ? *cpd : cp, q, qin, hits, n, acc, O, GI, blockIdx.x);
+}
+
+// The chunk's parameters and its camera-ray count (the count of level 0),
+// for levels replayed from a graph.
+__global__ void k_set_chunk(ChunkParams cp, ChunkParams *__restrict__ cpd, uint32_t *__restrict__ total, uint32_t n) {
+    if (threadIdx.x == 0) {
+        *cpd = cp;
+        total[0] = n;
+    }
 }
 
 // ---- learned guide: queries and training batches ------------------------------------------
@@ -1607,6 +1653,33 @@ struct RenderBuffers {
     DeviceBuffer recCum;                        // prefix of the training records' weights
     DeviceBuffer envCdf;                        // environment luminance distribution
     DeviceBuffer batchArgs, batchTotals;        // batched renders: per-scene arguments, queue totals
+    // Levels replayed from a graph: the chunk's parameters and per-level
+    // counts in device memory, and two plans (a render's full chunks and its
+    // last, smaller one): the level bounds learned on a host-driven chunk,
+    // what they were learned on, and the instantiated graph with the launch
+    // arguments it was recorded with.
+    struct LevelPlan {
+        std::vector<uint8_t> boundKey, graphKey;
+        std::vector<uint32_t> rayBound, shadowBound;   // per level: rays and shadow rays a level holds
+        hipGraphExec_t exec = nullptr;
+        uint64_t used = 0;
+        void dropGraph() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            exec = nullptr;
+            graphKey.clear();
+        }
+    };
+    DeviceBuffer chunkDev, levelTotals;
+    LevelPlan plans[2];
+    uint64_t planClock = 0;
+    uint64_t levelCaptures = 0, levelLaunches = 0;
+    void dropLevelGraph() {
+        for (LevelPlan &p : plans) {
+            p.dropGraph();
+            p = LevelPlan{};
+        }
+    }
+    ~RenderBuffers() { dropLevelGraph(); }
 };
 
 // Row and column CDFs of the environment map's luminance over its (u, v)
@@ -1917,6 +1990,7 @@ static CameraSetup setupCamera(const camera *Cam, u32 sppOverride) {
     };
     auto from = [&](const vec3 &v) { return mk(v.x, v.y, v.z); };
     CameraSetup c;
+    memset(&c, 0, sizeof(c));   // padding too: the struct is part of a graph's key
     memset(&c, 0, sizeof(c));
     H3 gaze = from(Cam->Gaze);
     if (Cam->GazeIsPoint) gaze = sub(from(Cam->Gaze), from(Cam->Position));
@@ -2117,11 +2191,135 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
     // split into fewer samples per pixel.
     const size_t budget = (size_t)48 << 30;
     AZ_HIPR(C, reserve_chunk(B, nPix * chunkSpp0, S.maxChildren, shadowPerRay, budget));
+    // Levels replayed from a graph: each level of a chunk is sized for a
+    // bound learned from a host-driven chunk of the same scene, camera, size
+    // and options (its rays and shadow rays plus an eighth and 1,024), the
+    // kernels read each level's count on the device from the previous
+    // level's segment scan, and the levels (extend, shade, segment scan,
+    // connect per bounce) are one graph launch with one read-back of the
+    // per-level counts at its end.  A level beyond its bound makes the chunk
+    // run again with the host-driven levels, which learn the bounds afresh,
+    // so the image never depends on the bounds.  The graph is recorded once
+    // per bounce count, bounds and set of buffers, and replayed for every
+    // chunk and later render with the same ones.
+    const bool levelGraph = C->levelMode == AZ_LEVELS_AUTO && !guided;
+    const u32 levels = S.maxDepth + 1u;
+    auto boundKeyOf = [&](u32 n) {
+        std::vector<uint8_t> k;
+        auto put = [&](const void *p, size_t b) {
+            const uint8_t *c = (const uint8_t *)p;
+            k.insert(k.end(), c, c + b);
+        };
+        put(&S, sizeof(S));
+        put(&C->dg, sizeof(C->dg));
+        put(&cam, sizeof(cam));
+        const uint32_t v[] = {n, row0, rows, Cull ? 1u : 0u, (uint32_t)Filter, envLum ? 1u : 0u, o.ScatterSampling};
+        put(v, sizeof(v));
+        return k;
+    };
+    RenderBuffers::LevelPlan *plan = nullptr;   // the current chunk's, when its levels replay a graph
+    auto levelCapsOf = [&](u32 d) { return level_caps(plan->rayBound[d], S.maxChildren, shadowPerRay); };
+    // Level d reads its count from levelTotals[2 d] and its scan writes the
+    // next level's count and its own shadow-ray count to [2 d + 2], [2 d + 3].
+    auto enqueueLevels = [&](u32 n) -> hipError_t {
+        const ChunkParams *cpd = (const ChunkParams *)B.chunkDev.ptr;
+        uint32_t *LT = (uint32_t *)B.levelTotals.ptr;
+        AccWord *acc = (AccWord *)B.acc.ptr;
+        ChunkParams unused;
+        memset(&unused, 0, sizeof(unused));
+        LevelCaps prev{};
+        for (u32 depth = 0; depth < levels; ++depth) {
+            const LevelCaps L = levelCapsOf(depth);
+            const u32 M = plan->rayBound[depth];
+            const int a = (int)(depth & 1u);
+            PathRay *qa = (PathRay *)(a ? B.q1.ptr : B.q0.ptr), *qb = (PathRay *)(a ? B.q0.ptr : B.q1.ptr);
+            const SegIn qin = depth == 0 ? SegIn{nullptr, 1u, n} : SegIn{offQ[a], prev.nseg, prev.childSegCap};
+            const uint32_t *cnt = LT + 2 * depth;
+            hipError_t e = launch_extend(C->renderTuning, C->cuCount, C->dg, qa, qin, M, (PathHit *)B.hits.ptr,
+                                         S.epsIntersection, Cull ? 1 : 0, st, cnt);
+            if (e != hipSuccess) return e;
+            GuideIO GI{nullptr, nullptr, recCnt, recSegCap, alpha};
+            ShadeOut O{qb, (ShadowRay *)B.shadow.ptr, childCnt, shadowCnt, L.nseg, L.childSegCap, L.shadowSegCap};
+            hipLaunchKernelGGL(k_shade<false>, dim3(L.blocks), dim3(kRenderBlock), 0, st, S, C->dg, cam, unused, qa,
+                               qin, (const PathHit *)B.hits.ptr, M, acc, O, GI, cnt, cpd);
+            SegScan sc;
+            memset(&sc, 0, sizeof(sc));
+            sc.counters[0] = childCnt;
+            sc.counters[1] = shadowCnt;
+            sc.offsets[0] = offQ[1 - a];
+            sc.offsets[1] = offShadow;
+            sc.segCap[0] = L.childSegCap;
+            sc.segCap[1] = L.shadowSegCap;
+            sc.nseg[0] = sc.nseg[1] = L.nseg;
+            sc.total = LT + 2 * depth + 2;
+            hipLaunchKernelGGL(k_seg_scan, dim3(2), dim3(kSegs), 0, st, sc);
+            e = launch_connect(C->renderTuning, C->cuCount, C->dg, (const ShadowRay *)B.shadow.ptr,
+                               SegIn{offShadow, L.nseg, L.shadowSegCap}, plan->shadowBound[depth], acc,
+                               (GuideSample *)B.recs.ptr, (ScatterLink *)B.links.ptr, st, LT + 2 * depth + 3);
+            if (e != hipSuccess) return e;
+            prev = L;
+        }
+        return hipGetLastError();
+    };
+    // Everything the recorded launches hold, compared byte for byte.
+    auto levelKey = [&](u32 n) {
+        std::vector<uint8_t> k;
+        auto put = [&](const void *p, size_t b) {
+            const uint8_t *c = (const uint8_t *)p;
+            k.insert(k.end(), c, c + b);
+        };
+        put(&S, sizeof(S));
+        put(&C->dg, sizeof(C->dg));
+        put(&cam, sizeof(cam));
+        put(&C->renderTuning, sizeof(C->renderTuning));
+        const void *ptrs[] = {B.q0.ptr,   B.q1.ptr,    B.hits.ptr,     B.shadow.ptr,        B.acc.ptr,
+                              B.recs.ptr, B.links.ptr, B.chunkDev.ptr, B.levelTotals.ptr, (void *)ctl};
+        put(ptrs, sizeof(ptrs));
+        const uint32_t v[] = {n, shadowPerRay, Cull ? 1u : 0u, (uint32_t)C->cuCount, recSegCap};
+        put(v, sizeof(v));
+        put(&alpha, sizeof(alpha));
+        put(plan->rayBound.data(), plan->rayBound.size() * sizeof(uint32_t));
+        put(plan->shadowBound.data(), plan->shadowBound.size() * sizeof(uint32_t));
+        return k;
+    };
     while (s0 < sppTotal) {
         const bool training = guided && s0 < trainSpp;
         u32 spp = std::min(chunkSpp, sppTotal - s0);
         if (training) spp = std::min(spp, std::max<u32>(1u, std::min(trainSpp - s0, trainChunkSpp)));
         u32 n = nPix * spp;
+        std::vector<uint8_t> boundKey;
+        bool graphChunk = false;
+        plan = nullptr;
+        if (levelGraph) {
+            boundKey = boundKeyOf(n);
+            for (RenderBuffers::LevelPlan &p : B.plans)
+                if (p.boundKey == boundKey && p.rayBound.size() == levels) plan = &p;
+            if (plan) {
+                plan->used = ++B.planClock;
+                size_t cap = n, hitCap = 0, shadowCap = 1;
+                bool fit = true;
+                for (u32 d = 0; d < levels; ++d) {
+                    const LevelCaps L = levelCapsOf(d);
+                    cap = std::max(cap, L.cap);
+                    hitCap = std::max<size_t>(hitCap, plan->rayBound[d]);
+                    shadowCap = std::max(shadowCap, L.shadowCap);
+                    fit = fit && L.cap <= 0xFFFFFFFFull && L.shadowCap <= 0xFFFFFFFFull;
+                }
+                const size_t need = 2 * cap * sizeof(PathRay) + hitCap * sizeof(PathHit) + shadowCap * sizeof(ShadowRay);
+                if (fit && need <= budget) {
+                    hipError_t ge = B.q0.alloc(cap * sizeof(PathRay));
+                    if (ge == hipSuccess) ge = B.q1.alloc(cap * sizeof(PathRay));
+                    if (ge == hipSuccess) ge = B.hits.alloc(hitCap * sizeof(PathHit));
+                    if (ge == hipSuccess) ge = B.shadow.alloc(shadowCap * sizeof(ShadowRay));
+                    if (ge == hipSuccess) ge = B.chunkDev.alloc(sizeof(ChunkParams));
+                    if (ge == hipSuccess) ge = B.levelTotals.alloc((size_t)2 * (levels + 1) * sizeof(uint32_t));
+                    if (!out_of_memory(ge)) {
+                        AZ_HIPR(C, ge);
+                        graphChunk = true;
+                    }
+                }
+            }
+        }
         hipError_t ae = B.q0.alloc((size_t)n * sizeof(PathRay));
         if (ae == hipSuccess) ae = B.acc.alloc((size_t)n * 3 * sizeof(AccWord));
         if (out_of_memory(ae) && chunkSpp > 1) {
@@ -2153,6 +2351,43 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
         SegIn qin{nullptr, 1u, n};   // camera rays are dense
         u32 count = n;
         bool overflow = false;
+        if (graphChunk) {
+            uint32_t *LT = (uint32_t *)B.levelTotals.ptr;
+            hipLaunchKernelGGL(k_set_chunk, dim3(1), dim3(64), 0, st, cp, (ChunkParams *)B.chunkDev.ptr, LT, n);
+            AZ_HIPR(C, hipGetLastError());
+            std::vector<uint8_t> key = levelKey(n);
+            if (!plan->exec || key != plan->graphKey) {
+                plan->dropGraph();
+                hipGraph_t g = nullptr;
+                AZ_HIPR(C, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                const hipError_t le = enqueueLevels(n);
+                const hipError_t ce = hipStreamEndCapture(st, &g);
+                if (le != hipSuccess || ce != hipSuccess) {
+                    if (g) (void)hipGraphDestroy(g);
+                    AZ_HIPR(C, le != hipSuccess ? le : ce);
+                }
+                const hipError_t ie = hipGraphInstantiate(&plan->exec, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ie != hipSuccess) plan->exec = nullptr;
+                AZ_HIPR(C, ie);
+                plan->graphKey = std::move(key);
+                ++B.levelCaptures;
+            }
+            AZ_HIPR(C, hipGraphLaunch(plan->exec, st));
+            ++B.levelLaunches;
+            std::vector<uint32_t> lt(2 * (levels + 1));
+            AZ_HIPR(C, hipMemcpyAsync(lt.data(), LT, lt.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            AZ_HIPR(C, hipStreamSynchronize(st));
+            bool fits = true;
+            for (u32 d = 0; d < levels; ++d)
+                fits = fits && lt[2 * d] <= plan->rayBound[d] && lt[2 * d + 3] <= plan->shadowBound[d];
+            if (!fits) {
+                plan->boundKey.clear();
+                continue;   // the chunk is redone with the host-driven levels; its accumulators are rebuilt
+            }
+            count = 0;   // the levels are done
+        }
+        std::vector<u32> seenRays(levels, 0u), seenShadow(levels, 0u);
         for (u32 depth = 0; depth <= S.maxDepth && count > 0; ++depth) {
             const LevelCaps L = level_caps(count, S.maxChildren, shadowPerRay);
             const u32 blocks = L.blocks, nseg = L.nseg, childSegCap = L.childSegCap, shadowSegCap = L.shadowSegCap;
@@ -2218,10 +2453,10 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             }
             if (guided)
                 hipLaunchKernelGGL(k_shade<true>, grid, dim3(kRenderBlock), 0, st, S, C->dg, cam, cp, qa, qin,
-                                   (const PathHit *)B.hits.ptr, count, acc, O, GI);
+                                   (const PathHit *)B.hits.ptr, count, acc, O, GI, nullptr, nullptr);
             else
                 hipLaunchKernelGGL(k_shade<false>, grid, dim3(kRenderBlock), 0, st, S, C->dg, cam, cp, qa, qin,
-                                   (const PathHit *)B.hits.ptr, count, acc, O, GI);
+                                   (const PathHit *)B.hits.ptr, count, acc, O, GI, nullptr, nullptr);
             AZ_HIPR(C, hipGetLastError());
             SegScan sc;
             sc.counters[0] = childCnt;
@@ -2237,6 +2472,8 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             uint32_t hc[2];
             AZ_HIPR(C, hipMemcpyAsync(hc, dTotal, 8, hipMemcpyDeviceToHost, st));
             AZ_HIPR(C, hipStreamSynchronize(st));
+            seenRays[depth] = count;
+            seenShadow[depth] = hc[1];
             if (hc[1]) {
                 SegIn sin{offShadow, nseg, shadowSegCap};
                 AZ_HIPR(C, launch_connect(C->renderTuning, C->cuCount, C->dg, (const ShadowRay *)B.shadow.ptr, sin,
@@ -2255,6 +2492,24 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             }
             chunkSpp = std::max<u32>(1u, chunkSpp / 2);
             continue;   // the chunk is redone with fewer samples; its accumulators are rebuilt
+        }
+        if (levelGraph && !graphChunk) {
+            // Bounds for the graph-replayed levels of the next chunks of this
+            // size, in the plan learned on the same key or the least recently
+            // used one.
+            RenderBuffers::LevelPlan *p = &B.plans[0];
+            for (RenderBuffers::LevelPlan &q : B.plans)
+                if (q.boundKey == boundKey) p = &q;
+            if (p->boundKey != boundKey && B.plans[1].used < B.plans[0].used) p = &B.plans[1];
+            p->dropGraph();
+            p->rayBound.assign(levels, 0u);
+            p->shadowBound.assign(levels, 0u);
+            for (u32 d = 0; d < levels; ++d) {
+                p->rayBound[d] = d == 0 ? n : (u32)std::min<uint64_t>(0xFFFFFFFFull, seenRays[d] + seenRays[d] / 8ull + 1024u);
+                p->shadowBound[d] = (u32)std::min<uint64_t>(0xFFFFFFFFull, seenShadow[d] + seenShadow[d] / 8ull + 1024u);
+            }
+            p->boundKey = boundKey;
+            p->used = ++B.planClock;
         }
         if (!cp.gaussian) {
             dim3 g((nPix + kRenderBlock - 1) / kRenderBlock);
@@ -2718,6 +2973,20 @@ az_status az_gpu_set_render_tuning(az_gpu_ctx *C, u32 chunkSamples, u32 traceBlo
     return AZ_OK;
 }
 
+az_status az_gpu_set_level_mode(az_gpu_ctx *C, u32 mode) {
+    if (!C || mode > AZ_LEVELS_HOST) return AZ_ERR_INVALID_ARGUMENT;
+    C->levelMode = mode;
+    return AZ_OK;
+}
+
+az_status az_gpu_level_graph_counts(az_gpu_ctx *C, u64 *Recorded, u64 *Launched) {
+    if (!C || !Recorded || !Launched) return AZ_ERR_INVALID_ARGUMENT;
+    RenderBuffers &B = buffersOf(C);
+    *Recorded = B.levelCaptures;
+    *Launched = B.levelLaunches;
+    return AZ_OK;
+}
+
 az_status az_gpu_trim(az_gpu_ctx *C) {
     if (!C) return AZ_ERR_INVALID_ARGUMENT;
     AZ_HIPR(C, hipSetDevice(C->device));
@@ -2726,7 +2995,10 @@ az_status az_gpu_trim(az_gpu_ctx *C) {
     DeviceBuffer *bufs[] = {&B.q0,      &B.q1,        &B.hits,  &B.shadow,      &B.acc,   &B.pixels, &B.queries,
                             &B.lobes,   &B.recs,      &B.batch, &B.batchArgs,   &B.batchTotals,     &B.scatLobes,
                             &B.srecs,   &B.links,     &B.qrec0, &B.qrec1,         &B.recCum};
+    B.dropLevelGraph();
     for (DeviceBuffer *b : bufs) b->release();
+    B.chunkDev.release();
+    B.levelTotals.release();
     C->outputScratch.release();
     if (C->guide) az_guide_destroy(C->guide);
     if (C->scatterGuide) az_guide_destroy(C->scatterGuide);

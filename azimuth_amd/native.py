@@ -170,6 +170,7 @@ AZ_ENV_COSINE, AZ_ENV_LUMINANCE = 0, 1
 AZ_SCATTER_REFERENCE, AZ_SCATTER_BRDF = 0, 1
 AZ_GUIDE_ENV, AZ_GUIDE_SCATTER, AZ_GUIDE_BOTH = 0, 1, 2
 AZ_BUILD_AUTO, AZ_BUILD_HOST, AZ_BUILD_DEVICE = 0, 1, 2
+AZ_LEVELS_AUTO, AZ_LEVELS_HOST = 0, 1
 
 
 class az_render_opts(C.Structure):
@@ -249,6 +250,8 @@ EXPORTS = {
     "az_gpu_trim": (s32, [_P]),
     "az_gpu_set_bvh_width": (s32, [_P, u32]),
     "az_gpu_set_build_mode": (s32, [_P, u32]),
+    "az_gpu_set_level_mode": (s32, [_P, u32]),
+    "az_gpu_level_graph_counts": (s32, [_P, C.POINTER(u64), C.POINTER(u64)]),
     "az_gpu_autotune_build": (s32, [_P, _P, _P, _P, u32, u32, _P]),
     "az_gpu_malloc": (s32, [_P, C.POINTER(_P), u64]),
     "az_gpu_free": (s32, [_P, _P]),

@@ -2,6 +2,7 @@
 
     python -m azimuth_amd.render_bench --workload cornell|interior|motion
         [--spp N] [--scale S] [--guide] [--autotune | --tuning B,W,L,C] [--cpu-seconds T]
+        [--levels graph|host]
     python -m azimuth_amd.render_bench --workload motion --scenes 16 [--autotune]
 
 Prints one JSON line: Msamples/s (width * height * spp_eff / render time,
@@ -51,6 +52,8 @@ def main(argv=None):
     ap.add_argument("--autotune", action="store_true", help="autotune chunk size and traversal variant first")
     ap.add_argument("--tuning", default="", help="fixed tuning instead: block,waves,loop,chunk (e.g. 128,6,1,4194304)")
     ap.add_argument("--cpu-seconds", type=float, default=0.0, help="time the CPU oracle on a strip of rows")
+    ap.add_argument("--levels", default="graph", choices=("graph", "host"),
+                    help="bounce levels replayed from a recorded graph per chunk, or launched from the host")
     ap.add_argument("--scenes", type=int, default=1,
                     help="motion workload only: render this many scenes (seeds 0..K-1) in one batched render")
     a = ap.parse_args(argv)
@@ -60,6 +63,7 @@ def main(argv=None):
     W, H = cam.Resolution.x, cam.Resolution.y
     spp_eff = int(math.isqrt(cam.SampleCount)) ** 2
     ctx = GpuContext(0)
+    ctx.set_level_mode(N.AZ_LEVELS_AUTO if a.levels == "graph" else N.AZ_LEVELS_HOST)
     t0 = time.time()
     ctx.upload_scene(sc)
     upload_s = time.time() - t0
@@ -85,7 +89,8 @@ def main(argv=None):
     out = {"workload": a.workload, "width": W, "height": H, "spp": spp_eff, "triangles": sc.triangle_count,
            "render_ms": round(best, 3), "msamples_per_s": round(W * H * spp_eff / (best * 1e-3) / 1e6, 3),
            "upload_s": round(upload_s, 2), "mean_radiance": [round(float(x), 6) for x in img.mean(0)],
-           "filter": "gaussian" if filt == N.FilterType_Gaussian else "box", "guide": bool(a.guide), "autotune": tuned}
+           "filter": "gaussian" if filt == N.FilterType_Gaussian else "box", "guide": bool(a.guide), "autotune": tuned,
+           "levels": a.levels, "level_graphs": dict(zip(("recorded", "launched"), ctx.level_graph_counts()))}
     if a.cpu_seconds > 0:
         import oracle
         threads = max(1, min(16, os.cpu_count() or 1))

@@ -431,6 +431,18 @@ class GpuContext:
         """Where the next uploads build their binary BVHs (N.AZ_BUILD_*)."""
         N.check(self.lib.az_gpu_set_build_mode(self.h, int(mode)), "az_gpu_set_build_mode", self.h)
 
+    def set_level_mode(self, mode):
+        """How renders launch their bounce levels (N.AZ_LEVELS_*): replayed
+        from a recorded graph per chunk, or one by one from the host."""
+        N.check(self.lib.az_gpu_set_level_mode(self.h, int(mode)), "az_gpu_set_level_mode", self.h)
+
+    def level_graph_counts(self):
+        """(graphs recorded, graph launches) of this context's renders."""
+        rec, lau = C.c_uint64(0), C.c_uint64(0)
+        N.check(self.lib.az_gpu_level_graph_counts(self.h, C.byref(rec), C.byref(lau)),
+                "az_gpu_level_graph_counts", self.h)
+        return int(rec.value), int(lau.value)
+
     def trim(self):
         """Frees the context's render queues (the scene stays uploaded)."""
         N.check(self.lib.az_gpu_trim(self.h), "az_gpu_trim", self.h)

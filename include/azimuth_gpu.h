@@ -229,6 +229,23 @@ az_status az_gpu_set_build_mode(az_gpu_ctx *Ctx, u32 Mode);
 #define AZ_BUILD_AUTO   0u
 #define AZ_BUILD_HOST   1u
 #define AZ_BUILD_DEVICE 2u
+/* How a render's bounce levels are launched (no counterpart in the
+ * reference, whose ProduceImageInLinearSpace follows each path to its end
+ * per sample, main.cpp:1256-1397):
+ * AZ_LEVELS_AUTO (default) replays the levels of each chunk (extend, shade,
+ * segment scan and connect per bounce) as one recorded hipGraph, one graph
+ * per bounce count and chunk size, reused while the scene, camera, tuning
+ * and buffers stay the same; every level is sized for the chunk's camera
+ * rays and the kernels read each level's count on the device.  That needs
+ * at most one child per shading point and no learned guide; other renders,
+ * and AZ_LEVELS_HOST, launch each level from the host after reading back
+ * the previous level's count.  Both give the same image, bit for bit.
+ * az_gpu_level_graph_counts reports how many graphs were recorded and
+ * launched on the context. */
+az_status az_gpu_set_level_mode(az_gpu_ctx *Ctx, u32 Mode);
+az_status az_gpu_level_graph_counts(az_gpu_ctx *Ctx, u64 *Recorded, u64 *Launched);
+#define AZ_LEVELS_AUTO 0u
+#define AZ_LEVELS_HOST 1u
 
 /* Autotuning.  Each candidate launch configuration is timed with HIP events
  * on the context stream; the fastest becomes the context's setting (and
