@@ -40,7 +40,7 @@ void setError(az_gpu_ctx *c, const char *what, hipError_t e) {
 
 // Triangle entities of a mesh in the order of its BVH leaves (left first),
 // which is the order of the mesh's triangle entity array.
-void collectLeaves(const entity *e, std::vector<const entity *> &out) {
+void walkLeaves(const entity *e, std::vector<const entity *> &out) {
     std::vector<const entity *> st;
     st.push_back(e);
     while (!st.empty()) {
@@ -54,6 +54,36 @@ void collectLeaves(const entity *e, std::vector<const entity *> &out) {
             out.push_back(x);
         }
     }
+}
+
+// The leaves of the entity tree at e, left to right.  A tree expected to
+// hold 65,536 leaves or more is split at its top levels (breadth first, in
+// order) into up to 64 subtrees, walked on host threads and concatenated in
+// order: the walk chases pointers through 288-byte entities.
+void collectLeaves(const entity *e, std::vector<const entity *> &out, size_t expected = 0) {
+    if (expected < 65536) {
+        walkLeaves(e, out);
+        return;
+    }
+    std::vector<const entity *> front{e};
+    for (bool grew = true; grew && front.size() < 64;) {
+        std::vector<const entity *> next;
+        grew = false;
+        for (const entity *x : front) {
+            if (x && x->Type == EntityType_BVHNode) {
+                next.push_back(x->Left);
+                next.push_back(x->Right);
+                grew = true;
+            } else {
+                next.push_back(x);
+            }
+        }
+        front.swap(next);
+    }
+    std::vector<std::vector<const entity *>> parts(front.size());
+    parallel_for(front.size(), [&](size_t i) { walkLeaves(front[i], parts[i]); }, 4);
+    out.reserve(out.size() + expected);
+    for (const std::vector<const entity *> &p : parts) out.insert(out.end(), p.begin(), p.end());
 }
 
 Affine toAffine(const mat4 &m) {
@@ -152,7 +182,7 @@ az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
     for (u32 m = 0; m < S->MeshCount; ++m) {
         const mesh &M = S->Meshes[m];
         std::vector<const entity *> &tl = info.meshTriEntities[m];
-        collectLeaves(M.BVH, tl);
+        collectLeaves(M.BVH, tl, M.TriangleCount);
         if (tl.size() != M.TriangleCount) {
             C->lastError = "mesh BVH leaf count differs from TriangleCount";
             return AZ_ERR_INVALID_ARGUMENT;
