@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cfloat>
 
 #include "bvh_build.h"
@@ -33,10 +34,14 @@ struct BuildRange {
     uint32_t begin, end, node;
 };
 
+// The host's BinNode, field for field, so the tree is copied into it as is.
 struct DevBinNode {
     float lo[3], hi[3];
-    uint32_t left, right, first, count, leaf, pad;
+    uint32_t left, right, first, count, leaf;
 };
+static_assert(sizeof(DevBinNode) == sizeof(BinNode) && offsetof(DevBinNode, left) == offsetof(BinNode, left) &&
+                  offsetof(DevBinNode, leaf) == offsetof(BinNode, leaf),
+              "device and host binary nodes share one layout");
 
 __device__ inline uint32_t fkey(float f) {
     const uint32_t u = __float_as_uint(f);
@@ -506,24 +511,12 @@ int BuildBinaryBvhDevice(void *stream, const std::vector<BuildBox> &boxes, const
         R = 2u * S;
         std::swap(cur, nxt);
     }
-    std::vector<DevBinNode> hn(nodeCount);
+    b2.resize(nodeCount);
     order.resize(n);
-    AZ_BUILD_TRY(hipMemcpyAsync(hn.data(), dNodes.p, (size_t)nodeCount * sizeof(DevBinNode), hipMemcpyDeviceToHost, st));
+    AZ_BUILD_TRY(hipMemcpyAsync(b2.data(), dNodes.p, (size_t)nodeCount * sizeof(BinNode), hipMemcpyDeviceToHost, st));
     AZ_BUILD_TRY(hipMemcpyAsync(order.data(), dOrder.p, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     AZ_BUILD_TRY(hipStreamSynchronize(st));
 #undef AZ_BUILD_TRY
-    b2.resize(nodeCount);
-    for (uint32_t i = 0; i < nodeCount; ++i) {
-        for (int k = 0; k < 3; ++k) {
-            b2[i].lo[k] = hn[i].lo[k];
-            b2[i].hi[k] = hn[i].hi[k];
-        }
-        b2[i].left = hn[i].left;
-        b2[i].right = hn[i].right;
-        b2[i].first = hn[i].first;
-        b2[i].count = hn[i].count;
-        b2[i].leaf = hn[i].leaf;
-    }
     return hipSuccess;
 }
 
