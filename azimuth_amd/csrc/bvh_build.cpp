@@ -7,6 +7,7 @@
 // the same closest hit when walked front to back with distance pruning; the
 // padding in PadBox keeps that containment under float rounding.
 #include "bvh_build.h"
+#include "host_parallel.h"
 
 #include <algorithm>
 #include <cfloat>
@@ -192,121 +193,220 @@ void BuildBinaryBvh(const std::vector<BuildBox> &boxes, const BuildParams &P, st
     }
 }
 
+namespace {
+
+Node4 emptyNode() {
+    Node4 z;
+    for (int k = 0; k < 4; ++k) {
+        z.lox[k] = z.loy[k] = z.loz[k] = INFINITY;
+        z.hix[k] = z.hiy[k] = z.hiz[k] = -INFINITY;
+        z.child[k] = kEmpty;
+        z.pad[k] = 0;
+    }
+    return z;
+}
+
+// The children of the wide node made from binary node b: a leaf's node
+// holds the leaf itself; otherwise its two children, then repeatedly the
+// two children of the internal child with the largest area in its place,
+// up to `width`.
+uint32_t gatherKids(const std::vector<BinNode> &b2, uint32_t b, uint32_t width, uint32_t kids[4]) {
+    const BinNode &src = b2[b];
+    if (src.leaf) {
+        kids[0] = b;
+        return 1;
+    }
+    uint32_t k = 0;
+    kids[k++] = src.left;
+    kids[k++] = src.right;
+    while (k < width) {
+        int best = -1;
+        float bestArea = -1.0f;
+        for (uint32_t c = 0; c < k; ++c) {
+            const BinNode &cn = b2[kids[c]];
+            if (cn.leaf) continue;
+            const float a = binArea(cn);
+            if (a > bestArea) {
+                bestArea = a;
+                best = (int)c;
+            }
+        }
+        if (best < 0) break;
+        const uint32_t open = kids[best];
+        kids[best] = b2[open].left;
+        kids[k++] = b2[open].right;
+    }
+    return k;
+}
+
+// Boxes and leaf references of a wide node's children (internal children
+// are linked by the caller).
+void fillNode(const std::vector<BinNode> &b2, const uint32_t *kids, uint32_t k, uint32_t itemBase, Node4 &dst,
+              BuildStats &st) {
+    for (uint32_t c = 0; c < k; ++c) {
+        const BinNode &cn = b2[kids[c]];
+        dst.lox[c] = cn.lo[0];
+        dst.loy[c] = cn.lo[1];
+        dst.loz[c] = cn.lo[2];
+        dst.hix[c] = cn.hi[0];
+        dst.hiy[c] = cn.hi[1];
+        dst.hiz[c] = cn.hi[2];
+        if (cn.leaf) {
+            dst.child[c] = make_leaf(itemBase + cn.first, cn.count);
+            st.leaves++;
+        }
+    }
+}
+
+// The wide nodes of the subtree at binary node `b` (depth `depth`), in the
+// order of a depth-first walk that takes the last child first: node 0 is
+// the subtree's root and child references are local indices.
+struct SubCollapse {
+    std::vector<Node4> nodes;
+    std::vector<uint32_t> kidCount;
+    BuildStats st;
+    void run(const std::vector<BinNode> &b2, uint32_t b, uint32_t depth, uint32_t width, uint32_t itemBase) {
+        struct Pend {
+            uint32_t b2node, parent, slot, depth;
+        };
+        std::vector<Pend> pend{{b, kEmpty, 0, depth}};
+        while (!pend.empty()) {
+            const Pend p = pend.back();
+            pend.pop_back();
+            const uint32_t me = (uint32_t)nodes.size();
+            nodes.push_back(emptyNode());
+            if (p.parent != kEmpty) nodes[p.parent].child[p.slot] = me;
+            st.maxDepth = std::max(st.maxDepth, p.depth);
+            uint32_t kids[4];
+            const uint32_t k = gatherKids(b2, p.b2node, width, kids);
+            kidCount.push_back(k);
+            fillNode(b2, kids, k, itemBase, nodes[me], st);
+            for (uint32_t c = 0; c < k; ++c)
+                if (!b2[kids[c]].leaf) pend.push_back({kids[c], me, c, p.depth + 1});
+        }
+    }
+};
+
+}  // namespace
+
 uint32_t CollapseWideBvh(const std::vector<BinNode> &b2, uint32_t root, uint32_t n, const BuildParams &P,
                          uint32_t itemBase, std::vector<Node4> &nodes, BuildStats *stats) {
     const uint32_t width = std::max(2u, std::min(P.width, 4u));
-    // ---- collapse to 4-wide -------------------------------------------------
     BuildStats st;
     const uint32_t rootIndex = (uint32_t)nodes.size();
-    struct Frame { uint32_t b2node, slotNode, slot, depth; };
-    // Each node is created from one binary node; its (up to `width`)
-    // children are gathered by repeatedly opening the internal child with the
-    // largest area.
-    std::vector<uint32_t> maxStackOf;   // per created node, filled bottom-up
-    std::vector<uint32_t> parentOf, childCountOf;
-    struct Pend { uint32_t b2node, parent, slot, depth; };
-    std::vector<Pend> pend;
-    auto emptyNode = []() {
-        Node4 z;
-        for (int k = 0; k < 4; ++k) {
-            z.lox[k] = z.loy[k] = z.loz[k] = INFINITY;
-            z.hix[k] = z.hiy[k] = z.hiz[k] = -INFINITY;
-            z.child[k] = kEmpty;
-            z.pad[k] = 0;
-        }
-        return z;
-    };
-    nodes.push_back(emptyNode());
-    parentOf.push_back(kEmpty);
-    childCountOf.push_back(0);
     if (n == 0) {
+        nodes.push_back(emptyNode());
         if (stats) *stats = st;
         return rootIndex;
     }
-    // The root: when the whole tree is one leaf, the root node holds it as
-    // its single child.
-    pend.push_back({root, kEmpty, 0, 1});
-    std::vector<uint32_t> depthOf;
-    depthOf.push_back(1);
-    bool first = true;
-    while (!pend.empty()) {
-        Pend p = pend.back();
-        pend.pop_back();
-        uint32_t me;
-        if (first) {
-            me = rootIndex;
-            first = false;
-        } else {
-            me = (uint32_t)nodes.size();
-            nodes.push_back(emptyNode());
-            parentOf.push_back(p.parent);
-            childCountOf.push_back(0);
-            depthOf.push_back(p.depth);
-            nodes[p.parent].child[p.slot] = me;
-        }
-        uint32_t local = me - rootIndex;
-        st.maxDepth = std::max(st.maxDepth, p.depth);
+    // Each wide node is made from one binary node and takes its children
+    // from gatherKids; nodes are numbered in the order of a depth-first walk
+    // from the root that takes the last child first, so every subtree is one
+    // contiguous run.  The nodes above depth kSplit are made here; the
+    // subtrees at depth kSplit are collapsed on host threads and their runs
+    // copied into place, which numbers every node as the serial walk does.
+    constexpr uint32_t kSplit = 4;
+    struct Top {
+        uint32_t b2node, depth, k;
         uint32_t kids[4];
-        uint32_t k = 0;
-        const BinNode &src = b2[p.b2node];
-        if (src.leaf) {
-            kids[k++] = p.b2node;
-        } else {
-            kids[k++] = src.left;
-            kids[k++] = src.right;
-            while (k < width) {
-                int best = -1;
-                float bestArea = -1.0f;
-                for (uint32_t c = 0; c < k; ++c) {
-                    const BinNode &cn = b2[kids[c]];
-                    if (cn.leaf) continue;
-                    float a = binArea(cn);
-                    if (a > bestArea) {
-                        bestArea = a;
-                        best = (int)c;
-                    }
-                }
-                if (best < 0) break;
-                uint32_t open = kids[best];
-                kids[best] = b2[open].left;
-                kids[k++] = b2[open].right;
+        int32_t sub[4];   // child c: a top node (index into tops), a subtree (-1 - index into subs), or none
+    };
+    std::vector<Top> tops;
+    std::vector<uint32_t> subRoots, subDepth;
+    std::vector<uint32_t> open{0};
+    tops.push_back(Top{root, 1, 0, {}, {0, 0, 0, 0}});
+    const bool parallel = n >= 65536;
+    while (!open.empty()) {
+        const uint32_t t = open.back();
+        open.pop_back();
+        Top cur = tops[t];
+        cur.k = gatherKids(b2, cur.b2node, width, cur.kids);
+        for (uint32_t c = 0; c < cur.k; ++c) {
+            cur.sub[c] = 0;
+            if (b2[cur.kids[c]].leaf) continue;
+            if (!parallel || cur.depth + 1 < kSplit) {
+                cur.sub[c] = (int32_t)tops.size();
+                tops.push_back(Top{cur.kids[c], cur.depth + 1, 0, {}, {0, 0, 0, 0}});
+                open.push_back((uint32_t)tops.size() - 1);
+            } else {
+                cur.sub[c] = -1 - (int32_t)subRoots.size();
+                subRoots.push_back(cur.kids[c]);
+                subDepth.push_back(cur.depth + 1);
             }
         }
-        childCountOf[local] = k;
-        for (uint32_t c = 0; c < k; ++c) {
-            const BinNode &cn = b2[kids[c]];
-            Node4 &dst = nodes[me];
-            dst.lox[c] = cn.lo[0];
-            dst.loy[c] = cn.lo[1];
-            dst.loz[c] = cn.lo[2];
-            dst.hix[c] = cn.hi[0];
-            dst.hiy[c] = cn.hi[1];
-            dst.hiz[c] = cn.hi[2];
-            if (cn.leaf) {
-                dst.child[c] = make_leaf(itemBase + cn.first, cn.count);
-                st.leaves++;
-            } else {
-                pend.push_back({kids[c], me, c, p.depth + 1});
+        tops[t] = cur;
+    }
+    std::vector<SubCollapse> subs(subRoots.size());
+    parallel_for(subs.size(), [&](size_t i) { subs[i].run(b2, subRoots[i], subDepth[i], width, itemBase); }, 2);
+    // Numbering: the serial walk's order.  A node's internal children are
+    // all top nodes (above depth kSplit - 1) or all subtrees (at kSplit - 1):
+    // top children are pushed in slot order, so the last is entered first;
+    // subtree runs follow their parent, last child first.
+    std::vector<uint32_t> topIndex(tops.size()), subBase(subs.size());
+    uint32_t next = 0;
+    std::vector<uint32_t> walk{0};
+    while (!walk.empty()) {
+        const uint32_t t = walk.back();
+        walk.pop_back();
+        topIndex[t] = next++;
+        for (uint32_t c = 0; c < tops[t].k; ++c) {
+            const int32_t s = tops[t].sub[c];
+            if (s > 0) walk.push_back((uint32_t)s);
+        }
+        for (uint32_t c = tops[t].k; c-- > 0;) {
+            const int32_t s = tops[t].sub[c];
+            if (s < 0) {
+                subBase[-1 - s] = next;
+                next += (uint32_t)subs[-1 - s].nodes.size();
             }
         }
     }
-    st.nodes = (uint32_t)(nodes.size() - rootIndex);
+    const uint32_t total = next;
+    nodes.resize((size_t)rootIndex + total);
+    std::vector<uint32_t> kidCount(total);
+    for (uint32_t t = 0; t < tops.size(); ++t) {
+        Node4 nd = emptyNode();
+        fillNode(b2, tops[t].kids, tops[t].k, itemBase, nd, st);
+        for (uint32_t c = 0; c < tops[t].k; ++c) {
+            const int32_t s = tops[t].sub[c];
+            if (s > 0) nd.child[c] = rootIndex + topIndex[s];
+            else if (s < 0) nd.child[c] = rootIndex + subBase[-1 - s];
+        }
+        nodes[rootIndex + topIndex[t]] = nd;
+        kidCount[topIndex[t]] = tops[t].k;
+        st.maxDepth = std::max(st.maxDepth, tops[t].depth);
+    }
+    parallel_for(subs.size(), [&](size_t i) {
+        const SubCollapse &sc = subs[i];
+        const uint32_t base = subBase[i];
+        for (size_t j = 0; j < sc.nodes.size(); ++j) {
+            Node4 nd = sc.nodes[j];
+            for (int c = 0; c < 4; ++c)
+                if (nd.child[c] != kEmpty && !ref_is_leaf(nd.child[c])) nd.child[c] += rootIndex + base;
+            nodes[(size_t)rootIndex + base + j] = nd;
+            kidCount[base + j] = sc.kidCount[j];
+        }
+    }, 2);
+    for (const SubCollapse &sc : subs) {
+        st.leaves += sc.st.leaves;
+        st.maxDepth = std::max(st.maxDepth, sc.st.maxDepth);
+    }
+    st.nodes = total;
     // Stack bound: at a node with k children the walk keeps one and pushes
     // k-1, then descends into any one child.
-    maxStackOf.assign(st.nodes, 0);
-    for (uint32_t i = st.nodes; i-- > 0;) {
+    std::vector<uint32_t> maxStackOf(total, 0);
+    for (uint32_t i = total; i-- > 0;) {
         uint32_t best = 0;
         const Node4 &nd = nodes[rootIndex + i];
         for (int c = 0; c < 4; ++c) {
-            uint32_t r = nd.child[c];
+            const uint32_t r = nd.child[c];
             if (r == kEmpty || ref_is_leaf(r)) continue;
             best = std::max(best, maxStackOf[r - rootIndex]);
         }
-        uint32_t kc = childCountOf[i];
+        const uint32_t kc = kidCount[i];
         maxStackOf[i] = (kc > 0 ? kc - 1 : 0) + best;
     }
     st.maxStack = maxStackOf[0];
-    (void)parentOf;
     if (stats) *stats = st;
     return rootIndex;
 }
