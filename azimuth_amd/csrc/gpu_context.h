@@ -14,10 +14,19 @@
 
 namespace az {
 
+// A cap on the bytes a group of buffers may hold (0: none).  An allocation
+// that would pass it fails with hipErrorOutOfMemory without calling the
+// runtime, as a full device would make it fail.
+struct AllocLimit {
+    size_t limit = 0;
+    size_t used = 0;
+};
+
 // Owning device allocation.
 struct DeviceBuffer {
     void *ptr = nullptr;
     size_t bytes = 0;
+    AllocLimit *lim = nullptr;   // not owned; null: uncapped
     // Grows to n plus a quarter, so that queues whose level sizes vary from
     // chunk to chunk settle after a few growths (each one a device-wide
     // synchronising free and a fresh allocation).
@@ -25,13 +34,22 @@ struct DeviceBuffer {
         if (n <= bytes && ptr) return hipSuccess;
         release();
         n += n / 4;
-        hipError_t e = hipMalloc(&ptr, n ? n : 1);
-        if (e == hipSuccess) bytes = n;
-        else ptr = nullptr;
+        if (!n) n = 1;
+        if (lim && lim->limit && lim->used + n > lim->limit) return hipErrorOutOfMemory;
+        hipError_t e = hipMalloc(&ptr, n);
+        if (e == hipSuccess) {
+            bytes = n;
+            if (lim) lim->used += n;
+        } else {
+            ptr = nullptr;
+        }
         return e;
     }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr) {
+            (void)hipFree(ptr);
+            if (lim) lim->used -= bytes;
+        }
         ptr = nullptr;
         bytes = 0;
     }

@@ -493,6 +493,27 @@ __device__ inline float env_lum_pdf(const DevScene &S, V3 L) {
     return p * (float)W * (float)H / (2.0f * kPiF * kPiF * fmaxf(st, 1e-7f));
 }
 
+// The luminance sampler and its density on caller-given inputs (for tests
+// and for callers that draw their own environment directions).
+__global__ __launch_bounds__(kRenderBlock) void k_env_lum_sample(DevScene S, const float *__restrict__ u, uint32_t n,
+                                                                 float *__restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float pdf;
+    const V3 d = env_lum_sample(S, u[2 * i], u[2 * i + 1], pdf);
+    out[4 * i] = d.x;
+    out[4 * i + 1] = d.y;
+    out[4 * i + 2] = d.z;
+    out[4 * i + 3] = pdf;
+}
+
+__global__ __launch_bounds__(kRenderBlock) void k_env_lum_pdf(DevScene S, const float *__restrict__ dirs, uint32_t n,
+                                                              float *__restrict__ pdf) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pdf[i] = env_lum_pdf(S, v3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+}
+
 struct ShadeOut {
     PathRay *next;
     ShadowRay *shadow;
@@ -1539,7 +1560,8 @@ namespace {
         hipError_t _e = (call);                                             \
         if (_e != hipSuccess) {                                             \
             (ctx)->lastError = std::string(#call ": ") + hipGetErrorString(_e); \
-            return AZ_ERR_DEVICE;                                           \
+            if (_e == hipErrorOutOfMemory) (void)hipGetLastError();        \
+            return _e == hipErrorOutOfMemory ? AZ_ERR_OUT_OF_MEMORY : AZ_ERR_DEVICE; \
         }                                                                   \
     } while (0)
 
@@ -1646,6 +1668,9 @@ hipError_t uploadVec(DeviceBuffer &b, const std::vector<T> &v, hipStream_t s) {
 }  // namespace
 
 struct RenderBuffers {
+    // Cap on the render queues' device memory (az_gpu_set_render_memory_limit);
+    // declared first so that it outlives the buffers it counts.
+    AllocLimit queueLimit;
     DeviceBuffer materials, textures, texels, perms, lights, lightTris, prims, itemShade, perlin, pcg;
     DeviceBuffer q0, q1, hits, shadow, acc, counters, pixels;
     DeviceBuffer queries, lobes, recs, batch;   // learned guide
@@ -1678,6 +1703,17 @@ struct RenderBuffers {
             p.dropGraph();
             p = LevelPlan{};
         }
+    }
+    // The per-render buffers: what az_gpu_trim frees and the memory cap counts.
+    template <typename F>
+    void forEachQueue(F f) {
+        DeviceBuffer *bufs[] = {&q0,    &q1,        &hits,      &shadow, &acc,   &counters, &pixels,  &queries,
+                                &lobes, &recs,      &batch,     &scatLobes, &srecs, &links, &qrec0,  &qrec1,
+                                &recCum, &batchArgs, &batchTotals, &chunkDev, &levelTotals};
+        for (DeviceBuffer *b : bufs) f(*b);
+    }
+    RenderBuffers() {
+        forEachQueue([this](DeviceBuffer &b) { b.lim = &queueLimit; });
     }
     ~RenderBuffers() { dropLevelGraph(); }
 };
@@ -2322,7 +2358,11 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
         }
         hipError_t ae = B.q0.alloc((size_t)n * sizeof(PathRay));
         if (ae == hipSuccess) ae = B.acc.alloc((size_t)n * 3 * sizeof(AccWord));
-        if (out_of_memory(ae) && chunkSpp > 1) {
+        if (out_of_memory(ae)) {
+            if (chunkSpp == 1) {
+                C->lastError = "camera rays of one sample per pixel exceed the free device memory (or the render memory limit)";
+                return AZ_ERR_OUT_OF_MEMORY;
+            }
             chunkSpp = std::max<u32>(1u, chunkSpp / 2);
             continue;
         }
@@ -2992,17 +3032,50 @@ az_status az_gpu_trim(az_gpu_ctx *C) {
     AZ_HIPR(C, hipSetDevice(C->device));
     AZ_HIPR(C, hipStreamSynchronize(C->stream));
     RenderBuffers &B = buffersOf(C);
-    DeviceBuffer *bufs[] = {&B.q0,      &B.q1,        &B.hits,  &B.shadow,      &B.acc,   &B.pixels, &B.queries,
-                            &B.lobes,   &B.recs,      &B.batch, &B.batchArgs,   &B.batchTotals,     &B.scatLobes,
-                            &B.srecs,   &B.links,     &B.qrec0, &B.qrec1,         &B.recCum};
     B.dropLevelGraph();
-    for (DeviceBuffer *b : bufs) b->release();
-    B.chunkDev.release();
-    B.levelTotals.release();
+    B.forEachQueue([](DeviceBuffer &b) { b.release(); });
     C->outputScratch.release();
     if (C->guide) az_guide_destroy(C->guide);
     if (C->scatterGuide) az_guide_destroy(C->scatterGuide);
     C->guide = C->scatterGuide = nullptr;
+    return AZ_OK;
+}
+
+static az_status env_probe_check(az_gpu_ctx *C, const void *a, const void *b) {
+    if (!C || !a || !b) return AZ_ERR_INVALID_ARGUMENT;
+    if (!C->hasScene || !C->render.ready) return AZ_ERR_NO_SCENE;
+    if (!C->devScene.useEnv || !C->devScene.envCdf) {
+        C->lastError = "luminance sampling needs an image environment map";
+        return AZ_ERR_UNSUPPORTED;
+    }
+    return AZ_OK;
+}
+
+az_status az_gpu_env_luminance_sample(az_gpu_ctx *C, const f32 *d_U, u32 N, f32 *d_Out) {
+    az_status s = env_probe_check(C, d_U, d_Out);
+    if (s != AZ_OK || N == 0) return s;
+    AZ_HIPR(C, hipSetDevice(C->device));
+    hipLaunchKernelGGL(k_env_lum_sample, dim3((N + kRenderBlock - 1) / kRenderBlock), dim3(kRenderBlock), 0, C->stream,
+                       C->devScene, d_U, N, d_Out);
+    AZ_HIPR(C, hipGetLastError());
+    return AZ_OK;
+}
+
+az_status az_gpu_env_luminance_pdf(az_gpu_ctx *C, const f32 *d_Dirs, u32 N, f32 *d_Pdf) {
+    az_status s = env_probe_check(C, d_Dirs, d_Pdf);
+    if (s != AZ_OK || N == 0) return s;
+    AZ_HIPR(C, hipSetDevice(C->device));
+    hipLaunchKernelGGL(k_env_lum_pdf, dim3((N + kRenderBlock - 1) / kRenderBlock), dim3(kRenderBlock), 0, C->stream,
+                       C->devScene, d_Dirs, N, d_Pdf);
+    AZ_HIPR(C, hipGetLastError());
+    return AZ_OK;
+}
+
+az_status az_gpu_set_render_memory_limit(az_gpu_ctx *C, u64 Bytes) {
+    if (!C) return AZ_ERR_INVALID_ARGUMENT;
+    az_status s = az_gpu_trim(C);
+    if (s != AZ_OK) return s;
+    buffersOf(C).queueLimit.limit = (size_t)Bytes;
     return AZ_OK;
 }
 

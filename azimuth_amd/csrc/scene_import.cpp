@@ -175,6 +175,9 @@ struct az_scene_builder {
     az_texture envmap;
     std::vector<u8> envPixels;
     b32 useEnv = 0;
+    az_texture background;
+    std::vector<u8> backgroundPixels;
+    b32 useBackground = 0;
     std::vector<light> lights;
     std::vector<az_mesh_desc> meshes;
     std::vector<std::vector<f32>> meshVerts, meshUVs;
@@ -190,7 +193,7 @@ struct SceneStorage {
     std::vector<light> lights;
     std::vector<az_texture> textures;
     std::vector<std::vector<u8>> texturePixels;
-    std::vector<u8> envPixels;
+    std::vector<u8> envPixels, backgroundPixels;
     std::vector<entity *> triEntities;   // one calloc'd array per mesh
     std::vector<triangle *> triangles;
     entity *topEntities = nullptr;
@@ -284,6 +287,16 @@ az_status az_sb_set_environment_map(az_scene_builder *B, const az_texture *T) {
     return AZ_OK;
 }
 
+az_status az_sb_set_background_texture(az_scene_builder *B, const az_texture *T) {
+    if (!B || !T || T->Type != TextureType_2D || !T->Pixels || !T->Width || !T->Height)
+        return AZ_ERR_INVALID_ARGUMENT;
+    size_t n = TexturePixelBytes(T);
+    B->background = *T;
+    B->backgroundPixels.assign((const u8 *)T->Pixels, (const u8 *)T->Pixels + n);
+    B->useBackground = 1;
+    return AZ_OK;
+}
+
 az_status az_sb_add_light(az_scene_builder *B, const light *L) {
     if (!B || !L || L->Type == LightType_EntityLight) return AZ_ERR_INVALID_ARGUMENT;
     B->lights.push_back(*L);
@@ -341,6 +354,12 @@ az_status az_sb_build(az_scene_builder *B, scene **Out) {
         sc.EnvironmentMap = B->envmap;
         sc.EnvironmentMap.Pixels = st->envPixels.data();
         sc.UseEnvironmentMap = 1;
+    }
+    if (B->useBackground) {
+        st->backgroundPixels = B->backgroundPixels;
+        sc.BackgroundTexture = B->background;
+        sc.BackgroundTexture.Pixels = st->backgroundPixels.data();
+        sc.UseBackgroundTexture = 1;
     }
 
     // Entities in the reference's order: spheres as parsed, then one instance

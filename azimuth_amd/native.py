@@ -220,6 +220,7 @@ EXPORTS = {
     "az_sb_add_material": (s32, [_P, C.POINTER(material), C.POINTER(u32)]),
     "az_sb_add_texture": (s32, [_P, C.POINTER(texture), C.POINTER(u32)]),
     "az_sb_set_environment_map": (s32, [_P, C.POINTER(texture)]),
+    "az_sb_set_background_texture": (s32, [_P, C.POINTER(texture)]),
     "az_sb_add_light": (s32, [_P, C.POINTER(light)]),
     "az_sb_add_mesh": (s32, [_P, C.POINTER(az_mesh_desc)]),
     "az_sb_add_sphere": (s32, [_P, C.POINTER(az_sphere_desc)]),
@@ -228,6 +229,8 @@ EXPORTS = {
     "az_mat4_inverse": (s32, [C.POINTER(mat4), C.POINTER(mat4)]),
     "az_scene_triangle_count": (u32, [_P]),
     "az_abi_sizes": (u32, [C.POINTER(u32), u32]),
+    # azimuth_platform_posix.h
+    "az_platform_posix_api": (None, [_P]),
     # azimuth_gpu.h
     "az_gpu_create": (s32, [C.POINTER(_P), C.c_int]),
     "az_gpu_destroy": (None, [_P]),
@@ -253,6 +256,9 @@ EXPORTS = {
     "az_gpu_set_level_mode": (s32, [_P, u32]),
     "az_gpu_level_graph_counts": (s32, [_P, C.POINTER(u64), C.POINTER(u64)]),
     "az_gpu_autotune_build": (s32, [_P, _P, _P, _P, u32, u32, _P]),
+    "az_gpu_set_render_memory_limit": (s32, [_P, u64]),
+    "az_gpu_env_luminance_sample": (s32, [_P, _P, u32, _P]),
+    "az_gpu_env_luminance_pdf": (s32, [_P, _P, u32, _P]),
     "az_gpu_malloc": (s32, [_P, C.POINTER(_P), u64]),
     "az_gpu_free": (s32, [_P, _P]),
     "az_gpu_upload": (s32, [_P, _P, _P, u64]),

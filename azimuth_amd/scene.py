@@ -164,6 +164,9 @@ class SceneBuilder:
     def set_environment_map(self, t):
         N.check(self.lib.az_sb_set_environment_map(self.h, C.byref(t)), "az_sb_set_environment_map")
 
+    def set_background_texture(self, t):
+        N.check(self.lib.az_sb_set_background_texture(self.h, C.byref(t)), "az_sb_set_background_texture")
+
     def add_light(self, L):
         N.check(self.lib.az_sb_add_light(self.h, C.byref(L)), "az_sb_add_light")
 
@@ -442,6 +445,40 @@ class GpuContext:
         N.check(self.lib.az_gpu_level_graph_counts(self.h, C.byref(rec), C.byref(lau)),
                 "az_gpu_level_graph_counts", self.h)
         return int(rec.value), int(lau.value)
+
+    def set_render_memory_limit(self, nbytes):
+        """Caps the render queues' device memory (0: no cap); frees them first."""
+        N.check(self.lib.az_gpu_set_render_memory_limit(self.h, int(nbytes)), "az_gpu_set_render_memory_limit",
+                self.h)
+
+    def env_luminance_sample(self, u):
+        """Directions and solid-angle densities of the environment luminance
+        sampler at uniforms u (n, 2); returns (n, 4) float32."""
+        u = np.ascontiguousarray(u, dtype=np.float32).reshape(-1, 2)
+        n = u.shape[0]
+        d_u, d_o = self.alloc(max(8, u.nbytes)), self.alloc(max(16, n * 16))
+        try:
+            d_u.upload(u)
+            N.check(self.lib.az_gpu_env_luminance_sample(self.h, d_u.ptr, n, d_o.ptr), "az_gpu_env_luminance_sample",
+                    self.h)
+            return d_o.download(np.float32, n * 4).reshape(n, 4)
+        finally:
+            d_u.free()
+            d_o.free()
+
+    def env_luminance_pdf(self, dirs):
+        """Solid-angle density of the environment luminance sampler at unit
+        directions dirs (n, 3)."""
+        d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+        n = d.shape[0]
+        d_d, d_p = self.alloc(max(12, d.nbytes)), self.alloc(max(4, n * 4))
+        try:
+            d_d.upload(d)
+            N.check(self.lib.az_gpu_env_luminance_pdf(self.h, d_d.ptr, n, d_p.ptr), "az_gpu_env_luminance_pdf", self.h)
+            return d_p.download(np.float32, n)
+        finally:
+            d_d.free()
+            d_p.free()
 
     def trim(self):
         """Frees the context's render queues (the scene stays uploaded)."""

@@ -235,11 +235,14 @@ az_status az_gpu_set_build_mode(az_gpu_ctx *Ctx, u32 Mode);
  * AZ_LEVELS_AUTO (default) replays the levels of each chunk (extend, shade,
  * segment scan and connect per bounce) as one recorded hipGraph, one graph
  * per bounce count and chunk size, reused while the scene, camera, tuning
- * and buffers stay the same; every level is sized for the chunk's camera
- * rays and the kernels read each level's count on the device.  That needs
- * at most one child per shading point and no learned guide; other renders,
- * and AZ_LEVELS_HOST, launch each level from the host after reading back
- * the previous level's count.  Both give the same image, bit for bit.
+ * and buffers stay the same.  Each level is sized from per-level bounds
+ * learned on a host-driven chunk of the same scene, camera and options (its
+ * rays and shadow rays plus an eighth and 1,024), and the kernels read each
+ * level's count on the device; a level past its bound makes the chunk run
+ * again host-driven, which learns the bounds afresh.  Guided and batched
+ * renders, and AZ_LEVELS_HOST, launch each level from the host after
+ * reading back the previous level's count.  Both give the same image, bit
+ * for bit.
  * az_gpu_level_graph_counts reports how many graphs were recorded and
  * launched on the context. */
 az_status az_gpu_set_level_mode(az_gpu_ctx *Ctx, u32 Mode);
@@ -301,6 +304,29 @@ az_status az_gpu_set_render_tuning(az_gpu_ctx *Ctx, u32 ChunkSamples, u32 TraceB
  * No reference counterpart: the reference frees its pixel buffer at the end
  * of RenderScene (main.cpp:1899). */
 az_status az_gpu_trim(az_gpu_ctx *Ctx);
+
+/* Caps the device memory a context's render queues, pixel staging and
+ * learned-guide buffers may hold (0: no cap, the default); frees them first
+ * (as az_gpu_trim).  An allocation past the cap fails as it would on a full
+ * device: a render halves its chunk of samples per pixel until its queues
+ * fit (the image is the same up to the float rounding of the per-chunk pixel
+ * sums) and returns AZ_ERR_OUT_OF_MEMORY when one sample per pixel does not
+ * fit.  For callers that share a device with other work.  No reference
+ * counterpart: the reference allocates one pixel buffer per render
+ * (main.cpp:1808-1812). */
+az_status az_gpu_set_render_memory_limit(az_gpu_ctx *Ctx, u64 Bytes);
+
+/* The environment luminance sampler of AZ_ENV_LUMINANCE on caller inputs:
+ * for i < N, the direction drawn at the uniforms d_U[2i], d_U[2i+1] in [0,1)
+ * into d_Out[4i..4i+2] and its solid-angle density into d_Out[4i+3]
+ * (az_gpu_env_luminance_sample), or the density of the unit direction
+ * d_Dirs[3i..3i+2] into d_Pdf[i] (az_gpu_env_luminance_pdf).  Device
+ * arrays.  AZ_ERR_UNSUPPORTED unless the uploaded scene has an image
+ * environment map.  No reference counterpart: the reference draws
+ * environment directions from the cosine distribution only
+ * (main.cpp:724-778). */
+az_status az_gpu_env_luminance_sample(az_gpu_ctx *Ctx, const f32 *d_U, u32 N, f32 *d_Out);
+az_status az_gpu_env_luminance_pdf(az_gpu_ctx *Ctx, const f32 *d_Dirs, u32 N, f32 *d_Pdf);
 
 /* Device memory helpers, ordered on the context stream. */
 az_status az_gpu_malloc(az_gpu_ctx *Ctx, void **d_Ptr, u64 Bytes);

@@ -87,6 +87,11 @@ az_status az_sb_add_material(az_scene_builder *Builder, const material *Material
 az_status az_sb_add_texture(az_scene_builder *Builder, const az_texture *Texture, u32 *Index);
 /* Sets the RGB F32 or U8 environment map (pixel data is copied). */
 az_status az_sb_set_environment_map(az_scene_builder *Builder, const az_texture *Texture);
+/* Sets the RGB F32 or U8 background texture (pixel data is copied): camera
+ * rays that miss every object sample it at (col / (W - 1), row / (H - 1))
+ * unless an environment map is set (main.cpp:1180-1207, the
+ * UseBackgroundTexture branch). */
+az_status az_sb_set_background_texture(az_scene_builder *Builder, const az_texture *Texture);
 /* Point, area, directional or spot light. */
 az_status az_sb_add_light(az_scene_builder *Builder, const light *Light);
 az_status az_sb_add_mesh(az_scene_builder *Builder, const az_mesh_desc *Mesh);

@@ -39,6 +39,8 @@ def load():
         lib.azo_evaluate_brdf.argtypes = [C.POINTER(N.brdf), P, P, P, C.c_float, P, P]
         lib.azo_output_pixels.argtypes = [P, C.c_uint32, C.c_uint32, C.POINTER(N.tonemap), C.c_float, C.c_uint32, P]
         lib.azo_output_pixels.restype = C.c_int
+        lib.azo_platform_atomic_sum.argtypes = [C.c_uint32, C.c_uint32, C.c_float]
+        lib.azo_platform_atomic_sum.restype = C.c_float
         _lib = lib
     return _lib
 

@@ -21,10 +21,23 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
-#include <thread>
+#include <functional>
 #include <vector>
 
+#include "azimuth_platform_posix.h"
+
 namespace {
+
+// The reference reaches threads and the float atomic through its platform
+// table (platform.h:26-52); the oracle goes through the POSIX one.
+const az_platform_api &platform() {
+    static const az_platform_api api = [] {
+        az_platform_api a;
+        az_platform_posix_api(&a);
+        return a;
+    }();
+    return api;
+}
 
 const f64 kPi = 3.14159265359;               // math.h:3
 const f32 kPiF = (f32)kPi;                   // math.h:4
@@ -915,12 +928,7 @@ CameraSetup setupCamera(const camera *Cam, u32 sppOverride) {                   
 
 struct PixelF64 { f64 c[4]; };
 
-inline void atomicAddF32(f32 *p, f32 v) {                                      // platform_win32.cpp:377-395
-    std::atomic_ref<f32> a(*p);
-    f32 cur = a.load();
-    while (!a.compare_exchange_weak(cur, cur + v)) {
-    }
-}
+inline void atomicAddF32(f32 *p, f32 v) { platform().AtomicAddF32(p, v); }   // platform_win32.cpp:377-395
 inline void atomicAddF64(f64 *p, f64 v) {
     std::atomic_ref<f64> a(*p);
     f64 cur = a.load();
@@ -1003,6 +1011,28 @@ void produceRows(const Ctx &C, const camera *Cam, const epsilon_values *eps, fil
     }
 }
 
+// One platform thread per job, then a wait for all of them and the handles
+// closed, as RenderScene does with its bands (main.cpp:1823-1880).
+PlatformThreadProc(runJob) {
+    (*static_cast<std::function<void()> *>(Parameters))();
+    return 0;
+}
+
+void runThreads(std::vector<std::function<void()>> &jobs) {
+    const az_platform_api &P = platform();
+    std::vector<thread_handle> h(jobs.size());
+    u32 started = 0;
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        if (P.CreateThread(&h[started], runJob, &jobs[i], nullptr, 0)) {
+            ++started;
+        } else {
+            jobs[i]();   // no thread to be had: run the job here
+        }
+    }
+    if (started) P.WaitForThreads(started, h.data(), 1, 1, 0);
+    for (u32 i = 0; i < started; ++i) P.CloseThreadHandle(h[i]);
+}
+
 template <typename PixelT>
 void renderBands(const scene *S, const camera *Cam, const epsilon_values *eps, filter_type filter,
                  b32 cull, const az_render_opts *opts, PixelT *px, u32 threads) {  // main.cpp:1799-1880
@@ -1018,7 +1048,7 @@ void renderBands(const scene *S, const camera *Cam, const epsilon_values *eps, f
     u32 rows = r1 - r0;
     if (threads > rows) threads = rows;
     u32 minPer = rows / threads, resid = rows - minPer * threads, consumed = r0;
-    std::vector<std::thread> pool;
+    std::vector<std::function<void()>> bands;
     for (u32 t = 0; t < threads; ++t) {
         u32 n = minPer;
         if (resid) {
@@ -1026,23 +1056,23 @@ void renderBands(const scene *S, const camera *Cam, const epsilon_values *eps, f
             --resid;
         }
         u32 a = consumed, b = consumed + n;
-        pool.emplace_back([&, a, b] { produceRows(C, Cam, eps, filter, cull != 0, o, px, a, b); });
+        bands.emplace_back([&, a, b] { produceRows(C, Cam, eps, filter, cull != 0, o, px, a, b); });
         consumed += n;
     }
-    for (auto &th : pool) th.join();
+    runThreads(bands);
 }
 
 template <typename F>
 void parallelFor(u32 n, u32 threads, F f) {
     if (threads == 0) threads = 1;
-    std::vector<std::thread> pool;
+    std::vector<std::function<void()>> parts;
     u32 chunk = (n + threads - 1) / threads;
     for (u32 t = 0; t < threads; ++t) {
         u32 a = t * chunk, b = a + chunk < n ? a + chunk : n;
         if (a >= b) break;
-        pool.emplace_back([=] { for (u32 i = a; i < b; ++i) f(i); });
+        parts.emplace_back([=] { for (u32 i = a; i < b; ++i) f(i); });
     }
-    for (auto &th : pool) th.join();
+    runThreads(parts);
 }
 
 inline Ray toRay(const az_ray &r) {
@@ -1167,6 +1197,17 @@ void azo_render(const scene *S, const camera *Cam, const epsilon_values *eps, fi
 void azo_render_f64(const scene *S, const camera *Cam, const epsilon_values *eps, filter_type filter,
                     b32 cull, const az_render_opts *opts, f64 *px, u32 threads) {
     renderBands(S, Cam, eps, filter, cull, opts, reinterpret_cast<PixelF64 *>(px), threads);
+}
+
+f32 azo_platform_atomic_sum(u32 threads, u32 addsPerThread, f32 value) {
+    f32 total = 0.0f;
+    std::vector<std::function<void()>> jobs;
+    for (u32 t = 0; t < threads; ++t)
+        jobs.emplace_back([&] {
+            for (u32 k = 0; k < addsPerThread; ++k) platform().AtomicAddF32(&total, value);
+        });
+    runThreads(jobs);
+    return total;
 }
 
 f32 azo_rng_draw(u64 seed, u32 pixel, u32 sample, u32 node, u32 dim) {

@@ -46,6 +46,11 @@ void azo_render_f64(const scene *Scene, const camera *Camera, const epsilon_valu
                     filter_type Filter, b32 Cull, const az_render_opts *Opts, f64 *Pixels,
                     u32 Threads);
 
+/* Threads threads (through the POSIX platform table) each adding Value
+ * AddsPerThread times to one float with PlatformAtomicAddF32; returns the
+ * total. */
+f32 azo_platform_atomic_sum(u32 Threads, u32 AddsPerThread, f32 Value);
+
 /* Counter-addressed random stream: the float drawn at (Seed, Pixel, Sample,
  * Node, Dim), Dim counted within one path vertex. */
 f32 azo_rng_draw(u64 Seed, u32 Pixel, u32 Sample, u32 Node, u32 Dim);
