@@ -222,7 +222,11 @@ def procedural_envmap(width=256, height=128, sun_peak=5000.0, sun_sigma_deg=0.5,
 
 
 def feature_scene(width=48, height=48, spp=4, max_depth=5, seed=3, env=True, importance=True, aperture=0.0,
-                  focus=0.0):
+                  focus=0.0, background_texture=False, image_bump=False):
+    """Every light type and BRDF model, textures, Perlin bump maps, glossy
+    mirrors, dielectrics, motion blur and (env) an environment map; with
+    background_texture (and env False) camera misses read a gradient
+    texture, with image_bump a back wall carries a 2D image bump map."""
     from .scene import directional_light, point_light
     g = np.random.default_rng(seed)
     b = SceneBuilder(max_depth=max_depth, path_trace=True, importance_sampling=importance,
@@ -275,6 +279,20 @@ def feature_scene(width=48, height=48, spp=4, max_depth=5, seed=3, env=True, imp
         envimg = procedural_envmap(64, 32)
         te, keep_env = image_texture(envimg, filt=N.TextureFilter_Nearest)
         b.set_environment_map(te)
+    if background_texture:
+        yy, xx = np.mgrid[0:24, 0:32]
+        bg = np.stack([xx * 6 + 20, yy * 8 + 10, 200 - xx * 3 - yy * 2], -1).astype(np.uint8)
+        tb, keep_bg = image_texture(bg, filt=N.TextureFilter_Bilinear, wrap=N.TextureWrapMode_Clamp)
+        b.set_background_texture(tb)
+    if image_bump:
+        yy, xx = np.mgrid[0:32, 0:32]
+        hgt = (np.sin(xx / 3.0) * 60 + np.cos(yy / 2.0) * 50 + 128).astype(np.float32)
+        tbump, keep_bump = image_texture(np.stack([hgt] * 3, -1), filt=N.TextureFilter_Nearest,
+                                         blend=N.TextureBlendMode_BlendDiffuse, bump=True, bump_multiplier=0.004)
+        tex_wall = b.add_texture(tbump)
+        wv = np.array([(-3.5, 0, -3), (3.5, 0, -3), (3.5, 3, -3), (-3.5, 3, -3)], np.float32)
+        wuv = np.array([(0, 0), (2, 0), (2, 1), (0, 1)], np.float32)
+        b.add_mesh(wv, np.array([[0, 1, 2], [0, 2, 3]], np.uint32), material=mats[6], uvs=wuv, texture=tex_wall)
     sc = b.build(meta={"name": "features"})
     cam = look_at_camera((0.0, 3.0, 6.5), (0.0, 0.5, 0.0), (0, 1, 0), width, height, 45.0, spp=spp,
                          aperture=aperture, focus=focus)

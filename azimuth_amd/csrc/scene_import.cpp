@@ -268,11 +268,31 @@ static size_t TexturePixelBytes(const az_texture *T) {
     return (size_t)T->Width * T->Height * texel;
 }
 
+// The degamma step of LoadTexture (texture.h:391-413, 441-448) on a copy of
+// a texture's texels: 8-bit channels c become (u8)(powf(c / 255, 2.2) *
+// 255.001), float channels powf(c, 2.2).  The builder stands in for the
+// reference's loader, so it applies the step when the texture asks for it;
+// the GPU upload never does (a reference scene's textures were converted
+// when they were loaded).
+static void DegammaTexels(const az_texture *T, std::vector<u8> &texels) {
+    if (T->Type != TextureType_2D || !T->Degamma) return;
+    if (T->PixelFormat == PixelFormat_RGB_U8) {
+        for (u8 &c : texels) {
+            f32 v = powf((f32)c / 255.0f, 2.2f) * 255.001f;
+            c = (u8)v;
+        }
+    } else {
+        f32 *p = (f32 *)texels.data();
+        for (size_t i = 0; i < texels.size() / sizeof(f32); ++i) p[i] = powf(p[i], 2.2f);
+    }
+}
+
 az_status az_sb_add_texture(az_scene_builder *B, const az_texture *T, u32 *Index) {
     if (!B || !T) return AZ_ERR_INVALID_ARGUMENT;
     size_t n = TexturePixelBytes(T);
     B->textures.push_back(*T);
     B->texturePixels.emplace_back((const u8 *)T->Pixels, (const u8 *)T->Pixels + n);
+    DegammaTexels(T, B->texturePixels.back());
     if (Index) *Index = (u32)B->textures.size();
     return AZ_OK;
 }
@@ -283,6 +303,7 @@ az_status az_sb_set_environment_map(az_scene_builder *B, const az_texture *T) {
     size_t n = TexturePixelBytes(T);
     B->envmap = *T;
     B->envPixels.assign((const u8 *)T->Pixels, (const u8 *)T->Pixels + n);
+    DegammaTexels(T, B->envPixels);
     B->useEnv = 1;
     return AZ_OK;
 }
@@ -293,6 +314,7 @@ az_status az_sb_set_background_texture(az_scene_builder *B, const az_texture *T)
     size_t n = TexturePixelBytes(T);
     B->background = *T;
     B->backgroundPixels.assign((const u8 *)T->Pixels, (const u8 *)T->Pixels + n);
+    DegammaTexels(T, B->backgroundPixels);
     B->useBackground = 1;
     return AZ_OK;
 }

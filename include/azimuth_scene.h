@@ -83,7 +83,11 @@ void      az_sb_destroy(az_scene_builder *Builder);
 az_status az_sb_set_params(az_scene_builder *Builder, const az_scene_params *Params);
 /* Appends a material; Index receives its position in scene->Materials. */
 az_status az_sb_add_material(az_scene_builder *Builder, const material *Material, u32 *Index);
-/* Appends a texture (pixel data is copied); Index receives the 1-based index. */
+/* Appends a texture (pixel data is copied); Index receives the 1-based index.
+ * With Texture->Degamma set, the copy gets LoadTexture's degamma step
+ * (texture.h:391-413, 441-448: 8-bit channels c -> (u8)(powf(c / 255, 2.2)
+ * * 255.001), float channels powf(c, 2.2)), as the reference's loader
+ * applies it; so do the environment map and background texture calls. */
 az_status az_sb_add_texture(az_scene_builder *Builder, const az_texture *Texture, u32 *Index);
 /* Sets the RGB F32 or U8 environment map (pixel data is copied). */
 az_status az_sb_set_environment_map(az_scene_builder *Builder, const az_texture *Texture);

@@ -316,10 +316,67 @@ def test_perlin_lattice_points_and_range():
 # ---------------------------------------------------------------- scene import
 
 
+def _reference_bvh_triangles_only(p0, idx, depth=0):
+    """ConstructBVHTrianglesOnly (bvh.cpp:223-328) on the triangles idx
+    (their P0 in p0, float32): midpoint of the node box on axis depth % 3,
+    the partition loop from index 1 swapping left items to the front, half
+    split when a side is empty.  Returns the tree as nested (left, right)
+    tuples with triangle indices at the leaves."""
+    idx = list(idx)
+    if len(idx) == 1:
+        return idx[0]
+    dim = depth % 3
+    lo = np.min(p0[idx][:, dim])
+    hi = np.max(p0[idx][:, dim])
+    return lo, hi, dim, idx
+
+
+def _bvh_shape(tris, idx, depth=0):
+    """The tree of _reference_bvh_triangles_only, with the node box over all
+    three vertices of each triangle (bvh.cpp:242-277)."""
+    idx = list(idx)
+    if len(idx) == 1:
+        return idx[0]
+    pts = tris[idx].reshape(-1, 3)
+    mn, mx = pts.min(0), pts.max(0)
+    centre = mn * np.float32(0.5) + mx * np.float32(0.5)
+    dim = depth % 3
+    split = centre[dim]
+    left = 0
+    for i in range(1, len(idx)):
+        if tris[idx[i]][0][dim] < split:
+            idx[left], idx[i] = idx[i], idx[left]
+            left += 1
+    if left == 0 or left == len(idx):
+        left = len(idx) // 2
+    return (_bvh_shape(tris, idx[:left], depth + 1), _bvh_shape(tris, idx[left:], depth + 1))
+
+
+def _walk(e, tris_by_p0, boxes_ok):
+    """The built entity tree as nested tuples of leaf triangle indices
+    (matched by their vertices), checking that each node box holds its
+    children."""
+    import ref_structs as R
+    if e.Type == R.EntityType_Triangle:
+        t = e.u.tri
+        key = (R.vec(t.P0), R.vec(t.P1), R.vec(t.P2))
+        return tris_by_p0[key], np.array(key, np.float32)
+    assert e.Type == R.EntityType_BVHNode
+    box = e.u.boxed.AABB
+    mn, mx = np.array(R.vec(box.Min), np.float32), np.array(R.vec(box.Max), np.float32)
+    l, lp = _walk(e.u.boxed.p.node.Left.contents, tris_by_p0, boxes_ok)
+    r, rp = _walk(e.u.boxed.p.node.Right.contents, tris_by_p0, boxes_ok)
+    pts = np.concatenate([lp.reshape(-1, 3), rp.reshape(-1, 3)])
+    boxes_ok.append(bool(np.all(pts >= mn) and np.all(pts <= mx)))
+    return (l, r), pts
+
+
 def test_reference_bvh_invariants():
-    """The entity tree built like bvh.cpp:1-328: every node box contains its
-    children, the leaves are exactly the mesh's triangles, and the walk over
-    it equals brute force (checked above)."""
+    """The mesh BVH built like bvh.cpp:223-328: the same tree, leaf for leaf,
+    as a Python run of the reference's construction over the mesh's
+    triangles in input order; every node box holds every vertex below it;
+    the leaves are exactly the mesh's triangles."""
+    import ref_structs as R
     from azimuth_amd.scene import SceneBuilder
     v, f = configs.displaced_icosphere(2, 1)
     b = SceneBuilder()
@@ -327,16 +384,219 @@ def test_reference_bvh_invariants():
     b.add_mesh(v, f)
     sc = b.build()
     assert sc.triangle_count == f.shape[0]
+    tris = v[f].astype(np.float32)                       # (F, 3, 3) in input order
+    keys = {tuple(map(tuple, t)): k for k, t in enumerate(tris)}
+    assert len(keys) == f.shape[0]
+    s = R.scene_of(sc)
+    m = s.Meshes[0]
+    boxes_ok = []
+    built, _ = _walk(m.BVH.contents, {tuple(tuple(float(x) for x in p) for p in k): i for k, i in keys.items()},
+                     boxes_ok)
+    want = _bvh_shape(tris, range(f.shape[0]))
+    assert built == want
+    assert len(boxes_ok) == f.shape[0] - 1 and all(boxes_ok)
+
+    def leaves(t):
+        return [t] if not isinstance(t, tuple) else leaves(t[0]) + leaves(t[1])
+    assert sorted(leaves(built)) == list(range(f.shape[0]))
 
 
 def test_smooth_vertex_normals_are_radial():
-    """xml.cpp:3632-3667: area-weighted normals of a sphere mesh point
-    outward along the radius."""
+    """xml.cpp:3632-3667: a vertex normal is the normalized sum of the
+    unnormalized cross products (P1 - P0) x (P2 - P0) of its faces; on an
+    icosphere they point along the radius.  Checked on the triangle entities
+    the builder made (N0, N1, N2 of each face)."""
+    import ref_structs as R
     v, f = configs.icosphere(3)
+    v32 = v.astype(np.float32)
     sc = SceneBuilder()
     sc.add_material(make_material())
-    sc.add_mesh(v.astype(np.float32), f.astype(np.uint32), smooth=True)
+    sc.add_mesh(v32, f.astype(np.uint32), smooth=True)
     s = sc.build()
-    r = configs.coherent_rays(32, 32)
-    h = oracle.trace(s, r)
-    assert (h["Prim"] != N.AZ_MISS).mean() > 0.3
+    acc = np.zeros((v.shape[0], 3))
+    tri = v32[f].astype(np.float64)
+    cr = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    for k in range(3):
+        np.add.at(acc, f[:, k], cr)
+    vn = acc / np.linalg.norm(acc, axis=1, keepdims=True)
+    assert float(np.min(np.sum(vn * v / np.linalg.norm(v, axis=1, keepdims=True), 1))) > 0.9995
+    mesh = R.scene_of(s).Meshes[0]
+    # The mesh's triangle entities sit in the BVH leaves; walk them.
+    seen = 0
+    stack = [mesh.BVH.contents]
+    by_pos = {tuple(np.round(p, 6)): k for k, p in enumerate(v32.astype(np.float64))}
+    while stack:
+        e = stack.pop()
+        if e.Type == R.EntityType_BVHNode:
+            stack += [e.u.boxed.p.node.Left.contents, e.u.boxed.p.node.Right.contents]
+            continue
+        t = e.u.tri
+        for P, Nv in ((t.P0, t.N0), (t.P1, t.N1), (t.P2, t.N2)):
+            k = by_pos[tuple(np.round(np.array(R.vec(P), np.float64), 6))]
+            np.testing.assert_allclose(R.vec(Nv), vn[k], atol=2e-6)
+        seen += 1
+    assert seen == f.shape[0]
+
+
+def test_builder_degamma_like_load_texture():
+    """texture.h:391-413, 441-448: with Degamma set, 8-bit texels become
+    (u8)(powf(c / 255, 2.2) * 255.001) and float texels powf(c, 2.2); the
+    flag unset leaves them as they are."""
+    import ref_structs as R
+    from azimuth_amd.scene import image_texture
+    u8 = np.arange(256, dtype=np.uint8).repeat(3).reshape(16, 16, 3)
+    f32 = (np.arange(48, dtype=np.float32).reshape(4, 4, 3) * np.float32(0.37))
+    for data in (u8, f32):
+        for flag in (0, 1):
+            t, keep = image_texture(data)
+            t.Degamma = flag
+            b = SceneBuilder()
+            b.add_material(make_material())
+            b.add_texture(t)
+            b.add_sphere((0, 0, 0), 1.0)
+            s = b.build()
+            tex = R.scene_of(s).Textures[0]
+            n = data.size
+            ctype = C.c_uint8 if data.dtype == np.uint8 else C.c_float
+            got = np.ctypeslib.as_array(C.cast(tex.u.img.Pixels, C.POINTER(ctype)), shape=(n,)).copy()
+            src = data.reshape(-1).astype(np.float64)
+            if not flag:
+                assert np.array_equal(got, data.reshape(-1))
+            elif data.dtype == np.uint8:
+                exact = (src / 255.0) ** 2.2 * 255.001
+                want = np.floor(exact)
+                near = np.abs(exact - np.round(exact)) < 1e-4       # binary32 rounding may land either side
+                assert np.all((got == want) | near), np.argwhere((got != want) & ~near)
+                assert got[-1] == 255 and got[0] == 0 and got[3 * 128] == 55 and got[3 * 200] == 149
+            else:
+                np.testing.assert_allclose(got, src ** 2.2, rtol=2e-6)
+            del keep
+
+
+# ---------------------------------------------------------------- BRDFs off the mirror direction
+
+
+def _textbook_brdf(kind, n, ior, L, Nn, Vray):
+    """The seven models of EvaluateBRDF (main.cpp:461-569) in float64 from
+    their definitions: Phong lobe (R . w_o)^n with R = reflect(L, N), Blinn
+    lobe (N . H)^n, normalizations (n+2)/2pi and (n+8)/8pi, the unnormalized
+    Phong and Blinn models divided by cos(theta_i), Torrance-Sparrow with the
+    Blinn distribution, G = min(1, 2 (N.H) min(N.w_o, N.L) / (w_o.H)) and
+    Schlick's F at w_o.H.  Returns (diffuse, specular) factors."""
+    wo = -Vray
+    cosI = float(Nn @ L)
+    R = 2.0 * (L @ Nn) * Nn - L
+    H = (wo + L) / np.linalg.norm(wo + L)
+    cp, cb = float(wo @ R), float(Nn @ H)
+    d = 1.0 / math.pi if kind in (N.BRDF_NormalizedModifiedPhong, N.BRDF_NormalizedModifiedBlinnPhong,
+                                  N.BRDF_TorranceSparrow) else 1.0
+    if kind == N.BRDF_Phong:
+        return d, (cp ** n / cosI if cp > 0 else 0.0)
+    if kind == N.BRDF_ModifiedPhong:
+        return d, (cp ** n if cp > 0 else 0.0)
+    if kind == N.BRDF_NormalizedModifiedPhong:
+        return d, ((n + 2) / (2 * math.pi) * cp ** n if cp > 0 else 0.0)
+    if kind == N.BRDF_BlinnPhong:
+        return d, (cb ** n / cosI if cb > 0 else 0.0)
+    if kind == N.BRDF_ModifiedBlinnPhong:
+        return d, (cb ** n if cb > 0 else 0.0)
+    if kind == N.BRDF_NormalizedModifiedBlinnPhong:
+        return d, ((n + 8) / (8 * math.pi) * cb ** n if cb > 0 else 0.0)
+    if cb <= 0:
+        return d, 0.0
+    D = (n + 2) / (2 * math.pi) * cb ** n
+    nwo, woh = float(Nn @ wo), float(wo @ H)
+    G = min(1.0, 2.0 * cb * min(nwo, cosI) / woh)
+    r0 = (ior - 1) ** 2 / (ior + 1) ** 2
+    F = r0 + (1 - r0) * (1 - woh) ** 5
+    return d, D * F * G / (4 * cosI * nwo)
+
+
+def _random_pair(g):
+    Nn = g.normal(size=3)
+    Nn /= np.linalg.norm(Nn)
+    while True:
+        L, wo = g.normal(size=3), g.normal(size=3)
+        L /= np.linalg.norm(L)
+        wo /= np.linalg.norm(wo)
+        if L @ Nn > 0.05 and wo @ Nn > 0.05:
+            return L, Nn, -wo
+
+
+def test_brdfs_at_random_directions_against_their_definitions():
+    g = np.random.default_rng(5)
+    for kind in range(7):
+        for _ in range(200):
+            L, Nn, V = _random_pair(g)
+            n = float(g.uniform(2, 60))
+            d, s = _eval(_brdf(kind, n, 1.6), L, Nn, V, float(L @ Nn))
+            wd, ws = _textbook_brdf(kind, n, 1.6, L, Nn, V)
+            assert d == pytest.approx(wd, rel=1e-6)
+            assert s == pytest.approx(ws, rel=3e-4, abs=1e-7), (kind, n)
+
+
+def test_brdf_reciprocity():
+    """Swapping the light and view directions leaves the reciprocal models'
+    value unchanged (and the Phong and Blinn models' value times
+    cos(theta_i), their lobe)."""
+    g = np.random.default_rng(9)
+    for kind in range(7):
+        for _ in range(100):
+            L, Nn, V = _random_pair(g)
+            n = float(g.uniform(2, 60))
+            _, a = _eval(_brdf(kind, n, 1.4), L, Nn, V, float(L @ Nn))
+            _, b = _eval(_brdf(kind, n, 1.4), -V, Nn, -L, float(-V @ Nn))
+            if kind in (N.BRDF_Phong, N.BRDF_BlinnPhong):
+                a, b = a * float(L @ Nn), b * float(-V @ Nn)
+            assert a == pytest.approx(b, rel=5e-4, abs=1e-7), kind
+
+
+def test_torrance_sparrow_fresnel_goes_to_one_at_grazing():
+    """Schlick's F = r0 + (1 - r0)(1 - w_o.H)^5 tends to 1 as w_o.H -> 0:
+    with light and view at the mirror configuration near grazing (H = N,
+    w_o.H = cos 89.5 deg), the specular value over D G / (4 cos_i cos_o)
+    approaches 1 whatever the index."""
+    Nn = np.array([0.0, 0.0, 1.0])
+    for ior in (1.3, 2.5):
+        th = math.radians(89.5)
+        L = np.array([math.sin(th), 0.0, math.cos(th)])
+        wo = np.array([-math.sin(th), 0.0, math.cos(th)])
+        d, s = _eval(_brdf(N.BRDF_TorranceSparrow, 4.0, ior), L, Nn, -wo, float(L @ Nn))
+        H = (wo + L) / np.linalg.norm(wo + L)
+        cb, woh = float(Nn @ H), float(wo @ H)
+        D = 6.0 / (2 * math.pi) * cb ** 4
+        G = min(1.0, 2.0 * cb * min(float(Nn @ wo), float(Nn @ L)) / woh)
+        F = s / (D * G / (4 * float(Nn @ L) * float(Nn @ wo)))
+        r0 = (ior - 1) ** 2 / (ior + 1) ** 2
+        assert F == pytest.approx(r0 + (1 - r0) * (1 - woh) ** 5, rel=1e-3)
+        assert F > 0.9
+
+
+# ---------------------------------------------------------------- Perlin noise
+
+
+def test_perlin_gradient_at_lattice_points_follows_the_hash():
+    """Near a lattice point Q the noise is dot(g_h, P - Q) to second order,
+    g_h the gradient table entry at h = Perm[(I + Perm[(J + Perm[K]) % 256]) % 256]
+    (PerlinHash, texture.h:134-144): finite differences of the patch value
+    (raw + 1) / 2 at Q recover g_h / 2.  The vein appearance is |raw|."""
+    from azimuth_amd.scene import perlin_texture
+    lib = oracle.load()
+    t = perlin_texture(21, scaling=1.0)
+    tv = perlin_texture(21, scaling=1.0, appearance=N.TexturePerlinAppearance_Vein)
+    perm = np.array(t.u.perlin.Perm[:], np.int64)
+    grad = np.zeros(3, np.float32)
+    eps = 1e-3
+    for (I, J, K) in ((0, 0, 0), (3, 7, 1), (12, 5, 200), (100, 2, 33)):
+        h = perm[(I + perm[(J + perm[K % 256]) % 256]) % 256]
+        lib.azo_perlin_gradient(int(h), grad.ctypes.data)
+        for k in range(3):
+            P = np.array([I, J, K], np.float32)
+            P[k] = np.float32(P[k] + eps)
+            step = float(P[k]) - (I, J, K)[k]                 # the binary32 offset actually taken
+            d = (lib.azo_perlin_sample(C.byref(t), *[float(x) for x in P]) - 0.5) * 2.0 / step
+            assert d == pytest.approx(float(grad[k]), abs=3e-3), (I, J, K, k)
+    q = np.random.default_rng(3).uniform(0, 50, (500, 3))
+    for p in q:
+        raw = (lib.azo_perlin_sample(C.byref(t), *p) - 0.5) * 2.0
+        assert lib.azo_perlin_sample(C.byref(tv), *p) == pytest.approx(abs(raw), abs=2e-6)
