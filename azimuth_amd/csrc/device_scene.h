@@ -87,6 +87,7 @@ struct DeviceGeometry {
     const TriPacked *tris;      // leaf-ordered triangles of all meshes
     const float *faceNormals;   // per leaf-ordered triangle, xyz (for culling)
     const TopItem *items;       // top-level items in TLAS leaf order
+    const float *refBoxes;      // per item: the reference entity's AABB (min xyz, max xyz), instances only
     uint32_t tlasRoot;
     uint32_t triTotal;          // sphere primitive ids start here
     uint32_t itemCount;

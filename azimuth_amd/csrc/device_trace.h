@@ -105,6 +105,25 @@ __device__ inline bool tri_test(const TriPacked &tp, float ox, float oy, float o
     return t > eps;
 }
 
+// RayAABBIntersection (ray.h:68-125) on a box given as min xyz, max xyz:
+// the slabs with 1/d, near and far planes chosen by the sign of 1/d, the
+// same comparisons (so NaN slab distances fall where the reference's do),
+// hit iff TEnterMax <= TExitMin and TExitMin > 0.
+__device__ inline bool ref_box_hit(const float *b, float ox, float oy, float oz, float dx, float dy, float dz) {
+    const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+    const float lx = b[0] - ox, ly = b[1] - oy, lz = b[2] - oz;
+    const float hx = b[3] - ox, hy = b[4] - oy, hz = b[5] - oz;
+    float ex, ey, ez, fx, fy, fz;
+    if (ix > 0.0f) { ex = lx * ix; fx = hx * ix; } else { ex = hx * ix; fx = lx * ix; }
+    if (iy > 0.0f) { ey = ly * iy; fy = hy * iy; } else { ey = hy * iy; fy = ly * iy; }
+    if (iz > 0.0f) { ez = lz * iz; fz = hz * iz; } else { ez = hz * iz; fz = lz * iz; }
+    float en = (ex > ey) ? ex : ey;
+    en = (ez > en) ? ez : en;
+    float fa = (fx < fy) ? fx : fy;
+    fa = (fz < fa) ? fz : fa;
+    return en <= fa && fa > 0.0f;
+}
+
 // The reference picks the closest hit by a recursive minimum that keeps the
 // right child on equal t (main.cpp:303-307), so among exactly equal t the
 // right-most leaf of its BVH wins.  Ranks order the leaves left to right:
@@ -416,7 +435,7 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
                     out.gamma = 0.0f;
                     if (any) finished = true;
                 }
-            } else {
+            } else if (ref_box_hit(G.refBoxes + 6 * (size_t)itemIdx, r.ox, r.oy, r.oz, r.dx, r.dy, r.dz)) {
                 mx = r.time * it.motion[0];
                 my = r.time * it.motion[1];
                 mz = r.time * it.motion[2];

@@ -316,6 +316,19 @@ az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
     AZ_HIP(C, C->geom.tris.alloc(std::max<size_t>(1, tris.size()) * sizeof(TriPacked)));
     AZ_HIP(C, C->geom.faceNormals.alloc(std::max<size_t>(1, faceN.size()) * sizeof(float)));
     AZ_HIP(C, C->geom.items.alloc(items.size() * sizeof(TopItem)));
+    // The reference tests an instance's own box before its mesh BVH
+    // (RayAABBIntersection(Entity->AABB, Ray), main.cpp:245); the walk repeats
+    // that test on these tight boxes, whatever its own (padded) boxes say.
+    std::vector<float> refBoxes(6 * items.size());
+    for (size_t i = 0; i < items.size(); ++i) {
+        const entity *e = topOrdered[i];
+        const aabb &bx = e->AABB;
+        const float v[6] = {bx.Min.x, bx.Min.y, bx.Min.z, bx.Max.x, bx.Max.y, bx.Max.z};
+        for (int k = 0; k < 6; ++k) refBoxes[6 * i + k] = e->Type == EntityType_MeshInstance ? v[k] : 0.0f;
+    }
+    AZ_HIP(C, C->geom.refBoxes.alloc(refBoxes.size() * sizeof(float)));
+    AZ_HIP(C, hipMemcpyAsync(C->geom.refBoxes.ptr, refBoxes.data(), refBoxes.size() * sizeof(float),
+                             hipMemcpyHostToDevice, C->stream));
     AZ_HIP(C, hipMemcpyAsync(C->geom.nodes.ptr, nodes.data(), nodes.size() * sizeof(Node4), hipMemcpyHostToDevice,
                              C->stream));
     if (!tris.empty()) {
@@ -332,6 +345,7 @@ az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
     G.tris = (const TriPacked *)C->geom.tris.ptr;
     G.faceNormals = (const float *)C->geom.faceNormals.ptr;
     G.items = (const TopItem *)C->geom.items.ptr;
+    G.refBoxes = (const float *)C->geom.refBoxes.ptr;
     G.tlasRoot = tlasRoot;
     G.triTotal = info.triTotal;
     G.itemCount = (uint32_t)items.size();

@@ -57,12 +57,13 @@ struct DeviceBuffer {
 };
 
 struct GeometryBuffers {
-    DeviceBuffer nodes, tris, faceNormals, items;
+    DeviceBuffer nodes, tris, faceNormals, items, refBoxes;
     void free() {
         nodes.release();
         tris.release();
         faceNormals.release();
         items.release();
+        refBoxes.release();
     }
 };
 

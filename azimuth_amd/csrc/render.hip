@@ -635,7 +635,7 @@ __device__ inline void reconstruct(const DevScene &S, const DeviceGeometry &G, c
         V3 hn = normalize(lhp);
         P = point_to_world(hp, it.fwd) + motion;
         float theta = Acos(hn.y);
-        float phi = atan2f(hn.z, hn.x);
+        float phi = Atan2F(hn.z, hn.x);
         float twoPi = kPiF * 2.0f;
         u = (-phi + kPiF) / twoPi;
         v = theta / kPiF;
@@ -643,7 +643,7 @@ __device__ inline void reconstruct(const DevScene &S, const DeviceGeometry &G, c
             N = hn;
             if (tex->bump) {
                 float pyPi = kPiF * lhp.y, rPi = kPiF * it.radius;
-                V3 Pv = v3(cosf(phi) * pyPi, -sinf(theta) * rPi, sinf(phi) * pyPi);
+                V3 Pv = v3(CosF(phi) * pyPi, -SinF(theta) * rPi, SinF(phi) * pyPi);
                 V3 Pu = v3(twoPi * lhp.z, 0.0f, -twoPi * lhp.x);
                 N = bump(S, *tex, N, P, u, v, Pu, Pv);
             }
@@ -730,7 +730,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
             V3 bg;
             if (S.useEnv) {
                 float th = Acos(D.y);
-                float ph = atan2f(D.z, D.x);
+                float ph = Atan2F(D.z, D.x);
                 float u = (-ph + kPiF) / (2.0f * kPiF), v = th / kPiF;
                 bg = sample_texture(S, S.env, v3(u, v, 1.0f), u, v);
             } else if (S.useBackgroundTexture) {
@@ -811,7 +811,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
             else if (ct > L.cosFall) falloff = 1.0f;
             else {
                 falloff = (ct - L.cosCov) / (L.cosFall - L.cosCov);
-                falloff = powf(falloff, 4.0f);
+                falloff = PowF(falloff, 4.0f);
                 if (falloff < 0.0f) falloff = 0.0f;
                 else if (falloff > 1.0f) falloff = 1.0f;
             }
@@ -826,7 +826,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
                 float cosT = 1.0f + rng.uniform() * (cosMax - 1.0f);
                 float sinT = sqrtf(1.0f - sq(cosT));
                 float phi = 2.0f * kPiF * rng.uniform();
-                float cp_ = cosf(phi), sp_ = sinf(phi);
+                float cp_ = CosF(phi), sp_ = SinF(phi);
                 V3 W = normalize(Dv), U, Vv;
                 orthonormal_basis(W, U, Vv);
                 dl = normalize(U * sinT * cp_ + Vv * sinT * sp_ + W * cosT);
@@ -881,14 +881,14 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
             L = env_lum_sample(S, u0, u1, pBase);
         } else {
             float phi = 2.0f * kPiF * rng.uniform();
-            float cp_ = cosf(phi), sp_ = sinf(phi);
+            float cp_ = CosF(phi), sp_ = SinF(phi);
             float theta = Asin(sqrtf(rng.uniform()));
-            float ct = cosf(theta), st = sinf(theta);
+            float ct = CosF(theta), st = SinF(theta);
             V3 U, Vv;
             orthonormal_basis(N, U, Vv);
             L = normalize(U * st * cp_ + Vv * st * sp_ + N * ct);
         }
-        float eu = (-atan2f(L.z, L.x) + kPiF) / (2.0f * kPiF);
+        float eu = (-Atan2F(L.z, L.x) + kPiF) / (2.0f * kPiF);
         float ev = Acos(L.y) / kPiF;
         V3 I = sample_texture(S, S.env, v3(eu, ev, 1.0f), eu, ev);
         float ca = dot(L, N);
@@ -956,9 +956,9 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
             Sd = sample_brdf_mixture(M, kd, N, -D, u0, u1, u2, pBase);
         } else {
             float theta = S.importance ? Asin(sqrtf(rng.uniform())) : Acos(rng.uniform());
-            float ct = cosf(theta), st = sinf(theta);
+            float ct = CosF(theta), st = SinF(theta);
             float phi = 2.0f * kPiF * rng.uniform();
-            float cp_ = cosf(phi), sp_ = sinf(phi);
+            float cp_ = CosF(phi), sp_ = SinF(phi);
             V3 U, Vv;
             orthonormal_basis(N, U, Vv);
             Sd = normalize(U * st * cp_ + Vv * st * sp_ + N * ct);
@@ -1074,7 +1074,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
             float cosAir = inside ? cosPhi : cosT;
             float inv = 1.0f - cosAir;
             float r0 = sq((from - to) / (from + to));
-            refl = r0 + (1.0f - r0) * powf(inv, 5.0f);
+            refl = r0 + (1.0f - r0) * PowF(inv, 5.0f);
             V3 T = (D + Nn * cosT) * ratio - Nn * cosPhi;
             T = normalize(T);
             V3 coef = inside ? zero : v3(tr.x - 1.0f, tr.y - 1.0f, tr.z - 1.0f);
@@ -1715,6 +1715,13 @@ struct RenderBuffers {
     RenderBuffers() {
         forEachQueue([this](DeviceBuffer &b) { b.lim = &queueLimit; });
     }
+    // A chunk that did not fit gives back its sample-sized queues before the
+    // render retries with half the samples per pixel, so that the smaller
+    // chunk is not squeezed by the larger one's allocations.
+    void releaseChunkQueues() {
+        DeviceBuffer *bufs[] = {&q0, &q1, &hits, &shadow, &acc, &qrec0, &qrec1};
+        for (DeviceBuffer *b : bufs) b->release();
+    }
     ~RenderBuffers() { dropLevelGraph(); }
 };
 
@@ -2352,6 +2359,12 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
                     if (!out_of_memory(ge)) {
                         AZ_HIPR(C, ge);
                         graphChunk = true;
+                    } else {
+                        // No room for the replayed levels' queues: the chunk runs
+                        // host-driven, sized by its own counts, with what was just
+                        // grown given back.
+                        AZ_HIPR(C, hipStreamSynchronize(st));
+                        B.releaseChunkQueues();
                     }
                 }
             }
@@ -2364,6 +2377,8 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
                 return AZ_ERR_OUT_OF_MEMORY;
             }
             chunkSpp = std::max<u32>(1u, chunkSpp / 2);
+            AZ_HIPR(C, hipStreamSynchronize(st));
+            B.releaseChunkQueues();
             continue;
         }
         AZ_HIPR(C, ae);
@@ -2531,6 +2546,8 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
                 return AZ_ERR_OUT_OF_MEMORY;
             }
             chunkSpp = std::max<u32>(1u, chunkSpp / 2);
+            AZ_HIPR(C, hipStreamSynchronize(st));
+            B.releaseChunkQueues();
             continue;   // the chunk is redone with fewer samples; its accumulators are rebuilt
         }
         if (levelGraph && !graphChunk) {

@@ -37,6 +37,16 @@ __device__ inline float sq(float v) { return v * v; }
 __device__ inline float Exp(float v) { return (float)exp((double)v); }                     // math.cpp:149-156
 __device__ inline float Asin(float v) { return (float)asin((double)v); }                   // math.cpp:221-228
 __device__ inline float Acos(float v) { return (float)acos((double)v); }                   // math.cpp:239-246
+// Sin, Cos, Atan and Power (math.cpp:136-147, 158-219, 248-260) wrap the
+// platform's float functions, whose last bit differs between C libraries and
+// the device.  Here, as in the CPU restatement, each is the double-precision
+// function of the float argument rounded to float: the correctly rounded value
+// in all but about one case in 2^29, so the device and the host draw the same
+// directions and weights bit for bit.
+__device__ inline float SinF(float v) { return (float)sin((double)v); }
+__device__ inline float CosF(float v) { return (float)cos((double)v); }
+__device__ inline float Atan2F(float y, float x) { return (float)atan2((double)y, (double)x); }
+__device__ inline float PowF(float x, float y) { return (float)pow((double)x, (double)y); }
 
 __device__ inline V3 point_to_world(V3 v, const Affine &m) {                               // math.cpp:1774-1800
     return v3(m.m[0][0] * v.x + m.m[0][1] * v.y + m.m[0][2] * v.z + m.m[0][3],
@@ -198,12 +208,12 @@ __device__ inline void evaluate_brdf(float &dif, float &spec, const DevMaterial 
     case 0: {   // Phong
         V3 R = reflect(L, N);
         float c = dot(-Vw, R);
-        spec = c > 0.0f ? powf(c, M.exponent) / cosI : 0.0f;
+        spec = c > 0.0f ? PowF(c, M.exponent) / cosI : 0.0f;
     } break;
     case 1: {   // ModifiedPhong
         V3 R = reflect(L, N);
         float c = dot(-Vw, R);
-        spec = c > 0.0f ? powf(c, M.exponent) : 0.0f;
+        spec = c > 0.0f ? PowF(c, M.exponent) : 0.0f;
     } break;
     case 2: {   // NormalizedModifiedPhong
         dif = (float)(1.0f / kPiD);
@@ -211,7 +221,7 @@ __device__ inline void evaluate_brdf(float &dif, float &spec, const DevMaterial 
         float c = dot(-Vw, R);
         if (c > 0.0f) {
             float e = M.exponent;
-            spec = ((e + 2.0f) / (2.0f * kPiF)) * powf(c, e);
+            spec = ((e + 2.0f) / (2.0f * kPiF)) * PowF(c, e);
         } else {
             spec = 0.0f;
         }
@@ -219,12 +229,12 @@ __device__ inline void evaluate_brdf(float &dif, float &spec, const DevMaterial 
     case 3: {   // BlinnPhong
         V3 H = normalize(-Vw + L);
         float c = dot(N, H);
-        spec = c > 0.0f ? powf(c, M.exponent) / cosI : 0.0f;
+        spec = c > 0.0f ? PowF(c, M.exponent) / cosI : 0.0f;
     } break;
     case 4: {   // ModifiedBlinnPhong
         V3 H = normalize(-Vw + L);
         float c = dot(N, H);
-        spec = c > 0.0f ? powf(c, M.exponent) : 0.0f;
+        spec = c > 0.0f ? PowF(c, M.exponent) : 0.0f;
     } break;
     case 5: {   // NormalizedModifiedBlinnPhong
         dif = (float)(1.0f / kPiD);
@@ -232,7 +242,7 @@ __device__ inline void evaluate_brdf(float &dif, float &spec, const DevMaterial 
         float c = dot(N, H);
         if (c > 0.0f) {
             float e = M.exponent;
-            spec = ((e + 8.0f) / (8.0f * kPiF)) * powf(c, e);
+            spec = ((e + 8.0f) / (8.0f * kPiF)) * PowF(c, e);
         } else {
             spec = 0.0f;
         }
@@ -243,7 +253,7 @@ __device__ inline void evaluate_brdf(float &dif, float &spec, const DevMaterial 
         float c = dot(N, H);
         if (c > 0.0f) {
             float e = M.exponent;
-            float ct = powf(c, e);
+            float ct = PowF(c, e);
             float D = ((e + 2.0f) / (2.0f * kPiF)) * ct;
             float nw0 = dot(N, -Vw);
             float w0wh = dot(-Vw, H);
@@ -252,7 +262,7 @@ __device__ inline void evaluate_brdf(float &dif, float &spec, const DevMaterial 
             G = 1.0f < G ? 1.0f : G;
             float ri = M.brdfIor;
             float R0 = sq(ri - 1.0f) / sq(ri + 1.0f);
-            float F = R0 + (1.0f - R0) * powf(1.0f - w0wh, 5.0f);
+            float F = R0 + (1.0f - R0) * PowF(1.0f - w0wh, 5.0f);
             spec = (D * F * G) / (4.0f * cosI * nw0);
         } else {
             spec = 0.0f;

@@ -67,6 +67,15 @@ inline f32 sq(f32 v) { return v * v; }
 inline f32 Exp(f32 v) { return (f32)exp((f64)v); }                              // math.cpp:149-156
 inline f32 Asin(f32 v) { return (f32)asin((f64)v); }                            // math.cpp:221-228
 inline f32 Acos(f32 v) { return (f32)acos((f64)v); }                            // math.cpp:239-246
+// Sin, Cos, Atan and Power (math.cpp:136-147, 158-219, 248-260) wrap the
+// platform's float functions, whose last bit is the C library's own choice;
+// this restatement defines each as the double-precision function of the
+// float argument rounded to float (the correctly rounded value in all but
+// about one case in 2^29), as the device does.
+inline f32 SinF(f32 v) { return (f32)sin((f64)v); }
+inline f32 CosF(f32 v) { return (f32)cos((f64)v); }
+inline f32 Atan2F(f32 y, f32 x) { return (f32)atan2((f64)y, (f64)x); }
+inline f32 PowF(f32 x, f32 y) { return (f32)pow((f64)x, (f64)y); }
 inline f32 minf_(f32 a, f32 b) { return a < b ? a : b; }                        // math.h:6
 inline f32 axis(V v, u32 d) { return d == 0 ? v.x : (d == 1 ? v.y : v.z); }
 
@@ -346,14 +355,14 @@ bool intersect(const Ctx &C, Hit *H, Ray ray, const entity *E, bool cull, f32 ep
                 f32 wt = length(wp - ray.o - motion) / length(ray.d);
                 if (wt > eps && wt < H->t) {
                     f32 theta = Acos(hn.y);
-                    f32 phi = atan2f(hn.z, hn.x);
+                    f32 phi = Atan2F(hn.z, hn.x);
                     f32 twoPi = kPiF * 2.0f;
                     H->P = wp;
                     if (E->TextureIndex) {
                         tex = S->Textures + (E->TextureIndex - 1);
                         if (tex->BumpMap) {
                             f32 pyPi = kPiF * lhp.y, rPi = kPiF * radius;
-                            Pv = mk(cosf(phi) * pyPi, -sinf(theta) * rPi, sinf(phi) * pyPi);
+                            Pv = mk(CosF(phi) * pyPi, -SinF(theta) * rPi, SinF(phi) * pyPi);
                             Pu = mk(twoPi * lhp.z, 0.0f, -twoPi * lhp.x);
                         } else {
                             tex = nullptr;
@@ -517,7 +526,7 @@ LightSample sampleEntityLight(const entity *LE, V hitP, VertexRng &rng) {       
         f32 cosT = 1.0f + rng.next() * (cosMax - 1.0f);
         f32 sinT = sqrtf(1.0f - sq(cosT));
         f32 phi = 2.0f * kPiF * rng.next();
-        f32 cp = cosf(phi), sp = sinf(phi);
+        f32 cp = CosF(phi), sp = SinF(phi);
         V W = normalize(D), U, Vv;
         orthonormalBasis(W, &U, &Vv);
         dl = normalize(U * sinT * cp + Vv * sinT * sp + W * cosT);
@@ -547,12 +556,12 @@ void evaluateBRDF(f32 *Dif, f32 *Spec, const brdf *B, V L, V N, V Vw, f32 cosI) 
     case BRDF_Phong: {
         V R = reflect(L, N);
         f32 c = dot(-Vw, R);
-        *Spec = c > 0.0f ? powf(c, B->Exponent) / cosI : 0.0f;
+        *Spec = c > 0.0f ? PowF(c, B->Exponent) / cosI : 0.0f;
     } break;
     case BRDF_ModifiedPhong: {
         V R = reflect(L, N);
         f32 c = dot(-Vw, R);
-        *Spec = c > 0.0f ? powf(c, B->Exponent) : 0.0f;
+        *Spec = c > 0.0f ? PowF(c, B->Exponent) : 0.0f;
     } break;
     case BRDF_NormalizedModifiedPhong: {
         *Dif = (f32)(1.0f / kPi);
@@ -560,7 +569,7 @@ void evaluateBRDF(f32 *Dif, f32 *Spec, const brdf *B, V L, V N, V Vw, f32 cosI) 
         f32 c = dot(-Vw, R);
         if (c > 0.0f) {
             f32 e = B->Exponent;
-            *Spec = ((e + 2.0f) / (2.0f * kPiF)) * powf(c, e);
+            *Spec = ((e + 2.0f) / (2.0f * kPiF)) * PowF(c, e);
         } else {
             *Spec = 0.0f;
         }
@@ -568,12 +577,12 @@ void evaluateBRDF(f32 *Dif, f32 *Spec, const brdf *B, V L, V N, V Vw, f32 cosI) 
     case BRDF_BlinnPhong: {
         V Hv = normalize(-Vw + L);
         f32 c = dot(N, Hv);
-        *Spec = c > 0.0f ? powf(c, B->Exponent) / cosI : 0.0f;
+        *Spec = c > 0.0f ? PowF(c, B->Exponent) / cosI : 0.0f;
     } break;
     case BRDF_ModifiedBlinnPhong: {
         V Hv = normalize(-Vw + L);
         f32 c = dot(N, Hv);
-        *Spec = c > 0.0f ? powf(c, B->Exponent) : 0.0f;
+        *Spec = c > 0.0f ? PowF(c, B->Exponent) : 0.0f;
     } break;
     case BRDF_NormalizedModifiedBlinnPhong: {
         *Dif = (f32)(1.0f / kPi);
@@ -581,7 +590,7 @@ void evaluateBRDF(f32 *Dif, f32 *Spec, const brdf *B, V L, V N, V Vw, f32 cosI) 
         f32 c = dot(N, Hv);
         if (c > 0.0f) {
             f32 e = B->Exponent;
-            *Spec = ((e + 8.0f) / (8.0f * kPiF)) * powf(c, e);
+            *Spec = ((e + 8.0f) / (8.0f * kPiF)) * PowF(c, e);
         } else {
             *Spec = 0.0f;
         }
@@ -592,7 +601,7 @@ void evaluateBRDF(f32 *Dif, f32 *Spec, const brdf *B, V L, V N, V Vw, f32 cosI) 
         f32 c = dot(N, Hv);
         if (c > 0.0f) {
             f32 e = B->Exponent;
-            f32 ct = powf(c, e);
+            f32 ct = PowF(c, e);
             f32 D = ((e + 2.0f) / (2.0f * kPiF)) * ct;
             f32 nw0 = dot(N, -Vw);
             f32 w0wh = dot(-Vw, Hv);
@@ -601,7 +610,7 @@ void evaluateBRDF(f32 *Dif, f32 *Spec, const brdf *B, V L, V N, V Vw, f32 cosI) 
             G = minf_(1.0f, G);
             f32 ri = B->RefractiveIndex;
             f32 R0 = sq(ri - 1.0f) / sq(ri + 1.0f);
-            f32 F = R0 + (1.0f - R0) * powf(1.0f - w0wh, 5.0f);
+            f32 F = R0 + (1.0f - R0) * PowF(1.0f - w0wh, 5.0f);
             *Spec = (D * F * G) / (4.0f * cosI * nw0);
         } else {
             *Spec = 0.0f;
@@ -669,7 +678,7 @@ void calculateLighting(const Env &X, V *DC, V *SC, const Ray &ray, const brdf *B
             else if (ct > cf) falloff = 1.0f;
             else {
                 falloff = (ct - ch) / (cf - ch);
-                falloff = powf(falloff, 4.0f);
+                falloff = PowF(falloff, 4.0f);
                 if (falloff < 0.0f) falloff = 0.0f;
                 else if (falloff > 1.0f) falloff = 1.0f;
             }
@@ -697,16 +706,16 @@ void calculateLighting(const Env &X, V *DC, V *SC, const Ray &ray, const brdf *B
     }
     if (S->UseEnvironmentMap) {
         f32 phi = 2.0f * kPiF * rng.next();
-        f32 cp = cosf(phi), sp = sinf(phi);
+        f32 cp = CosF(phi), sp = SinF(phi);
         f32 theta = Asin(sqrtf(rng.next()));
-        f32 ct = cosf(theta), st = sinf(theta);
+        f32 ct = CosF(theta), st = SinF(theta);
         V U, Vv;
         orthonormalBasis(hn, &U, &Vv);
         V L = normalize(U * st * cp + Vv * st * sp + hn * ct);
         Ray er = {hp + (L * shadowEps), L, ray.T};
         Hit eh;
         if (!closest(X.C, &eh, er, false, 0.0f)) {
-            f32 u = (-atan2f(L.z, L.x) + kPiF) / (2.0f * kPiF);
+            f32 u = (-Atan2F(L.z, L.x) + kPiF) / (2.0f * kPiF);
             f32 v = Acos(L.y) / kPiF;
             V I = sampleTexture(&S->EnvironmentMap, mk(u, v, 1.0f), u, v);
             f32 ca = dot(L, hn);
@@ -770,7 +779,7 @@ V transparencyContribution(const Env &X, const material *Mt, const Ray &ray, V h
         f32 cosAir = inside ? cosPhi : cosT;
         f32 inv = 1.0f - cosAir;
         f32 r0 = sq((from - to) / (from + to));
-        refl = r0 + (1.0f - r0) * powf(inv, 5);
+        refl = r0 + (1.0f - r0) * PowF(inv, 5);
         V T = (ray.d + N * cosT) * ratio - N * cosPhi;
         T = normalize(T);
         const f32 refrEps = 0.000100f;
@@ -834,9 +843,9 @@ V traceRay(const Env &X, Ray ray, u32 depth, bool inside, f32 *travel, u32 node)
                 calculateLighting(X, &DC, &SC, ray, B, hp, hn, X.eps->ShadowRay, inside, rng);
                 if (S->PathTrace) {
                     f32 theta = S->ImportanceSampling ? Asin(sqrtf(rng.next())) : Acos(rng.next());
-                    f32 ct = cosf(theta), st = sinf(theta);
+                    f32 ct = CosF(theta), st = SinF(theta);
                     f32 phi = 2.0f * kPiF * rng.next();
-                    f32 cp = cosf(phi), sp = sinf(phi);
+                    f32 cp = CosF(phi), sp = SinF(phi);
                     V U, Vv;
                     orthonormalBasis(hn, &U, &Vv);
                     V s = normalize(U * st * cp + Vv * st * sp + hn * ct);
@@ -980,7 +989,7 @@ void produceRows(const Ctx &C, const camera *Cam, const epsilon_values *eps, fil
                     if (travel == 0.0f) {                                        // main.cpp:1180-1207
                         if (S->UseEnvironmentMap) {
                             f32 th = Acos(ray.d.y);
-                            f32 ph = atan2f(ray.d.z, ray.d.x);
+                            f32 ph = Atan2F(ray.d.z, ray.d.x);
                             f32 u = (-ph + kPiF) / (2.0f * kPiF), v = th / kPiF;
                             col3 = sampleTexture(&S->EnvironmentMap, mk(u, v, 1.0f), u, v);
                         } else if (S->UseBackgroundTexture) {

@@ -58,8 +58,7 @@ def test_background_texture_and_image_bump(gpu):
     for filt in (N.FilterType_Box, N.FilterType_Gaussian):
         got = gpu.render(cam, filt=filt, seed=17)
         want = oracle.render(sc, cam, filt=filt, seed=17)
-        compare(got, want, f"features, background texture and image bump, filter {filt}",
-                pix_frac=0.99 if filt == N.FilterType_Box else 0.98)
+        compare(got, want, f"features, background texture and image bump, filter {filt}")
 
 
 def test_cornell_box_full_size(gpu):
@@ -99,12 +98,14 @@ def test_render_scene_drop_in(gpu):
 
 
 def test_render_memory_limit_halves_the_chunk_then_fails(gpu):
-    sc, cam = configs.cornell_scene(64, 64, spp=16, max_depth=8)
+    # Depth 3: the path queues of the deepest level hold about twice the
+    # camera rays (dielectric splits, mirror children), some 60 MB for the
+    # render's single 16-spp chunk and under 4 MB at 1 spp.
+    sc, cam = configs.cornell_scene(64, 64, spp=16, max_depth=3)
     gpu.upload_scene(sc)
     free = gpu.render(cam, seed=3)
     try:
-        # 8 MB holds the queues of 1-2 samples per pixel, not the 16 of one chunk.
-        gpu.set_render_memory_limit(8 << 20)
+        gpu.set_render_memory_limit(12 << 20)
         capped = gpu.render(cam, seed=3)
         assert np.array_equal(capped["TotalWeight"], free["TotalWeight"])
         np.testing.assert_allclose(capped["Color"], free["Color"], rtol=2e-6, atol=1e-7)

@@ -20,7 +20,7 @@ def test_interior_reduced(gpu):
     gpu.upload_scene(sc)
     got = gpu.render(cam, filt=N.FilterType_Gaussian, seed=11)
     want = oracle.render(sc, cam, filt=N.FilterType_Gaussian, seed=11)
-    compare(got, want, "interior 40x24 4spp gaussian", pix_frac=0.98)
+    compare(got, want, "interior 40x24 4spp gaussian")
 
 
 def test_motion_reduced(gpu):
