@@ -68,6 +68,11 @@ struct Affine {
 
 // Top-level item: a mesh instance (BLAS reference) or a sphere.
 enum : uint32_t { kItemInstance = 0u, kItemSphere = 1u };
+// Set in an instance's kind when its mesh has triangles lying in a face plane
+// of its own box (quads, walls, boxes): hits there lie on the box boundary,
+// where the reference's box test on the instance (main.cpp:245) depends on
+// rounding, so the walk repeats that test on such an instance's hits.
+constexpr uint32_t kItemBoxTest = 0x100u;
 
 struct alignas(16) TopItem {
     uint32_t kind;

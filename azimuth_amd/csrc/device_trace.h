@@ -124,6 +124,25 @@ __device__ inline bool ref_box_hit(const float *b, float ox, float oy, float oz,
     return en <= fa && fa > 0.0f;
 }
 
+// Whether the reference's box test on an instance (main.cpp:245) passes for
+// a ray whose hit on that instance is at world t w.  A hit point inside the
+// box by a margin in every dimension of non-zero extent passes the slab test
+// whatever the rounding of the slab distances (its t lies inside every slab
+// interval; a dimension of zero extent gives the interval [t, t] at the hit's
+// own plane crossing); nearer the boundary the test itself decides.
+__device__ inline bool ref_box_accepts(const float *b, float ox, float oy, float oz, float dx, float dy, float dz,
+                                       float w) {
+    const float p[3] = {ox + dx * w, oy + dy * w, oz + dz * w}, o[3] = {ox, oy, oz};
+    bool inside = true;
+    for (int k = 0; k < 3; ++k) {
+        const float lo = b[k], hi = b[3 + k];
+        if (lo == hi) continue;
+        const float m = 1e-5f * ((hi - lo) + fabsf(p[k] - o[k]) + fabsf(lo) + fabsf(hi));
+        inside = inside && p[k] > lo + m && p[k] < hi - m;
+    }
+    return inside || ref_box_hit(b, ox, oy, oz, dx, dy, dz);
+}
+
 // The reference picks the closest hit by a recursive minimum that keeps the
 // right child on equal t (main.cpp:303-307), so among exactly equal t the
 // right-most leaf of its BVH wins.  Ranks order the leaves left to right:
@@ -282,6 +301,8 @@ struct TraceCounters {
 //   bool fetch(uint32_t id, Ray3 &r, float &tMax)   -> true: any-hit ray
 //   void finish(uint32_t id, bool found, const HitRec &h)
 // Every lane of the wave must call it (it ends when the whole range is done).
+// ldsLane is the lane's column of a [RING + 1][BLOCK] LDS array: RING stack
+// slots and one word for an excluded instance.
 template <int MODE, bool COUNT, int RING, int BLOCK, bool WW, class Pool>
 __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t begin, uint32_t end, float eps,
                                   bool cull, uint32_t *ldsLane, TraceCounters &cnt) {
@@ -292,6 +313,11 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
     bool have = false;
     Ray3 r{};
     bool any = MODE == 1;
+    // An instance whose hit the reference's own box test rejected: the walk
+    // of the ray is redone once without it.  Kept in the lane's LDS word
+    // after its ring (read only at top-level instance leaves), not in a
+    // register, so that the walk's register budget is unchanged.
+    uint32_t *const excl = ldsLane + RING * BLOCK;
     float dirLen = 1.0f, bestW = kFltMax;
     uint32_t bestRank = 0;
     bool found = false;
@@ -321,6 +347,7 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
                 const bool a = pool.fetch(id, r, tMax);
                 any = MODE == 2 ? a : MODE == 1;
                 dirLen = len3(r.dx, r.dy, r.dz);
+                *excl = kEmpty;
                 bestW = any ? tMax : kFltMax;
                 bestRank = any ? kEmpty : 0u;       // any: a hit must be strictly below tMax
                 found = false;
@@ -378,7 +405,7 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
         } else if (cur == kEmpty) {
             // Leaving a BLAS: convert its closest local hit (main.cpp:256-276).
             if (!any && localPrim != kEmpty) {
-                const TopItem &it = G.items[curItem];
+                const TopItem &it = G.items[curItem & 0x7FFFFFFFu];
                 float px = ox + (dx * localBest), py = oy + (dy * localBest), pz = oz + (dz * localBest);
                 float w = world_t(it.fwd, px, py, pz, mx, my, mz, sox, soy, soz, dirLen);
                 if (w > eps && closer(w, it.rank, bestW, bestRank)) {
@@ -435,7 +462,7 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
                     out.gamma = 0.0f;
                     if (any) finished = true;
                 }
-            } else if (ref_box_hit(G.refBoxes + 6 * (size_t)itemIdx, r.ox, r.oy, r.oz, r.dx, r.dy, r.dz)) {
+            } else if (itemIdx != *excl) {
                 mx = r.time * it.motion[0];
                 my = r.time * it.motion[1];
                 mz = r.time * it.motion[2];
@@ -448,7 +475,8 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
                 iz = 1.0f / dz;
                 np = near_planes(ix, iy, iz);
                 inBlas = true;
-                curItem = itemIdx;
+                // Bit 31 marks an instance whose hits take the reference's box test.
+                curItem = itemIdx | ((it.kind & kItemBoxTest) ? 0x80000000u : 0u);
                 // Local distances scale with |M^-1 d|; the bound is widened so
                 // that rounding never prunes a hit that could still win.
                 localBest = bestW >= kFltMax ? kFltMax : bestW * dscale * 1.00001f + 1e-30f;
@@ -470,7 +498,7 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
                 if (tri_test(tp, ox, oy, oz, dx, dy, dz, eps, t, b, g) &&
                     (any ? t < localBest : closer(t, tp.prim, localBest, localPrim))) {
                     if (any) {
-                        const TopItem &it = G.items[curItem];
+                        const TopItem &it = G.items[curItem & 0x7FFFFFFFu];
                         float px = ox + (dx * t), py = oy + (dy * t), pz = oz + (dz * t);
                         float w = world_t(it.fwd, px, py, pz, mx, my, mz, sox, soy, soz, dirLen);
                         if (w > eps && w < bestW) {
@@ -497,7 +525,38 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
             if (sp == 0) finished = true;
             else cur = stack.pop(sp);
         }
+        if (finished && found && (out.item >> 31) && *excl == kEmpty &&
+            !ref_box_accepts(G.refBoxes + 6 * (size_t)(out.item & 0x7FFFFFFFu), r.ox, r.oy, r.oz, r.dx, r.dy, r.dz,
+                             out.t)) {
+            // The reference never enters this instance: walk the ray again
+            // without it.
+            *excl = out.item & 0x7FFFFFFFu;
+            finished = false;
+            float tMax = kFltMax;
+            (void)pool.fetch(id, r, tMax);
+            bestW = any ? tMax : kFltMax;
+            bestRank = any ? kEmpty : 0u;
+            found = false;
+            out.prim = kEmpty;
+            out.item = kEmpty;
+            ox = r.ox;
+            oy = r.oy;
+            oz = r.oz;
+            dx = r.dx;
+            dy = r.dy;
+            dz = r.dz;
+            ix = 1.0f / dx;
+            iy = 1.0f / dy;
+            iz = 1.0f / dz;
+            np = near_planes(ix, iy, iz);
+            inBlas = false;
+            localBest = kFltMax;
+            localPrim = kEmpty;
+            sp = 0;
+            cur = G.tlasRoot;
+        }
         if (finished) {
+            out.item &= 0x7FFFFFFFu;
             pool.finish(id, found, out);
             have = false;
         }

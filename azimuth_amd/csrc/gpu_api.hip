@@ -231,6 +231,26 @@ az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
     }
     info.triTotal = primBase;
 
+    // Meshes with a triangle in a face plane of their own box (all three
+    // vertices on the box's min or max in one axis).
+    std::vector<uint8_t> meshFaceOnBox(S->MeshCount, 0);
+    parallel_for(S->MeshCount, [&](size_t m) {
+        const std::vector<const entity *> &tl = info.meshTriEntities[m];
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        auto axis = [](const vec3 &v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); };
+        for (const entity *e : tl)
+            for (const vec3 *p : {&e->P0, &e->P1, &e->P2})
+                for (int k = 0; k < 3; ++k) {
+                    lo[k] = std::min(lo[k], axis(*p, k));
+                    hi[k] = std::max(hi[k], axis(*p, k));
+                }
+        for (const entity *e : tl)
+            for (int k = 0; k < 3 && !meshFaceOnBox[m]; ++k) {
+                const float a = axis(e->P0, k), b = axis(e->P1, k), c = axis(e->P2, k);
+                if (a == b && b == c && (a == lo[k] || a == hi[k])) meshFaceOnBox[m] = 1;
+            }
+    });
+
     // ---- top level ------------------------------------------------------------
     std::vector<const entity *> tops;
     collectLeaves(S->BVHRoot, tops);
@@ -263,6 +283,7 @@ az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
             }
             it.index = mi;
             it.blasRoot = info.meshBlasRoot[mi];
+            if (meshFaceOnBox[mi]) it.kind |= kItemBoxTest;
             vec3 p[2] = {e->AABB.Min, e->AABB.Max};
             b = boxOfPoints(p, 2);
         } else {
@@ -356,7 +377,7 @@ az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, UploadInfo &info) {
     C->stats.SphereCount = 0;
     C->stats.InstanceCount = 0;
     for (const TopItem &it : items) {
-        if (it.kind == kItemSphere) C->stats.SphereCount++;
+        if ((it.kind & 0xFFu) == kItemSphere) C->stats.SphereCount++;
         else C->stats.InstanceCount++;
     }
     C->stats.NodeBytes = sizeof(Node4);

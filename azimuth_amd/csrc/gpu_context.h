@@ -116,6 +116,7 @@ struct az_gpu_ctx {
     az_guide *guide = nullptr;       // owned; the learned guide of guided renders (environment NEE)
     az_guide *scatterGuide = nullptr;   // owned; the learned guide of the hemisphere scatter
     az_stats stats{};
+    az_render_stats renderStats{};   // the last render's counts (while counting)
     bool recordLaunches = false;
     std::vector<hipEvent_t> launchEvents;   // pairs: start, stop
     size_t launchUsed = 0;                  // pairs in use since the last collect

@@ -213,17 +213,30 @@ struct ExtendPool {
     }
 };
 
-template <int BLOCK, bool WW>
+// A wave's traversal counters added to counters[0..2] (nodes, triangles,
+// top-level items) with one atomic each.
+__device__ inline void add_render_counters(unsigned long long *counters, const TraceCounters &c) {
+    const unsigned long long v[3] = {c.nodes, c.tris, c.prims};
+    for (int k = 0; k < 3; ++k) {
+        unsigned long long x = v[k];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+        if ((threadIdx.x & 63) == 0) atomicAdd(counters + k, x);
+    }
+}
+
+template <int BLOCK, bool WW, bool COUNT = false>
 __device__ __forceinline__ void extend_body(const DeviceGeometry &G, const PathRay *__restrict__ q, SegIn qin, uint32_t n,
-                                   PathHit *__restrict__ hits, float eps, int cull, uint32_t perWave, uint32_t block) {
-    __shared__ uint32_t smem[kRenderRing * BLOCK];
+                                   PathHit *__restrict__ hits, float eps, int cull, uint32_t perWave, uint32_t block,
+                                   unsigned long long *counters = nullptr) {
+    __shared__ uint32_t smem[(kRenderRing + 1) * BLOCK];
     __shared__ uint32_t soff[kSegs + 1];
     const uint32_t *off = stage_offsets(qin, soff);
     const uint32_t begin = (block * (BLOCK / 64) + (threadIdx.x >> 6)) * perWave;
     const uint32_t end = min(begin + perWave, n);
     ExtendPool pool{q, off, qin, hits};
     TraceCounters c{0u, 0u, 0u};
-    trace_pool<0, false, kRenderRing, BLOCK, WW>(G, pool, begin, end, eps, cull != 0, smem + threadIdx.x, c);
+    trace_pool<0, COUNT, kRenderRing, BLOCK, WW>(G, pool, begin, end, eps, cull != 0, smem + threadIdx.x, c);
+    if (COUNT) add_render_counters(counters, c);
 }
 
 // Rays per wave for a level whose count is read on the device (nDev), over
@@ -248,6 +261,16 @@ __global__ __launch_bounds__(BLOCK, WAVES) void k_extend(DeviceGeometry G, const
         if ((uint64_t)blockIdx.x * (BLOCK / 64) * perWave >= n) return;
     }
     extend_body<BLOCK, WW>(G, q, qin, n, hits, eps, cull, perWave, blockIdx.x);
+}
+
+// The same walk counting nodes, triangles and items (untimed renders that
+// measure a render's algorithmic bytes; host-driven levels only).
+template <int BLOCK, int WAVES, bool WW>
+__global__ __launch_bounds__(BLOCK, WAVES) void k_extend_count(DeviceGeometry G, const PathRay *__restrict__ q,
+                                                              SegIn qin, uint32_t n, PathHit *__restrict__ hits,
+                                                              float eps, int cull, uint32_t perWave,
+                                                              unsigned long long *__restrict__ counters) {
+    extend_body<BLOCK, WW, true>(G, q, qin, n, hits, eps, cull, perWave, blockIdx.x, counters);
 }
 
 // ---- connect: shadow rays ------------------------------------------------------
@@ -288,18 +311,20 @@ struct ConnectPool {
     }
 };
 
-template <int BLOCK, bool WW>
+template <int BLOCK, bool WW, bool COUNT = false>
 __device__ __forceinline__ void connect_body(const DeviceGeometry &G, const ShadowRay *__restrict__ sq, SegIn sin, uint32_t n,
                                     AccWord *__restrict__ acc, GuideSample *__restrict__ recs,
-                                    ScatterLink *__restrict__ links, uint32_t perWave, uint32_t block) {
-    __shared__ uint32_t smem[kRenderRing * BLOCK];
+                                    ScatterLink *__restrict__ links, uint32_t perWave, uint32_t block,
+                                    unsigned long long *counters = nullptr) {
+    __shared__ uint32_t smem[(kRenderRing + 1) * BLOCK];
     __shared__ uint32_t soff[kSegs + 1];
     const uint32_t *off = stage_offsets(sin, soff);
     const uint32_t begin = (block * (BLOCK / 64) + (threadIdx.x >> 6)) * perWave;
     const uint32_t end = min(begin + perWave, n);
     ConnectPool pool{sq, off, sin, &G, acc, recs, links};
     TraceCounters c{0u, 0u, 0u};
-    trace_pool<2, false, kRenderRing, BLOCK, WW>(G, pool, begin, end, 0.0f, false, smem + threadIdx.x, c);
+    trace_pool<2, COUNT, kRenderRing, BLOCK, WW>(G, pool, begin, end, 0.0f, false, smem + threadIdx.x, c);
+    if (COUNT) add_render_counters(counters, c);
 }
 
 template <int BLOCK, int WAVES, bool WW>
@@ -316,14 +341,33 @@ __global__ __launch_bounds__(BLOCK, WAVES) void k_connect(DeviceGeometry G, cons
     connect_body<BLOCK, WW>(G, sq, sin, n, acc, recs, links, perWave, blockIdx.x);
 }
 
+template <int BLOCK, int WAVES, bool WW>
+__global__ __launch_bounds__(BLOCK, WAVES) void k_connect_count(DeviceGeometry G, const ShadowRay *__restrict__ sq,
+                                                               SegIn sin, uint32_t n, AccWord *__restrict__ acc,
+                                                               GuideSample *__restrict__ recs,
+                                                               ScatterLink *__restrict__ links, uint32_t perWave,
+                                                               unsigned long long *__restrict__ counters) {
+    connect_body<BLOCK, WW, true>(G, sq, sin, n, acc, recs, links, perWave, blockIdx.x, counters);
+}
+
 // n is the level's ray count, or with nDev set the largest count the level
 // can hold (the count itself is read from nDev on the device).
 static hipError_t launch_extend(const RenderTuning &T, int cuCount, const DeviceGeometry &G, const PathRay *q,
                                 SegIn qin, uint32_t n, PathHit *hits, float eps, int cull, hipStream_t st,
-                                const uint32_t *nDev = nullptr) {
+                                const uint32_t *nDev = nullptr, unsigned long long *counters = nullptr) {
     // With the count on the device, the grid has a wave per 64 rays of the
     // largest count, so that every count gets the split it gets on the host.
     const uint32_t per = nDev ? 64u : rays_per_wave(n, cuCount, T.traceWaves);
+    if (counters) {   // counting: the 128-lane, 6-wave walk in the tuned loop form
+        const uint32_t perC = rays_per_wave(n, cuCount, 6), perBlock = 2 * perC;
+        if (T.traceLoop)
+            hipLaunchKernelGGL((k_extend_count<128, 6, true>), dim3((n + perBlock - 1) / perBlock), dim3(128), 0, st, G,
+                               q, qin, n, hits, eps, cull, perC, counters);
+        else
+            hipLaunchKernelGGL((k_extend_count<128, 6, false>), dim3((n + perBlock - 1) / perBlock), dim3(128), 0, st, G,
+                               q, qin, n, hits, eps, cull, perC, counters);
+        return hipGetLastError();
+    }
 #define AZ_CASE(B, W, F)                                                                                 \
     if (T.traceBlock == B && T.traceWaves == W && T.traceLoop == F) {                                  \
         const uint32_t perBlock = (B / 64) * per;                                                       \
@@ -338,8 +382,18 @@ static hipError_t launch_extend(const RenderTuning &T, int cuCount, const Device
 
 static hipError_t launch_connect(const RenderTuning &T, int cuCount, const DeviceGeometry &G, const ShadowRay *sq,
                                  SegIn sin, uint32_t n, AccWord *acc, GuideSample *recs, ScatterLink *links,
-                                 hipStream_t st, const uint32_t *nDev = nullptr) {
+                                 hipStream_t st, const uint32_t *nDev = nullptr, unsigned long long *counters = nullptr) {
     const uint32_t per = nDev ? 64u : rays_per_wave(n, cuCount, T.traceWaves);
+    if (counters) {
+        const uint32_t perC = rays_per_wave(n, cuCount, 6), perBlock = 2 * perC;
+        if (T.traceLoop)
+            hipLaunchKernelGGL((k_connect_count<128, 6, true>), dim3((n + perBlock - 1) / perBlock), dim3(128), 0, st, G,
+                               sq, sin, n, acc, recs, links, perC, counters);
+        else
+            hipLaunchKernelGGL((k_connect_count<128, 6, false>), dim3((n + perBlock - 1) / perBlock), dim3(128), 0, st,
+                               G, sq, sin, n, acc, recs, links, perC, counters);
+        return hipGetLastError();
+    }
 #define AZ_CASE(B, W, F)                                                                                  \
     if (T.traceBlock == B && T.traceWaves == W && T.traceLoop == F) {                                   \
         const uint32_t perBlock = (B / 64) * per;                                                        \
@@ -643,7 +697,9 @@ __device__ inline void reconstruct(const DevScene &S, const DeviceGeometry &G, c
             N = hn;
             if (tex->bump) {
                 float pyPi = kPiF * lhp.y, rPi = kPiF * it.radius;
-                V3 Pv = v3(CosF(phi) * pyPi, -SinF(theta) * rPi, SinF(phi) * pyPi);
+                float sph, cph;
+                SinCosF(phi, sph, cph);
+                V3 Pv = v3(cph * pyPi, -SinF(theta) * rPi, sph * pyPi);
                 V3 Pu = v3(twoPi * lhp.z, 0.0f, -twoPi * lhp.x);
                 N = bump(S, *tex, N, P, u, v, Pu, Pv);
             }
@@ -826,7 +882,8 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
                 float cosT = 1.0f + rng.uniform() * (cosMax - 1.0f);
                 float sinT = sqrtf(1.0f - sq(cosT));
                 float phi = 2.0f * kPiF * rng.uniform();
-                float cp_ = CosF(phi), sp_ = SinF(phi);
+                float cp_, sp_;
+                SinCosF(phi, sp_, cp_);
                 V3 W = normalize(Dv), U, Vv;
                 orthonormal_basis(W, U, Vv);
                 dl = normalize(U * sinT * cp_ + Vv * sinT * sp_ + W * cosT);
@@ -881,9 +938,11 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
             L = env_lum_sample(S, u0, u1, pBase);
         } else {
             float phi = 2.0f * kPiF * rng.uniform();
-            float cp_ = CosF(phi), sp_ = SinF(phi);
+            float cp_, sp_;
+            SinCosF(phi, sp_, cp_);
             float theta = Asin(sqrtf(rng.uniform()));
-            float ct = CosF(theta), st = SinF(theta);
+            float ct, st;
+            SinCosF(theta, st, ct);
             V3 U, Vv;
             orthonormal_basis(N, U, Vv);
             L = normalize(U * st * cp_ + Vv * st * sp_ + N * ct);
@@ -956,9 +1015,11 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
             Sd = sample_brdf_mixture(M, kd, N, -D, u0, u1, u2, pBase);
         } else {
             float theta = S.importance ? Asin(sqrtf(rng.uniform())) : Acos(rng.uniform());
-            float ct = CosF(theta), st = SinF(theta);
+            float ct, st;
+            SinCosF(theta, st, ct);
             float phi = 2.0f * kPiF * rng.uniform();
-            float cp_ = CosF(phi), sp_ = SinF(phi);
+            float cp_, sp_;
+            SinCosF(phi, sp_, cp_);
             V3 U, Vv;
             orthonormal_basis(N, U, Vv);
             Sd = normalize(U * st * cp_ + Vv * st * sp_ + N * ct);
@@ -1676,6 +1737,7 @@ struct RenderBuffers {
     DeviceBuffer queries, lobes, recs, batch;   // learned guide
     DeviceBuffer scatLobes, srecs, links, qrec0, qrec1;   // scatter guide: lobes, records, per-slot record of the queues
     DeviceBuffer recCum;                        // prefix of the training records' weights
+    DeviceBuffer workCounters;                  // counted renders: extend and connect walk counts (6 x u64)
     DeviceBuffer envCdf;                        // environment luminance distribution
     DeviceBuffer batchArgs, batchTotals;        // batched renders: per-scene arguments, queue totals
     // Levels replayed from a graph: the chunk's parameters and per-level
@@ -1709,7 +1771,7 @@ struct RenderBuffers {
     void forEachQueue(F f) {
         DeviceBuffer *bufs[] = {&q0,    &q1,        &hits,      &shadow, &acc,   &counters, &pixels,  &queries,
                                 &lobes, &recs,      &batch,     &scatLobes, &srecs, &links, &qrec0,  &qrec1,
-                                &recCum, &batchArgs, &batchTotals, &chunkDev, &levelTotals};
+                                &recCum, &batchArgs, &batchTotals, &chunkDev, &levelTotals, &workCounters};
         for (DeviceBuffer *b : bufs) f(*b);
     }
     RenderBuffers() {
@@ -2245,7 +2307,18 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
     // so the image never depends on the bounds.  The graph is recorded once
     // per bounce count, bounds and set of buffers, and replayed for every
     // chunk and later render with the same ones.
-    const bool levelGraph = C->levelMode == AZ_LEVELS_AUTO && !guided;
+    const bool counting = C->counting;
+    const bool levelGraph = C->levelMode == AZ_LEVELS_AUTO && !guided && !counting;
+    memset(&C->renderStats, 0, sizeof(C->renderStats));
+    unsigned long long *workCnt = nullptr;
+    if (counting) {
+        AZ_HIPR(C, B.workCounters.alloc(6 * sizeof(unsigned long long)));
+        workCnt = (unsigned long long *)B.workCounters.ptr;
+        hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, (uint32_t *)workCnt, 12u);
+        AZ_HIPR(C, hipGetLastError());
+        C->renderStats.Gaussian = Filter == FilterType_Gaussian ? 1u : 0u;
+        C->renderStats.Pixels = nPix;
+    }
     const u32 levels = S.maxDepth + 1u;
     auto boundKeyOf = [&](u32 n) {
         std::vector<uint8_t> k;
@@ -2400,6 +2473,10 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
         const bool scatTraining = scatGuide && training;
         int offA = 0;
         AccWord *acc = (AccWord *)B.acc.ptr;
+        if (counting) {
+            C->renderStats.CameraSamples += n;
+            C->renderStats.Chunks += 1;
+        }
         hipLaunchKernelGGL(k_generate, dim3((n + kRenderBlock - 1) / kRenderBlock), dim3(kRenderBlock), 0, st, S, cam,
                            cp, (PathRay *)bufA->ptr, acc, n);
         AZ_HIPR(C, hipGetLastError());
@@ -2467,7 +2544,8 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             PathRay *qa = (PathRay *)bufA->ptr, *qb = (PathRay *)bufB->ptr;
             dim3 grid(blocks);
             AZ_HIPR(C, launch_extend(C->renderTuning, C->cuCount, C->dg, qa, qin, count, (PathHit *)B.hits.ptr,
-                                     S.epsIntersection, (int)cp.cull, st));
+                                     S.epsIntersection, (int)cp.cull, st, nullptr, workCnt));
+            if (counting) C->renderStats.PathRays += count;
             GuideIO GI{nullptr, nullptr, recCnt, recSegCap, alpha};
             ShadeOut O{qb, (ShadowRay *)B.shadow.ptr, childCnt, shadowCnt, nseg, childSegCap, shadowSegCap};
             const bool guideLevel = o.GuideMaxDepth == 0 || depth <= o.GuideMaxDepth;
@@ -2532,7 +2610,9 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             if (hc[1]) {
                 SegIn sin{offShadow, nseg, shadowSegCap};
                 AZ_HIPR(C, launch_connect(C->renderTuning, C->cuCount, C->dg, (const ShadowRay *)B.shadow.ptr, sin,
-                                          hc[1], acc, (GuideSample *)B.recs.ptr, (ScatterLink *)B.links.ptr, st));
+                                          hc[1], acc, (GuideSample *)B.recs.ptr, (ScatterLink *)B.links.ptr, st,
+                                          nullptr, counting ? workCnt + 3 : nullptr));
+                if (counting) C->renderStats.ShadowRays += hc[1];
             }
             count = hc[0];
             offA = 1 - offA;
@@ -2640,6 +2720,17 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             ++trainRound;
         }
         s0 += spp;
+    }
+    if (counting) {
+        unsigned long long w[6];
+        AZ_HIPR(C, hipMemcpyAsync(w, workCnt, sizeof(w), hipMemcpyDeviceToHost, st));
+        AZ_HIPR(C, hipStreamSynchronize(st));
+        C->renderStats.ExtendNodes = w[0];
+        C->renderStats.ExtendTriangles = w[1];
+        C->renderStats.ExtendItems = w[2];
+        C->renderStats.ConnectNodes = w[3];
+        C->renderStats.ConnectTriangles = w[4];
+        C->renderStats.ConnectItems = w[5];
     }
     return AZ_OK;
 }
@@ -3085,6 +3176,12 @@ az_status az_gpu_env_luminance_pdf(az_gpu_ctx *C, const f32 *d_Dirs, u32 N, f32 
     hipLaunchKernelGGL(k_env_lum_pdf, dim3((N + kRenderBlock - 1) / kRenderBlock), dim3(kRenderBlock), 0, C->stream,
                        C->devScene, d_Dirs, N, d_Pdf);
     AZ_HIPR(C, hipGetLastError());
+    return AZ_OK;
+}
+
+az_status az_gpu_get_render_stats(az_gpu_ctx *C, az_render_stats *Out) {
+    if (!C || !Out) return AZ_ERR_INVALID_ARGUMENT;
+    *Out = C->renderStats;
     return AZ_OK;
 }
 

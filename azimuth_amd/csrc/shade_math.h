@@ -45,6 +45,12 @@ __device__ inline float Acos(float v) { return (float)acos((double)v); }        
 // directions and weights bit for bit.
 __device__ inline float SinF(float v) { return (float)sin((double)v); }
 __device__ inline float CosF(float v) { return (float)cos((double)v); }
+__device__ inline void SinCosF(float v, float &sv, float &cv) {   // Sin and Cos of one angle, one reduction
+    double sd, cd;
+    sincos((double)v, &sd, &cd);
+    sv = (float)sd;
+    cv = (float)cd;
+}
 __device__ inline float Atan2F(float y, float x) { return (float)atan2((double)y, (double)x); }
 __device__ inline float PowF(float x, float y) { return (float)pow((double)x, (double)y); }
 

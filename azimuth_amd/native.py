@@ -187,6 +187,12 @@ class az_stats(C.Structure):
                 ("TriangleCount", u32), ("SphereCount", u32), ("InstanceCount", u32)]
 
 
+class az_render_stats(C.Structure):
+    _fields_ = [("CameraSamples", u64), ("PathRays", u64), ("ShadowRays", u64), ("ExtendNodes", u64),
+                ("ExtendTriangles", u64), ("ExtendItems", u64), ("ConnectNodes", u64), ("ConnectTriangles", u64),
+                ("ConnectItems", u64), ("Pixels", u64), ("Gaussian", u32), ("Chunks", u32)]
+
+
 class az_tune_entry(C.Structure):
     _fields_ = [("Block", u32), ("LdsStackEntries", u32), ("MinWaves", s32), ("ChunkSamples", u32),
                 ("LoopForm", u32), ("BvhWidth", u32), ("MaxLeafItems", u32), ("Reserved", u32),
@@ -244,6 +250,7 @@ EXPORTS = {
     "az_gpu_output_pixels": (s32, [_P, _P, u32, u32, C.POINTER(tonemap), f32, u32, _P]),
     "az_gpu_sum_f32_sequential": (s32, [_P, _P, u32, _P]),
     "az_gpu_set_counting": (s32, [_P, u32]),
+    "az_gpu_get_render_stats": (s32, [_P, C.POINTER(az_render_stats)]),
     "az_gpu_set_launch": (s32, [_P, u32, u32, s32, s32]),
     "az_gpu_autotune_trace": (s32, [_P, _P, u32, u32, _P]),
     "az_gpu_render_batch": (s32, [_P, _P, u32, s32, u32, _P, _P]),

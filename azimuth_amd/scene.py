@@ -294,6 +294,12 @@ class GpuContext:
         N.check(self.lib.az_gpu_get_stats(self.h, C.byref(s)), "az_gpu_get_stats", self.h)
         return s
 
+    def render_stats(self):
+        """Counts of the last render while counting is on (az_render_stats)."""
+        r = N.az_render_stats()
+        N.check(self.lib.az_gpu_get_render_stats(self.h, C.byref(r)), "az_gpu_get_render_stats", self.h)
+        return {name: getattr(r, name) for name, _ in r._fields_}
+
     def set_counting(self, on):
         N.check(self.lib.az_gpu_set_counting(self.h, 1 if on else 0), "az_gpu_set_counting", self.h)
 

@@ -10,11 +10,22 @@ value is closest-hit Mrays/s over both batches.
 
 Before timing, the per-scene autotuner picks the traversal variant (LDS
 stack entries, workgroup size, register target) on both batches.
-Also reported: the path tracer's throughput on the architectural interior
-(render_interior), the live roofline of the traversal kernel (algorithmic bytes
-per launch from kernel-counted node visits and triangle tests, over the
-launch's HIP-event duration), any-hit rates, and the oracle's reference
-traversal timed on the host cores as a CPU baseline.
+Also reported:
+- roofline: the traversal kernel's algorithmic bytes per launch (32 B ray in,
+  16 B hit out, 128 B per node visited, 48 B per triangle tested, from an
+  untimed counting launch) over the launch's HIP-event duration;
+- any-hit rates;
+- render_interior: the path tracer on the architectural interior
+  (1920x1080, 64 spp, Gaussian filter) with its own roofline fraction from a
+  counted render's algorithmic bytes; render_other: the Cornell box and one
+  motion-blur scene;
+- equal_time_mse: the metric's second half, MSE against a 4096-spp
+  double-precision reference of the guided and the plain tracer at equal
+  wall-clock on the guided interior, beside the environment luminance
+  distribution (the strongest sampler that does not learn);
+- cpu_baseline / cpu_renders: the oracle's reference traversal and recursive
+  renderer timed on the host cores (the interior and motion renders
+  extrapolated from a strip of rows at 1 spp).
 """
 import argparse
 import json
@@ -47,7 +58,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile", action="store_true", help="short run for profilers: no CPU baseline, no any-hit")
-    ap.add_argument("--no-render", action="store_true", help="skip the interior render line")
+    ap.add_argument("--no-render", action="store_true", help="skip the render lines")
+    ap.add_argument("--no-mse", action="store_true", help="skip the equal-time MSE leg (about a minute)")
     ap.add_argument("--tune-build", action="store_true",
                     help="also autotune the BVH build (node width 2-4 x leaf size 2/4/8, one upload each)")
     args = ap.parse_args()
@@ -128,12 +140,16 @@ def main():
     rays_per_step = n_coh + n_inc
     value = rays_per_step * args.steps / (total_ms * 1e-3) / 1e6
 
-    def alg_bytes(name, n):
+    def alg_bytes(name, n, items=False):
         nodes, tris, prims = work[name]
-        return n * (RAY_BYTES + HIT_BYTES) + nodes * node_b + tris * tri_b + prims * 144
+        return n * (RAY_BYTES + HIT_BYTES) + nodes * node_b + tris * tri_b + (prims * 144 if items else 0)
 
     b_coh, b_inc = alg_bytes("coherent", n_coh), alg_bytes("incoherent", n_inc)
     achieved = (b_coh + b_inc) / ((ms_coh + ms_inc) * 1e-3) / 1e9
+    # The same with the 144-byte top-level item record each walk reads (not
+    # in the per-ray formula above).
+    achieved_items = (alg_bytes("coherent", n_coh, True) + alg_bytes("incoherent", n_inc, True)) / \
+        ((ms_coh + ms_inc) * 1e-3) / 1e9
 
     extra = {
         "coherent_mrays": n_coh / (ms_coh * 1e-3) / 1e6,
@@ -144,6 +160,7 @@ def main():
         "tris_per_ray": {k: work[k][1] / n for k, n in (("coherent", n_coh), ("incoherent", n_inc))},
         "coherent_gbps": b_coh / (ms_coh * 1e-3) / 1e9,
         "incoherent_gbps": b_inc / (ms_inc * 1e-3) / 1e9,
+        "achieved_gbps_with_item_records": achieved_items,
     }
 
     if args.profile:
@@ -190,8 +207,16 @@ def main():
     cpu = None
     if not (args.no_cpu_baseline or args.profile):
         cpu = cpu_baseline(sc, coh, inc, args.cpu_seconds)
+        log(f"cpu baseline: {cpu}")
     if not (args.profile or args.no_render):
         extra["render_interior"] = render_interior(ctx)
+        log(f"render interior: {extra['render_interior']}")
+        extra["render_other"] = render_other(ctx)
+    if not (args.profile or args.no_render or args.no_mse):
+        extra["equal_time_mse"] = equal_time_mse(ctx)
+    if not (args.no_cpu_baseline or args.profile):
+        extra["cpu_renders"] = cpu_renders()
+        log(f"cpu renders: {extra['cpu_renders']}")
 
     line = {
         "metric": "Mrays/s closest-hit; equal-time MSE vs 4096spp ref, guided vs plain (1 GPU)",
@@ -230,11 +255,51 @@ def main():
     ctx.close()
 
 
+def cpu_threads():
+    """Host threads for the CPU baselines: the cores this process may run
+    on, at most the host share of a one-GPU box (OMP_NUM_THREADS there)."""
+    n = len(os.sched_getaffinity(0))
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return max(1, min(n, cap) if cap > 0 else n), n
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def render_bytes(st, node_b, tri_b, item_b=144):
+    """Algorithmic bytes of a render from its counts (az_render_stats): camera
+    rays written (64 B) with their accumulators (24 B); per traced path ray
+    64 B in and a 32 B hit out plus the node, triangle and item records its
+    walk fetched; per shading point the ray, the hit and 112 B of attributes
+    in, 64 B per child and per shadow ray out; per shadow ray 64 B in, its
+    walk's records and a 24 B accumulator add; the splat reads 24 B per
+    sample (31 % more with the Gaussian filter's halo) and 32 B per pixel and
+    chunk."""
+    gen = 88 * st["CameraSamples"]
+    ext = 96 * st["PathRays"] + node_b * st["ExtendNodes"] + tri_b * st["ExtendTriangles"] + item_b * st["ExtendItems"]
+    shade = 208 * st["PathRays"] + 64 * (st["PathRays"] - st["CameraSamples"]) + 64 * st["ShadowRays"]
+    conn = 88 * st["ShadowRays"] + node_b * st["ConnectNodes"] + tri_b * st["ConnectTriangles"] + \
+        item_b * st["ConnectItems"]
+    splat = 24 * st["CameraSamples"] * (1.31 if st["Gaussian"] else 1.0) + 32 * st["Pixels"] * st["Chunks"]
+    return {"generate": gen, "extend": ext, "shade": shade, "connect": conn, "splat": splat,
+            "total": gen + ext + shade + conn + splat}
+
+
 def render_interior(ctx):
     """Beside the traversal line: the full path tracer on the architectural
     interior (299,722 triangles, 8 spot lights, HDR sky with a sun, 1920x1080,
     64 spp, depth 6, Gaussian filter), autotuned, one timed render after a
-    warm-up render (HIP events around the render call)."""
+    warm-up render (HIP events around the render call); then an untimed
+    counted render of the same samples gives the render's algorithmic bytes,
+    whose rate over the timed render is its HBM roofline fraction."""
     from azimuth_amd import native as N, workloads
     sc, cam = workloads.interior_scene(1920, 1080, spp=64, max_depth=6)
     ctx.upload_scene(sc)
@@ -243,10 +308,99 @@ def render_interior(ctx):
     ctx.timer_start()
     ctx.render(cam, filt=N.FilterType_Gaussian, seed=1)
     ms = ctx.timer_stop()
+    ctx.set_counting(True)
+    ctx.render(cam, filt=N.FilterType_Gaussian, seed=1)
+    ctx.set_counting(False)
+    st = ctx.render_stats()
+    g = ctx.stats()
+    nb = render_bytes(st, g.NodeBytes, g.TriangleBytes)
+    achieved = nb["total"] / (ms * 1e-3) / 1e9
     e = entries[best]
     return {"msamples_per_s": round(1920 * 1080 * 64 / (ms * 1e-3) / 1e6, 2), "render_ms": round(ms, 2),
             "tuning": f'{e["block"]} lanes, {e["waves"]} waves, {"while-while" if e["loop"] else "if-if"}, '
-                      f'{e["chunk"]} samples per chunk'}
+                      f'{e["chunk"]} samples per chunk',
+            "work": {"path_rays": st["PathRays"], "shadow_rays": st["ShadowRays"],
+                     "nodes_per_path_ray": round(st["ExtendNodes"] / max(1, st["PathRays"]), 3),
+                     "tris_per_path_ray": round(st["ExtendTriangles"] / max(1, st["PathRays"]), 3),
+                     "nodes_per_shadow_ray": round(st["ConnectNodes"] / max(1, st["ShadowRays"]), 3),
+                     "tris_per_shadow_ray": round(st["ConnectTriangles"] / max(1, st["ShadowRays"]), 3)},
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                         "bytes_by_stage_gb": {k: round(v / 1e9, 3) for k, v in nb.items()}}}
+
+
+def render_other(ctx):
+    """The Cornell box (256x256, 16 spp, depth 8, box filter) and one motion-
+    blur scene (2,000 moving spheres and 327,680 triangles, 1280x720, 121 spp,
+    depth 10), default tuning, timed after a warm-up render."""
+    from azimuth_amd import configs, native as N, workloads
+    out = {}
+    for name, (sc, cam), spp_eff in (("cornell", configs.cornell_scene(256, 256, spp=16, max_depth=8), 16),
+                                     ("motion", workloads.motion_scene(1280, 720, spp=128, max_depth=10), 121)):
+        ctx.upload_scene(sc)
+        ctx.render(cam, seed=1)
+        ctx.timer_start()
+        ctx.render(cam, seed=1)
+        ms = ctx.timer_stop()
+        W, H = cam.Resolution.x, cam.Resolution.y
+        out[name] = {"msamples_per_s": round(W * H * spp_eff / (ms * 1e-3) / 1e6, 2), "render_ms": round(ms, 2)}
+        log(f"render {name}: {out[name]}")
+    return out
+
+
+def equal_time_mse(ctx, ref_spp=4096, spp=64):
+    """The guided-interior workload: MSE against a ref_spp plain render
+    accumulated in double precision, of the plain tracer at spp and of the
+    guided tracer (learned environment-direction guide, one-sample MIS at
+    fraction 0.5 against the reference's cosine sampling, trained on the
+    first 32 spp) at the largest square spp whose render time stays within
+    the plain render's; also the environment luminance distribution (the
+    strongest sampler that does not learn) at equal time, and the guide over
+    it.  Wall-clock times around each render call, guide training included."""
+    import math
+    from azimuth_amd import native as N, workloads
+    sc, cam = workloads.interior_scene(1920, 1080, spp=spp, max_depth=6)
+    ctx.upload_scene(sc)
+    filt = N.FilterType_Gaussian
+
+    def run(seed, n, guide=False, env=N.AZ_ENV_COSINE):
+        px = ctx.render_f64(cam, filt=filt, seed=seed, spp=n, guide=guide, env_sampling=env)
+        return px[:, :3] / np.maximum(px[:, 3:4], 1e-300), ctx.last_render_s
+
+    t0 = time.time()
+    acc = None
+    for i in range(ref_spp // 256):
+        px = ctx.render_f64(cam, filt=filt, seed=5000 + i, spp=256)
+        acc = px if acc is None else acc + px
+    ref = acc[:, :3] / np.maximum(acc[:, 3:4], 1e-300)
+    t_ref = time.time() - t0
+    log(f"reference {ref_spp} spp in {t_ref:.1f}s")
+    mse = lambda img: float(np.mean((img - ref) ** 2))
+
+    def equal_spp(guide, env, t_budget):
+        _, t_same = run(3, spp, guide, env)
+        s = int(math.isqrt(max(1, int(spp * t_budget / t_same))))
+        while s > 1 and run(3, s * s, guide, env)[1] > t_budget * 1.02:
+            s -= 1
+        if run(3, (s + 1) ** 2, guide, env)[1] <= t_budget * 1.02:
+            s += 1
+        return s * s
+
+    run(1, spp)                                           # warm-up
+    plain, t_plain = run(1, spp)
+    res = {"workload": "guided architectural interior, 1920x1080, depth 6, Gaussian filter",
+           "reference": {"spp": ref_spp, "accumulation": "f64", "seconds": round(t_ref, 1)},
+           "plain": {"spp": spp, "seconds": round(t_plain, 3), "mse": mse(plain)}}
+    for name, guide, env in (("guided", True, N.AZ_ENV_COSINE), ("luminance", False, N.AZ_ENV_LUMINANCE),
+                             ("guided_over_luminance", True, N.AZ_ENV_LUMINANCE)):
+        n = equal_spp(guide, env, t_plain)
+        img, t = run(1, n, guide, env)
+        res[name] = {"spp": n, "seconds": round(t, 3), "mse": mse(img)}
+        log(f"{name}: {res[name]}")
+    res["mse_ratio_plain_over_guided"] = round(res["plain"]["mse"] / res["guided"]["mse"], 3)
+    res["mse_ratio_luminance_over_guided_over_luminance"] = round(
+        res["luminance"]["mse"] / res["guided_over_luminance"]["mse"], 3)
+    return res
 
 
 def cpu_baseline(sc, coh, inc, budget_s):
@@ -254,7 +408,7 @@ def cpu_baseline(sc, coh, inc, budget_s):
     cores, on an equal number of coherent and incoherent rays sized to about
     budget_s seconds."""
     import oracle
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads, nproc = cpu_threads()
     probe = 4096
     sel_c = coh[:: max(1, coh.shape[0] // probe)][:probe]
     sel_i = inc[:probe]
@@ -270,8 +424,41 @@ def cpu_baseline(sc, coh, inc, budget_s):
     oracle.trace(sc, sel_i, threads=threads)
     dt = time.time() - t0
     return {"value": round(2 * m / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(), "nproc": nproc,
             "sample": f"{m} coherent (every {max(1, coh.shape[0] // m)}th) + {m} incoherent rays of the same "
                       f"batches, closest hit through the reference BVH without pruning, {dt:.1f}s"}
+
+
+def cpu_renders(rows=8):
+    """The oracle's recursive renderer (the reference's ProduceImageInLinearSpace
+    over RenderScene's scanline bands, through the POSIX platform table) on
+    the host cores: the full Cornell box, and 1 spp over `rows` rows of the
+    interior and motion-blur workloads, extrapolated linearly in rows and spp
+    to the full renders."""
+    import oracle
+    from azimuth_amd import configs, native as N, workloads
+    threads, _ = cpu_threads()
+    out = {}
+    sc, cam = configs.cornell_scene(256, 256, spp=16, max_depth=8)
+    t0 = time.time()
+    oracle.render(sc, cam, seed=1, threads=threads)
+    dt = time.time() - t0
+    out["cornell"] = {"msamples_per_s": round(256 * 256 * 16 / dt / 1e6, 3), "seconds": round(dt, 2),
+                      "sample": "full render, 256x256, 16 spp, depth 8"}
+    for name, make, full_spp, filt in (
+            ("interior", lambda: workloads.interior_scene(1920, 1080, spp=1, max_depth=6), 64, N.FilterType_Gaussian),
+            ("motion", lambda: workloads.motion_scene(1280, 720, spp=1, max_depth=10), 121, N.FilterType_Box)):
+        sc, cam = make()
+        W, H = cam.Resolution.x, cam.Resolution.y
+        r0 = H // 2 - rows // 2
+        t0 = time.time()
+        oracle.render(sc, cam, filt=filt, seed=1, spp=1, row_begin=r0, row_end=r0 + rows, threads=threads)
+        dt = time.time() - t0
+        rate = W * rows / dt
+        out[name] = {"msamples_per_s": round(rate / 1e6, 4), "extrapolated": True,
+                     "full_render_seconds": round(W * H * full_spp / rate, 1),
+                     "sample": f"{rows} rows x {W} pixels x 1 spp in {dt:.2f}s, extrapolated to {W}x{H}x{full_spp}"}
+    return {"cores": threads, "cpu": cpu_model(), "kind": "port", "renders": out}
 
 
 if __name__ == "__main__":

@@ -196,8 +196,24 @@ az_status az_gpu_output_pixels(az_gpu_ctx *Ctx, const pixel *d_Pixels, u32 Width
 az_status az_gpu_sum_f32_sequential(az_gpu_ctx *Ctx, const f32 *d_X, u32 N, f32 *d_Sum);
 
 az_status az_gpu_get_stats(az_gpu_ctx *Ctx, az_stats *Stats);
-/* Enables per-ray node/triangle counting in trace calls (slower). */
+/* Enables per-ray node/triangle counting in trace calls and renders
+ * (slower; a counted render launches its levels from the host). */
 az_status az_gpu_set_counting(az_gpu_ctx *Ctx, b32 Enable);
+
+/* What the last render of a context did, filled while counting is enabled
+ * (az_gpu_set_counting), zero otherwise: the work behind a render's
+ * algorithmic bytes.  No reference counterpart. */
+typedef struct {
+    u64 CameraSamples;      /* camera rays generated */
+    u64 PathRays;           /* rays the closest-hit stage traced, camera rays included */
+    u64 ShadowRays;         /* shadow rays the connect stage traced */
+    u64 ExtendNodes, ExtendTriangles, ExtendItems;      /* node visits, triangle tests, top-level item tests */
+    u64 ConnectNodes, ConnectTriangles, ConnectItems;
+    u64 Pixels;             /* pixels splatted */
+    u32 Gaussian;           /* 1: the Gaussian filter */
+    u32 Chunks;             /* chunks of camera samples */
+} az_render_stats;
+az_status az_gpu_get_render_stats(az_gpu_ctx *Ctx, az_render_stats *Stats);
 
 /* Traversal launch parameters, as az_gpu_autotune_trace chooses them: workgroup
  * size (64, 128, 256), LDS stack entries per lane (8, 16, 32: the top of

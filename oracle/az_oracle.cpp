@@ -74,6 +74,12 @@ inline f32 Acos(f32 v) { return (f32)acos((f64)v); }                            
 // about one case in 2^29), as the device does.
 inline f32 SinF(f32 v) { return (f32)sin((f64)v); }
 inline f32 CosF(f32 v) { return (f32)cos((f64)v); }
+inline void SinCosF(f32 v, f32 &sv, f32 &cv) {   // Sin and Cos of one angle (as the device computes them)
+    f64 sd, cd;
+    sincos((f64)v, &sd, &cd);
+    sv = (f32)sd;
+    cv = (f32)cd;
+}
 inline f32 Atan2F(f32 y, f32 x) { return (f32)atan2((f64)y, (f64)x); }
 inline f32 PowF(f32 x, f32 y) { return (f32)pow((f64)x, (f64)y); }
 inline f32 minf_(f32 a, f32 b) { return a < b ? a : b; }                        // math.h:6
@@ -362,7 +368,9 @@ bool intersect(const Ctx &C, Hit *H, Ray ray, const entity *E, bool cull, f32 ep
                         tex = S->Textures + (E->TextureIndex - 1);
                         if (tex->BumpMap) {
                             f32 pyPi = kPiF * lhp.y, rPi = kPiF * radius;
-                            Pv = mk(CosF(phi) * pyPi, -SinF(theta) * rPi, SinF(phi) * pyPi);
+                            f32 sph, cph;
+                            SinCosF(phi, sph, cph);
+                            Pv = mk(cph * pyPi, -SinF(theta) * rPi, sph * pyPi);
                             Pu = mk(twoPi * lhp.z, 0.0f, -twoPi * lhp.x);
                         } else {
                             tex = nullptr;
@@ -526,7 +534,8 @@ LightSample sampleEntityLight(const entity *LE, V hitP, VertexRng &rng) {       
         f32 cosT = 1.0f + rng.next() * (cosMax - 1.0f);
         f32 sinT = sqrtf(1.0f - sq(cosT));
         f32 phi = 2.0f * kPiF * rng.next();
-        f32 cp = CosF(phi), sp = SinF(phi);
+        f32 cp, sp;
+        SinCosF(phi, sp, cp);
         V W = normalize(D), U, Vv;
         orthonormalBasis(W, &U, &Vv);
         dl = normalize(U * sinT * cp + Vv * sinT * sp + W * cosT);
@@ -706,9 +715,11 @@ void calculateLighting(const Env &X, V *DC, V *SC, const Ray &ray, const brdf *B
     }
     if (S->UseEnvironmentMap) {
         f32 phi = 2.0f * kPiF * rng.next();
-        f32 cp = CosF(phi), sp = SinF(phi);
+        f32 cp, sp;
+        SinCosF(phi, sp, cp);
         f32 theta = Asin(sqrtf(rng.next()));
-        f32 ct = CosF(theta), st = SinF(theta);
+        f32 ct, st;
+        SinCosF(theta, st, ct);
         V U, Vv;
         orthonormalBasis(hn, &U, &Vv);
         V L = normalize(U * st * cp + Vv * st * sp + hn * ct);
@@ -843,9 +854,11 @@ V traceRay(const Env &X, Ray ray, u32 depth, bool inside, f32 *travel, u32 node)
                 calculateLighting(X, &DC, &SC, ray, B, hp, hn, X.eps->ShadowRay, inside, rng);
                 if (S->PathTrace) {
                     f32 theta = S->ImportanceSampling ? Asin(sqrtf(rng.next())) : Acos(rng.next());
-                    f32 ct = CosF(theta), st = SinF(theta);
+                    f32 ct, st;
+                    SinCosF(theta, st, ct);
                     f32 phi = 2.0f * kPiF * rng.next();
-                    f32 cp = CosF(phi), sp = SinF(phi);
+                    f32 cp, sp;
+                    SinCosF(phi, sp, cp);
                     V U, Vv;
                     orthonormalBasis(hn, &U, &Vv);
                     V s = normalize(U * st * cp + Vv * st * sp + hn * ct);
