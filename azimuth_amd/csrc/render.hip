@@ -867,7 +867,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
             else if (ct > L.cosFall) falloff = 1.0f;
             else {
                 falloff = (ct - L.cosCov) / (L.cosFall - L.cosCov);
-                falloff = PowF(falloff, 4.0f);
+                falloff = PowIF<4>(falloff);
                 if (falloff < 0.0f) falloff = 0.0f;
                 else if (falloff > 1.0f) falloff = 1.0f;
             }
@@ -1135,7 +1135,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
             float cosAir = inside ? cosPhi : cosT;
             float inv = 1.0f - cosAir;
             float r0 = sq((from - to) / (from + to));
-            refl = r0 + (1.0f - r0) * PowF(inv, 5.0f);
+            refl = r0 + (1.0f - r0) * PowIF<5>(inv);
             V3 T = (D + Nn * cosT) * ratio - Nn * cosPhi;
             T = normalize(T);
             V3 coef = inside ? zero : v3(tr.x - 1.0f, tr.y - 1.0f, tr.z - 1.0f);

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "device_render.h"
+#include "dmath.h"
 
 namespace az {
 
@@ -34,25 +35,6 @@ __device__ inline V3 normalize(V3 a) {                                          
 }
 __device__ inline V3 reflect(V3 v, V3 r) { return ((2.0f * dot(v, r)) * r) - v; }           // math.cpp:1335-1348
 __device__ inline float sq(float v) { return v * v; }
-__device__ inline float Exp(float v) { return (float)exp((double)v); }                     // math.cpp:149-156
-__device__ inline float Asin(float v) { return (float)asin((double)v); }                   // math.cpp:221-228
-__device__ inline float Acos(float v) { return (float)acos((double)v); }                   // math.cpp:239-246
-// Sin, Cos, Atan and Power (math.cpp:136-147, 158-219, 248-260) wrap the
-// platform's float functions, whose last bit differs between C libraries and
-// the device.  Here, as in the CPU restatement, each is the double-precision
-// function of the float argument rounded to float: the correctly rounded value
-// in all but about one case in 2^29, so the device and the host draw the same
-// directions and weights bit for bit.
-__device__ inline float SinF(float v) { return (float)sin((double)v); }
-__device__ inline float CosF(float v) { return (float)cos((double)v); }
-__device__ inline void SinCosF(float v, float &sv, float &cv) {   // Sin and Cos of one angle, one reduction
-    double sd, cd;
-    sincos((double)v, &sd, &cd);
-    sv = (float)sd;
-    cv = (float)cd;
-}
-__device__ inline float Atan2F(float y, float x) { return (float)atan2((double)y, (double)x); }
-__device__ inline float PowF(float x, float y) { return (float)pow((double)x, (double)y); }
 
 __device__ inline V3 point_to_world(V3 v, const Affine &m) {                               // math.cpp:1774-1800
     return v3(m.m[0][0] * v.x + m.m[0][1] * v.y + m.m[0][2] * v.z + m.m[0][3],
@@ -210,69 +192,45 @@ __device__ inline V3 sample_texture(const DevScene &S, const DevTexture &T, V3 P
 // ---- BRDFs and light transport (main.cpp:461-603) ---------------------------
 __device__ inline void evaluate_brdf(float &dif, float &spec, const DevMaterial &M, V3 L, V3 N, V3 Vw,
                                      float cosI) {
-    switch (M.brdfType) {
-    case 0: {   // Phong
+    // Types 0-2 (Phong family) take the cosine to the mirror direction, types
+    // 3-6 (Blinn-Phong family, Torrance-Sparrow) the cosine to the half vector;
+    // every type raises it to the exponent once.
+    const int type = M.brdfType >= 0 && M.brdfType <= 5 ? M.brdfType : 6;   // 6 and unknown: Torrance-Sparrow
+    const bool half = type >= 3;
+    V3 H = v3(0.0f, 0.0f, 0.0f);
+    float c;
+    if (half) {
+        H = normalize(-Vw + L);
+        c = dot(N, H);
+    } else {
         V3 R = reflect(L, N);
-        float c = dot(-Vw, R);
-        spec = c > 0.0f ? PowF(c, M.exponent) / cosI : 0.0f;
-    } break;
-    case 1: {   // ModifiedPhong
-        V3 R = reflect(L, N);
-        float c = dot(-Vw, R);
-        spec = c > 0.0f ? PowF(c, M.exponent) : 0.0f;
-    } break;
-    case 2: {   // NormalizedModifiedPhong
-        dif = (float)(1.0f / kPiD);
-        V3 R = reflect(L, N);
-        float c = dot(-Vw, R);
-        if (c > 0.0f) {
-            float e = M.exponent;
-            spec = ((e + 2.0f) / (2.0f * kPiF)) * PowF(c, e);
-        } else {
-            spec = 0.0f;
-        }
-    } break;
-    case 3: {   // BlinnPhong
-        V3 H = normalize(-Vw + L);
-        float c = dot(N, H);
-        spec = c > 0.0f ? PowF(c, M.exponent) / cosI : 0.0f;
-    } break;
-    case 4: {   // ModifiedBlinnPhong
-        V3 H = normalize(-Vw + L);
-        float c = dot(N, H);
-        spec = c > 0.0f ? PowF(c, M.exponent) : 0.0f;
-    } break;
-    case 5: {   // NormalizedModifiedBlinnPhong
-        dif = (float)(1.0f / kPiD);
-        V3 H = normalize(-Vw + L);
-        float c = dot(N, H);
-        if (c > 0.0f) {
-            float e = M.exponent;
-            spec = ((e + 8.0f) / (8.0f * kPiF)) * PowF(c, e);
-        } else {
-            spec = 0.0f;
-        }
-    } break;
-    default: {  // TorranceSparrow
-        dif = (float)(1.0f / kPiD);
-        V3 H = normalize(-Vw + L);
-        float c = dot(N, H);
-        if (c > 0.0f) {
-            float e = M.exponent;
-            float ct = PowF(c, e);
-            float D = ((e + 2.0f) / (2.0f * kPiF)) * ct;
-            float nw0 = dot(N, -Vw);
-            float w0wh = dot(-Vw, H);
-            float G = nw0 < cosI ? nw0 : cosI;
-            G = (2.0f * G * c) / w0wh;
-            G = 1.0f < G ? 1.0f : G;
-            float ri = M.brdfIor;
-            float R0 = sq(ri - 1.0f) / sq(ri + 1.0f);
-            float F = R0 + (1.0f - R0) * PowF(1.0f - w0wh, 5.0f);
-            spec = (D * F * G) / (4.0f * cosI * nw0);
-        } else {
-            spec = 0.0f;
-        }
+        c = dot(-Vw, R);
+    }
+    if (type == 2 || type >= 5) dif = (float)(1.0f / kPiD);
+    if (!(c > 0.0f)) {
+        spec = 0.0f;
+        return;
+    }
+    const float e = M.exponent;
+    const float p = PowF(c, e);
+    switch (type) {
+    case 0: spec = p / cosI; break;                                  // Phong
+    case 1: spec = p; break;                                         // ModifiedPhong
+    case 2: spec = ((e + 2.0f) / (2.0f * kPiF)) * p; break;          // NormalizedModifiedPhong
+    case 3: spec = p / cosI; break;                                  // BlinnPhong
+    case 4: spec = p; break;                                         // ModifiedBlinnPhong
+    case 5: spec = ((e + 8.0f) / (8.0f * kPiF)) * p; break;          // NormalizedModifiedBlinnPhong
+    default: {                                                       // TorranceSparrow
+        float D = ((e + 2.0f) / (2.0f * kPiF)) * p;
+        float nw0 = dot(N, -Vw);
+        float w0wh = dot(-Vw, H);
+        float G = nw0 < cosI ? nw0 : cosI;
+        G = (2.0f * G * c) / w0wh;
+        G = 1.0f < G ? 1.0f : G;
+        float ri = M.brdfIor;
+        float R0 = sq(ri - 1.0f) / sq(ri + 1.0f);
+        float F = R0 + (1.0f - R0) * PowIF<5>(1.0f - w0wh);
+        spec = (D * F * G) / (4.0f * cosI * nw0);
     } break;
     }
 }

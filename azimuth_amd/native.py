@@ -266,6 +266,8 @@ EXPORTS = {
     "az_gpu_set_render_memory_limit": (s32, [_P, u64]),
     "az_gpu_env_luminance_sample": (s32, [_P, _P, u32, _P]),
     "az_gpu_env_luminance_pdf": (s32, [_P, _P, u32, _P]),
+    "az_gpu_math_eval": (s32, [_P, u32, _P, _P, u32, _P]),
+    "az_math_eval": (s32, [u32, _P, _P, u32, _P]),
     "az_gpu_malloc": (s32, [_P, C.POINTER(_P), u64]),
     "az_gpu_free": (s32, [_P, _P]),
     "az_gpu_upload": (s32, [_P, _P, _P, u64]),
@@ -330,3 +332,21 @@ def check(status, what="", ctx=None):
         if ctx is not None:
             msg += ": " + load().az_gpu_last_error(ctx).decode(errors="replace")
         raise AzError(f"{what} failed with {msg}")
+
+
+# az_math_fn codes of az_math_eval / az_gpu_math_eval.
+MATH_FUNCTIONS = {"sin": 0, "cos": 1, "atan2": 2, "pow": 3, "exp": 4, "asin": 5, "acos": 6, "pow4": 7, "pow5": 8}
+
+
+def math_eval_host(name, x, y=None):
+    """The renderer's float elementary function `name` of float32 arrays x
+    (and y), evaluated on the host from the device routines' source."""
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+    yp = None
+    if y is not None:
+        y = np.ascontiguousarray(y, dtype=np.float32).reshape(-1)
+        assert y.size == x.size
+        yp = y.ctypes.data
+    out = np.empty_like(x)
+    check(load().az_math_eval(MATH_FUNCTIONS[name], x.ctypes.data, yp, x.size, out.ctypes.data), "az_math_eval")
+    return out

@@ -486,6 +486,29 @@ class GpuContext:
             d_d.free()
             d_p.free()
 
+    def math_eval(self, name, x, y=None):
+        """The device's float elementary function `name` (MATH_FUNCTIONS) of
+        float32 arrays x (and y), elementwise."""
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+        n = x.size
+        d_x, d_o = self.alloc(max(4, x.nbytes)), self.alloc(max(4, x.nbytes))
+        d_y = None
+        try:
+            d_x.upload(x)
+            if y is not None:
+                y = np.ascontiguousarray(y, dtype=np.float32).reshape(-1)
+                assert y.size == n
+                d_y = self.alloc(max(4, y.nbytes))
+                d_y.upload(y)
+            N.check(self.lib.az_gpu_math_eval(self.h, N.MATH_FUNCTIONS[name], d_x.ptr, d_y.ptr if d_y else None, n,
+                                              d_o.ptr), "az_gpu_math_eval", self.h)
+            return d_o.download(np.float32, n)
+        finally:
+            d_x.free()
+            d_o.free()
+            if d_y:
+                d_y.free()
+
     def trim(self):
         """Frees the context's render queues (the scene stays uploaded)."""
         N.check(self.lib.az_gpu_trim(self.h), "az_gpu_trim", self.h)

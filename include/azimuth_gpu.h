@@ -344,6 +344,27 @@ az_status az_gpu_set_render_memory_limit(az_gpu_ctx *Ctx, u64 Bytes);
 az_status az_gpu_env_luminance_sample(az_gpu_ctx *Ctx, const f32 *d_U, u32 N, f32 *d_Out);
 az_status az_gpu_env_luminance_pdf(az_gpu_ctx *Ctx, const f32 *d_Dirs, u32 N, f32 *d_Pdf);
 
+/* The renderer's float elementary functions, which replace the reference's
+ * Sin, Cos, Atan2, Power, Exp, ArcSine and ArcCosine (math.cpp:136-260):
+ * each is the double-precision function of the float argument rounded to
+ * float.  Out[i] = f(X[i]) or f(X[i], Y[i]) for i < N; Y is read only by
+ * AZ_MATH_ATAN2 (X the y-coordinate) and AZ_MATH_POW.  az_gpu_math_eval runs
+ * the device routines on device arrays, ordered on the context stream;
+ * az_math_eval runs the same source on the host on host arrays. */
+typedef enum az_math_fn {
+    AZ_MATH_SIN = 0,
+    AZ_MATH_COS = 1,
+    AZ_MATH_ATAN2 = 2,
+    AZ_MATH_POW = 3,
+    AZ_MATH_EXP = 4,
+    AZ_MATH_ASIN = 5,
+    AZ_MATH_ACOS = 6,
+    AZ_MATH_POW4 = 7,   /* X^4, the spot light falloff (main.cpp:665) */
+    AZ_MATH_POW5 = 8,   /* X^5, the Fresnel terms (main.cpp:560, 896) */
+} az_math_fn;
+az_status az_gpu_math_eval(az_gpu_ctx *Ctx, u32 Fn, const f32 *d_X, const f32 *d_Y, u32 N, f32 *d_Out);
+az_status az_math_eval(u32 Fn, const f32 *X, const f32 *Y, u32 N, f32 *Out);
+
 /* Device memory helpers, ordered on the context stream. */
 az_status az_gpu_malloc(az_gpu_ctx *Ctx, void **d_Ptr, u64 Bytes);
 az_status az_gpu_free(az_gpu_ctx *Ctx, void *d_Ptr);

@@ -29,6 +29,8 @@ def load():
         lib.azo_render.argtypes = [P, C.POINTER(N.camera), C.POINTER(N.epsilon_values), C.c_int, C.c_uint32,
                                    C.POINTER(N.az_render_opts), P, C.c_uint32]
         lib.azo_render_f64.argtypes = lib.azo_render.argtypes
+        lib.azo_math_eval.argtypes = [C.c_uint32, P, P, C.c_uint32, P]
+        lib.azo_math_eval.restype = C.c_int
         lib.azo_rng_draw.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
         lib.azo_rng_draw.restype = C.c_float
         lib.azo_pcg32_at.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
@@ -83,6 +85,23 @@ def render(scene, camera, filt=N.FilterType_Box, cull=False, seed=0, spp=0, row_
     out = np.zeros(W * H, dtype=N.PIXEL_DTYPE)
     lib.azo_render(scene.ptr, C.byref(camera), C.byref(e), int(filt), 1 if cull else 0, C.byref(o),
                    out.ctypes.data, th)
+    return out
+
+
+MATH_FUNCTIONS = {"sin": 0, "cos": 1, "atan2": 2, "pow": 3, "exp": 4, "asin": 5, "acos": 6, "pow4": 7, "pow5": 8}
+
+
+def math_eval(name, x, y=None):
+    """The renderer's float elementary function `name` (MATH_FUNCTIONS) of
+    float32 arrays x (and y), elementwise."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    yp = None
+    if y is not None:
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        assert y.shape == x.shape
+        yp = y.ctypes.data
+    out = np.empty_like(x)
+    assert load().azo_math_eval(MATH_FUNCTIONS[name], x.ctypes.data, yp, x.size, out.ctypes.data) == 1
     return out
 
 

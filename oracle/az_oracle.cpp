@@ -71,7 +71,8 @@ inline f32 Acos(f32 v) { return (f32)acos((f64)v); }                            
 // platform's float functions, whose last bit is the C library's own choice;
 // this restatement defines each as the double-precision function of the
 // float argument rounded to float (the correctly rounded value in all but
-// about one case in 2^29), as the device does.
+// about one case in 2^29); the device computes the same values with its own
+// double-precision routines.
 inline f32 SinF(f32 v) { return (f32)sin((f64)v); }
 inline f32 CosF(f32 v) { return (f32)cos((f64)v); }
 inline void SinCosF(f32 v, f32 &sv, f32 &cv) {   // Sin and Cos of one angle (as the device computes them)
@@ -1230,6 +1231,30 @@ f32 azo_platform_atomic_sum(u32 threads, u32 addsPerThread, f32 value) {
         });
     runThreads(jobs);
     return total;
+}
+
+// The float elementary functions as the renderer calls them: 0 Sin, 1 Cos,
+// 2 Atan2(x = y-coordinate, y = x-coordinate), 3 Power, 4 Exp, 5 ArcSine,
+// 6 ArcCosine, 7 x^4, 8 x^5 (math.cpp:136-260).
+int azo_math_eval(u32 fn, const f32 *x, const f32 *y, u32 n, f32 *out) {
+    for (u32 i = 0; i < n; ++i) {
+        const f32 a = x[i], b = y ? y[i] : 0.0f;
+        f32 r;
+        switch (fn) {
+        case 0: r = SinF(a); break;
+        case 1: r = CosF(a); break;
+        case 2: r = Atan2F(a, b); break;
+        case 3: r = PowF(a, b); break;
+        case 4: r = Exp(a); break;
+        case 5: r = Asin(a); break;
+        case 6: r = Acos(a); break;
+        case 7: r = PowF(a, 4.0f); break;
+        case 8: r = PowF(a, 5.0f); break;
+        default: return 0;
+        }
+        out[i] = r;
+    }
+    return 1;
 }
 
 f32 azo_rng_draw(u64 seed, u32 pixel, u32 sample, u32 node, u32 dim) {

@@ -51,6 +51,12 @@ void azo_render_f64(const scene *Scene, const camera *Camera, const epsilon_valu
  * total. */
 f32 azo_platform_atomic_sum(u32 Threads, u32 AddsPerThread, f32 Value);
 
+/* The float elementary functions of the renderer on arrays: Fn 0 Sin(X),
+ * 1 Cos(X), 2 Atan2(X, Y) (X the y-coordinate), 3 Power(X, Y), 4 Exp(X),
+ * 5 ArcSine(X), 6 ArcCosine(X), 7 Power(X, 4), 8 Power(X, 5); Y may be null
+ * for the one-argument functions.  Returns 0 for an unknown Fn. */
+int azo_math_eval(u32 Fn, const f32 *X, const f32 *Y, u32 N, f32 *Out);
+
 /* Counter-addressed random stream: the float drawn at (Seed, Pixel, Sample,
  * Node, Dim), Dim counted within one path vertex. */
 f32 azo_rng_draw(u64 Seed, u32 Pixel, u32 Sample, u32 Node, u32 Dim);
