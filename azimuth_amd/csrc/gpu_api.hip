@@ -415,6 +415,8 @@ az_status az_gpu_create(az_gpu_ctx **Out, int dev) {
     if (hipDeviceGetAttribute(&c->cuCount, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) c->cuCount = 256;
     c->cuCount = std::max(1, c->cuCount);
     memset(&c->stats, 0, sizeof(c->stats));
+    void *ws = nullptr;
+    (void)az::noalloc_workspace(c, &ws);   // set up here; retried by the entry point if this failed
     *Out = c;
     return AZ_OK;
 }

@@ -135,3 +135,8 @@ struct az_gpu_ctx {
 
 az_status az_upload_geometry(az_gpu_ctx *C, const scene *S, az::UploadInfo &info);
 az_status az_upload_shading(az_gpu_ctx *C, const scene *S, const az::UploadInfo &info);
+// Device address of the fixed workspace of az_gpu_sum_f32_sequential_noalloc on
+// the context's device, which must be current (looked up once per device).
+namespace az {
+az_status noalloc_workspace(az_gpu_ctx *C, void **Out);
+}

@@ -17,7 +17,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "azimuth_gpu.h"
 #include "gpu_context.h"
@@ -423,6 +425,19 @@ void launch_sequential_sum(const float *x, uint32_t n, ChunkFold *folds, double 
                        sumOut);
 }
 
+// Workspace of az_gpu_sum_f32_sequential_noalloc: the chunk folds and chunk
+// sums for up to kNoAllocMaxValues values (2.5 MiB), one copy on each device,
+// held until the process ends.  Host code reaches it only through the address
+// hipGetSymbolAddress gives for the current device (noalloc_workspace).
+constexpr uint32_t kNoAllocMaxValues = 1u << 26;
+constexpr uint32_t kNoAllocChunks = kNoAllocMaxValues / kChunk;
+static __device__ __align__(256) unsigned char scratch[(size_t)kNoAllocChunks * (sizeof(ChunkFold) + sizeof(double))];
+
+__global__ void k_store_zero(float *p) { *p = 0.0f; }
+
+std::mutex gWorkspaceMu;
+std::vector<void *> gWorkspace;      // device address of `scratch`, by device ordinal
+
 }  // namespace
 }  // namespace az
 
@@ -515,6 +530,47 @@ az_status az_gpu_sum_f32_sequential(az_gpu_ctx *C, const f32 *d_X, u32 N, f32 *d
     ChunkFold *folds = (ChunkFold *)C->outputScratch.ptr;
     double *sums = (double *)(folds + nch);
     launch_sequential_sum(d_X, N, folds, sums, 0.0f, 0u, nullptr, nullptr, d_Sum, st);
+    AZ_HIPO(C, hipGetLastError());
+    return AZ_OK;
+}
+
+// The device address of the workspace on the context's device (current),
+// looked up once per device.
+az_status az::noalloc_workspace(az_gpu_ctx *C, void **Out) {
+    std::lock_guard<std::mutex> lock(gWorkspaceMu);
+    if ((size_t)C->device >= gWorkspace.size()) gWorkspace.resize((size_t)C->device + 1, nullptr);
+    if (!gWorkspace[C->device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            C->lastError = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return AZ_ERR_DEVICE;
+        }
+        gWorkspace[C->device] = p;
+    }
+    *Out = gWorkspace[C->device];
+    return AZ_OK;
+}
+
+az_status az_gpu_sum_f32_sequential_noalloc(az_gpu_ctx *C, const f32 *d_X, u32 N, f32 *d_Sum) {
+    if (!C || !d_Sum || (N && !d_X)) return AZ_ERR_INVALID_ARGUMENT;
+    if (N > kNoAllocMaxValues) {
+        C->lastError = "az_gpu_sum_f32_sequential_noalloc: more than 2^26 values (its fixed workspace)";
+        return AZ_ERR_INVALID_ARGUMENT;
+    }
+    AZ_HIPO(C, hipSetDevice(C->device));
+    void *ws = nullptr;
+    const az_status s = noalloc_workspace(C, &ws);
+    if (s != AZ_OK) return s;
+    hipStream_t st = C->stream;
+    if (N == 0) {
+        hipLaunchKernelGGL(k_store_zero, dim3(1), dim3(1), 0, st, (float *)d_Sum);
+    } else {
+        // Every chunk's fold and sum is written before it is read: nothing to clear.
+        ChunkFold *folds = (ChunkFold *)ws;
+        double *sums = (double *)(folds + kNoAllocChunks);
+        launch_sequential_sum(d_X, N, folds, sums, 0.0f, 0u, nullptr, nullptr, d_Sum, st);
+    }
     AZ_HIPO(C, hipGetLastError());
     return AZ_OK;
 }

@@ -249,6 +249,7 @@ EXPORTS = {
     "az_gpu_get_stats": (s32, [_P, C.POINTER(az_stats)]),
     "az_gpu_output_pixels": (s32, [_P, _P, u32, u32, C.POINTER(tonemap), f32, u32, _P]),
     "az_gpu_sum_f32_sequential": (s32, [_P, _P, u32, _P]),
+    "az_gpu_sum_f32_sequential_noalloc": (s32, [_P, _P, u32, _P]),
     "az_gpu_set_counting": (s32, [_P, u32]),
     "az_gpu_get_render_stats": (s32, [_P, C.POINTER(az_render_stats)]),
     "az_gpu_set_launch": (s32, [_P, u32, u32, s32, s32]),

@@ -194,6 +194,15 @@ az_status az_gpu_output_pixels(az_gpu_ctx *Ctx, const pixel *d_Pixels, u32 Width
  * counterpart as an entry point: the primitive behind az_gpu_output_pixels's
  * log-average. */
 az_status az_gpu_sum_f32_sequential(az_gpu_ctx *Ctx, const f32 *d_X, u32 N, f32 *d_Sum);
+/* az_gpu_sum_f32_sequential without any device allocation: the same kernels
+ * and result, with their chunk workspace in a fixed 2.5 MiB array that the
+ * library holds on each device it runs on until the process ends.  N <= 2^26
+ * (larger N returns AZ_ERR_INVALID_ARGUMENT).  All calls on one device share
+ * that workspace, so calls that overlap on one device (several streams or
+ * host threads) race.  No counterpart in the reference: for callers that
+ * cannot allocate (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc). */
+az_status az_gpu_sum_f32_sequential_noalloc(az_gpu_ctx *Ctx, const f32 *d_X, u32 N, f32 *d_Sum);
 
 az_status az_gpu_get_stats(az_gpu_ctx *Ctx, az_stats *Stats);
 /* Enables per-ray node/triangle counting in trace calls and renders
