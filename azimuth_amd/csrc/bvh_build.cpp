@@ -315,7 +315,7 @@ uint32_t CollapseWideBvh(const std::vector<BinNode> &b2, uint32_t root, uint32_t
     std::vector<uint32_t> subRoots, subDepth;
     std::vector<uint32_t> open{0};
     tops.push_back(Top{root, 1, 0, {}, {0, 0, 0, 0}});
-    const bool parallel = n >= 65536;
+    const bool parallel = n >= 65536 && !P.serialCollapse;
     while (!open.empty()) {
         const uint32_t t = open.back();
         open.pop_back();

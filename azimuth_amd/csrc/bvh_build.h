@@ -19,6 +19,7 @@ struct BuildParams {
     uint32_t bins = 32;            // SAH bins per axis
     float nodeCost = 1.0f;         // SAH cost of visiting one 4-wide node
     float itemCost = 1.0f;         // SAH cost of testing one item
+    bool serialCollapse = false;   // collapse into wide nodes on one thread (for checks)
 };
 
 struct BuildStats {
@@ -50,8 +51,13 @@ void BuildBinaryBvh(const std::vector<BuildBox> &boxes, const BuildParams &param
 uint32_t CollapseWideBvh(const std::vector<BinNode> &b2, uint32_t root, uint32_t n, const BuildParams &params,
                          uint32_t itemBase, std::vector<Node4> &nodes, BuildStats *stats);
 
+// The device builder bins with this many bins per axis only; BuildParams
+// with other bins are built on the host.
+constexpr uint32_t kDeviceBuildBins = 32;
+
 // Builds the binary tree on the device (hipSuccess, or the error of the
-// first failing call; all device memory is freed either way).
+// first failing call, hipErrorInvalidValue when params.bins is not
+// kDeviceBuildBins; all device memory is freed either way).
 int BuildBinaryBvhDevice(void *stream, const std::vector<BuildBox> &boxes, const BuildParams &params,
                          std::vector<BinNode> &b2, std::vector<uint32_t> &order);
 

@@ -26,7 +26,7 @@ namespace az {
 namespace {
 
 constexpr uint32_t kBuildBlock = 256;
-constexpr uint32_t kBuildBins = 32;
+constexpr uint32_t kBuildBins = kDeviceBuildBins;
 // Levels with this many ranges or more split each range with one wave.
 constexpr uint32_t kOneWaveLevel = 4096;
 
@@ -444,6 +444,7 @@ int BuildBinaryBvhDevice(void *stream, const std::vector<BuildBox> &boxes, const
     const uint32_t n = (uint32_t)boxes.size();
     b2.clear();
     order.clear();
+    if (P.bins != kBuildBins) return (int)hipErrorInvalidValue;   // the host binning would differ
     if (n == 0) {
         b2.emplace_back();
         return hipSuccess;

@@ -147,8 +147,10 @@ constexpr size_t kDeviceBuildMin = 65536;
 
 static az_status buildWide(az_gpu_ctx *C, const std::vector<BuildBox> &boxes, const BuildParams &bp, uint32_t itemBase,
                            std::vector<Node4> &nodes, std::vector<uint32_t> &order, BuildStats *bs, uint32_t *root) {
-    const bool device = C->buildMode == AZ_BUILD_DEVICE ||
-                        (C->buildMode == AZ_BUILD_AUTO && boxes.size() >= kDeviceBuildMin);
+    // The device builder gives the host's tree for its own bin count only.
+    const bool device = bp.bins == kDeviceBuildBins &&
+                        (C->buildMode == AZ_BUILD_DEVICE ||
+                         (C->buildMode == AZ_BUILD_AUTO && boxes.size() >= kDeviceBuildMin));
     if (!device) {
         *root = BuildWideBvh(boxes, bp, itemBase, nodes, order, bs);
         return AZ_OK;

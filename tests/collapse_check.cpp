@@ -4,12 +4,15 @@
 //              first visits the nodes as 0, 1, 2, ... (each subtree is one
 //              contiguous run), as the serial collapse numbers them;
 //   items    : every item lies in exactly one leaf;
-//   boxes    : every child box is the union of the item boxes below it.
+//   boxes    : every child box is the union of the item boxes below it;
+//   threads  : from 65536 items, the threaded collapse equals the serial one
+//              byte for byte.
 // Prints one line per case and "ok" at the end; exits non-zero on a failure.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <random>
 #include <vector>
 
@@ -101,6 +104,22 @@ int main() {
                         hi[0] != nodes[v].hix[c] || hi[1] != nodes[v].hiy[c] || hi[2] != nodes[v].hiz[c])
                         return fail("inner box", n, width);
                 }
+            // The threaded collapse gives the serial collapse's nodes byte for byte.
+            if (n >= 65536) {
+                BuildParams Ps = P;
+                Ps.serialCollapse = true;
+                std::vector<Node4> serial(5);
+                std::vector<uint32_t> serialOrder;
+                BuildStats ss;
+                const uint32_t sroot = BuildWideBvh(boxes, Ps, itemBase, serial, serialOrder, &ss);
+                if (sroot != root || serial.size() != nodes.size() || serialOrder != order)
+                    return fail("serial collapse shape", n, width);
+                if (std::memcmp(serial.data(), nodes.data(), nodes.size() * sizeof(Node4)) != 0)
+                    return fail("serial collapse bytes", n, width);
+                if (ss.nodes != st.nodes || ss.leaves != st.leaves || ss.maxDepth != st.maxDepth ||
+                    ss.maxStack != st.maxStack)
+                    return fail("serial collapse stats", n, width);
+            }
             std::printf("n=%u width=%u nodes=%u leaves=%u depth=%u stack=%u\n", n, width, st.nodes, st.leaves,
                         st.maxDepth, st.maxStack);
         }
