@@ -37,7 +37,7 @@ struct GuideSample {
 struct GuideWeights {
     const short *wt;    // bf16 W^T per layer, [out_pad][in]
     const short *wb;    // bf16 W per layer, [in][out_pad]
-    const short *wf;    // bf16 W^T per layer as MFMA A fragments: tile (mt, kt), then lane, then 4 values
+    const short *wf;    // bf16 W^T per layer as 16x16x32 MFMA A fragments: tile (mt, t), then lane, then 8 values
     const float *bias;  // 5 x kGuideBiasStride
 };
 

@@ -64,6 +64,19 @@ __device__ inline v4s to_bf16x4(v4f x) { return __builtin_bit_cast(v4s, __builti
 
 __device__ inline v4s load4(const short *p) { return *reinterpret_cast<const v4s *>(p); }
 
+// gfx950's 16x16x32 bf16 MFMA: lane l holds A row l % 16 and B column l % 16
+// at k = 8 (l / 16) .. +7, the result as in the 16x16x16 form.  A K-block of
+// 32 is built from two 16-row tiles in the accumulator layout: lane quarter q
+// takes rows 4q..4q+3 of the first tile as k = 8q..8q+3 and of the second as
+// k = 8q+4..8q+7, and the weight fragments are packed in the same k order.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+__device__ inline v4f mfma32(v8s a, v4s b0, v4s b1, v4f c) {
+    const v8s b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0, 0,
+                                                   0);
+}
+
 // ---- input encoding ----------------------------------------------------------------
 // Features 0..95 are sin and cos of pi * 2^j * p[c] for 16 frequencies j and
 // the 3 coordinates c; the order follows the B-operand lanes: the lanes of
@@ -82,9 +95,12 @@ __device__ inline void encode_tiles(float px, float py, float pz, float nx, floa
     float S[4][3], Co[4][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const float th = scale * p[c];
-        S[0][c] = sinpif(th);
-        Co[0][c] = cospif(th);
+        // sin(pi th) and cos(pi th) with the hardware's sine and cosine of a
+        // whole-turn argument (v_sin/v_cos take x in turns, |x| <= 256), fed
+        // the fraction of th/2: exact reduction, about 1e-6 absolute error.
+        const float turns = __builtin_amdgcn_fractf(0.5f * (scale * p[c]));
+        S[0][c] = __builtin_amdgcn_sinf(turns);
+        Co[0][c] = __builtin_amdgcn_cosf(turns);
 #pragma unroll
         for (int j = 1; j < 4; ++j) {
             const float a = S[j - 1][c], b = Co[j - 1][c];
@@ -181,13 +197,14 @@ __global__ __launch_bounds__(64) void k_mfma_selftest(const float *A, const floa
 // reads one fragment as 64 consecutive 8-byte words: no bank conflicts),
 // then loops over groups of 256 queries; a wave takes 64 of them, four
 // 16-sample column tiles that share each A fragment.
-constexpr int kInferGroups = 4;
-constexpr int kInferSamples = 4 * 16 * kInferGroups;   // per block iteration
+constexpr int kInferGroups = 4;                         // 16-sample tiles a wave carries through the layers
+constexpr int kInferWaves = 12;                         // waves per block sharing one LDS copy of the weights
+constexpr int kInferSamples = 16 * kInferGroups;        // per wave iteration
 
 struct alignas(16) InferLds {
     short w[kHalfCount];
     float bias[5 * kGuideBiasStride];
-    float outs[4][16][kGuideOutPad];
+    float outs[kInferWaves][16][kGuideOutPad];
 };
 
 template <int NI, int NO, bool RELU>
@@ -200,10 +217,10 @@ __device__ inline void layer_lds(const short *w, const float *b, const v4s (*x)[
 #pragma unroll
         for (int g = 0; g < kInferGroups; ++g) c[g] = bias;
 #pragma unroll
-        for (int kt = 0; kt < NI; ++kt) {
-            const v4s a = *reinterpret_cast<const v4s *>(w + ((mt * NI + kt) * 64 + lane) * 4);
+        for (int t = 0; t < NI / 2; ++t) {
+            const v8s a = *reinterpret_cast<const v8s *>(w + ((mt * (NI / 2) + t) * 64 + lane) * 8);
 #pragma unroll
-            for (int g = 0; g < kInferGroups; ++g) c[g] = mfma16(a, x[g][kt], c[g]);
+            for (int g = 0; g < kInferGroups; ++g) c[g] = mfma32(a, x[g][2 * t], x[g][2 * t + 1], c[g]);
         }
 #pragma unroll
         for (int g = 0; g < kInferGroups; ++g) {
@@ -216,20 +233,32 @@ __device__ inline void layer_lds(const short *w, const float *b, const v4s (*x)[
     }
 }
 
-__global__ __launch_bounds__(256, 2) void k_guide_infer(GuideWeights W, const GuideQuery *__restrict__ q, uint32_t n,
-                                                        float *__restrict__ lobes) {
+// Orders a wave's LDS writes before its later LDS reads of other lanes'
+// words (a wave's LDS operations complete in order; this keeps the compiler
+// from moving them).
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One LDS copy of the weights per block of kInferWaves waves (3 per SIMD at
+// the kernel's 155 VGPRs; the copy and the waves' output tiles take 83 KB
+// of the CU's 160 KB).  After the copy the waves run independently, each
+// looping over its own 64-query groups.
+__global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideWeights W, const GuideQuery *__restrict__ q,
+                                                                 uint32_t n, float *__restrict__ lobes) {
     __shared__ InferLds L;
-    for (int k = threadIdx.x; k < kHalfCount / 8; k += 256)
+    for (int k = threadIdx.x; k < kHalfCount / 8; k += 64 * kInferWaves)
         reinterpret_cast<uint4 *>(L.w)[k] = reinterpret_cast<const uint4 *>(W.wf)[k];
-    for (int k = threadIdx.x; k < 5 * kGuideBiasStride; k += 256) L.bias[k] = W.bias[k];
+    for (int k = threadIdx.x; k < 5 * kGuideBiasStride; k += 64 * kInferWaves) L.bias[k] = W.bias[k];
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, kq = 4 * (lane >> 4);
     const uint32_t groups = (n + kInferSamples - 1) / kInferSamples;
-    // The loop bounds are uniform over the block, so every wave reaches every
-    // barrier; queries past n are computed on a clamped copy and not stored.
-    for (uint32_t it = blockIdx.x; it < groups; it += gridDim.x) {
-        const uint32_t base = it * kInferSamples + wave * (16 * kInferGroups);
+    // Queries past n are computed on a clamped copy and not stored.
+    for (uint32_t it = blockIdx.x * kInferWaves + wave; it < groups; it += gridDim.x * kInferWaves) {
+        const uint32_t base = it * kInferSamples;
         v4s x[kInferGroups][8], h1[kInferGroups][8], h2[kInferGroups][8];
 #pragma unroll
         for (int g = 0; g < kInferGroups; ++g) {
@@ -250,11 +279,12 @@ __global__ __launch_bounds__(256, 2) void k_guide_infer(GuideWeights W, const Gu
             for (int mt = 0; mt < 3; ++mt) {
                 v4f c = *reinterpret_cast<const v4f *>(b4 + 16 * mt + kq);
 #pragma unroll
-                for (int kt = 0; kt < 4; ++kt)
-                    c = mfma16(*reinterpret_cast<const v4s *>(w4 + ((mt * 4 + kt) * 64 + lane) * 4), h2[g][kt], c);
+                for (int t = 0; t < 2; ++t)
+                    c = mfma32(*reinterpret_cast<const v8s *>(w4 + ((mt * 2 + t) * 64 + lane) * 8), h2[g][2 * t],
+                               h2[g][2 * t + 1], c);
                 *reinterpret_cast<v4f *>(&L.outs[wave][col][16 * mt + kq]) = c;
             }
-            __syncthreads();
+            wave_lds_sync();
             // Lobe k of sample sl for the 128 (sample, lobe) items, two per
             // lane: the 8 lanes of a sample form its softmax with shuffles,
             // and consecutive lanes write consecutive 20-byte lobes.
@@ -268,25 +298,25 @@ __global__ __launch_bounds__(256, 2) void k_guide_infer(GuideWeights W, const Gu
                 m = fmaxf(m, __shfl_xor(m, 1, 8));
                 m = fmaxf(m, __shfl_xor(m, 2, 8));
                 m = fmaxf(m, __shfl_xor(m, 4, 8));
-                const float e = expf(logit - m);
+                const float e = __builtin_amdgcn_exp2f((logit - m) * 1.44269504f);
                 float sum = e;
                 sum += __shfl_xor(sum, 1, 8);
                 sum += __shfl_xor(sum, 2, 8);
                 sum += __shfl_xor(sum, 4, 8);
                 const float x0 = o[8 + 3 * k], y0 = o[9 + 3 * k], z0 = o[10 + 3 * k];
-                const float len = sqrtf(x0 * x0 + y0 * y0 + z0 * z0);
-                const float inv = len > 1e-12f ? 1.0f / len : 0.0f;
+                const float len2 = x0 * x0 + y0 * y0 + z0 * z0;
+                const float inv = len2 > 1e-24f ? __builtin_amdgcn_rsqf(len2) : 0.0f;
                 const float kr = fminf(fmaxf(o[32 + k], kGuideKMin), kGuideKMax);
                 if (s < n) {
                     float *dst = lobes + (size_t)s * 40 + 5 * k;
-                    dst[0] = e / sum;
+                    dst[0] = e * __builtin_amdgcn_rcpf(sum);
                     dst[1] = x0 * inv;
                     dst[2] = y0 * inv;
                     dst[3] = z0 * inv;
-                    dst[4] = expf(kr);
+                    dst[4] = __builtin_amdgcn_exp2f(kr * 1.44269504f);
                 }
             }
-            __syncthreads();
+            wave_lds_sync();   // this tile's reads before the next tile's writes
         }
     }
 }
@@ -494,9 +524,10 @@ __global__ __launch_bounds__(256) void k_guide_refresh(const float *__restrict__
             wt[h_off(l) + i] = f2bf(r < out ? p[w_off(l) + c * out + r] : 0.0f);
             int r2 = i / op, c2 = i % op;        // W [in][out_pad]
             wb[h_off(l) + i] = f2bf(c2 < out ? p[w_off(l) + r2 * out + c2] : 0.0f);
-            // A fragment (mt, kt), lane, value e: W^T[16 mt + lane % 16][16 kt + 4 (lane / 16) + e]
-            int frag = i >> 8, ln = (i >> 2) & 63, e = i & 3, nkt = in / 16;
-            int r3 = 16 * (frag / nkt) + (ln & 15), c3 = 16 * (frag % nkt) + 4 * (ln >> 4) + e;
+            // 16x16x32 A fragment (mt, t), lane, value e:
+            // W^T[16 mt + lane % 16][32 t + 16 (e / 4) + 4 (lane / 16) + e % 4]
+            int frag = i >> 9, ln = (i >> 3) & 63, e = i & 7, nt = in / 32;
+            int r3 = 16 * (frag / nt) + (ln & 15), c3 = 32 * (frag % nt) + 16 * (e >> 2) + 4 * (ln >> 4) + (e & 3);
             wf[h_off(l) + i] = f2bf(r3 < out ? p[w_off(l) + c3 * out + r3] : 0.0f);
         }
         if (i < kGuideBiasStride) bias[l * kGuideBiasStride + i] = i < out ? p[b_off(l) + i] : 0.0f;
@@ -642,11 +673,11 @@ az_status az_guide_get_grad(az_guide *G, f32 *Grad, u32 Count) {
 az_status az_guide_infer_device(az_guide *G, const void *d_Queries, u32 N, f32 *d_Lobes) {
     if (!G || (N && (!d_Queries || !d_Lobes))) return AZ_ERR_INVALID_ARGUMENT;
     if (N == 0) return AZ_OK;
-    // Persistent blocks: two per CU (the LDS a block holds allows two), each
-    // looping over groups of kInferSamples queries.
+    // Persistent blocks, one per CU (the LDS a block holds allows one), each
+    // wave looping over groups of kInferSamples queries.
     const u32 groups = (N + kInferSamples - 1) / kInferSamples;
-    dim3 grid(std::min<u32>(groups, 2u * (u32)G->cuCount));
-    hipLaunchKernelGGL(k_guide_infer, grid, dim3(256), 0, G->stream, weightsOf(G), (const GuideQuery *)d_Queries, N,
+    dim3 grid(std::min<u32>((groups + kInferWaves - 1) / kInferWaves, (u32)G->cuCount));
+    hipLaunchKernelGGL(k_guide_infer, grid, dim3(64 * kInferWaves), 0, G->stream, weightsOf(G), (const GuideQuery *)d_Queries, N,
                        d_Lobes);
     AZ_GHIP(G, hipGetLastError());
     return AZ_OK;

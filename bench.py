@@ -353,7 +353,7 @@ def equal_time_mse(ctx, ref_spp=4096, spp=64):
     accumulated in double precision, of the plain tracer at spp and of the
     guided tracer (learned environment-direction guide, one-sample MIS at
     fraction 0.5 against the reference's cosine sampling, trained on the
-    first 32 spp) at the largest square spp whose render time stays within
+    first quarter of its spp) at the largest square spp whose render time stays within
     the plain render's; also the environment luminance distribution (the
     strongest sampler that does not learn) at equal time, and the guide over
     it.  Wall-clock times around each render call, guide training included."""

@@ -71,7 +71,8 @@ typedef struct {
                                distribution (one-sample MIS; GuideTarget says which directions);
                                the guide starts from seeded weights and trains online */
     f32 GuideFraction;      /* probability of drawing the direction from the guide (0 = 0.5) */
-    u32 GuideTrainSpp;      /* samples per pixel rendered while training (0 = 32) */
+    u32 GuideTrainSpp;      /* samples per pixel rendered while training (0: a quarter of the
+                               render's, at least 4 and at most 32) */
     u32 GuideStepsPerChunk; /* Adam steps after each training chunk (0: spread 256 steps over the
                                training chunks) */
     u32 EnvSampling;        /* environment next-event directions: AZ_ENV_COSINE (the reference's
