@@ -16,6 +16,7 @@
 // (seed, pixel, sample, tree node, draw), so the breadth-first order draws the
 // same numbers as the depth-first CPU oracle.
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -447,6 +448,11 @@ struct GuideIO {
     uint32_t *srecCounters;
     uint32_t srecSegCap;
     const uint32_t *rayRec;
+    // Per record slot, the record's path vertex (sample << 32 | random-stream
+    // node): the order in which the records are drawn from, whatever order
+    // the waves appended them in.
+    uint64_t *recKeys;
+    uint64_t *srecKeys;
 };
 
 // Guide query of a hit: position in the scene box, normal, roughness.
@@ -988,6 +994,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
                 g.wgt = guide_lum(I) * fmaxf(ca, 0.0f) * invPdf;
                 g.pad = 0.0f;
                 GI.recs[idx] = g;
+                GI.recKeys[idx] = ((uint64_t)r.sample << 32) | r.node;
                 sr.rec = idx;
             }
         }
@@ -1069,6 +1076,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
                 g.wgt = lw > 0.0f ? fmaxf(dot(Sd, N), 0.0f) * invPdf / lw : 0.0f;
                 g.pad = 0.0f;
                 GI.srecs[idx] = g;
+                GI.srecKeys[idx] = ((uint64_t)r.sample << 32) | r.node;
                 GI.links[idx] = ScatterLink{0ull, rin, depth};
                 crec = idx;
             }
@@ -1213,16 +1221,31 @@ __global__ __launch_bounds__(kRenderBlock) void k_guide_query(DevScene S, Device
 // (occluded, or whose path delivered nothing) are never drawn.  The prefix
 // of the weights in dense record order, in three passes (block scans, one
 // block over the block sums, offsets):
-__global__ __launch_bounds__(kRenderBlock) void k_rec_prefix_blocks(const GuideSample *__restrict__ recs, SegIn rin,
+// The records of a chunk in the order of their keys (sample, then random-
+// stream node), so that the weighted draws below, and the guide trained on
+// them, do not depend on the order in which waves appended the records:
+// the dense index's slot and key, for a radix sort of the pairs.
+__global__ __launch_bounds__(kRenderBlock) void k_rec_keys(const uint64_t *__restrict__ keys, SegIn rin,
+                                                           uint32_t count, uint64_t *__restrict__ kout,
+                                                           uint32_t *__restrict__ sout) {
+    __shared__ uint32_t soff[kSegs + 1];
+    const uint32_t *off = stage_offsets(rin, soff);
+    const uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t slot = seg_slot(off, rin.nseg, rin.segCap, i);
+    kout[i] = keys[slot];
+    sout[i] = slot;
+}
+
+__global__ __launch_bounds__(kRenderBlock) void k_rec_prefix_blocks(const GuideSample *__restrict__ recs,
+                                                                    const uint32_t *__restrict__ order,
                                                                     uint32_t count, float *__restrict__ cum,
                                                                     float *__restrict__ sums) {
-    __shared__ uint32_t soff[kSegs + 1];
     __shared__ float s[kRenderBlock];
-    const uint32_t *off = stage_offsets(rin, soff);
     const uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
     float w = 0.0f;
     if (i < count) {
-        w = recs[seg_slot(off, rin.nseg, rin.segCap, i)].wgt;
+        w = recs[order[i]].wgt;
         w = w > 0.0f && w < INFINITY ? w : 0.0f;          // NaN, negative and infinite weights are not drawn
     }
     s[threadIdx.x] = w;
@@ -1267,12 +1290,11 @@ __global__ __launch_bounds__(kRenderBlock) void k_rec_prefix_add(float *__restri
 // ones (hash of (seed, slot), splitmix64 finalizer, then the first record
 // whose prefix exceeds u times the total); all weights zero: drawn
 // uniformly with their weights.
-__global__ __launch_bounds__(kRenderBlock) void k_guide_gather_weighted(const GuideSample *__restrict__ recs, SegIn rin,
+__global__ __launch_bounds__(kRenderBlock) void k_guide_gather_weighted(const GuideSample *__restrict__ recs,
+                                                                        const uint32_t *__restrict__ order,
                                                                         uint32_t count, const float *__restrict__ cum,
                                                                         GuideSample *__restrict__ batch, uint32_t n,
                                                                         uint64_t seed) {
-    __shared__ uint32_t soff[kSegs + 1];
-    const uint32_t *off = stage_offsets(rin, soff);
     uint32_t i = blockIdx.x * kRenderBlock + threadIdx.x;
     if (i >= n) return;
     uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(i + 1);
@@ -1281,7 +1303,7 @@ __global__ __launch_bounds__(kRenderBlock) void k_guide_gather_weighted(const Gu
     z ^= z >> 31;
     const float total = cum[count - 1];
     if (!(total > 0.0f)) {
-        batch[i] = recs[seg_slot(off, rin.nseg, rin.segCap, (uint32_t)(z % count))];
+        batch[i] = recs[order[(uint32_t)(z % count)]];
         return;
     }
     const float target = (float)(z >> 40) * 0x1p-24f * total;
@@ -1291,7 +1313,7 @@ __global__ __launch_bounds__(kRenderBlock) void k_guide_gather_weighted(const Gu
         if (cum[mid] > target) hi = mid;
         else lo = mid + 1u;
     }
-    GuideSample g = recs[seg_slot(off, rin.nseg, rin.segCap, lo)];
+    GuideSample g = recs[order[lo]];
     g.wgt = 1.0f;
     batch[i] = g;
 }
@@ -1737,6 +1759,8 @@ struct RenderBuffers {
     DeviceBuffer queries, lobes, recs, batch;   // learned guide
     DeviceBuffer scatLobes, srecs, links, qrec0, qrec1;   // scatter guide: lobes, records, per-slot record of the queues
     DeviceBuffer recCum;                        // prefix of the training records' weights
+    DeviceBuffer recKeys, srecKeys;             // per record slot: its path vertex, the draw order
+    DeviceBuffer recSort, recSortTmp;           // key-ordered records: keys, slots, radix sort workspace
     DeviceBuffer workCounters;                  // counted renders: extend and connect walk counts (6 x u64)
     DeviceBuffer envCdf;                        // environment luminance distribution
     DeviceBuffer batchArgs, batchTotals;        // batched renders: per-scene arguments, queue totals
@@ -1771,7 +1795,8 @@ struct RenderBuffers {
     void forEachQueue(F f) {
         DeviceBuffer *bufs[] = {&q0,    &q1,        &hits,      &shadow, &acc,   &counters, &pixels,  &queries,
                                 &lobes, &recs,      &batch,     &scatLobes, &srecs, &links, &qrec0,  &qrec1,
-                                &recCum, &batchArgs, &batchTotals, &chunkDev, &levelTotals, &workCounters};
+                                &recCum, &batchArgs, &batchTotals, &chunkDev, &levelTotals, &workCounters,
+                                &recKeys, &srecKeys, &recSort, &recSortTmp};
         for (DeviceBuffer *b : bufs) f(*b);
     }
     RenderBuffers() {
@@ -2262,6 +2287,7 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             return gs;
         }
         AZ_HIPR(C, B.recs.alloc((size_t)recCap * sizeof(GuideSample)));
+        AZ_HIPR(C, B.recKeys.alloc((size_t)recCap * sizeof(uint64_t)));
     }
     if (scatGuide) {
         az_status gs = az_guide_create(&C->scatterGuide, C->device, o.Seed ^ 0x7363617474657221ull);
@@ -2270,6 +2296,7 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             return gs;
         }
         AZ_HIPR(C, B.srecs.alloc((size_t)recCap * sizeof(GuideSample)));
+        AZ_HIPR(C, B.srecKeys.alloc((size_t)recCap * sizeof(uint64_t)));
         AZ_HIPR(C, B.links.alloc((size_t)recCap * sizeof(ScatterLink)));
     }
     if (guided) AZ_HIPR(C, B.batch.alloc((size_t)65536 * sizeof(GuideSample)));
@@ -2554,9 +2581,13 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             GuideIO GI{nullptr, nullptr, recCnt, recSegCap, alpha};
             ShadeOut O{qb, (ShadowRay *)B.shadow.ptr, childCnt, shadowCnt, nseg, childSegCap, shadowSegCap};
             const bool guideLevel = o.GuideMaxDepth == 0 || depth <= o.GuideMaxDepth;
-            if (envGuide && training) GI.recs = (GuideSample *)B.recs.ptr;
+            if (envGuide && training) {
+                GI.recs = (GuideSample *)B.recs.ptr;
+                GI.recKeys = (uint64_t *)B.recKeys.ptr;
+            }
             if (scatTraining) {
                 GI.srecs = (GuideSample *)B.srecs.ptr;
+                GI.srecKeys = (uint64_t *)B.srecKeys.ptr;
                 GI.links = (ScatterLink *)B.links.ptr;
                 GI.srecCounters = srecCnt;
                 GI.srecSegCap = recSegCap;
@@ -2705,15 +2736,27 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
                 const u32 pb = (R[g] + kRenderBlock - 1) / kRenderBlock;
                 AZ_HIPR(C, B.recCum.alloc(((size_t)R[g] + pb) * sizeof(float)));
                 float *cum = (float *)B.recCum.ptr, *sums = cum + R[g];
-                hipLaunchKernelGGL(k_rec_prefix_blocks, dim3(pb), dim3(kRenderBlock), 0, st, recs, rin, R[g], cum, sums);
+                // Records in key order (see k_rec_keys).
+                AZ_HIPR(C, B.recSort.alloc((size_t)R[g] * 2 * (sizeof(uint64_t) + sizeof(uint32_t))));
+                uint64_t *kin = (uint64_t *)B.recSort.ptr, *kout = kin + R[g];
+                uint32_t *vin = (uint32_t *)(kout + R[g]), *order = vin + R[g];
+                hipLaunchKernelGGL(k_rec_keys, rg, dim3(kRenderBlock), 0, st,
+                                   (const uint64_t *)(g == 0 ? B.recKeys.ptr : B.srecKeys.ptr), rin, R[g], kin, vin);
+                AZ_HIPR(C, hipGetLastError());
+                size_t tmpBytes = 0;
+                AZ_HIPR(C, rocprim::radix_sort_pairs(nullptr, tmpBytes, kin, kout, vin, order, R[g], 0, 64, st));
+                AZ_HIPR(C, B.recSortTmp.alloc(std::max<size_t>(tmpBytes, 16)));
+                AZ_HIPR(C, rocprim::radix_sort_pairs(B.recSortTmp.ptr, tmpBytes, kin, kout, vin, order, R[g], 0, 64, st));
+                hipLaunchKernelGGL(k_rec_prefix_blocks, dim3(pb), dim3(kRenderBlock), 0, st, recs,
+                                   (const uint32_t *)order, R[g], cum, sums);
                 hipLaunchKernelGGL(k_rec_prefix_sums, dim3(1), dim3(kRenderBlock), 0, st, sums, pb);
                 hipLaunchKernelGGL(k_rec_prefix_add, dim3(pb), dim3(kRenderBlock), 0, st, cum, R[g], (const float *)sums);
                 AZ_HIPR(C, hipGetLastError());
                 for (u32 k = 0; k < stepsPerChunk; ++k) {
                     uint64_t seed = (o.Seed + 0x9E3779B97F4A7C15ull * (uint64_t)(g + 1)) * (uint64_t)(trainRound * 131 + k + 1);
                     hipLaunchKernelGGL(k_guide_gather_weighted, dim3((bn + kRenderBlock - 1) / kRenderBlock),
-                                       dim3(kRenderBlock), 0, st, recs, rin, R[g], (const float *)cum,
-                                       (GuideSample *)B.batch.ptr, bn, seed);
+                                       dim3(kRenderBlock), 0, st, recs, (const uint32_t *)order, R[g],
+                                       (const float *)cum, (GuideSample *)B.batch.ptr, bn, seed);
                     AZ_HIPR(C, hipGetLastError());
                     az_status gs = az_guide_train_on_stream(net, B.batch.ptr, bn, st);
                     if (gs != AZ_OK) {
