@@ -84,7 +84,9 @@ struct UploadInfo {
 void az_release_render_buffers(void *buffers);
 
 // Guide calls ordered on a caller's stream (the renderer's).
-az_status az_guide_infer_on_stream(az_guide *G, const void *d_Queries, u32 N, float *d_Lobes, hipStream_t s);
+// Packed lobes (guide_lobes.h) of the queries of non-negative roughness,
+// ordered on stream s.
+az_status az_guide_infer_on_stream(az_guide *G, const void *d_Queries, u32 N, void *d_Packed, hipStream_t s);
 az_status az_guide_train_on_stream(az_guide *G, const void *d_Samples, u32 N, hipStream_t s);
 
 struct az_gpu_ctx {

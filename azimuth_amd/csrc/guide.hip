@@ -25,6 +25,7 @@
 #include "azimuth_guide.h"
 #include "gpu_context.h"
 #include "guide.h"
+#include "guide_lobes.h"
 
 namespace az {
 
@@ -205,6 +206,7 @@ struct alignas(16) InferLds {
     short w[kHalfCount];
     float bias[5 * kGuideBiasStride];
     float outs[kInferWaves][16][kGuideOutPad];
+    uint32_t idx[kInferWaves][2 * kInferSamples];   // packed form: each wave's gathered query indices
 };
 
 template <int NI, int NO, bool RELU>
@@ -242,82 +244,134 @@ __device__ inline void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The network on one wave's 64 queries: the query of tile g, column col is
+// number at(16 g + col); the first cnt of them are stored (the others are
+// computed on a copy of query at(cnt - 1)).  Plain form: 40 floats per query
+// at lobes + 40 s; packed form: 8 packed lobes at lobes + 64 s bytes.
+template <bool PACKED, class At>
+__device__ inline void infer_batch(InferLds &L, const GuideQuery *__restrict__ q, uint32_t cnt, At at, void *lobes,
+                                   int wave, int lane) {
+    const int col = lane & 15, kq = 4 * (lane >> 4);
+    v4s x[kInferGroups][8], h1[kInferGroups][8], h2[kInferGroups][8];
+#pragma unroll
+    for (int g = 0; g < kInferGroups; ++g) {
+        const uint32_t j = 16 * g + col;
+        const GuideQuery *Q = q + at(j < cnt ? j : cnt - 1);
+        encode_tiles(Q->p[0], Q->p[1], Q->p[2], Q->n[0], Q->n[1], Q->n[2], Q->rough, lane, x[g]);
+    }
+    layer_lds<8, 4, true>(L.w + h_off(0), L.bias + 0 * kGuideBiasStride, x, h1, lane);
+    layer_lds<4, 4, true>(L.w + h_off(1), L.bias + 1 * kGuideBiasStride, h1, h2, lane);
+    layer_lds<4, 4, true>(L.w + h_off(2), L.bias + 2 * kGuideBiasStride, h2, h1, lane);
+    layer_lds<4, 4, true>(L.w + h_off(3), L.bias + 3 * kGuideBiasStride, h1, h2, lane);
+    const short *w4 = L.w + h_off(4);
+    const float *b4 = L.bias + 4 * kGuideBiasStride;
+#pragma unroll
+    for (int g = 0; g < kInferGroups; ++g) {
+        // Output layer of this 16-sample tile, staged in LDS.
+#pragma unroll
+        for (int mt = 0; mt < 3; ++mt) {
+            v4f c = *reinterpret_cast<const v4f *>(b4 + 16 * mt + kq);
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+                c = mfma32(*reinterpret_cast<const v8s *>(w4 + ((mt * 2 + t) * 64 + lane) * 8), h2[g][2 * t],
+                           h2[g][2 * t + 1], c);
+            *reinterpret_cast<v4f *>(&L.outs[wave][col][16 * mt + kq]) = c;
+        }
+        wave_lds_sync();
+        // Lobe k of sample sl for the 128 (sample, lobe) items, two per
+        // lane: the 8 lanes of a sample form its softmax with shuffles, and
+        // consecutive lanes write consecutive lobes.
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int item = lane + 64 * r, sl = item >> 3, k = item & 7;
+            const uint32_t j = 16 * g + sl;
+            const float *o = L.outs[wave][sl];
+            const float logit = o[k];
+            float m = logit;
+            m = fmaxf(m, __shfl_xor(m, 1, 8));
+            m = fmaxf(m, __shfl_xor(m, 2, 8));
+            m = fmaxf(m, __shfl_xor(m, 4, 8));
+            const float e = __builtin_amdgcn_exp2f((logit - m) * 1.44269504f);
+            float sum = e;
+            sum += __shfl_xor(sum, 1, 8);
+            sum += __shfl_xor(sum, 2, 8);
+            sum += __shfl_xor(sum, 4, 8);
+            const float x0 = o[8 + 3 * k], y0 = o[9 + 3 * k], z0 = o[10 + 3 * k];
+            const float len2 = x0 * x0 + y0 * y0 + z0 * z0;
+            const float inv = len2 > 1e-24f ? __builtin_amdgcn_rsqf(len2) : 0.0f;
+            const float kr = fminf(fmaxf(o[32 + k], kGuideKMin), kGuideKMax);
+            if (j < cnt) {
+                const uint32_t s = at(j);
+                const float w = e * __builtin_amdgcn_rcpf(sum), kap = __builtin_amdgcn_exp2f(kr * 1.44269504f);
+                if (PACKED) {
+                    reinterpret_cast<uint2 *>(lobes)[(size_t)s * 8 + k] = lobe_pack(w, kap, x0 * inv, y0 * inv, z0 * inv);
+                } else {
+                    float *dst = reinterpret_cast<float *>(lobes) + (size_t)s * 40 + 5 * k;
+                    dst[0] = w;
+                    dst[1] = x0 * inv;
+                    dst[2] = y0 * inv;
+                    dst[3] = z0 * inv;
+                    dst[4] = kap;
+                }
+            }
+        }
+        wave_lds_sync();   // this tile's reads before the next tile's writes
+    }
+}
+
 // One LDS copy of the weights per block of kInferWaves waves (3 per SIMD at
-// the kernel's 155 VGPRs; the copy and the waves' output tiles take 83 KB
-// of the CU's 160 KB).  After the copy the waves run independently, each
-// looping over its own 64-query groups.
+// the kernel's VGPR count; the copy, the waves' output tiles and index lists
+// take 91 KB of the CU's 160 KB).  After the copy the waves run
+// independently, each looping over its own 64-query chunks.
+// Plain form: every query, 40 floats of lobes each.  Packed form (the
+// renderer's): queries of negative roughness (misses and emitters, which
+// shading never guides) are skipped, the others gathered into full batches of
+// 64 in a per-wave LDS list, and their lobes stored packed; a skipped
+// query's lobes are left as they were.
+template <bool PACKED>
 __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideWeights W, const GuideQuery *__restrict__ q,
-                                                                 uint32_t n, float *__restrict__ lobes) {
+                                                                 uint32_t n, void *__restrict__ lobes) {
     __shared__ InferLds L;
     for (int k = threadIdx.x; k < kHalfCount / 8; k += 64 * kInferWaves)
         reinterpret_cast<uint4 *>(L.w)[k] = reinterpret_cast<const uint4 *>(W.wf)[k];
     for (int k = threadIdx.x; k < 5 * kGuideBiasStride; k += 64 * kInferWaves) L.bias[k] = W.bias[k];
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int col = lane & 15, kq = 4 * (lane >> 4);
-    const uint32_t groups = (n + kInferSamples - 1) / kInferSamples;
-    // Queries past n are computed on a clamped copy and not stored.
-    for (uint32_t it = blockIdx.x * kInferWaves + wave; it < groups; it += gridDim.x * kInferWaves) {
-        const uint32_t base = it * kInferSamples;
-        v4s x[kInferGroups][8], h1[kInferGroups][8], h2[kInferGroups][8];
-#pragma unroll
-        for (int g = 0; g < kInferGroups; ++g) {
-            const uint32_t s = base + 16 * g + col;
-            const GuideQuery *Q = q + (s < n ? s : n - 1);
-            encode_tiles(Q->p[0], Q->p[1], Q->p[2], Q->n[0], Q->n[1], Q->n[2], Q->rough, lane, x[g]);
+    const uint32_t chunks = (n + kInferSamples - 1) / kInferSamples;
+    const uint32_t first = blockIdx.x * kInferWaves + wave, stride = gridDim.x * kInferWaves;
+    if (!PACKED) {
+        for (uint32_t it = first; it < chunks; it += stride) {
+            const uint32_t base = it * kInferSamples;
+            infer_batch<false>(L, q, min(n - base, (uint32_t)kInferSamples),
+                               [base](uint32_t j) { return base + j; }, lobes, wave, lane);
         }
-        layer_lds<8, 4, true>(L.w + h_off(0), L.bias + 0 * kGuideBiasStride, x, h1, lane);
-        layer_lds<4, 4, true>(L.w + h_off(1), L.bias + 1 * kGuideBiasStride, h1, h2, lane);
-        layer_lds<4, 4, true>(L.w + h_off(2), L.bias + 2 * kGuideBiasStride, h2, h1, lane);
-        layer_lds<4, 4, true>(L.w + h_off(3), L.bias + 3 * kGuideBiasStride, h1, h2, lane);
-        const short *w4 = L.w + h_off(4);
-        const float *b4 = L.bias + 4 * kGuideBiasStride;
-#pragma unroll
-        for (int g = 0; g < kInferGroups; ++g) {
-            // Output layer of this 16-sample tile, staged in LDS.
-#pragma unroll
-            for (int mt = 0; mt < 3; ++mt) {
-                v4f c = *reinterpret_cast<const v4f *>(b4 + 16 * mt + kq);
-#pragma unroll
-                for (int t = 0; t < 2; ++t)
-                    c = mfma32(*reinterpret_cast<const v8s *>(w4 + ((mt * 2 + t) * 64 + lane) * 8), h2[g][2 * t],
-                               h2[g][2 * t + 1], c);
-                *reinterpret_cast<v4f *>(&L.outs[wave][col][16 * mt + kq]) = c;
-            }
+        return;
+    }
+    uint32_t *list = L.idx[wave];
+    uint32_t pending = 0;                            // wave-uniform: gathered indices not yet run
+    for (uint32_t it = first;; it += stride) {
+        const bool more = it < chunks;
+        if (more) {
+            const uint32_t s = it * kInferSamples + lane;
+            const bool valid = s < n && q[s].rough >= 0.0f;
+            const uint64_t m = __ballot(valid);
+            if (valid)
+                list[pending + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = s;
+            pending += (uint32_t)__popcll(m);
             wave_lds_sync();
-            // Lobe k of sample sl for the 128 (sample, lobe) items, two per
-            // lane: the 8 lanes of a sample form its softmax with shuffles,
-            // and consecutive lanes write consecutive 20-byte lobes.
-#pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                const int item = lane + 64 * r, sl = item >> 3, k = item & 7;
-                const uint32_t s = base + 16 * g + sl;
-                const float *o = L.outs[wave][sl];
-                const float logit = o[k];
-                float m = logit;
-                m = fmaxf(m, __shfl_xor(m, 1, 8));
-                m = fmaxf(m, __shfl_xor(m, 2, 8));
-                m = fmaxf(m, __shfl_xor(m, 4, 8));
-                const float e = __builtin_amdgcn_exp2f((logit - m) * 1.44269504f);
-                float sum = e;
-                sum += __shfl_xor(sum, 1, 8);
-                sum += __shfl_xor(sum, 2, 8);
-                sum += __shfl_xor(sum, 4, 8);
-                const float x0 = o[8 + 3 * k], y0 = o[9 + 3 * k], z0 = o[10 + 3 * k];
-                const float len2 = x0 * x0 + y0 * y0 + z0 * z0;
-                const float inv = len2 > 1e-24f ? __builtin_amdgcn_rsqf(len2) : 0.0f;
-                const float kr = fminf(fmaxf(o[32 + k], kGuideKMin), kGuideKMax);
-                if (s < n) {
-                    float *dst = lobes + (size_t)s * 40 + 5 * k;
-                    dst[0] = e * __builtin_amdgcn_rcpf(sum);
-                    dst[1] = x0 * inv;
-                    dst[2] = y0 * inv;
-                    dst[3] = z0 * inv;
-                    dst[4] = __builtin_amdgcn_exp2f(kr * 1.44269504f);
-                }
-            }
-            wave_lds_sync();   // this tile's reads before the next tile's writes
         }
+        if (pending >= (uint32_t)kInferSamples || (!more && pending > 0u)) {
+            const uint32_t cnt = min(pending, (uint32_t)kInferSamples);
+            infer_batch<true>(L, q, cnt, [list](uint32_t j) { return list[j]; }, lobes, wave, lane);
+            // The indices past the batch move to the front of the list.
+            const uint32_t rest = pending - cnt;
+            const uint32_t v = (uint32_t)lane < rest ? list[kInferSamples + lane] : 0u;
+            wave_lds_sync();
+            if ((uint32_t)lane < rest) list[lane] = v;
+            wave_lds_sync();
+            pending = rest;
+        }
+        if (!more) break;
     }
 }
 
@@ -670,16 +724,46 @@ az_status az_guide_get_grad(az_guide *G, f32 *Grad, u32 Count) {
     return AZ_OK;
 }
 
+// Persistent blocks, at most one per CU (the LDS a block holds allows one),
+// each wave looping over chunks of kInferSamples queries.
+static az_status launchInfer(az_guide *G, const void *d_Queries, u32 N, void *d_Lobes, bool packed) {
+    const u32 chunks = (N + kInferSamples - 1) / kInferSamples;
+    dim3 grid(std::min<u32>((chunks + kInferWaves - 1) / kInferWaves, (u32)G->cuCount));
+    if (packed)
+        hipLaunchKernelGGL(k_guide_infer<true>, grid, dim3(64 * kInferWaves), 0, G->stream, weightsOf(G),
+                           (const GuideQuery *)d_Queries, N, d_Lobes);
+    else
+        hipLaunchKernelGGL(k_guide_infer<false>, grid, dim3(64 * kInferWaves), 0, G->stream, weightsOf(G),
+                           (const GuideQuery *)d_Queries, N, d_Lobes);
+    AZ_GHIP(G, hipGetLastError());
+    return AZ_OK;
+}
+
 az_status az_guide_infer_device(az_guide *G, const void *d_Queries, u32 N, f32 *d_Lobes) {
     if (!G || (N && (!d_Queries || !d_Lobes))) return AZ_ERR_INVALID_ARGUMENT;
     if (N == 0) return AZ_OK;
-    // Persistent blocks, one per CU (the LDS a block holds allows one), each
-    // wave looping over groups of kInferSamples queries.
-    const u32 groups = (N + kInferSamples - 1) / kInferSamples;
-    dim3 grid(std::min<u32>((groups + kInferWaves - 1) / kInferWaves, (u32)G->cuCount));
-    hipLaunchKernelGGL(k_guide_infer, grid, dim3(64 * kInferWaves), 0, G->stream, weightsOf(G), (const GuideQuery *)d_Queries, N,
-                       d_Lobes);
-    AZ_GHIP(G, hipGetLastError());
+    return launchInfer(G, d_Queries, N, d_Lobes, false);
+}
+
+az_status az_guide_infer_packed_device(az_guide *G, const void *d_Queries, u32 N, void *d_Packed) {
+    if (!G || (N && (!d_Queries || !d_Packed))) return AZ_ERR_INVALID_ARGUMENT;
+    if (N == 0) return AZ_OK;
+    return launchInfer(G, d_Queries, N, d_Packed, true);
+}
+
+az_status az_guide_infer_packed(az_guide *G, const f32 *Queries, u32 N, u32 *Packed) {
+    if (!G || (N && (!Queries || !Packed))) return AZ_ERR_INVALID_ARGUMENT;
+    if (N == 0) return AZ_OK;
+    const size_t pb = (size_t)N * kGuideLobeBytes;
+    AZ_GHIP(G, G->queries.alloc((size_t)N * sizeof(GuideQuery)));
+    AZ_GHIP(G, G->lobes.alloc(pb));
+    AZ_GHIP(G, hipMemcpyAsync(G->queries.ptr, Queries, (size_t)N * sizeof(GuideQuery), hipMemcpyHostToDevice, G->stream));
+    // Skipped queries keep what the caller's array held.
+    AZ_GHIP(G, hipMemcpyAsync(G->lobes.ptr, Packed, pb, hipMemcpyHostToDevice, G->stream));
+    az_status s = launchInfer(G, G->queries.ptr, N, G->lobes.ptr, true);
+    if (s != AZ_OK) return s;
+    AZ_GHIP(G, hipMemcpyAsync(Packed, G->lobes.ptr, pb, hipMemcpyDeviceToHost, G->stream));
+    AZ_GHIP(G, hipStreamSynchronize(G->stream));
     return AZ_OK;
 }
 
@@ -785,10 +869,10 @@ az_status az_guide_set_learning_rate(az_guide *G, f32 Lr) {
 }  // extern "C"
 
 // The renderer's calls: the guide's kernels ordered on the render stream.
-az_status az_guide_infer_on_stream(az_guide *G, const void *d_Queries, u32 N, float *d_Lobes, hipStream_t s) {
+az_status az_guide_infer_on_stream(az_guide *G, const void *d_Queries, u32 N, void *d_Packed, hipStream_t s) {
     hipStream_t own = G->stream;
     G->stream = s;
-    az_status st = az_guide_infer_device(G, d_Queries, N, d_Lobes);
+    az_status st = az_guide_infer_packed_device(G, d_Queries, N, d_Packed);
     G->stream = own;
     return st;
 }

@@ -27,6 +27,7 @@
 #include "device_trace.h"
 #include "gpu_context.h"
 #include "azimuth_guide.h"
+#include "guide_lobes.h"
 #include "guide.h"
 #include "host_parallel.h"
 #include "shade_math.h"
@@ -433,7 +434,7 @@ bool render_trace_variant_exists(int block, int waves, int loop) {
 // ---- shade ------------------------------------------------------------------------
 // Learned guide inputs and training outputs of the shade kernel.
 struct GuideIO {
-    const float *lobes;              // per queued ray: 8 x {w, mu, kappa}, or null (plain sampling)
+    const uint2 *lobes;              // per queued ray: 8 packed lobes (guide_lobes.h), or null (plain sampling)
     GuideSample *recs;               // training samples (kSegs segments), or null when not training
     uint32_t *recCounters;           // kSegs counters, kSegStride apart
     uint32_t recSegCap;
@@ -442,7 +443,7 @@ struct GuideIO {
     // training records, their links and kSegs counters (null when not
     // training); per slot of the current queue, the record whose subtree the
     // ray is in (null at the first level).
-    const float *scatLobes;
+    const uint2 *scatLobes;
     GuideSample *srecs;
     ScatterLink *links;
     uint32_t *srecCounters;
@@ -468,39 +469,48 @@ __device__ inline void guide_position(const DevScene &S, V3 P, float *p) {
     p[2] = fminf(fmaxf((P.z - S.boxMin[2]) * S.boxInv[2], 0.0f), 1.0f);
 }
 
-// Mixture pdf q(w) of 8 vMF lobes, each kappa / (2 pi (1 - e^-2kappa)) *
-// exp(kappa (mu.w - 1)) (Jakob 2012).
-__device__ inline float guide_pdf(const float *lb, V3 w) {
-    float q = 0.0f;
+// Mixture pdf q(w) of 8 packed vMF lobes: the weights divided by their sum,
+// each lobe kappa / (2 pi (1 - e^-2kappa)) exp(kappa (mu.w - 1)) (Jakob 2012).
+__device__ inline float guide_pdf(const uint2 *lb, V3 w) {
+    float q = 0.0f, wsum = 0.0f;
     for (int k = 0; k < 8; ++k) {
-        const float *l = lb + 5 * k;
-        float kap = l[4];
-        float c = l[1] * w.x + l[2] * w.y + l[3] * w.z;
-        q += l[0] * kap / (2.0f * kPiF * -expm1f(-2.0f * kap)) * expf(kap * (c - 1.0f));
+        const uint2 l = lb[k];
+        const float wk = lobe_weight(l), kap = lobe_kappa(l);
+        float mx, my, mz;
+        lobe_mean(l, mx, my, mz);
+        const float c = mx * w.x + my * w.y + mz * w.z;
+        wsum += wk;
+        q += wk * kap / (2.0f * kPiF * -expm1f(-2.0f * kap)) * expf(kap * (c - 1.0f));
     }
-    return q;
+    return wsum > 0.0f ? q / wsum : 0.0f;
 }
 
-// Draws a direction from the mixture with three uniforms: lobe choice, then
-// Jakob's inversion for the cosine to the mean and a uniform azimuth.
-__device__ inline V3 guide_sample(const float *lb, float u0, float u1, float u2) {
+// Draws a direction from the mixture with three uniforms: lobe choice by the
+// weights, then Jakob's inversion for the cosine to the mean and a uniform
+// azimuth.
+__device__ inline V3 guide_sample(const uint2 *lb, float u0, float u1, float u2) {
+    float wsum = 0.0f;
+    for (int k = 0; k < 8; ++k) wsum += lobe_weight(lb[k]);
+    const float t = u0 * wsum;
     int k = 0;
-    float c = lb[0];
-    while (k < 7 && u0 >= c) {
+    float c = lobe_weight(lb[0]);
+    while (k < 7 && t >= c) {
         ++k;
-        c += lb[5 * k];
+        c += lobe_weight(lb[k]);
     }
-    const float *l = lb + 5 * k;
-    V3 mu = v3(l[1], l[2], l[3]);
-    float kap = l[4];
+    const uint2 l = lb[k];
+    float mx, my, mz;
+    lobe_mean(l, mx, my, mz);
+    V3 mu = v3(mx, my, mz);
+    float kap = lobe_kappa(l);
     float xi = 1.0f - u1;                                            // (0, 1]
     float wz = 1.0f + logf(xi + (1.0f - xi) * expf(-2.0f * kap)) / kap;
     wz = fminf(fmaxf(wz, -1.0f), 1.0f);
     float s = sqrtf(fmaxf(0.0f, 1.0f - wz * wz));
-    V3 t, b;
-    orthonormal_basis(mu, t, b);
+    V3 tb, b;
+    orthonormal_basis(mu, tb, b);
     float ph = 2.0f * kPiF * u2;
-    return normalize(mu * wz + t * (s * cosf(ph)) + b * (s * sinf(ph)));
+    return normalize(mu * wz + tb * (s * cosf(ph)) + b * (s * sinf(ph)));
 }
 
 // Outputs of shade: the next level's rays and the shadow rays, both
@@ -930,7 +940,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
     // estimator unbiased: the cosine term covers every direction with
     // non-zero contribution.
     if (S.useEnv) {
-        const float *lb = GUIDED && GI.lobes ? GI.lobes + (size_t)i * 40 : nullptr;
+        const uint2 *lb = GUIDED && GI.lobes ? GI.lobes + (size_t)i * 8 : nullptr;
         const bool lum = cp.envLuminance != 0u;
         V3 L;
         bool fromGuide = false;
@@ -1009,7 +1019,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
     // keeps the estimator unbiased: the base distribution covers every
     // direction with non-zero BRDF.
     if (S.pathTrace) {
-        const float *sl = GUIDED && GI.scatLobes ? GI.scatLobes + (size_t)i * 40 : nullptr;
+        const uint2 *sl = GUIDED && GI.scatLobes ? GI.scatLobes + (size_t)i * 8 : nullptr;
         bool fromGuide = false;
         if (sl) fromGuide = rng.uniform() < GI.alpha;
         V3 Sd;
@@ -1195,8 +1205,11 @@ __global__ __launch_bounds__(kRenderBlock) void k_guide_query(DevScene S, Device
     if (i >= n) return;
     const PathRay r = q[seg_slot(off, qin.nseg, qin.segCap, i)];
     const PathHit h = hits[i];
+    // Misses and emitters are never guided: roughness -1 makes the
+    // inference skip them.
     GuideQuery g;
     memset(&g, 0, sizeof(g));
+    g.rough = -1.0f;
     if (h.prim != kEmpty) {
         const DevItemShade sh = S.itemShade[h.item];
         if (!sh.isEmitter) {
@@ -2601,23 +2614,22 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
                                    (const PathHit *)B.hits.ptr, count, (GuideQuery *)B.queries.ptr);
                 AZ_HIPR(C, hipGetLastError());
                 if (envGuide) {
-                    AZ_HIPR(C, B.lobes.alloc((size_t)count * 40 * sizeof(float)));
-                    az_status gs = az_guide_infer_on_stream(C->guide, B.queries.ptr, count, (float *)B.lobes.ptr, st);
+                    AZ_HIPR(C, B.lobes.alloc((size_t)count * kGuideLobeBytes));
+                    az_status gs = az_guide_infer_on_stream(C->guide, B.queries.ptr, count, B.lobes.ptr, st);
                     if (gs != AZ_OK) {
                         C->lastError = "guide inference failed";
                         return gs;
                     }
-                    GI.lobes = (const float *)B.lobes.ptr;
+                    GI.lobes = (const uint2 *)B.lobes.ptr;
                 }
                 if (scatGuide) {
-                    AZ_HIPR(C, B.scatLobes.alloc((size_t)count * 40 * sizeof(float)));
-                    az_status gs = az_guide_infer_on_stream(C->scatterGuide, B.queries.ptr, count,
-                                                            (float *)B.scatLobes.ptr, st);
+                    AZ_HIPR(C, B.scatLobes.alloc((size_t)count * kGuideLobeBytes));
+                    az_status gs = az_guide_infer_on_stream(C->scatterGuide, B.queries.ptr, count, B.scatLobes.ptr, st);
                     if (gs != AZ_OK) {
                         C->lastError = "guide inference failed";
                         return gs;
                     }
-                    GI.scatLobes = (const float *)B.scatLobes.ptr;
+                    GI.scatLobes = (const uint2 *)B.scatLobes.ptr;
                 }
             }
             if (guided)

@@ -63,6 +63,16 @@ class Guide:
         self._check(self.lib.az_guide_infer(self.h, q.ctypes.data, n, out.ctypes.data), "az_guide_infer")
         return out.reshape(n, 8, 5)
 
+    def infer_packed(self, p01, nrm, rough, fill=0):
+        """The renderer's packed lobes: (n, 8, 2) uint32 words per query;
+        queries of negative roughness keep the fill value."""
+        n = len(p01)
+        q = np.zeros((n, 8), np.float32)
+        q[:, 0:3], q[:, 3:6], q[:, 6] = p01, nrm, rough
+        out = np.full((n, 16), fill, np.uint32)
+        self._check(self.lib.az_guide_infer_packed(self.h, q.ctypes.data, n, out.ctypes.data), "az_guide_infer_packed")
+        return out.reshape(n, 8, 2)
+
     def train(self, p01, nrm, rough, dirs, wgt, update=True):
         n = len(p01)
         s = np.zeros((n, 12), np.float32)

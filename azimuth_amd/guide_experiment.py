@@ -5,7 +5,10 @@ architectural interior (environment with a sun lobe, 8 spots):
         [--spp 64] [--seed 1] [--seeds 4]
 
 The reference is a plain render with --ref-spp samples per pixel accumulated
-in double precision (chunks of 256 spp with distinct seeds).  The plain tracer renders --spp samples per pixel (time
+in double precision (chunks of 256 spp with distinct seeds), its environment
+directions drawn from the map's luminance distribution; its own remaining
+noise, estimated from its two halves, is reported and subtracted in the
+'net' ratios.  The plain tracer renders --spp samples per pixel (time
 T); the guided tracer (environment directions from the learned guide mixed
 50/50 with the reference's cosine sampling, trained online during its first
 32 samples per pixel) renders the largest square spp whose measured time stays
@@ -66,15 +69,25 @@ def measure(ctx, cam, filt, ref, seeds, spp, guide, env=N.AZ_ENV_COSINE, max_dep
 
 
 def reference(ctx, cam, filt, seed, spp):
-    """Plain render of spp samples per pixel in double precision, as chunks
-    of 256 spp with distinct seeds (sums of colour and weight add up)."""
-    chunk = min(spp, 256)
-    acc = None
-    for i in range(spp // chunk):
-        px = ctx.render_f64(cam, filt=filt, seed=seed + i, spp=chunk)
-        acc = px if acc is None else acc + px
-        print(f"reference chunk {i + 1}/{spp // chunk} ({ctx.last_render_s:.1f}s)", file=sys.stderr, flush=True)
-    return resolve(acc)
+    """Plain render of spp samples per pixel in double precision with the
+    environment's luminance distribution (unbiased, the same expectation as
+    the cosine sampling, far less variance on this scene), as two halves of
+    chunks of 256 spp with distinct seeds (sums of colour and weight add up).
+    Returns the image and E[(a - b)^2] / 4 over the pixels of the two halves'
+    images a and b: the variance left in the reference, which adds to every
+    MSE measured against it."""
+    chunk = min(spp // 2, 256)
+    halves = []
+    for h in range(2):
+        acc = None
+        for i in range(spp // (2 * chunk)):
+            px = ctx.render_f64(cam, filt=filt, seed=seed + 100 * h + i, spp=chunk, env_sampling=N.AZ_ENV_LUMINANCE)
+            acc = px if acc is None else acc + px
+            print(f"reference half {h} chunk {i + 1}/{spp // (2 * chunk)} ({ctx.last_render_s:.1f}s)",
+                  file=sys.stderr, flush=True)
+        halves.append(acc)
+    noise = float(np.mean((resolve(halves[0]) - resolve(halves[1])) ** 2)) / 4.0
+    return resolve(halves[0] + halves[1]), noise
 
 
 def main(argv=None):
@@ -93,9 +106,9 @@ def main(argv=None):
     ctx.upload_scene(sc)
     ctx.render(cam, filt=filt, spp=1, row_end=4)                      # warm-up
     t0 = time.time()
-    ref = reference(ctx, cam, filt, 1000 * (a.seed + 1), a.ref_spp)
+    ref, ref_noise = reference(ctx, cam, filt, 1000 * (a.seed + 1), a.ref_spp)
     t_ref = time.time() - t0
-    print(f"reference {a.ref_spp} spp in {t_ref:.1f}s", file=sys.stderr, flush=True)
+    print(f"reference {a.ref_spp} spp in {t_ref:.1f}s, its own noise {ref_noise:.3e}", file=sys.stderr, flush=True)
     timed(ctx, cam, filt, a.seed, a.spp, False)                        # warm-up at full size
     mse_plain, t_plain, per_plain = measure(ctx, cam, filt, ref, seeds, a.spp, False)
     print(f"plain {a.spp} spp {t_plain:.2f}s mse {mse_plain:.4e}", file=sys.stderr, flush=True)
@@ -127,18 +140,22 @@ def main(argv=None):
     out = {
         "experiment": "equal-time MSE vs reference, guided vs plain",
         "workload": f"architectural interior {W}x{H} ({sc.triangle_count} triangles), Gaussian filter, depth 6",
-        "reference": {"spp": a.ref_spp, "accumulation": "f64", "seconds": round(t_ref, 2)},
+        "reference": {"spp": a.ref_spp, "accumulation": "f64", "environment_sampling": "luminance",
+                      "seconds": round(t_ref, 2), "noise_mse": ref_noise},
         "seeds": seeds,
         "plain": {"spp": a.spp, "seconds": round(t_plain, 3), "mse": mse_plain, "mse_per_seed": per_plain},
         "guided_same_spp": {"spp": a.spp, "seconds": round(t_g_same, 3), "mse": mse_g_same},
         "guided_equal_time": {"spp": spp_eq, "seconds": round(t_g_eq, 3), "mse": mse_g_eq, "mse_per_seed": per_g_eq},
         "mse_ratio_equal_time": round(mse_plain / mse_g_eq, 3),
+        "net_mse_ratio_equal_time": round((mse_plain - ref_noise) / (mse_g_eq - ref_noise), 3),
         "mse_ratio_time_normalized": round((mse_plain * t_plain) / (mse_g_eq * t_g_eq), 3),
         "mse_ratio_same_spp": round(mse_plain / mse_g_same, 3),
         "guided_to_depth3_equal_time": {"spp": spp_d3, "seconds": round(t_d3, 3), "mse": mse_d3,
                                         "mse_ratio": round(mse_plain / mse_d3, 3)},
         "luminance_env_sampling": {"spp": a.spp, "seconds": round(t_lum, 3), "mse": mse_lum},
         "guided_over_luminance": {"spp": a.spp, "seconds": round(t_glum, 3), "mse": mse_glum},
+        "net_mse_ratio_luminance_over_guided_over_luminance_same_spp":
+            round((mse_lum - ref_noise) / (mse_glum - ref_noise), 3),
         "scatter_guide_same_spp": {"spp": a.spp, "seconds": round(t_sc, 3), "mse": mse_sc},
         "both_guides_same_spp": {"spp": a.spp, "seconds": round(t_both, 3), "mse": mse_both},
     }

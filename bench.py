@@ -348,34 +348,55 @@ def render_other(ctx):
     return out
 
 
-def equal_time_mse(ctx, ref_spp=4096, spp=64):
-    """The guided-interior workload: MSE against a ref_spp plain render
-    accumulated in double precision, of the plain tracer at spp and of the
-    guided tracer (learned environment-direction guide, one-sample MIS at
-    fraction 0.5 against the reference's cosine sampling, trained on the
-    first quarter of its spp) at the largest square spp whose render time stays within
-    the plain render's; also the environment luminance distribution (the
-    strongest sampler that does not learn) at equal time, and the guide over
-    it.  Wall-clock times around each render call, guide training included."""
+def equal_time_mse(ctx, ref_spp=4096, spp=64, seeds=(1, 12)):
+    """The guided-interior workload: MSE against a ref_spp render accumulated
+    in double precision, of the plain tracer at spp and of the guided tracer
+    (learned environment-direction guide, one-sample MIS at fraction 0.5
+    against the reference's cosine sampling, trained on the first quarter of
+    its spp) at the largest square spp whose render time stays within the
+    plain render's; also the environment luminance distribution (the strongest
+    sampler that does not learn) at equal time, and the guide over it.  Wall-
+    clock times around each render call, guide training included; each MSE is
+    the mean over the given seeds.
+
+    The reference is the plain tracer with the luminance distribution for the
+    environment (unbiased, the same expectation as the cosine sampling, about
+    60x less variance per sample on this scene), rendered as two halves of
+    ref_spp/2 samples with distinct seeds: E[(a - b)^2] / 4 over the pixels
+    estimates the variance left in their mean, which adds to every MSE
+    measured against it; 'mse_net' subtracts it."""
     import math
     from azimuth_amd import native as N, workloads
     sc, cam = workloads.interior_scene(1920, 1080, spp=spp, max_depth=6)
     ctx.upload_scene(sc)
     filt = N.FilterType_Gaussian
+    resolve = lambda px: px[:, :3] / np.maximum(px[:, 3:4], 1e-300)
 
     def run(seed, n, guide=False, env=N.AZ_ENV_COSINE):
         px = ctx.render_f64(cam, filt=filt, seed=seed, spp=n, guide=guide, env_sampling=env)
-        return px[:, :3] / np.maximum(px[:, 3:4], 1e-300), ctx.last_render_s
+        return resolve(px), ctx.last_render_s
 
     t0 = time.time()
-    acc = None
-    for i in range(ref_spp // 256):
-        px = ctx.render_f64(cam, filt=filt, seed=5000 + i, spp=256)
-        acc = px if acc is None else acc + px
-    ref = acc[:, :3] / np.maximum(acc[:, 3:4], 1e-300)
+    halves = []
+    for h in range(2):
+        acc = None
+        for i in range(ref_spp // 512):
+            px = ctx.render_f64(cam, filt=filt, seed=5000 + 100 * h + i, spp=256, env_sampling=N.AZ_ENV_LUMINANCE)
+            acc = px if acc is None else acc + px
+        halves.append(acc)
+    ref = resolve(halves[0] + halves[1])
+    ref_noise = float(np.mean((resolve(halves[0]) - resolve(halves[1])) ** 2)) / 4.0
     t_ref = time.time() - t0
-    log(f"reference {ref_spp} spp in {t_ref:.1f}s")
-    mse = lambda img: float(np.mean((img - ref) ** 2))
+    log(f"reference {ref_spp} spp in {t_ref:.1f}s, its own noise {ref_noise:.3e}")
+
+    def mse(n, guide=False, env=N.AZ_ENV_COSINE):
+        vals, ts = [], []
+        for sd in seeds:
+            img, t = run(sd, n, guide, env)
+            vals.append(float(np.mean((img - ref) ** 2)))
+            ts.append(t)
+        m = float(np.mean(vals))
+        return {"spp": n, "seconds": round(float(np.mean(ts)), 3), "mse": m, "mse_net": m - ref_noise}
 
     def equal_spp(guide, env, t_budget):
         _, t_same = run(3, spp, guide, env)
@@ -387,19 +408,19 @@ def equal_time_mse(ctx, ref_spp=4096, spp=64):
         return s * s
 
     run(1, spp)                                           # warm-up
-    plain, t_plain = run(1, spp)
     res = {"workload": "guided architectural interior, 1920x1080, depth 6, Gaussian filter",
-           "reference": {"spp": ref_spp, "accumulation": "f64", "seconds": round(t_ref, 1)},
-           "plain": {"spp": spp, "seconds": round(t_plain, 3), "mse": mse(plain)}}
+           "reference": {"spp": ref_spp, "accumulation": "f64", "environment_sampling": "luminance",
+                         "seconds": round(t_ref, 1), "noise_mse": ref_noise},
+           "seeds": list(seeds), "guide_target": "environment NEE", "plain": mse(spp)}
+    t_plain = res["plain"]["seconds"]
     for name, guide, env in (("guided", True, N.AZ_ENV_COSINE), ("luminance", False, N.AZ_ENV_LUMINANCE),
                              ("guided_over_luminance", True, N.AZ_ENV_LUMINANCE)):
-        n = equal_spp(guide, env, t_plain)
-        img, t = run(1, n, guide, env)
-        res[name] = {"spp": n, "seconds": round(t, 3), "mse": mse(img)}
+        res[name] = mse(equal_spp(guide, env, t_plain), guide, env)
         log(f"{name}: {res[name]}")
+    ratio = lambda a, b: round(res[a]["mse_net"] / max(res[b]["mse_net"], 1e-30), 3)
     res["mse_ratio_plain_over_guided"] = round(res["plain"]["mse"] / res["guided"]["mse"], 3)
-    res["mse_ratio_luminance_over_guided_over_luminance"] = round(
-        res["luminance"]["mse"] / res["guided_over_luminance"]["mse"], 3)
+    res["net_mse_ratio_plain_over_guided"] = ratio("plain", "guided")
+    res["net_mse_ratio_luminance_over_guided_over_luminance"] = ratio("luminance", "guided_over_luminance")
     return res
 
 

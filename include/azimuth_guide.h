@@ -40,6 +40,13 @@ az_status az_guide_set_learning_rate(az_guide *Guide, f32 LearningRate);
 /* Host and device variants; N queries -> N x 40 lobe floats. */
 az_status az_guide_infer(az_guide *Guide, const f32 *Queries, u32 N, f32 *Lobes);
 az_status az_guide_infer_device(az_guide *Guide, const void *d_Queries, u32 N, f32 *d_Lobes);
+/* The renderer's form: N queries -> N x 16 words of packed lobes (8 bytes a
+ * lobe: word 0 = half-precision weight | half-precision kappa << 16, word 1 =
+ * the mean direction's octahedral map, x | y << 16 as unsigned 16-bit fixed
+ * point over [-1, 1]).  Queries of negative roughness are skipped and their
+ * 16 words left as the caller's array held them. */
+az_status az_guide_infer_packed(az_guide *Guide, const f32 *Queries, u32 N, u32 *Packed);
+az_status az_guide_infer_packed_device(az_guide *Guide, const void *d_Queries, u32 N, void *d_Packed);
 /* One step on N samples (N a multiple of 16): gradient of the mean loss,
  * then an Adam update when ApplyUpdate is non-zero.  MeanLoss may be NULL. */
 az_status az_guide_train(az_guide *Guide, const f32 *Samples, u32 N, b32 ApplyUpdate, f32 *MeanLoss);

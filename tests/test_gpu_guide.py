@@ -117,6 +117,62 @@ def test_inference_matches_reference(guide):
     assert np.max(np.abs(lobes[:, :, 0] - pi64)) < 0.05
 
 
+def half_to_float(h):
+    return np.asarray(h, np.uint16).view(np.float16).astype(np.float64)
+
+
+def octahedral_decode(u, v):
+    """Meyer et al. 2010: the point (x, y) of the unit octahedron's map, with
+    z = 1 - |x| - |y| and the lower half folded over the diagonals."""
+    x = u.astype(np.float64) * (2.0 / 65535.0) - 1.0
+    y = v.astype(np.float64) * (2.0 / 65535.0) - 1.0
+    z = 1.0 - np.abs(x) - np.abs(y)
+    fold = z < 0
+    fx = (1.0 - np.abs(y)) * np.where(x >= 0, 1.0, -1.0)
+    fy = (1.0 - np.abs(x)) * np.where(y >= 0, 1.0, -1.0)
+    x, y = np.where(fold, fx, x), np.where(fold, fy, y)
+    d = np.stack([x, y, z], -1)
+    return d / np.linalg.norm(d, axis=-1, keepdims=True)
+
+
+def test_packed_lobes_match_the_float_lobes(guide):
+    """The renderer's packed lobes (half-precision weight and kappa,
+    octahedral mean direction) decode to the float lobes of the same queries
+    within the formats' rounding, and queries of negative roughness (misses,
+    emitters) are skipped: their words keep the caller's fill."""
+    Ws, bs = R.init_params(11)
+    g = np.random.default_rng(5)
+    bs = [b + g.normal(0, 0.3, b.shape).astype(np.float32) for b in bs]
+    guide.set_params(pack_params(Ws, bs))
+    p, nrm, rough, _, _ = random_batch(3001, 6)
+    skip = g.random(3001) < 0.4
+    skip[:70] = True                  # a whole first chunk of 64 skipped, and more
+    skip[-5:] = False
+    rough_q = np.where(skip, -1.0, rough).astype(np.float32)
+    want = guide.infer(p, nrm, rough)
+    fill = 0xDEADBEEF
+    got = guide.infer_packed(p, nrm, rough_q, fill=fill)
+    assert np.all(got[skip] == fill)
+    keep = ~skip
+    w = got[keep][:, :, 0]
+    pi = half_to_float(w & 0xFFFF)
+    kappa = half_to_float(w >> 16)
+    mu = octahedral_decode(got[keep][:, :, 1] & 0xFFFF, got[keep][:, :, 1] >> 16)
+    ref = want[keep]
+    # half precision: 2^-11 relative (weights below the half normal range
+    # are absolute 2^-25); the mean within about 1e-4 rad.
+    np.testing.assert_allclose(pi, ref[:, :, 0], rtol=5e-4, atol=6e-8)
+    np.testing.assert_allclose(kappa, ref[:, :, 4], rtol=5e-4)
+    # (the float lobes' means are normalized with a 1-ulp reciprocal square
+    # root: renormalized here, or their 1e-7 length error alone reads as
+    # 4.5e-4 rad through arccos)
+    mref = ref[:, :, 1:4].astype(np.float64)
+    mref /= np.linalg.norm(mref, axis=-1, keepdims=True)
+    ang = np.arccos(np.clip(np.sum(mu * mref, -1), -1.0, 1.0))
+    assert ang.max() < 1e-4, ang.max()
+    assert np.all(np.abs(np.sum(pi, axis=1) - 1.0) < 4e-3)
+
+
 def grads_bf16(Ws, bs, x, w, wgt):
     """The float64 reference's backward with bf16 rounding where the kernels
     round: activations, the output gradient and each back-propagated

@@ -316,24 +316,13 @@ def test_perlin_lattice_points_and_range():
 # ---------------------------------------------------------------- scene import
 
 
-def _reference_bvh_triangles_only(p0, idx, depth=0):
-    """ConstructBVHTrianglesOnly (bvh.cpp:223-328) on the triangles idx
-    (their P0 in p0, float32): midpoint of the node box on axis depth % 3,
-    the partition loop from index 1 swapping left items to the front, half
+def _bvh_shape(tris, idx, depth=0):
+    """ConstructBVHTrianglesOnly (bvh.cpp:223-328) on the triangles idx, in
+    Python: the node box over all three vertices of each triangle
+    (bvh.cpp:242-277), the split at the box centre on axis depth % 3, the
+    partition loop from index 1 swapping left items to the front, a half
     split when a side is empty.  Returns the tree as nested (left, right)
     tuples with triangle indices at the leaves."""
-    idx = list(idx)
-    if len(idx) == 1:
-        return idx[0]
-    dim = depth % 3
-    lo = np.min(p0[idx][:, dim])
-    hi = np.max(p0[idx][:, dim])
-    return lo, hi, dim, idx
-
-
-def _bvh_shape(tris, idx, depth=0):
-    """The tree of _reference_bvh_triangles_only, with the node box over all
-    three vertices of each triangle (bvh.cpp:242-277)."""
     idx = list(idx)
     if len(idx) == 1:
         return idx[0]
