@@ -377,194 +377,364 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_guide_infer(GuideWeights W
 
 // Training pass 1: forward, loss gradient, backward through the activations.
 // Writes H_l^T and dY_l^T (features x batch, bf16) for the weight-gradient
-// GEMMs, the bias gradients (atomics) and the per-sample loss.
-__global__ __launch_bounds__(256) void k_guide_train_pass1(GuideWeights W, const GuideSample *__restrict__ smp,
-                                                           uint32_t n, float invN, GuideTrainBuffers T) {
-    __shared__ float outs[4][16][kGuideOutPad];
-    // n is a multiple of 16 (checked on the host), so a wave is either wholly
-    // inside the batch or wholly past it; the latter computes on a clamped
-    // copy of the last sample, reaches every barrier and stores nothing.
+// pass and the per-sample loss.  Persistent blocks of kTrainWaves waves hold
+// every layer's weights in LDS, the forward copy as 16x16x32 A fragments
+// (W^T) and the backward copy as 16x16x16 A fragments (W), each fragment 64
+// consecutive lane words (no bank conflicts); each wave then loops over
+// groups of 16 samples (n is a multiple of 16, checked on the host).
+constexpr int kTrainWaves = 8;
+
+struct alignas(16) TrainLds {
+    short wf[kHalfCount];                      // forward: W^T fragments (mt, K-block of 32), lane, 8 values
+    short wbf[kHalfCount];                     // backward: W fragments (in tile, out tile), lane, 4 values
+    float bias[5 * kGuideBiasStride];
+    float outs[kTrainWaves][16][kGuideOutPad];
+};
+
+template <int NI, int NO, bool RELU>
+__device__ inline void layer_lds1(const short *w, const float *b, const v4s *x, v4f *acc, v4s *y, int lane) {
+    const int kq = 4 * (lane >> 4);
+#pragma unroll
+    for (int mt = 0; mt < NO; ++mt) {
+        v4f c = *reinterpret_cast<const v4f *>(b + 16 * mt + kq);     // the bias starts the sum
+#pragma unroll
+        for (int t = 0; t < NI / 2; ++t)
+            c = mfma32(*reinterpret_cast<const v8s *>(w + ((mt * (NI / 2) + t) * 64 + lane) * 8), x[2 * t], x[2 * t + 1],
+                       c);
+        if (RELU) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) c[j] = fmaxf(c[j], 0.0f);
+        }
+        acc[mt] = c;
+        if (y) y[mt] = to_bf16x4(c);
+    }
+}
+
+__global__ __launch_bounds__(64 * kTrainWaves) void k_guide_train_pass1(GuideWeights W, const GuideSample *__restrict__ smp,
+                                                                        uint32_t n, float invN, GuideTrainBuffers T) {
+    __shared__ TrainLds Ls;
+    for (int k = threadIdx.x; k < kHalfCount / 8; k += 64 * kTrainWaves)
+        reinterpret_cast<uint4 *>(Ls.wf)[k] = reinterpret_cast<const uint4 *>(W.wf)[k];
+    // W [in][out_pad] into 16x16x16 A fragments: fragment (mt, kt) of layer
+    // l, lane r + 16 q holds W[16 mt + r][16 kt + 4 q .. + 3].
+    for (int k = threadIdx.x; k < kHalfCount / 4; k += 64 * kTrainWaves) {
+        int l = 0;
+        while (l < 4 && 4 * k >= h_off(l + 1)) ++l;
+        const int op = layer_out_pad(l), local = k - h_off(l) / 4, frag = local >> 6, ln = local & 63;
+        const int mt = frag / (op / 16), kt = frag % (op / 16);
+        const short *src = W.wb + h_off(l) + (16 * mt + (ln & 15)) * op + 16 * kt + 4 * (ln >> 4);
+        *reinterpret_cast<uint2 *>(Ls.wbf + 4 * k) = *reinterpret_cast<const uint2 *>(src);
+    }
+    for (int k = threadIdx.x; k < 5 * kGuideBiasStride; k += 64 * kTrainWaves) Ls.bias[k] = W.bias[k];
+    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t base = (blockIdx.x * 4 + wave) * 16;
-    const bool active = base < n;
     const int col = lane & 15, kq = 4 * (lane >> 4);
-    const uint32_t s = active ? base + col : n - 1;
-    const GuideSample g = smp[s];
-    v4s x[8], h[4][4];
-    v4f acc[4], out[3];
-    encode_tiles(g.p[0], g.p[1], g.p[2], g.n[0], g.n[1], g.n[2], g.rough, lane, x);
-    layer_fwd<8, 4, true>(W.wt + h_off(0), W.bias + 0 * kGuideBiasStride, kGuideIn, x, acc, h[0], lane);
-    layer_fwd<4, 4, true>(W.wt + h_off(1), W.bias + 1 * kGuideBiasStride, kGuideHid, h[0], acc, h[1], lane);
-    layer_fwd<4, 4, true>(W.wt + h_off(2), W.bias + 2 * kGuideBiasStride, kGuideHid, h[1], acc, h[2], lane);
-    layer_fwd<4, 4, true>(W.wt + h_off(3), W.bias + 3 * kGuideBiasStride, kGuideHid, h[2], acc, h[3], lane);
-    layer_fwd<4, 3, false>(W.wt + h_off(4), W.bias + 4 * kGuideBiasStride, kGuideHid, h[3], out, nullptr, lane);
-    // Store H^T (the inputs of every layer) for the weight gradients.
-    auto storeT = [&](short *dst, int tile, v4s v) {
-        if (!active) return;
+    float (*outs)[kGuideOutPad] = Ls.outs[wave];
+#pragma unroll 1
+    for (uint32_t grp = blockIdx.x * kTrainWaves + wave; grp < n / 16; grp += gridDim.x * kTrainWaves) {
+        // The fragment addresses are formed from ln inside the loop: an empty
+        // asm marks it as changed each iteration, so that the compiler does
+        // not hoist one address register per fragment (about 90) out of the
+        // loop and spill.
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        const uint32_t s = grp * 16 + col;
+        const GuideSample g = smp[s];
+        v4s x[8], h[4][4];
+        v4f acc[4], out[3];
+        encode_tiles(g.p[0], g.p[1], g.p[2], g.n[0], g.n[1], g.n[2], g.rough, lane, x);
+        layer_lds1<8, 4, true>(Ls.wf + h_off(0), Ls.bias + 0 * kGuideBiasStride, x, acc, h[0], ln);
+        layer_lds1<4, 4, true>(Ls.wf + h_off(1), Ls.bias + 1 * kGuideBiasStride, h[0], acc, h[1], ln);
+        layer_lds1<4, 4, true>(Ls.wf + h_off(2), Ls.bias + 2 * kGuideBiasStride, h[1], acc, h[2], ln);
+        layer_lds1<4, 4, true>(Ls.wf + h_off(3), Ls.bias + 3 * kGuideBiasStride, h[2], acc, h[3], ln);
+        layer_lds1<4, 3, false>(Ls.wf + h_off(4), Ls.bias + 4 * kGuideBiasStride, h[3], out, nullptr, ln);
+        // Store H^T (the inputs of every layer) for the weight gradients.
+        const int kqs = 4 * (ln >> 4);                   // kq, formed inside the loop like the fragment addresses
+        auto storeT = [&](short *dst, int tile, v4s v) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) dst[(size_t)(16 * tile + kq + j) * T.batch + s] = v[j];
-    };
+            for (int j = 0; j < 4; ++j) dst[(size_t)(16 * tile + kqs + j) * T.batch + s] = v[j];
+        };
 #pragma unroll
-    for (int t = 0; t < 8; ++t) storeT(T.hT + T.hOff[0], t, x[t]);
+        for (int t = 0; t < 8; ++t) storeT(T.hT + T.hOff[0], t, x[t]);
 #pragma unroll
-    for (int l = 0; l < 4; ++l)
+        for (int l = 0; l < 4; ++l)
 #pragma unroll
-        for (int t = 0; t < 4; ++t) storeT(T.hT + T.hOff[l + 1], t, h[l][t]);
-    // Output gradient per sample (lanes 0..15 own one sample each).
+            for (int t = 0; t < 4; ++t) storeT(T.hT + T.hOff[l + 1], t, h[l][t]);
+        // The raw outputs staged in LDS, a row of 48 per sample.
 #pragma unroll
-    for (int t = 0; t < 3; ++t)
+        for (int t = 0; t < 3; ++t)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) outs[wave][col][16 * t + kq + j] = out[t][j];
-    __syncthreads();
-    if (lane < 16) {
-        float *o = outs[wave][lane];
-        float lb[40];
-        lobes_from_outputs(o, lb);
-        float w[3] = {g.dir[0], g.dir[1], g.dir[2]};
-        float lp[8], mx = -INFINITY;
-        for (int k = 0; k < 8; ++k) {
-            float kap = lb[5 * k + 4];
-            float c = lb[5 * k + 1] * w[0] + lb[5 * k + 2] * w[1] + lb[5 * k + 3] * w[2];
-            lp[k] = logf(kap) - logf(2.0f * 3.14159265f) - log1pf(-expf(-2.0f * kap)) + kap * (c - 1.0f) +
-                    logf(fmaxf(lb[5 * k], 1e-30f));
-            mx = fmaxf(mx, lp[k]);
+            for (int j = 0; j < 4; ++j) outs[col][16 * t + kq + j] = out[t][j];
+        wave_lds_sync();
+        // Loss and output gradient, one (sample, lobe) item per lane, two
+        // items per lane: the 8 lanes of a sample form the softmax of its
+        // weights and the log-sum-exp of its mixture with shuffles.  All
+        // reads of the raw outputs come before the gradient overwrites them.
+        float dv[2][5];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int item = lane + 64 * r, sl = item >> 3, k = item & 7;
+            const float *o = outs[sl];
+            const GuideSample &gs = smp[grp * 16 + sl];
+            const float w0 = gs.dir[0], w1 = gs.dir[1], w2 = gs.dir[2], wgt = gs.wgt;
+            const float logit = o[k];
+            float m = logit;
+            m = fmaxf(m, __shfl_xor(m, 1, 8));
+            m = fmaxf(m, __shfl_xor(m, 2, 8));
+            m = fmaxf(m, __shfl_xor(m, 4, 8));
+            const float e = expf(logit - m);
+            float se = e;
+            se += __shfl_xor(se, 1, 8);
+            se += __shfl_xor(se, 2, 8);
+            se += __shfl_xor(se, 4, 8);
+            const float pi = e / se;
+            const float rx = o[8 + 3 * k], ry = o[9 + 3 * k], rz = o[10 + 3 * k];
+            const float len = sqrtf(rx * rx + ry * ry + rz * rz);
+            const float inv = len > 1e-12f ? 1.0f / len : 0.0f, rn = fmaxf(len, 1e-12f);
+            const float mx = rx * inv, my = ry * inv, mz = rz * inv;
+            const float kr = o[32 + k], kap = expf(fminf(fmaxf(kr, kGuideKMin), kGuideKMax));
+            const float c = mx * w0 + my * w1 + mz * w2;
+            const float lp = logf(kap) - logf(2.0f * 3.14159265f) - log1pf(-expf(-2.0f * kap)) + kap * (c - 1.0f) +
+                             logf(fmaxf(pi, 1e-30f));
+            float lm = lp;
+            lm = fmaxf(lm, __shfl_xor(lm, 1, 8));
+            lm = fmaxf(lm, __shfl_xor(lm, 2, 8));
+            lm = fmaxf(lm, __shfl_xor(lm, 4, 8));
+            float ls = expf(lp - lm);
+            ls += __shfl_xor(ls, 1, 8);
+            ls += __shfl_xor(ls, 2, 8);
+            ls += __shfl_xor(ls, 4, 8);
+            const float lq = lm + logf(ls);
+            if (k == 0) T.loss[grp * 16 + sl] = -wgt * lq;
+            const float gq = -wgt * invN;                    // dL / dlog q
+            const float rr = expf(lp - lq);
+            dv[r][0] = gq * (rr - pi);
+            const float f = gq * rr * kap / rn;
+            dv[r][1] = f * (w0 - c * mx);
+            dv[r][2] = f * (w1 - c * my);
+            dv[r][3] = f * (w2 - c * mz);
+            const bool inside = kr > kGuideKMin && kr < kGuideKMax;
+            const float dl = 1.0f / kap - 2.0f / expm1f(2.0f * kap) + (c - 1.0f);
+            dv[r][4] = inside ? gq * rr * dl * kap : 0.0f;
         }
-        float se = 0.0f;
-        for (int k = 0; k < 8; ++k) se += expf(lp[k] - mx);
-        float lq = mx + logf(se);
-        if (active) T.loss[s] = -g.wgt * lq;
-        float gq = -g.wgt * invN;                        // dL / dlog q
-        float d[kGuideOutPad];
-        for (int k = 0; k < kGuideOutPad; ++k) d[k] = 0.0f;
-        for (int k = 0; k < 8; ++k) {
-            float r = expf(lp[k] - lq);
-            float pi = lb[5 * k], kap = lb[5 * k + 4];
-            d[k] = gq * (r - pi);
-            float rx = o[8 + 3 * k], ry = o[9 + 3 * k], rz = o[10 + 3 * k];
-            float rn = fmaxf(sqrtf(rx * rx + ry * ry + rz * rz), 1e-12f);
-            float c = lb[5 * k + 1] * w[0] + lb[5 * k + 2] * w[1] + lb[5 * k + 3] * w[2];
-            for (int a = 0; a < 3; ++a) d[8 + 3 * k + a] = gq * r * kap * (w[a] - c * lb[5 * k + 1 + a]) / rn;
-            float kr = o[32 + k];
-            bool inside = kr > kGuideKMin && kr < kGuideKMax;
-            float dl = 1.0f / kap - 2.0f / expm1f(2.0f * kap) + (c - 1.0f);
-            d[32 + k] = inside ? gq * r * dl * kap : 0.0f;
+        wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int item = lane + 64 * r, sl = item >> 3, k = item & 7;
+            float *o = outs[sl];
+            o[k] = dv[r][0];
+            o[8 + 3 * k] = dv[r][1];
+            o[9 + 3 * k] = dv[r][2];
+            o[10 + 3 * k] = dv[r][3];
+            o[32 + k] = dv[r][4];
+            o[40 + k] = 0.0f;                                  // padding rows of the output tiles
         }
-        for (int k = 0; k < kGuideOutPad; ++k) o[k] = d[k];
-    }
-    __syncthreads();
-    // dY of the output layer as B tiles; bias gradient; store dY^T.
-    v4f dy[4];
-    v4s dyb[4];
+        wave_lds_sync();
+        // dY of the output layer as B tiles; bias gradient; store dY^T.
+        v4f dy[4];
+        v4s dyb[4];
 #pragma unroll
-    for (int t = 0; t < 3; ++t) {
+        for (int t = 0; t < 3; ++t) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) dy[t][j] = outs[wave][col][16 * t + kq + j];
-    }
+            for (int j = 0; j < 4; ++j) dy[t][j] = outs[col][16 * t + kq + j];
+        }
 #pragma unroll
-    for (int t = 0; t < 3; ++t) {
+        for (int t = 0; t < 3; ++t) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) dyb[t][j] = f2bf(dy[t][j]);
-        storeT(T.dyT + T.dyOff[4], t, dyb[t]);
-    }
-    // Backward: dX^T = W dY^T (A = W [in][out_pad]), masked by ReLU'.
-    const int row = lane & 15;
-    auto back = [&](int l, int NO, const v4s *dyIn, const v4s *hPrev, v4s *dyOut, v4f *dyF) {
-        const int out = layer_out_pad(l);
-        for (int mt = 0; mt < 4; ++mt) {
-            v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
-            for (int kt = 0; kt < NO; ++kt) {
-                v4s a = load4(W.wb + h_off(l) + (size_t)(16 * mt + row) * out + 16 * kt + kq);
-                c = mfma16(a, dyIn[kt], c);
+            for (int j = 0; j < 4; ++j) dyb[t][j] = f2bf(dy[t][j]);
+            storeT(T.dyT + T.dyOff[4], t, dyb[t]);
+        }
+        // Backward: dX^T = W dY^T (A = W [in][out_pad]), masked by ReLU'.
+        auto back = [&](int l, int NO, const v4s *dyIn, const v4s *hPrev, v4s *dyOut, v4f *dyF) {
+            const int out = layer_out_pad(l);
+            for (int mt = 0; mt < 4; ++mt) {
+                v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+                for (int kt = 0; kt < NO; ++kt) {
+                    v4s a = *reinterpret_cast<const v4s *>(Ls.wbf + h_off(l) + ((mt * (out / 16) + kt) * 64 + ln) * 4);
+                    c = mfma16(a, dyIn[kt], c);
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) c[j] = bf2f(hPrev[mt][j]) > 0.0f ? c[j] : 0.0f;
+                dyF[mt] = c;
+                v4s hb;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) hb[j] = f2bf(c[j]);
+                dyOut[mt] = hb;
             }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) c[j] = bf2f(hPrev[mt][j]) > 0.0f ? c[j] : 0.0f;
-            dyF[mt] = c;
-            v4s hb;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) hb[j] = f2bf(c[j]);
-            dyOut[mt] = hb;
+        };
+        v4s g4[4], g3[4];
+        v4f gf[4];
+        back(4, 3, dyb, h[3], g4, gf);                      // dY of layer 3 (hidden 4)
+        for (int t = 0; t < 4; ++t) {
+            storeT(T.dyT + T.dyOff[3], t, g4[t]);
         }
-    };
-    v4s g4[4], g3[4];
-    v4f gf[4];
-    back(4, 3, dyb, h[3], g4, gf);                      // dY of layer 3 (hidden 4)
-    for (int t = 0; t < 4; ++t) {
-        storeT(T.dyT + T.dyOff[3], t, g4[t]);
-    }
-    back(3, 4, g4, h[2], g3, gf);
-    for (int t = 0; t < 4; ++t) {
-        storeT(T.dyT + T.dyOff[2], t, g3[t]);
-    }
-    back(2, 4, g3, h[1], g4, gf);
-    for (int t = 0; t < 4; ++t) {
-        storeT(T.dyT + T.dyOff[1], t, g4[t]);
-    }
-    back(1, 4, g4, h[0], g3, gf);
-    for (int t = 0; t < 4; ++t) {
-        storeT(T.dyT + T.dyOff[0], t, g3[t]);
+        back(3, 4, g4, h[2], g3, gf);
+        for (int t = 0; t < 4; ++t) {
+            storeT(T.dyT + T.dyOff[2], t, g3[t]);
+        }
+        back(2, 4, g3, h[1], g4, gf);
+        for (int t = 0; t < 4; ++t) {
+            storeT(T.dyT + T.dyOff[1], t, g4[t]);
+        }
+        back(1, 4, g4, h[0], g3, gf);
+        for (int t = 0; t < 4; ++t) {
+            storeT(T.dyT + T.dyOff[0], t, g3[t]);
+        }
+        wave_lds_sync();   // this group's reads of outs before the next group's writes
     }
 }
 
-// Training pass 2: the weight and bias gradients of one layer over one
-// K-slice of the batch.  A wave computes one 16x16 tile of dW_l [in][out] =
-// H_l^T (in x batch) x dY_l (batch x out) on MFMA (A[m = in][k = sample]
-// from H^T, B[k = sample][n = out] from dY^T) and writes it to the slice's
-// partial (laid out like the master parameters); waves of the first row
-// tile also sum dY over the slice, the bias gradient.  No atomics: the
-// partials are added in slice order by k_guide_reduce.
-__global__ __launch_bounds__(64) void k_guide_train_dw(GuideTrainBuffers T, float *__restrict__ part, int layer,
-                                                       uint32_t kSlice) {
-    const int lane = threadIdx.x;
-    const int in = layer_in(layer), out = layer_out(layer);
-    const int tilesN = (out + 15) / 16;
-    const int tile = blockIdx.x, mt = tile / tilesN, nt = tile % tilesN;
-    const uint32_t k0 = blockIdx.y * kSlice, k1 = min(k0 + kSlice, T.batch);
-    const int row = lane & 15, kq = 4 * (lane >> 4);
-    const short *hT = T.hT + T.hOff[layer] + (size_t)(16 * mt + row) * T.batch;
-    const int ocol = 16 * nt + row;
-    const short *dT = T.dyT + T.dyOff[layer] + (size_t)ocol * T.batch;
-    v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
-    float bsum = 0.0f;
-    for (uint32_t k = k0; k < k1; k += 16) {
-        v4s a = load4(hT + k + kq);
-        v4s b = load4(dT + k + kq);
-        c = mfma16(a, b, c);
-        bsum += (bf2f(b[0]) + bf2f(b[1])) + (bf2f(b[2]) + bf2f(b[3]));
-    }
-    float *dst = part + (size_t)blockIdx.y * kParamCount;
+// Training pass 2: the weight and bias gradients of every layer over one
+// slice of kDwSlice samples per block (blockIdx.x the slice, blockIdx.y the
+// layer): dW_l [in][out] = H_l^T (in x samples) x dY_l (samples x out) on
+// 16x16x32 MFMA.  The block stages kDwSub samples of H_l^T and dY_l^T at a
+// time in LDS with coalesced 16-byte loads (rows padded to 272 bytes, so
+// that the 16 lanes of a 16-byte operand read touch every bank once); each
+// wave accumulates its tiles over the slice in registers and writes them to
+// the slice's partial (laid out like the master parameters), with the bias
+// gradients summed in sample order.  No atomics: k_guide_update adds the
+// partials in slice order.
+constexpr int kDwSlice = 512;
+constexpr int kDwSub = 128;
+constexpr int kDwRow = kDwSub + 8;
+
+struct alignas(16) DwLds {
+    short h[kGuideIn][kDwRow];
+    short d[kGuideHid][kDwRow];
+};
+
+__device__ inline v4f mfma32x(v8s a, v8s b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0, 0,
+                                                   0);
+}
+
+__global__ __launch_bounds__(256) void k_guide_train_dw(GuideTrainBuffers T, float *__restrict__ part) {
+    __shared__ DwLds L;
+    const int layer = blockIdx.y;
+    const int in = layer_in(layer), out = layer_out(layer), op = layer_out_pad(layer);
+    const int tilesN = op / 16, tiles = (in / 16) * tilesN;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, row = lane & 15, q = lane >> 4;
+    const uint32_t s0 = blockIdx.x * kDwSlice, s1 = min(s0 + (uint32_t)kDwSlice, T.batch);
+    const short *hT = T.hT + T.hOff[layer];
+    const short *dT = T.dyT + T.dyOff[layer];
+    v4f acc[8];
+    float bsum[8];                                       // bias gradient partials of the tiles with mt = 0
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        int i = 16 * mt + kq + j, o = 16 * nt + row;
-        if (i < in && o < out) dst[w_off(layer) + i * out + o] = c[j];
+    for (int j = 0; j < 8; ++j) {
+        acc[j] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+        bsum[j] = 0.0f;
     }
-    bsum += __shfl_xor(bsum, 16);
-    bsum += __shfl_xor(bsum, 32);
-    if (mt == 0 && lane < 16 && ocol < out) dst[b_off(layer) + ocol] = bsum;
+    for (uint32_t b = s0; b < s1; b += kDwSub) {
+        __syncthreads();                                 // the previous sub-slice's reads are done
+        // Batches are multiples of 16 samples, so an 8-sample chunk is
+        // wholly inside the batch or wholly past it (zeros).
+        for (int c = threadIdx.x; c < in * (kDwSub / 8); c += 256) {
+            const int r = c / (kDwSub / 8), k8 = (c % (kDwSub / 8)) * 8;
+            const uint32_t sm = b + (uint32_t)k8;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (sm < s1) v = *reinterpret_cast<const uint4 *>(hT + (size_t)r * T.batch + sm);
+            *reinterpret_cast<uint4 *>(&L.h[r][k8]) = v;
+        }
+        for (int c = threadIdx.x; c < op * (kDwSub / 8); c += 256) {
+            const int r = c / (kDwSub / 8), k8 = (c % (kDwSub / 8)) * 8;
+            const uint32_t sm = b + (uint32_t)k8;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (sm < s1) v = *reinterpret_cast<const uint4 *>(dT + (size_t)r * T.batch + sm);
+            *reinterpret_cast<uint4 *>(&L.d[r][k8]) = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int t = wave + 4 * j;
+            if (t < tiles) {
+                const int mt = t / tilesN, nt = t % tilesN;
+#pragma unroll
+                for (int kk = 0; kk < kDwSub / 32; ++kk) {
+                    const v8s a = *reinterpret_cast<const v8s *>(&L.h[16 * mt + row][32 * kk + 8 * q]);
+                    const v8s d = *reinterpret_cast<const v8s *>(&L.d[16 * nt + row][32 * kk + 8 * q]);
+                    acc[j] = mfma32x(a, d, acc[j]);
+                    if (mt == 0) {
+                        float e = 0.0f;
+#pragma unroll
+                        for (int x = 0; x < 8; ++x) e += bf2f(d[x]);
+                        bsum[j] += e;
+                    }
+                }
+            }
+        }
+    }
+    float *dst = part + (size_t)blockIdx.x * kParamCount;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int t = wave + 4 * j;
+        if (t < tiles) {
+            const int mt = t / tilesN, nt = t % tilesN, o = 16 * nt + row;
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const int i = 16 * mt + 4 * q + jj;
+                if (o < out) dst[w_off(layer) + i * out + o] = acc[j][jj];
+            }
+            // The bias partial of output o: the four lane quarters' sums.
+            float b = bsum[j];
+            b += __shfl_xor(b, 16);
+            b += __shfl_xor(b, 32);
+            if (mt == 0 && q == 0 && o < out) dst[b_off(layer) + o] = b;
+        }
+    }
 }
 
-// grad += the sum of the slices' partial gradients.
-__global__ __launch_bounds__(256) void k_guide_reduce(const float *__restrict__ part, uint32_t slices,
-                                                      float *__restrict__ grad) {
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= kParamCount) return;
+// grad + the slices' partials (added in slice order); with apply, Adam on the
+// FP32 master (gradient zeroed) and the BF16 copies of the updated parameter
+// in the three weight layouts (or the FP32 bias copy).  One thread per
+// parameter.
+// A block takes 32 parameters; its 8 thread rows sum the slices k with
+// k % 8 = row (32 consecutive parameters of one slice: one 128-byte read),
+// and the first row adds the 8 row sums in order.
+__global__ __launch_bounds__(256) void k_guide_update(const float *__restrict__ part, uint32_t slices,
+                                                      float *__restrict__ grad, float *__restrict__ p,
+                                                      float *__restrict__ m, float *__restrict__ v, short *__restrict__ wt,
+                                                      short *__restrict__ wb, short *__restrict__ wf,
+                                                      float *__restrict__ bias, int apply, float lr, float b1, float b2,
+                                                      float eps, float bc1, float bc2) {
+    __shared__ float rows[8][32];
+    const int i = blockIdx.x * 32 + (threadIdx.x & 31), sub = threadIdx.x >> 5;
+    float ps = 0.0f;
+    if (i < kParamCount)
+        for (uint32_t k = sub; k < slices; k += 8) ps += part[(size_t)k * kParamCount + i];
+    rows[sub][threadIdx.x & 31] = ps;
+    __syncthreads();
+    if (sub != 0 || i >= kParamCount) return;
     float s = 0.0f;
-    for (uint32_t k = 0; k < slices; ++k) s += part[(size_t)k * kParamCount + p];
-    grad[p] += s;
-}
-
-// Adam on the FP32 masters, then the BF16 copies in both layouts.
-__global__ __launch_bounds__(256) void k_guide_adam(float *__restrict__ p, float *__restrict__ g, float *__restrict__ m,
-                                                    float *__restrict__ v, float lr, float b1, float b2, float eps,
-                                                    float bc1, float bc2, int count) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= count) return;
-    float gr = g[i];
-    float mm = b1 * m[i] + (1.0f - b1) * gr;
-    float vv = b2 * v[i] + (1.0f - b2) * gr * gr;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) s += rows[r][threadIdx.x];
+    const float gr = grad[i] + s;
+    if (!apply) {
+        grad[i] = gr;
+        return;
+    }
+    const float mm = b1 * m[i] + (1.0f - b1) * gr;
+    const float vv = b2 * v[i] + (1.0f - b2) * gr * gr;
     m[i] = mm;
     v[i] = vv;
-    p[i] -= lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
-    g[i] = 0.0f;
+    const float np = p[i] - lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
+    p[i] = np;
+    grad[i] = 0.0f;
+    int l = 0;
+    while (l < 4 && i >= w_off(l + 1)) ++l;
+    const int in = layer_in(l), out = layer_out(l), op = layer_out_pad(l), r = i - w_off(l);
+    if (r < in * out) {
+        const int ii = r / out, o = r % out, c32 = ii % 32;
+        const short bv = f2bf(np);
+        wt[h_off(l) + o * in + ii] = bv;                 // W^T [out_pad][in]
+        wb[h_off(l) + ii * op + o] = bv;                 // W [in][out_pad]
+        // 16x16x32 A fragment (o / 16, ii / 32): lane o % 16 + 16 ((ii % 16) / 4),
+        // value 4 ((ii % 32) / 16) + ii % 4 (the inverse of k_guide_refresh's map)
+        const int frag = (o / 16) * (in / 32) + ii / 32;
+        const int ln = (o % 16) + 16 * ((c32 % 16) / 4), e = 4 * (c32 / 16) + (c32 % 4);
+        wf[h_off(l) + frag * 512 + ln * 8 + e] = bv;
+    } else {
+        bias[l * kGuideBiasStride + (r - in * out)] = np;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_guide_refresh(const float *__restrict__ p, short *__restrict__ wt,
@@ -807,20 +977,24 @@ az_status az_guide_train_device(az_guide *G, const void *d_Samples, u32 N, b32 A
         T.dyOff[l] = off;
         off += (size_t)layer_out_pad(l) * N;
     }
-    dim3 g1((N + 63) / 64);
-    hipLaunchKernelGGL(k_guide_train_pass1, g1, dim3(256), 0, G->stream, weightsOf(G), (const GuideSample *)d_Samples, N,
-                       1.0f / (float)N, T);
+    dim3 g1(std::min<u32>((N / 16 + kTrainWaves - 1) / kTrainWaves, (u32)G->cuCount));
+    hipLaunchKernelGGL(k_guide_train_pass1, g1, dim3(64 * kTrainWaves), 0, G->stream, weightsOf(G),
+                       (const GuideSample *)d_Samples, N, 1.0f / (float)N, T);
     AZ_GHIP(G, hipGetLastError());
-    const uint32_t kSlice = 1024, slices = (N + kSlice - 1) / kSlice;
+    const uint32_t slices = (N + kDwSlice - 1) / kDwSlice;
     AZ_GHIP(G, G->part.alloc((size_t)slices * kParamCount * sizeof(float)));
-    for (int l = 0; l < 5; ++l) {
-        int tiles = ((layer_in(l) + 15) / 16) * ((layer_out(l) + 15) / 16);
-        dim3 g2(tiles, slices);
-        hipLaunchKernelGGL(k_guide_train_dw, g2, dim3(64), 0, G->stream, T, (float *)G->part.ptr, l, kSlice);
-        AZ_GHIP(G, hipGetLastError());
+    hipLaunchKernelGGL(k_guide_train_dw, dim3(slices, 5), dim3(256), 0, G->stream, T, (float *)G->part.ptr);
+    AZ_GHIP(G, hipGetLastError());
+    float bc1 = 1.0f, bc2 = 1.0f;
+    if (ApplyUpdate) {
+        G->step++;
+        bc1 = 1.0f - powf(0.9f, (float)G->step);
+        bc2 = 1.0f - powf(0.999f, (float)G->step);
     }
-    hipLaunchKernelGGL(k_guide_reduce, dim3((kParamCount + 255) / 256), dim3(256), 0, G->stream,
-                       (const float *)G->part.ptr, slices, (float *)G->grad.ptr);
+    hipLaunchKernelGGL(k_guide_update, dim3((kParamCount + 31) / 32), dim3(256), 0, G->stream,
+                       (const float *)G->part.ptr, slices, (float *)G->grad.ptr, (float *)G->params.ptr,
+                       (float *)G->m.ptr, (float *)G->v.ptr, (short *)G->wt.ptr, (short *)G->wb.ptr, (short *)G->wf.ptr,
+                       (float *)G->bias.ptr, ApplyUpdate ? 1 : 0, G->lr, 0.9f, 0.999f, 1e-8f, bc1, bc2);
     AZ_GHIP(G, hipGetLastError());
     if (MeanLoss) {
         std::vector<float> h(N);
@@ -829,16 +1003,6 @@ az_status az_guide_train_device(az_guide *G, const void *d_Samples, u32 N, b32 A
         double s = 0.0;
         for (float x : h) s += x;
         *MeanLoss = (f32)(s / N);
-    }
-    if (ApplyUpdate) {
-        G->step++;
-        float bc1 = 1.0f - powf(0.9f, (float)G->step), bc2 = 1.0f - powf(0.999f, (float)G->step);
-        hipLaunchKernelGGL(k_guide_adam, dim3((kParamCount + 255) / 256), dim3(256), 0, G->stream, (float *)G->params.ptr,
-                           (float *)G->grad.ptr, (float *)G->m.ptr, (float *)G->v.ptr, G->lr, 0.9f, 0.999f, 1e-8f, bc1,
-                           bc2, kParamCount);
-        AZ_GHIP(G, hipGetLastError());
-        az_status s = refresh(G);
-        if (s != AZ_OK) return s;
     }
     return AZ_OK;
 }

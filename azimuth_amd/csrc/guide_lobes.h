@@ -32,8 +32,9 @@ __device__ inline uint2 lobe_pack(float w, float kappa, float mx, float my, floa
     const float a = fabsf(mx) + fabsf(my) + fabsf(mz);
     float px = 0.0f, py = 0.0f;
     if (a > 0.0f) {
-        px = mx / a;
-        py = my / a;
+        const float ia = __builtin_amdgcn_rcpf(a);
+        px = mx * ia;
+        py = my * ia;
         if (mz < 0.0f) {
             const float tx = (1.0f - fabsf(py)) * (px >= 0.0f ? 1.0f : -1.0f);
             const float ty = (1.0f - fabsf(px)) * (py >= 0.0f ? 1.0f : -1.0f);
@@ -60,7 +61,7 @@ __device__ inline void lobe_mean(uint2 l, float &x, float &y, float &z) {
         px = tx;
         py = ty;
     }
-    const float inv = 1.0f / sqrtf(px * px + py * py + pz * pz);
+    const float inv = __builtin_amdgcn_rsqf(px * px + py * py + pz * pz);
     x = px * inv;
     y = py * inv;
     z = pz * inv;

@@ -352,7 +352,7 @@ class GpuContext:
 
     def render(self, camera, filt=N.FilterType_Box, cull=False, seed=0, spp=0, row_begin=0, row_end=0,
                guide=False, guide_fraction=0.5, env_sampling=N.AZ_ENV_COSINE, guide_max_depth=0,
-               guide_train_spp=0, guide_chunk_spp=0, scatter_sampling=N.AZ_SCATTER_REFERENCE,
+               guide_train_spp=0, guide_chunk_spp=0, scatter_sampling=N.AZ_SCATTER_REFERENCE, guide_steps_per_chunk=0,
                guide_target=N.AZ_GUIDE_ENV):
         o = N.az_render_opts()
         o.ScatterSampling = int(scatter_sampling)
@@ -361,6 +361,7 @@ class GpuContext:
         o.GuideMaxDepth = int(guide_max_depth)
         o.GuideTrainSpp = int(guide_train_spp)
         o.GuideChunkSpp = int(guide_chunk_spp)
+        o.GuideStepsPerChunk = int(guide_steps_per_chunk)
         o.Seed = int(seed)
         o.SampleCountOverride = int(spp)
         o.RowBegin, o.RowEnd = int(row_begin), int(row_end)
@@ -515,6 +516,7 @@ class GpuContext:
 
     def render_f64(self, camera, filt=N.FilterType_Box, seed=0, spp=0, guide=False, guide_fraction=0.5,
                    env_sampling=N.AZ_ENV_COSINE, guide_max_depth=0, guide_train_spp=0, guide_chunk_spp=0,
+                   guide_steps_per_chunk=0,
                    scatter_sampling=N.AZ_SCATTER_REFERENCE, guide_target=N.AZ_GUIDE_ENV):
         """Device render accumulated in double precision; returns (W*H, 4)
         float64 (colour sums, weight sums).  The wall time of the render call
@@ -532,6 +534,7 @@ class GpuContext:
         o.GuideMaxDepth = int(guide_max_depth)
         o.GuideTrainSpp = int(guide_train_spp)
         o.GuideChunkSpp = int(guide_chunk_spp)
+        o.GuideStepsPerChunk = int(guide_steps_per_chunk)
         o.GuideEnable = 1 if guide else 0
         o.GuideFraction = float(guide_fraction)
         self.synchronize()
