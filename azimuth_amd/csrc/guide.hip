@@ -65,6 +65,14 @@ __device__ inline v4s to_bf16x4(v4f x) { return __builtin_bit_cast(v4s, __builti
 
 __device__ inline v4s load4(const short *p) { return *reinterpret_cast<const v4s *>(p); }
 
+// ReLU of four accumulators, one v_med3_f32 each (the median of x, 0 and
+// FLT_MAX; fmaxf in IEEE mode would quiet each input first), then bf16.
+__device__ inline v4s relu_bf16(v4f c) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) c[j] = __builtin_amdgcn_fmed3f(c[j], 0.0f, 3.402823466e+38f);
+    return to_bf16x4(c);
+}
+
 // gfx950's 16x16x32 bf16 MFMA: lane l holds A row l % 16 and B column l % 16
 // at k = 8 (l / 16) .. +7, the result as in the 16x16x16 form.  A K-block of
 // 32 is built from two 16-row tiles in the accumulator layout: lane quarter q
@@ -131,51 +139,6 @@ __device__ inline void encode_tiles(float px, float py, float pz, float nx, floa
     }
 }
 
-// One dense layer for the wave's 16 samples: out tiles (NO of them) from in
-// tiles (NI), A blocks read from W^T [out_pad][in] at `wt`, bias from `b`.
-template <int NI, int NO, bool RELU>
-__device__ inline void layer_fwd(const short *__restrict__ wt, const float *__restrict__ b, int in, const v4s *x,
-                                 v4f *acc, v4s *y, int lane) {
-    const int row = lane & 15, kq = 4 * (lane >> 4);
-#pragma unroll
-    for (int mt = 0; mt < NO; ++mt) {
-        v4f c = *reinterpret_cast<const v4f *>(b + 16 * mt + kq);     // the bias starts the sum
-#pragma unroll
-        for (int kt = 0; kt < NI; ++kt) {
-            v4s a = load4(wt + (size_t)(16 * mt + row) * in + 16 * kt + kq);
-            c = mfma16(a, x[kt], c);
-        }
-        if (RELU) {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) c[j] = fmaxf(c[j], 0.0f);
-        }
-        acc[mt] = c;
-        if (y) y[mt] = to_bf16x4(c);
-    }
-}
-
-// Mixture parameters from the 40 raw outputs of one sample.
-__device__ inline void lobes_from_outputs(const float *o, float *lobes /* 8 x {w, mx, my, mz, k} */) {
-    float m = o[0];
-    for (int k = 1; k < 8; ++k) m = fmaxf(m, o[k]);
-    float e[8], s = 0.0f;
-    for (int k = 0; k < 8; ++k) {
-        e[k] = expf(o[k] - m);
-        s += e[k];
-    }
-    for (int k = 0; k < 8; ++k) {
-        float x = o[8 + 3 * k], y = o[9 + 3 * k], z = o[10 + 3 * k];
-        float l = sqrtf(x * x + y * y + z * z);
-        float inv = l > 1e-12f ? 1.0f / l : 0.0f;
-        float kr = fminf(fmaxf(o[32 + k], kGuideKMin), kGuideKMax);
-        lobes[5 * k + 0] = e[k] / s;
-        lobes[5 * k + 1] = x * inv;
-        lobes[5 * k + 2] = y * inv;
-        lobes[5 * k + 3] = z * inv;
-        lobes[5 * k + 4] = expf(kr);
-    }
-}
-
 // ---- kernels ----------------------------------------------------------------------------
 
 // Lane-layout check of the MFMA builtin: D = A * B for 16x16 matrices given
@@ -225,13 +188,7 @@ __device__ inline void layer_lds(const short *w, const float *b, const v4s (*x)[
             for (int g = 0; g < kInferGroups; ++g) c[g] = mfma32(a, x[g][2 * t], x[g][2 * t + 1], c[g]);
         }
 #pragma unroll
-        for (int g = 0; g < kInferGroups; ++g) {
-            if (RELU) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) c[g][j] = fmaxf(c[g][j], 0.0f);
-            }
-            y[g][mt] = to_bf16x4(c[g]);
-        }
+        for (int g = 0; g < kInferGroups; ++g) y[g][mt] = RELU ? relu_bf16(c[g]) : to_bf16x4(c[g]);
     }
 }
 
@@ -401,12 +358,27 @@ __device__ inline void layer_lds1(const short *w, const float *b, const v4s *x, 
         for (int t = 0; t < NI / 2; ++t)
             c = mfma32(*reinterpret_cast<const v8s *>(w + ((mt * (NI / 2) + t) * 64 + lane) * 8), x[2 * t], x[2 * t + 1],
                        c);
-        if (RELU) {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) c[j] = fmaxf(c[j], 0.0f);
-        }
         acc[mt] = c;
-        if (y) y[mt] = to_bf16x4(c);
+        if (y) y[mt] = RELU ? relu_bf16(c) : to_bf16x4(c);
+    }
+}
+
+// Gradient of layer L's inputs for 16 samples: dX^T = W dY^T on 16x16x16
+// MFMA (A = W [in][out_pad] fragments from LDS, B = the NO tiles of dY^T),
+// masked by the derivative of the ReLU that produced the inputs (hPrev, the
+// bf16 activations), rounded to bf16.
+template <int L, int NO>
+__device__ inline void back_layer(const short *wbf, int ln, const v4s *dyIn, const v4s *hPrev, v4s *dyOut) {
+    constexpr int op = layer_out_pad(L);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kt = 0; kt < NO; ++kt)
+            c = mfma16(*reinterpret_cast<const v4s *>(wbf + h_off(L) + ((mt * (op / 16) + kt) * 64 + ln) * 4), dyIn[kt], c);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) c[j] = bf2f(hPrev[mt][j]) > 0.0f ? c[j] : 0.0f;
+        dyOut[mt] = to_bf16x4(c);
     }
 }
 
@@ -449,10 +421,15 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_guide_train_pass1(GuideWei
         layer_lds1<4, 4, true>(Ls.wf + h_off(3), Ls.bias + 3 * kGuideBiasStride, h[2], acc, h[3], ln);
         layer_lds1<4, 3, false>(Ls.wf + h_off(4), Ls.bias + 4 * kGuideBiasStride, h[3], out, nullptr, ln);
         // Store H^T (the inputs of every layer) for the weight gradients.
-        const int kqs = 4 * (ln >> 4);                   // kq, formed inside the loop like the fragment addresses
+        // Row 16 tile + kq + j of a features x batch array: a wave-uniform
+        // row base (16 tile + j) plus the lane's 32-bit offset kq batch + s
+        // (kq formed inside the loop like the fragment addresses; the host
+        // keeps 32 batch below 2^32).
+        const uint32_t laneOff = 2u * ((uint32_t)(4 * (ln >> 4)) * T.batch + s);     // bytes
         auto storeT = [&](short *dst, int tile, v4s v) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) dst[(size_t)(16 * tile + kqs + j) * T.batch + s] = v[j];
+            for (int j = 0; j < 4; ++j)
+                *reinterpret_cast<short *>(reinterpret_cast<char *>(dst + (size_t)(16 * tile + j) * T.batch) + laneOff) = v[j];
         };
 #pragma unroll
         for (int t = 0; t < 8; ++t) storeT(T.hT + T.hOff[0], t, x[t]);
@@ -545,41 +522,19 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_guide_train_pass1(GuideWei
             storeT(T.dyT + T.dyOff[4], t, dyb[t]);
         }
         // Backward: dX^T = W dY^T (A = W [in][out_pad]), masked by ReLU'.
-        auto back = [&](int l, int NO, const v4s *dyIn, const v4s *hPrev, v4s *dyOut, v4f *dyF) {
-            const int out = layer_out_pad(l);
-            for (int mt = 0; mt < 4; ++mt) {
-                v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
-                for (int kt = 0; kt < NO; ++kt) {
-                    v4s a = *reinterpret_cast<const v4s *>(Ls.wbf + h_off(l) + ((mt * (out / 16) + kt) * 64 + ln) * 4);
-                    c = mfma16(a, dyIn[kt], c);
-                }
-#pragma unroll
-                for (int j = 0; j < 4; ++j) c[j] = bf2f(hPrev[mt][j]) > 0.0f ? c[j] : 0.0f;
-                dyF[mt] = c;
-                v4s hb;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) hb[j] = f2bf(c[j]);
-                dyOut[mt] = hb;
-            }
-        };
         v4s g4[4], g3[4];
-        v4f gf[4];
-        back(4, 3, dyb, h[3], g4, gf);                      // dY of layer 3 (hidden 4)
-        for (int t = 0; t < 4; ++t) {
-            storeT(T.dyT + T.dyOff[3], t, g4[t]);
-        }
-        back(3, 4, g4, h[2], g3, gf);
-        for (int t = 0; t < 4; ++t) {
-            storeT(T.dyT + T.dyOff[2], t, g3[t]);
-        }
-        back(2, 4, g3, h[1], g4, gf);
-        for (int t = 0; t < 4; ++t) {
-            storeT(T.dyT + T.dyOff[1], t, g4[t]);
-        }
-        back(1, 4, g4, h[0], g3, gf);
-        for (int t = 0; t < 4; ++t) {
-            storeT(T.dyT + T.dyOff[0], t, g3[t]);
-        }
+        back_layer<4, 3>(Ls.wbf, ln, dyb, h[3], g4);         // dY of layer 3 (hidden 4)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) storeT(T.dyT + T.dyOff[3], t, g4[t]);
+        back_layer<3, 4>(Ls.wbf, ln, g4, h[2], g3);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) storeT(T.dyT + T.dyOff[2], t, g3[t]);
+        back_layer<2, 4>(Ls.wbf, ln, g3, h[1], g4);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) storeT(T.dyT + T.dyOff[1], t, g4[t]);
+        back_layer<1, 4>(Ls.wbf, ln, g4, h[0], g3);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) storeT(T.dyT + T.dyOff[0], t, g3[t]);
         wave_lds_sync();   // this group's reads of outs before the next group's writes
     }
 }
@@ -953,7 +908,7 @@ az_status az_guide_infer(az_guide *G, const f32 *Queries, u32 N, f32 *Lobes) {
 // One training step on N device-resident samples (N a multiple of 16):
 // gradients of the mean loss, then Adam unless ApplyUpdate is 0.
 az_status az_guide_train_device(az_guide *G, const void *d_Samples, u32 N, b32 ApplyUpdate, f32 *MeanLoss) {
-    if (!G || !d_Samples || N == 0 || (N % 16) != 0) return AZ_ERR_INVALID_ARGUMENT;
+    if (!G || !d_Samples || N == 0 || (N % 16) != 0 || N >= (1u << 27)) return AZ_ERR_INVALID_ARGUMENT;
     AZ_GHIP(G, hipSetDevice(G->device));
     if (N > G->batchCap) {
         AZ_GHIP(G, G->hT.alloc((size_t)N * (kGuideIn + 4 * kGuideHid) * sizeof(short)));
