@@ -2290,7 +2290,7 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
     const u32 trainWant = o.GuideChunkSpp ? o.GuideChunkSpp : std::max<u32>(1u, (2u << 20) / std::max<u32>(1u, nPix));
     const u32 trainChunkSpp = std::max<u32>(1u, std::min<u32>(std::min(chunkSpp0, trainWant), trainSpp / 8u));
     const u32 trainChunks = trainSpp ? (std::min(trainSpp, sppTotal) + trainChunkSpp - 1) / trainChunkSpp : 1u;
-    const u32 kGuideSteps = 256;
+    const u32 kGuideSteps = 512;
     const u32 stepsPerChunk =
         o.GuideStepsPerChunk ? o.GuideStepsPerChunk : std::max<u32>(1u, (kGuideSteps + trainChunks - 1) / trainChunks);
     const float alpha = o.GuideFraction > 0.0f && o.GuideFraction < 1.0f ? o.GuideFraction : 0.5f;

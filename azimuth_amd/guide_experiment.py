@@ -129,6 +129,7 @@ def main(argv=None):
     # (no learning; samples the sun directly), alone and as the guide's base.
     mse_lum, t_lum, _ = measure(ctx, cam, filt, ref, seeds, a.spp, False, N.AZ_ENV_LUMINANCE)
     mse_glum, t_glum, _ = measure(ctx, cam, filt, ref, seeds, a.spp, True, N.AZ_ENV_LUMINANCE)
+    mse_glum3, t_glum3, _ = measure(ctx, cam, filt, ref, seeds, a.spp, True, N.AZ_ENV_LUMINANCE, max_depth=3)
     print(f"luminance {a.spp} spp {t_lum:.2f}s mse {mse_lum:.4e}; guide+luminance {t_glum:.2f}s mse {mse_glum:.4e}",
           file=sys.stderr, flush=True)
     # The scatter guide (hemisphere scatter directions mixed with the
@@ -156,6 +157,9 @@ def main(argv=None):
         "guided_over_luminance": {"spp": a.spp, "seconds": round(t_glum, 3), "mse": mse_glum},
         "net_mse_ratio_luminance_over_guided_over_luminance_same_spp":
             round((mse_lum - ref_noise) / (mse_glum - ref_noise), 3),
+        "guided_over_luminance_to_depth3": {"spp": a.spp, "seconds": round(t_glum3, 3), "mse": mse_glum3},
+        "time_normalized_ratio_luminance_over_guided_over_luminance_to_depth3":
+            round((mse_lum - ref_noise) * t_lum / ((mse_glum3 - ref_noise) * t_glum3), 3),
         "scatter_guide_same_spp": {"spp": a.spp, "seconds": round(t_sc, 3), "mse": mse_sc},
         "both_guides_same_spp": {"spp": a.spp, "seconds": round(t_both, 3), "mse": mse_both},
     }

@@ -355,7 +355,8 @@ def equal_time_mse(ctx, ref_spp=4096, spp=64, seeds=(1, 12)):
     against the reference's cosine sampling, trained on the first quarter of
     its spp) at the largest square spp whose render time stays within the
     plain render's; also the environment luminance distribution (the strongest
-    sampler that does not learn) at equal time, and the guide over it.  Wall-
+    sampler that does not learn) at equal time, and the guide over it (up to
+    path depth 3).  Besides the equal-time ratios, MSE x seconds ratios.  Wall-
     clock times around each render call, guide training included; each MSE is
     the mean over the given seeds.
 
@@ -372,8 +373,8 @@ def equal_time_mse(ctx, ref_spp=4096, spp=64, seeds=(1, 12)):
     filt = N.FilterType_Gaussian
     resolve = lambda px: px[:, :3] / np.maximum(px[:, 3:4], 1e-300)
 
-    def run(seed, n, guide=False, env=N.AZ_ENV_COSINE):
-        px = ctx.render_f64(cam, filt=filt, seed=seed, spp=n, guide=guide, env_sampling=env)
+    def run(seed, n, guide=False, env=N.AZ_ENV_COSINE, md=0):
+        px = ctx.render_f64(cam, filt=filt, seed=seed, spp=n, guide=guide, env_sampling=env, guide_max_depth=md)
         return resolve(px), ctx.last_render_s
 
     t0 = time.time()
@@ -389,21 +390,22 @@ def equal_time_mse(ctx, ref_spp=4096, spp=64, seeds=(1, 12)):
     t_ref = time.time() - t0
     log(f"reference {ref_spp} spp in {t_ref:.1f}s, its own noise {ref_noise:.3e}")
 
-    def mse(n, guide=False, env=N.AZ_ENV_COSINE):
+    def mse(n, guide=False, env=N.AZ_ENV_COSINE, md=0):
         vals, ts = [], []
         for sd in seeds:
-            img, t = run(sd, n, guide, env)
+            img, t = run(sd, n, guide, env, md)
             vals.append(float(np.mean((img - ref) ** 2)))
             ts.append(t)
-        m = float(np.mean(vals))
-        return {"spp": n, "seconds": round(float(np.mean(ts)), 3), "mse": m, "mse_net": m - ref_noise}
+        m, t = float(np.mean(vals)), float(np.mean(ts))
+        return {"spp": n, "seconds": round(t, 3), "mse": m, "mse_net": m - ref_noise,
+                "mse_net_x_seconds": (m - ref_noise) * t}
 
-    def equal_spp(guide, env, t_budget):
-        _, t_same = run(3, spp, guide, env)
+    def equal_spp(guide, env, t_budget, md=0):
+        _, t_same = run(3, spp, guide, env, md)
         s = int(math.isqrt(max(1, int(spp * t_budget / t_same))))
-        while s > 1 and run(3, s * s, guide, env)[1] > t_budget * 1.02:
+        while s > 1 and run(3, s * s, guide, env, md)[1] > t_budget * 1.02:
             s -= 1
-        if run(3, (s + 1) ** 2, guide, env)[1] <= t_budget * 1.02:
+        if run(3, (s + 1) ** 2, guide, env, md)[1] <= t_budget * 1.02:
             s += 1
         return s * s
 
@@ -413,14 +415,24 @@ def equal_time_mse(ctx, ref_spp=4096, spp=64, seeds=(1, 12)):
                          "seconds": round(t_ref, 1), "noise_mse": ref_noise},
            "seeds": list(seeds), "guide_target": "environment NEE", "plain": mse(spp)}
     t_plain = res["plain"]["seconds"]
-    for name, guide, env in (("guided", True, N.AZ_ENV_COSINE), ("luminance", False, N.AZ_ENV_LUMINANCE),
-                             ("guided_over_luminance", True, N.AZ_ENV_LUMINANCE)):
-        res[name] = mse(equal_spp(guide, env, t_plain), guide, env)
+    # The guide over the luminance distribution is used up to path depth 3
+    # (deeper vertices take the luminance distribution alone), which pays on
+    # this scene; over the cosine distribution it is used at every depth.
+    for name, guide, env, md in (("guided", True, N.AZ_ENV_COSINE, 0), ("luminance", False, N.AZ_ENV_LUMINANCE, 0),
+                                 ("guided_over_luminance", True, N.AZ_ENV_LUMINANCE, 3)):
+        res[name] = mse(equal_spp(guide, env, t_plain, md), guide, env, md)
+        if guide:
+            res[name]["guide_max_depth"] = md
         log(f"{name}: {res[name]}")
-    ratio = lambda a, b: round(res[a]["mse_net"] / max(res[b]["mse_net"], 1e-30), 3)
+    ratio = lambda a, b, k="mse_net": round(res[a][k] / max(res[b][k], 1e-30), 3)
     res["mse_ratio_plain_over_guided"] = round(res["plain"]["mse"] / res["guided"]["mse"], 3)
     res["net_mse_ratio_plain_over_guided"] = ratio("plain", "guided")
     res["net_mse_ratio_luminance_over_guided_over_luminance"] = ratio("luminance", "guided_over_luminance")
+    # MSE x seconds (unbiased estimators: MSE falls as 1 / time), free of the
+    # square-spp steps of the equal-time renders.
+    res["time_normalized_ratio_plain_over_guided"] = ratio("plain", "guided", "mse_net_x_seconds")
+    res["time_normalized_ratio_luminance_over_guided_over_luminance"] = ratio(
+        "luminance", "guided_over_luminance", "mse_net_x_seconds")
     return res
 
 

@@ -73,7 +73,7 @@ typedef struct {
     f32 GuideFraction;      /* probability of drawing the direction from the guide (0 = 0.5) */
     u32 GuideTrainSpp;      /* samples per pixel rendered while training (0: a quarter of the
                                render's, at least 4 and at most 32) */
-    u32 GuideStepsPerChunk; /* Adam steps after each training chunk (0: spread 256 steps over the
+    u32 GuideStepsPerChunk; /* Adam steps after each training chunk (0: spread 512 steps over the
                                training chunks) */
     u32 EnvSampling;        /* environment next-event directions: AZ_ENV_COSINE (the reference's
                                cosine distribution about the normal) or AZ_ENV_LUMINANCE (the
