@@ -233,6 +233,10 @@ EXPORTS = {
     "az_sb_build": (s32, [_P, C.POINTER(_P)]),
     "az_scene_free": (None, [_P]),
     "az_mat4_inverse": (s32, [C.POINTER(mat4), C.POINTER(mat4)]),
+    "az_ply_read": (s32, [C.c_char_p, _P, C.POINTER(u32), _P, C.POINTER(u32), _P, C.POINTER(u32)]),
+    "az_sb_add_ply_mesh": (s32, [_P, C.c_char_p, C.POINTER(az_mesh_desc)]),
+    "az_read_binary_vertices": (s32, [C.c_char_p, _P, C.POINTER(u32)]),
+    "az_read_binary_faces": (s32, [C.c_char_p, u32, u32, _P, C.POINTER(u32)]),
     "az_scene_triangle_count": (u32, [_P]),
     "az_abi_sizes": (u32, [C.POINTER(u32), u32]),
     # azimuth_platform_posix.h
@@ -353,3 +357,40 @@ def math_eval_host(name, x, y=None):
     out = np.empty_like(x)
     check(load().az_math_eval(MATH_FUNCTIONS[name], x.ctypes.data, yp, x.size, out.ctypes.data), "az_math_eval")
     return out
+
+
+def ply_read(path):
+    """az_ply_read: (vertices (V, 3) f32, faces (F, 3) u32, uvs (V, 2) f32 or
+    None) of a PLY file."""
+    lib = load()
+    p = str(path).encode()
+    nv, nf, nuv = u32(0), u32(0), u32(0)
+    check(lib.az_ply_read(p, None, C.byref(nv), None, C.byref(nf), None, C.byref(nuv)), "az_ply_read")
+    v = np.zeros((nv.value, 3), np.float32)
+    f = np.zeros((nf.value, 3), np.uint32)
+    uv = np.zeros((nuv.value, 2), np.float32)
+    check(lib.az_ply_read(p, v.ctypes.data, C.byref(nv), f.ctypes.data, C.byref(nf),
+                          uv.ctypes.data if nuv.value else None, C.byref(nuv)), "az_ply_read")
+    return v, f, (uv if nuv.value else None)
+
+
+def read_binary_vertices(path):
+    lib = load()
+    p = str(path).encode()
+    n = u32(0)
+    check(lib.az_read_binary_vertices(p, None, C.byref(n)), "az_read_binary_vertices")
+    v = np.zeros((n.value, 3), np.float32)
+    check(lib.az_read_binary_vertices(p, v.ctypes.data, C.byref(n)), "az_read_binary_vertices")
+    return v
+
+
+def read_binary_faces(path, vertex_offset=0, zero_based=False):
+    lib = load()
+    p = str(path).encode()
+    n = u32(0)
+    check(lib.az_read_binary_faces(p, vertex_offset, 1 if zero_based else 0, None, C.byref(n)),
+          "az_read_binary_faces")
+    f = np.zeros((n.value, 3), np.uint32)
+    check(lib.az_read_binary_faces(p, vertex_offset, 1 if zero_based else 0, f.ctypes.data, C.byref(n)),
+          "az_read_binary_faces")
+    return f

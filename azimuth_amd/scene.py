@@ -193,6 +193,20 @@ class SceneBuilder:
         d.ShadingMode = N.ShadingMode_Smooth if smooth else N.ShadingMode_Flat
         N.check(self.lib.az_sb_add_mesh(self.h, C.byref(d)), "az_sb_add_mesh")
 
+    def add_ply_mesh(self, path, material=0, transform=None, motion=(0, 0, 0), emitter=False, radiance=(0, 0, 0),
+                     smooth=False, texture=0, uv_offset=0):
+        """A mesh read from a PLY file (az_sb_add_ply_mesh)."""
+        d = N.az_mesh_desc()
+        d.Transformation = N.mat4.from_numpy(np.eye(4) if transform is None else transform)
+        d.MotionVector = _vec3(motion)
+        d.Radiance = _vec3(radiance)
+        d.IsEmitter = 1 if emitter else 0
+        d.TextureIndex = int(texture)
+        d.MaterialIndex = int(material)
+        d.UVOffset = int(uv_offset)
+        d.ShadingMode = N.ShadingMode_Smooth if smooth else N.ShadingMode_Flat
+        N.check(self.lib.az_sb_add_ply_mesh(self.h, str(path).encode(), C.byref(d)), "az_sb_add_ply_mesh")
+
     def add_sphere(self, center, radius, material=0, transform=None, motion=(0, 0, 0), emitter=False,
                    radiance=(0, 0, 0), texture=0):
         d = N.az_sphere_desc()

@@ -2,9 +2,10 @@
  * azimuth_scene.h -- building a reference-shaped `scene` from geometry held in
  * memory, in place of the reference's XML importer.
  *
- * The reference fills `scene` in ImportXML (xml.cpp:3021-3930).  Its parsing
- * half reads XML/PLY files and is not part of this library; its post-process
- * half (xml.cpp:3632-3918) is restated here:
+ * The reference fills `scene` in ImportXML (xml.cpp:3021-3930).  Its XML
+ * parsing is not part of this library; its mesh file readers (PLY and the
+ * binary vertex and face files) and its post-process half (xml.cpp:3632-3918)
+ * are restated here:
  *   - per-vertex normals as the normalized sum of unnormalized face cross
  *     products (xml.cpp:3632-3667);
  *   - one triangle entity per face with flat/smooth normals and UVs, the
@@ -104,6 +105,34 @@ az_status az_sb_add_sphere(az_scene_builder *Builder, const az_sphere_desc *Sphe
  * caller and released with az_scene_free. */
 az_status az_sb_build(az_scene_builder *Builder, scene **Out);
 void      az_scene_free(scene *Scene);
+
+/* The reference importer's mesh files, read into memory (xml.cpp:2397-2420,
+ * 2621-2700, 3311-3630).  Malformed or truncated files give
+ * AZ_ERR_INVALID_ARGUMENT, PLY files outside what the reference reads
+ * (big-endian, property types other than float and the face list) give
+ * AZ_ERR_UNSUPPORTED.
+ *
+ * PLY, ASCII or binary little-endian: the vertex element's float properties
+ * (x y z, and u/s v/t when present: the reference's files carry x y z nx ny
+ * nz u v, whose normals it drops), the face element's "list uchar int"
+ * indices; quads become (a, b, c) and (a, c, d); faces of other sizes are an
+ * error.  With Vertices, Faces and UVs NULL only the counts are returned;
+ * with arrays, the counts give their capacity on entry (vertices, faces and
+ * uv pairs).  UVCount is 0 when the file has no u v. */
+az_status az_ply_read(const char *Path, f32 *Vertices, u32 *VertexCount, u32 *Faces, u32 *FaceCount, f32 *UVs,
+                      u32 *UVCount);
+/* A mesh whose vertices, faces and uvs come from a PLY file (the plyFile of
+ * a <Mesh>, xml.cpp:2621-2660); Mesh gives the rest (transform, material,
+ * texture, shading mode, UVOffset added to the vertex index for uv lookup);
+ * its Vertices, Faces and UVs are ignored. */
+az_status az_sb_add_ply_mesh(az_scene_builder *Builder, const char *Path, const az_mesh_desc *Mesh);
+/* <VertexData binaryFile=...>: a u32 count, then count x y z f32 triples.
+ * Vertices NULL: the count only; else *Count is the capacity on entry. */
+az_status az_read_binary_vertices(const char *Path, f32 *Vertices, u32 *Count);
+/* <Faces binaryFile=...>: a u32 count, then count u32 index triples;
+ * VertexOffset is added to every index and 1 subtracted unless
+ * ZeroBasedIndexing.  Faces NULL: the count only. */
+az_status az_read_binary_faces(const char *Path, u32 VertexOffset, b32 ZeroBasedIndexing, u32 *Faces, u32 *Count);
 
 /* 4x4 inverse used for entity transforms (double precision, then rounded). */
 az_status az_mat4_inverse(const mat4 *In, mat4 *Out);
