@@ -1,7 +1,7 @@
 """Render throughput of the wavefront path tracer on one GPU:
 
     python -m azimuth_amd.render_bench --workload cornell|interior|motion
-        [--spp N] [--scale S] [--guide] [--env cosine|luminance] [--autotune | --tuning B,W,L,C] [--cpu-seconds T]
+        [--spp N] [--scale S] [--guide] [--env cosine|luminance] [--guide-max-depth D] [--autotune | --tuning B,W,L,C] [--cpu-seconds T]
         [--levels graph|host]
     python -m azimuth_amd.render_bench --workload motion --scenes 16 [--autotune]
 
@@ -49,6 +49,7 @@ def main(argv=None):
     ap.add_argument("--repeat", type=int, default=1)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--guide", action="store_true", help="render with the learned path guide")
+    ap.add_argument("--guide-max-depth", type=int, default=0, help="path depth up to which the guide is used (0: all)")
     ap.add_argument("--env", default="cosine", choices=("cosine", "luminance"),
                     help="environment next-event directions: the reference's cosine distribution or the map's "
                          "luminance distribution")
@@ -82,18 +83,18 @@ def main(argv=None):
     env = N.AZ_ENV_LUMINANCE if a.env == "luminance" else N.AZ_ENV_COSINE
     # Warm-up: a full render, so that kernels are loaded and every queue
     # buffer has its steady-state size before the timed renders.
-    ctx.render(cam, filt=filt, seed=a.seed, guide=a.guide, env_sampling=env)
+    ctx.render(cam, filt=filt, seed=a.seed, guide=a.guide, env_sampling=env, guide_max_depth=a.guide_max_depth)
     best = None
     for _ in range(a.repeat):
         ctx.timer_start()
-        px = ctx.render(cam, filt=filt, seed=a.seed, guide=a.guide, env_sampling=env)
+        px = ctx.render(cam, filt=filt, seed=a.seed, guide=a.guide, env_sampling=env, guide_max_depth=a.guide_max_depth)
         ms = ctx.timer_stop()
         best = ms if best is None else min(best, ms)
     img = px["Color"] / np.maximum(px["TotalWeight"], 1e-30)[:, None]
     out = {"workload": a.workload, "width": W, "height": H, "spp": spp_eff, "triangles": sc.triangle_count,
            "render_ms": round(best, 3), "msamples_per_s": round(W * H * spp_eff / (best * 1e-3) / 1e6, 3),
            "upload_s": round(upload_s, 2), "mean_radiance": [round(float(x), 6) for x in img.mean(0)],
-           "filter": "gaussian" if filt == N.FilterType_Gaussian else "box", "guide": bool(a.guide), "env": a.env,
+           "filter": "gaussian" if filt == N.FilterType_Gaussian else "box", "guide": bool(a.guide), "env": a.env, "guide_max_depth": a.guide_max_depth,
            "autotune": tuned,
            "levels": a.levels, "level_graphs": dict(zip(("recorded", "launched"), ctx.level_graph_counts()))}
     if a.cpu_seconds > 0:
