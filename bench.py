@@ -247,6 +247,13 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
             "traffic": traffic,
+            # The HBM side of the same launches: the profiled bytes per launch
+            # over this run's mean launch time.  Algorithmic bytes above the
+            # peak mean the L2 and the MALL serve most node fetches; the
+            # kernel is then bound by VALU issue, not by HBM.
+            "hbm_side_gbps": round(traffic / ((ms_coh + ms_inc) / 2 * 1e-3) / 1e9, 1) if traffic else None,
+            "hbm_side_frac": round(traffic / ((ms_coh + ms_inc) / 2 * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)
+            if traffic else None,
         },
         "cpu_baseline": cpu,
     }
