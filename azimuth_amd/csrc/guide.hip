@@ -973,6 +973,12 @@ az_status az_guide_train(az_guide *G, const f32 *Samples, u32 N, b32 ApplyUpdate
     return AZ_OK;
 }
 
+az_status az_guide_synchronize(az_guide *G) {
+    if (!G) return AZ_ERR_INVALID_ARGUMENT;
+    AZ_GHIP(G, hipStreamSynchronize(G->stream));
+    return AZ_OK;
+}
+
 az_status az_guide_zero_grad(az_guide *G) {
     if (!G) return AZ_ERR_INVALID_ARGUMENT;
     AZ_GHIP(G, hipMemsetAsync(G->grad.ptr, 0, (size_t)kParamCount * sizeof(float), G->stream));

@@ -299,6 +299,7 @@ EXPORTS = {
     "az_guide_infer_device": (s32, [_P, _P, u32, _P]),
     "az_guide_infer_packed": (s32, [_P, _P, u32, _P]),
     "az_guide_infer_packed_device": (s32, [_P, _P, u32, _P]),
+    "az_guide_synchronize": (s32, [_P]),
     "az_guide_train": (s32, [_P, _P, u32, u32, C.POINTER(f32)]),
     "az_guide_train_device": (s32, [_P, _P, u32, u32, C.POINTER(f32)]),
     "az_guide_mfma_selftest": (s32, [_P, _P, _P, _P]),

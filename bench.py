@@ -19,6 +19,9 @@ Also reported:
   (1920x1080, 64 spp, Gaussian filter) with its own roofline fraction from a
   counted render's algorithmic bytes; render_other: the Cornell box and one
   motion-blur scene;
+- guide: the learned guide's packed inference (8.4M queries) and Adam
+  training step (65,536 samples) on their own, algorithmic TFLOP/s against
+  the 2.5 PFLOP/s dense BF16 peak;
 - equal_time_mse: the metric's second half, MSE against a 4096-spp
   double-precision reference of the guided and the plain tracer at equal
   wall-clock on the guided interior, beside the environment luminance
@@ -210,6 +213,8 @@ def main():
         log(f"cpu baseline: {cpu}")
     if not (args.profile or args.no_render):
         extra["render_interior"] = render_interior(ctx)
+        extra["guide"] = guide_kernels(ctx)
+        log(f"guide kernels: {extra['guide']}")
         log(f"render interior: {extra['render_interior']}")
         extra["render_other"] = render_other(ctx)
     if not (args.profile or args.no_render or args.no_mse):
@@ -334,6 +339,67 @@ def render_interior(ctx):
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBPS, 4),
                          "bytes_by_stage_gb": {k: round(v / 1e9, 3) for k, v in nb.items()}}}
+
+
+GUIDE_FLOPS_PER_QUERY = 2 * 23040                      # forward: 128-64-64-64-64-40 MACs x 2
+# forward, the gradient of layers 1-4's inputs (64 x 64 x 3 + 64 x 40 MACs), weight gradients
+GUIDE_FLOPS_PER_SAMPLE = 2 * (23040 + 14848 + 23040)
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def guide_kernels(ctx, n_query=8388608, batch=65536, reps=10, steps=64):
+    """The learned guide's MFMA kernels on their own, inputs in HBM: the
+    renderer's packed inference over n_query valid queries (random positions,
+    normals and roughness), and Adam training steps on batches of `batch`
+    samples; algorithmic TFLOP/s against the 2.5 PFLOP/s dense BF16 peak.
+    Wall-clock around the launches with the guide's stream synchronized on
+    both sides."""
+    import ctypes as C
+    from azimuth_amd import native as N
+    lib = N.load()
+    g = C.c_void_p()
+    N.check(lib.az_guide_create(C.byref(g), 0, 7), "az_guide_create")
+    rng = np.random.default_rng(3)
+    nrm = rng.normal(size=(n_query, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    q = np.zeros((n_query, 8), np.float32)
+    q[:, 0:3], q[:, 3:6], q[:, 6] = rng.random((n_query, 3)), nrm, rng.random(n_query)
+    dq, dl = ctx.alloc(q.nbytes), ctx.alloc(n_query * 64)
+    dq.upload(q)
+    smp = np.zeros((batch, 12), np.float32)
+    d = rng.normal(size=(batch, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    smp[:, 0:3], smp[:, 3:6], smp[:, 6], smp[:, 7:10], smp[:, 10] = q[:batch, 0:3], q[:batch, 3:6], \
+        q[:batch, 6], d, 1.0
+    ds = ctx.alloc(smp.nbytes)
+    ds.upload(smp)
+    ctx.synchronize()
+    chk = lambda st, what: N.check(st, what)
+    chk(lib.az_guide_infer_packed_device(g, dq.ptr, n_query, dl.ptr), "az_guide_infer_packed_device")
+    chk(lib.az_guide_synchronize(g), "az_guide_synchronize")
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        chk(lib.az_guide_infer_packed_device(g, dq.ptr, n_query, dl.ptr), "az_guide_infer_packed_device")
+    chk(lib.az_guide_synchronize(g), "az_guide_synchronize")
+    t_inf = (time.perf_counter() - t0) / reps
+    chk(lib.az_guide_train_device(g, ds.ptr, batch, 1, None), "az_guide_train_device")
+    chk(lib.az_guide_synchronize(g), "az_guide_synchronize")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        chk(lib.az_guide_train_device(g, ds.ptr, batch, 1, None), "az_guide_train_device")
+    chk(lib.az_guide_synchronize(g), "az_guide_synchronize")
+    t_step = (time.perf_counter() - t0) / steps
+    lib.az_guide_destroy(g)
+    for b in (dq, dl, ds):
+        b.free()
+    inf_tf = n_query * GUIDE_FLOPS_PER_QUERY / t_inf / 1e12
+    step_tf = batch * GUIDE_FLOPS_PER_SAMPLE / t_step / 1e12
+    return {"inference": {"queries": n_query, "ms": round(t_inf * 1e3, 3), "tflops": round(inf_tf, 1),
+                          "frac_of_bf16_peak": round(inf_tf / BF16_PEAK_TFLOPS, 4)},
+            "train_step": {"batch": batch, "ms": round(t_step * 1e3, 4), "tflops": round(step_tf, 1),
+                           "frac_of_bf16_peak": round(step_tf / BF16_PEAK_TFLOPS, 4)},
+            "peak_tflops": BF16_PEAK_TFLOPS, "flops_per_query": GUIDE_FLOPS_PER_QUERY,
+            "flops_per_training_sample": GUIDE_FLOPS_PER_SAMPLE}
 
 
 def render_other(ctx):

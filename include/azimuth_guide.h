@@ -51,6 +51,8 @@ az_status az_guide_infer_packed_device(az_guide *Guide, const void *d_Queries, u
  * then an Adam update when ApplyUpdate is non-zero.  MeanLoss may be NULL. */
 az_status az_guide_train(az_guide *Guide, const f32 *Samples, u32 N, b32 ApplyUpdate, f32 *MeanLoss);
 az_status az_guide_train_device(az_guide *Guide, const void *d_Samples, u32 N, b32 ApplyUpdate, f32 *MeanLoss);
+/* Waits for the guide's queued device work (its own HIP stream). */
+az_status az_guide_synchronize(az_guide *Guide);
 /* D = A * B for row-major 16x16 matrices through one BF16 MFMA in the
  * operand layouts the guide kernels use (a layout self-check). */
 az_status az_guide_mfma_selftest(az_guide *Guide, const f32 *A, const f32 *B, f32 *D);
