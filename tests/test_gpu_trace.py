@@ -213,3 +213,36 @@ def test_exact_ties_follow_reference_order(gpu):
     assert np.array_equal(want["Prim"], brute["Prim"])
     assert np.array_equal(got["Prim"], want["Prim"]), f"{int((got['Prim'] != want['Prim']).sum())} tie mismatches"
     assert np.array_equal(got["t"], want["t"])
+
+
+def test_degenerate_triangles(gpu):
+    """A displaced icosphere in which every fifth face is collapsed to a point
+    and every seventh made collinear (zero area, as meshes exported with
+    welded vertices carry them): the zero-area faces never hit (the
+    reference's Cramer's rule divides by a zero determinant), their padded
+    boxes do not disturb the walk, and the other faces' hits are the
+    oracle's, ids and t, beta, gamma, closest and any hit."""
+    from azimuth_amd.scene import SceneBuilder, make_material
+    v, f = configs.displaced_icosphere(4, 5)
+    v = v.astype(np.float32).copy()
+    f = f.astype(np.uint32).copy()
+    f[::5, 1] = f[::5, 0]
+    f[::5, 2] = f[::5, 0]                                  # a point
+    col = np.arange(3, f.shape[0], 7)
+    v = np.concatenate([v, 0.5 * (v[f[col, 0]] + v[f[col, 1]])]).astype(np.float32)
+    f[col, 2] = np.arange(v.shape[0] - col.size, v.shape[0], dtype=np.uint32)   # on the edge (0, 1)
+    b = SceneBuilder(max_depth=1)
+    b.add_material(make_material())
+    b.add_mesh(v, f)
+    sc = b.build()
+    gpu.upload_scene(sc)
+    lo, hi = v.min(0), v.max(0)
+    rays = np.concatenate([configs.coherent_rays(192, 192), configs.incoherent_rays(32768, (lo, hi), 21)])
+    got = gpu.trace(rays)
+    want = oracle.trace(sc, rays)
+    exact = compare_closest(got, want, "degenerate faces")
+    assert exact == 1.0
+    assert int((want["Prim"] != N.AZ_MISS).sum()) > 10000
+    anyg = gpu.trace(rays, any_hit=True)
+    anyw = oracle.trace(sc, rays, any_hit=True)
+    assert np.array_equal(anyg["Prim"] != N.AZ_MISS, anyw["Prim"] != N.AZ_MISS)
