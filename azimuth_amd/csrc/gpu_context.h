@@ -10,6 +10,7 @@
 #include "azimuth_guide.h"
 #include "device_render.h"
 #include "device_scene.h"
+#include "guide.h"
 #include "kernels.h"
 
 namespace az {
@@ -88,6 +89,9 @@ void az_release_render_buffers(void *buffers);
 // ordered on stream s.
 az_status az_guide_infer_on_stream(az_guide *G, const void *d_Queries, u32 N, void *d_Packed, hipStream_t s);
 az_status az_guide_train_on_stream(az_guide *G, const void *d_Samples, u32 N, hipStream_t s);
+// The device copies of the guide's weights, for kernels that run the network
+// themselves (the renderer's fused query and inference).
+az::GuideWeights az_guide_device_weights(az_guide *G);
 
 struct az_gpu_ctx {
     int device = 0;

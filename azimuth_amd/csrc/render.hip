@@ -27,6 +27,7 @@
 #include "device_trace.h"
 #include "gpu_context.h"
 #include "azimuth_guide.h"
+#include "guide_infer.h"
 #include "guide_lobes.h"
 #include "guide.h"
 #include "host_parallel.h"
@@ -1234,6 +1235,89 @@ __global__ __launch_bounds__(kRenderBlock) void k_guide_query(DevScene S, Device
         }
     }
     out[i] = g;
+}
+
+// The environment guide's queries and inference in one pass over a level's
+// hits.  Each wave takes 64 rays at a time and gathers, in an LDS list, the
+// indices of those that hit a surface that does not emit (misses and
+// emitters are never guided); for every 64 gathered it builds their queries
+// (one ray per lane, as k_guide_query builds them) into LDS and runs the
+// network on them (infer_batch, guide_infer.h), writing each ray's packed
+// lobes at its index.  The memory-bound query building of some waves runs
+// beside the MFMA work of others, and the queries never go through HBM.
+struct alignas(16) FusedInferLds {
+    InferLds net;
+    GuideQuery q[kInferWaves][kInferSamples];
+    uint32_t soff[kSegs + 1];
+};
+
+__global__ __launch_bounds__(64 * kInferWaves) void k_guide_query_infer(DevScene S, DeviceGeometry G, GuideWeights W,
+                                                                       const PathRay *__restrict__ queue, SegIn qin,
+                                                                       const PathHit *__restrict__ hits, uint32_t n,
+                                                                       uint2 *__restrict__ lobes) {
+    __shared__ FusedInferLds L;
+    for (int k = threadIdx.x; k < kHalfCount / 8; k += 64 * kInferWaves)
+        reinterpret_cast<uint4 *>(L.net.w)[k] = reinterpret_cast<const uint4 *>(W.wf)[k];
+    for (int k = threadIdx.x; k < 5 * kGuideBiasStride; k += 64 * kInferWaves) L.net.bias[k] = W.bias[k];
+    const uint32_t *off = nullptr;
+    if (qin.nseg != 1u) {
+        for (uint32_t k = threadIdx.x; k <= qin.nseg; k += blockDim.x) L.soff[k] = qin.offsets[k];
+        off = L.soff;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t chunks = (n + kInferSamples - 1) / kInferSamples;
+    const uint32_t first = blockIdx.x * kInferWaves + wave, stride = gridDim.x * kInferWaves;
+    uint32_t *list = L.net.idx[wave];
+    GuideQuery *qs = L.q[wave];
+    uint32_t pending = 0;                            // wave-uniform: gathered indices not yet run
+    for (uint32_t it = first;; it += stride) {
+        const bool more = it < chunks;
+        if (more) {
+            const uint32_t s = it * kInferSamples + lane;
+            bool valid = false;
+            if (s < n) {
+                const PathHit h = hits[s];
+                valid = h.prim != kEmpty && !S.itemShade[h.item].isEmitter;
+            }
+            const uint64_t m = __ballot(valid);
+            if (valid)
+                list[pending + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))] = s;
+            pending += (uint32_t)__popcll(m);
+            wave_lds_sync();
+        }
+        if (pending >= (uint32_t)kInferSamples || (!more && pending > 0u)) {
+            const uint32_t cnt = min(pending, (uint32_t)kInferSamples);
+            if ((uint32_t)lane < cnt) {
+                const uint32_t s = list[lane];
+                const PathRay r = queue[seg_slot(off, qin.nseg, qin.segCap, s)];
+                const PathHit h = hits[s];
+                const DevItemShade sh = S.itemShade[h.item];
+                V3 P, N;
+                float u, v;
+                reconstruct(S, G, r, h, sh, P, N, u, v);
+                GuideQuery g;
+                guide_position(S, P, g.p);
+                g.n[0] = N.x;
+                g.n[1] = N.y;
+                g.n[2] = N.z;
+                g.rough = guide_roughness(S.materials[sh.material]);
+                g.pad = 0.0f;
+                qs[lane] = g;
+            }
+            wave_lds_sync();
+            infer_batch<true>(L.net, [qs](uint32_t j) { return qs[j]; }, cnt, [list](uint32_t j) { return list[j]; },
+                              lobes, wave, lane);
+            // The indices past the batch move to the front of the list.
+            const uint32_t rest = pending - cnt;
+            const uint32_t v = (uint32_t)lane < rest ? list[kInferSamples + lane] : 0u;
+            wave_lds_sync();
+            if ((uint32_t)lane < rest) list[lane] = v;
+            wave_lds_sync();
+            pending = rest;
+        }
+        if (!more) break;
+    }
 }
 
 // Training batches are drawn in proportion to the records' weights (a
@@ -2617,7 +2701,17 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
                 GI.rayRec = depth > 0 ? (const uint32_t *)recA->ptr : nullptr;
                 O.nextRec = (uint32_t *)recB->ptr;
             }
-            if (guided && guideLevel) {
+            if (guided && guideLevel && envGuide && !scatGuide) {
+                // The environment guide alone: queries and inference fused.
+                AZ_HIPR(C, B.lobes.alloc((size_t)count * kGuideLobeBytes));
+                const u32 chunks = (count + kInferSamples - 1) / kInferSamples;
+                dim3 ig(std::min<u32>((chunks + kInferWaves - 1) / kInferWaves, (u32)C->cuCount));
+                hipLaunchKernelGGL(k_guide_query_infer, ig, dim3(64 * kInferWaves), 0, st, S, C->dg,
+                                   az_guide_device_weights(C->guide), qa, qin, (const PathHit *)B.hits.ptr, count,
+                                   (uint2 *)B.lobes.ptr);
+                AZ_HIPR(C, hipGetLastError());
+                GI.lobes = (const uint2 *)B.lobes.ptr;
+            } else if (guided && guideLevel) {
                 // Query every hit, then one batched MFMA inference per guide.
                 AZ_HIPR(C, B.queries.alloc((size_t)count * sizeof(GuideQuery)));
                 hipLaunchKernelGGL(k_guide_query, grid, dim3(kRenderBlock), 0, st, S, C->dg, qa, qin,
