@@ -206,7 +206,7 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_guide_train_pass1(GuideWei
         wave_lds_sync();
         // Loss and output gradient, one (sample, lobe) item per lane, two
         // items per lane: the 8 lanes of a sample form the softmax of its
-        // weights and the log-sum-exp of its mixture with shuffles.  All
+        // weights and the log-sum-exp of its mixture with DPP moves.  All
         // reads of the raw outputs come before the gradient overwrites them.
         float dv[2][5];
 #pragma unroll
@@ -216,15 +216,9 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_guide_train_pass1(GuideWei
             const GuideSample &gs = smp[grp * 16 + sl];
             const float w0 = gs.dir[0], w1 = gs.dir[1], w2 = gs.dir[2], wgt = gs.wgt;
             const float logit = o[k];
-            float m = logit;
-            m = fmaxf(m, __shfl_xor(m, 1, 8));
-            m = fmaxf(m, __shfl_xor(m, 2, 8));
-            m = fmaxf(m, __shfl_xor(m, 4, 8));
+            const float m = max8(logit);
             const float e = expf(logit - m);
-            float se = e;
-            se += __shfl_xor(se, 1, 8);
-            se += __shfl_xor(se, 2, 8);
-            se += __shfl_xor(se, 4, 8);
+            const float se = sum8(e);
             const float pi = e / se;
             const float rx = o[8 + 3 * k], ry = o[9 + 3 * k], rz = o[10 + 3 * k];
             const float len = sqrtf(rx * rx + ry * ry + rz * rz);
@@ -234,14 +228,8 @@ __global__ __launch_bounds__(64 * kTrainWaves) void k_guide_train_pass1(GuideWei
             const float c = mx * w0 + my * w1 + mz * w2;
             const float lp = logf(kap) - logf(2.0f * 3.14159265f) - log1pf(-expf(-2.0f * kap)) + kap * (c - 1.0f) +
                              logf(fmaxf(pi, 1e-30f));
-            float lm = lp;
-            lm = fmaxf(lm, __shfl_xor(lm, 1, 8));
-            lm = fmaxf(lm, __shfl_xor(lm, 2, 8));
-            lm = fmaxf(lm, __shfl_xor(lm, 4, 8));
-            float ls = expf(lp - lm);
-            ls += __shfl_xor(ls, 1, 8);
-            ls += __shfl_xor(ls, 2, 8);
-            ls += __shfl_xor(ls, 4, 8);
+            const float lm = max8(lp);
+            const float ls = sum8(expf(lp - lm));
             const float lq = lm + logf(ls);
             if (k == 0) T.loss[grp * 16 + sl] = -wgt * lq;
             const float gq = -wgt * invN;                    // dL / dlog q

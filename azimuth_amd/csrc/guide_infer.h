@@ -80,6 +80,26 @@ __device__ inline v4f mfma32(v8s a, v4s b0, v4s b1, v4f c) {
                                                    0);
 }
 
+// Maximum and sum over the 8 consecutive lanes of a sample with DPP lane
+// moves: quad swaps for lane ^ 1 and lane ^ 2, then the half-row mirror,
+// which pairs lane i with lane 7 - i of each group of 8.  The operands meet
+// as with xor shuffles of 1, 2 and 4 (the last step adds the same two values
+// in the other order), so the sums are the same bit for bit; no LDS traffic.
+template <int CTRL>
+__device__ inline float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+__device__ inline float max8(float v) {
+    v = fmaxf(v, dpp_f<0xB1>(v));      // quad_perm [1, 0, 3, 2]
+    v = fmaxf(v, dpp_f<0x4E>(v));      // quad_perm [2, 3, 0, 1]
+    return fmaxf(v, dpp_f<0x141>(v));  // row_half_mirror
+}
+__device__ inline float sum8(float v) {
+    v += dpp_f<0xB1>(v);
+    v += dpp_f<0x4E>(v);
+    return v + dpp_f<0x141>(v);
+}
+
 // ---- input encoding ----------------------------------------------------------------
 // Features 0..95 are sin and cos of pi * 2^j * p[c] for 16 frequencies j and
 // the 3 coordinates c; the order follows the B-operand lanes: the lanes of
@@ -212,7 +232,7 @@ __device__ inline void infer_batch(InferLds &L, QueryOf query, uint32_t cnt, At 
         }
         wave_lds_sync();
         // Lobe k of sample sl for the 128 (sample, lobe) items, two per
-        // lane: the 8 lanes of a sample form its softmax with shuffles, and
+        // lane: the 8 lanes of a sample form its softmax with DPP moves, and
         // consecutive lanes write consecutive lobes.
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
@@ -220,15 +240,9 @@ __device__ inline void infer_batch(InferLds &L, QueryOf query, uint32_t cnt, At 
             const uint32_t j = 16 * g + sl;
             const float *o = L.outs[wave][sl];
             const float logit = o[k];
-            float m = logit;
-            m = fmaxf(m, __shfl_xor(m, 1, 8));
-            m = fmaxf(m, __shfl_xor(m, 2, 8));
-            m = fmaxf(m, __shfl_xor(m, 4, 8));
+            const float m = max8(logit);
             const float e = __builtin_amdgcn_exp2f((logit - m) * 1.44269504f);
-            float sum = e;
-            sum += __shfl_xor(sum, 1, 8);
-            sum += __shfl_xor(sum, 2, 8);
-            sum += __shfl_xor(sum, 4, 8);
+            const float sum = sum8(e);
             const float x0 = o[8 + 3 * k], y0 = o[9 + 3 * k], z0 = o[10 + 3 * k];
             const float len2 = x0 * x0 + y0 * y0 + z0 * z0;
             const float inv = len2 > 1e-24f ? __builtin_amdgcn_rsqf(len2) : 0.0f;
