@@ -184,7 +184,7 @@ __device__ inline bool sphere_test(const TopItem &it, const Ray3 &r, float eps, 
 // Near/far planes are chosen by the sign of 1/d as in RayAABBIntersection
 // (ray.h:78-102); empty children (lo=+inf, hi=-inf) never hit.
 struct NodeHits {
-    float tn[4];
+    float key[4];       // the visiting order's key; +inf for a miss
     uint32_t ref[4];
 };
 
@@ -201,9 +201,24 @@ __device__ inline NearPlanes near_planes(float ix, float iy, float iz) {
     return NearPlanes{ix >= 0.0f ? 0u : 16u, iy >= 0.0f ? 32u : 48u, iz >= 0.0f ? 64u : 80u};
 }
 
+// Order in which a node's children hit are visited.  A closest-hit walk
+// takes the nearest entry first, which tightens its clip distance soonest.
+// An occlusion query has the same answer in any order and stops at its
+// first hit, so it takes the child most likely to block the ray first:
+// measured on a 1.3M-triangle mesh and on the interior render's shadow rays,
+//   kOverlapFirst (longest stretch of the ray inside the box):
+//     camera rays +2 %, rays from random points +2 %;
+//   kFarExitFirst (furthest box exit; closest-hit lanes nearest first):
+//     shadow rays from surface points to lights 9.8 -> 5.7 nodes per ray,
+//     where nearest-first spends its steps in the boxes around the origin.
+enum ChildOrder { kNearFirst = 0, kOverlapFirst = 1, kFarExitFirst = 2 };
+
 // Node byte offsets are 32-bit: the upload keeps BVHs below 2^25 nodes.
+// far (kFarExitFirst only): this lane's ray is an occlusion query.
+template <int ORDER = kNearFirst>
 __device__ inline int node_test(const Node4 *__restrict__ nodes, uint32_t idx, float ox, float oy,
-                                float oz, float ix, float iy, float iz, NearPlanes np, float tclip, NodeHits &h) {
+                                float oz, float ix, float iy, float iz, NearPlanes np, float tclip, NodeHits &h,
+                                bool far = false) {
     const char *base = reinterpret_cast<const char *>(nodes);
     const uint32_t o = idx * (uint32_t)sizeof(Node4);
     const float4 nx = *reinterpret_cast<const float4 *>(base + (o + np.x));
@@ -235,19 +250,22 @@ __device__ inline int node_test(const Node4 *__restrict__ nodes, uint32_t idx, f
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         bool hit = tn[k] <= tf[k];
-        h.tn[k] = hit ? tn[k] : INFINITY;
+        const float key = ORDER == kNearFirst      ? tn[k]
+                          : ORDER == kOverlapFirst ? tn[k] - tf[k]
+                                                   : (far ? -tf[k] : tn[k]);
+        h.key[k] = hit ? key : INFINITY;
         h.ref[k] = cr[k];
         n += hit ? 1 : 0;
     }
-    // Sort the four (distance, ref) pairs ascending; misses sort last.
-#define AZ_CSWAP(i, j)                                                                  \
-    if (h.tn[j] < h.tn[i]) {                                                            \
-        float tt = h.tn[i];                                                             \
-        h.tn[i] = h.tn[j];                                                              \
-        h.tn[j] = tt;                                                                   \
-        uint32_t rr = h.ref[i];                                                         \
-        h.ref[i] = h.ref[j];                                                            \
-        h.ref[j] = rr;                                                                  \
+    // Sort the four (key, ref) pairs ascending; misses sort last.
+#define AZ_CSWAP(i, j)                                                              \
+    if (h.key[j] < h.key[i]) {                                                      \
+        float tt = h.key[i];                                                        \
+        h.key[i] = h.key[j];                                                        \
+        h.key[j] = tt;                                                              \
+        uint32_t rr = h.ref[i];                                                     \
+        h.ref[i] = h.ref[j];                                                        \
+        h.ref[j] = rr;                                                              \
     }
     AZ_CSWAP(0, 1)
     AZ_CSWAP(2, 3)
@@ -306,6 +324,7 @@ struct TraceCounters {
 template <int MODE, bool COUNT, int RING, int BLOCK, bool WW, class Pool>
 __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t begin, uint32_t end, float eps,
                                   bool cull, uint32_t *ldsLane, TraceCounters &cnt) {
+    constexpr int kOrder = MODE == 0 ? kNearFirst : MODE == 1 ? kOverlapFirst : kFarExitFirst;
     RingStack<RING, BLOCK> stack;
     stack.base = ldsLane;
     uint32_t next = begin;                          // wave-uniform: first unclaimed ray
@@ -405,7 +424,7 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
             if (COUNT) cnt.nodes++;
             float tclip = inBlas ? localBest : bestW;
             NodeHits h;
-            int n = node_test(G.nodes, cur, ox, oy, oz, ix, iy, iz, np, tclip, h);
+            int n = node_test<kOrder>(G.nodes, cur, ox, oy, oz, ix, iy, iz, np, tclip, h, any);
             if (n > 0) {
                 cur = h.ref[0];
                 if (n > 3) stack.push(sp, h.ref[3]);
@@ -454,7 +473,7 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
             if (COUNT) cnt.nodes++;
             float tclip = inBlas ? localBest : bestW;
             NodeHits h;
-            int n = node_test(G.nodes, cur, ox, oy, oz, ix, iy, iz, np, tclip, h);
+            int n = node_test<kOrder>(G.nodes, cur, ox, oy, oz, ix, iy, iz, np, tclip, h, any);
             if (n > 0) {
                 cur = h.ref[0];
                 if (n > 3) stack.push(sp, h.ref[3]);
