@@ -250,8 +250,12 @@ __device__ inline int node_test(const Node4 *__restrict__ nodes, uint32_t idx, f
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         bool hit = tn[k] <= tf[k];
+        // A hit's key stays below +inf, so that no hit sorts with the misses:
+        // tn <= tf <= the clip distance, finite but for a batch occlusion
+        // query's own tmax, where tn = tf = +inf (nothing reachable) makes
+        // the overlap key NaN, taken as -FLT_MAX.
         const float key = ORDER == kNearFirst      ? tn[k]
-                          : ORDER == kOverlapFirst ? tn[k] - tf[k]
+                          : ORDER == kOverlapFirst ? fmaxf(tn[k] - tf[k], -kFltMax)
                                                    : (far ? -tf[k] : tn[k]);
         h.key[k] = hit ? key : INFINITY;
         h.ref[k] = cr[k];

@@ -759,6 +759,137 @@ __device__ inline void reconstruct(const DevScene &S, const DeviceGeometry &G, c
 
 }
 
+// Direction to, distance of, falloff and shadow-ray reach of a point, spot or
+// directional light at P, as CalculateLighting computes them for these types
+// (main.cpp:611-627, 643-672, 697-699); they draw no random numbers.
+__device__ inline void delta_light(const DevLight &L, V3 P, float eps, V3 &Ld, float &ldist, float &falloff,
+                                   float &tmax) {
+    ldist = 1.0f;
+    falloff = 1.0f;
+    tmax = 1.0f - eps;
+    if (L.type == kLightPoint) {
+        Ld = v3(L.pos) - P;
+        ldist = length(Ld);
+    } else if (L.type == kLightDirectional) {
+        Ld = normalize(-v3(L.dir));
+        tmax = INFINITY;
+    } else {
+        Ld = v3(L.pos) - P;
+        ldist = length(Ld);
+        float ct = dot(normalize(-Ld), normalize(v3(L.dir)));
+        if (ct < L.cosCov) falloff = 0.0f;
+        else if (ct > L.cosFall) falloff = 1.0f;
+        else {
+            falloff = (ct - L.cosCov) / (L.cosFall - L.cosCov);
+            falloff = PowIF<4>(falloff);
+            if (falloff < 0.0f) falloff = 0.0f;
+            else if (falloff > 1.0f) falloff = 1.0f;
+        }
+    }
+    if (ldist == 0.0f) ldist = 0.001f;
+}
+
+// A cheap test for the same light: false only where the full computation
+// gives the vertex nothing, that is where the light lies behind the surface
+// (cosI <= 0) or outside a spot's cone (falloff 0).  Estimates by the hardware
+// reciprocal square root, within about 1e-6 of the exact cosines, decide with
+// a margin of 1e-4 (N is a unit vector); vectors too short or too long for
+// that estimate pass.
+__device__ inline bool delta_light_may_add(const DevLight &L, V3 P, V3 N) {
+    const float m = 1e-4f;
+    const V3 Ld = L.type == kLightDirectional ? -v3(L.dir) : v3(L.pos) - P;
+    const float l2 = dot(Ld, Ld);
+    if (!(l2 > 1e-30f && l2 < 1e30f)) return true;
+    const float il = __builtin_amdgcn_rsqf(l2);
+    if (dot(N, Ld) * il < -m) return false;
+    if (L.type == kLightSpot) {
+        const V3 sd = v3(L.dir);
+        const float s2 = dot(sd, sd);
+        if (s2 > 1e-30f && s2 < 1e30f && -dot(Ld, sd) * il * __builtin_amdgcn_rsqf(s2) < L.cosCov - m) return false;
+    }
+    return true;
+}
+
+__device__ inline bool is_delta_light(int32_t type) {
+    return type == kLightPoint || type == kLightSpot || type == kLightDirectional;
+}
+
+// Shadow rays of a wave's (lane, light) pairs for point, spot and directional
+// lights.  Each lane's cand has bit li set for the lights li < 32 that may
+// add to its vertex (delta_light_may_add: every light left out would have
+// added exactly nothing, and a vertex inside a dielectric takes none).  The wave's
+// lanes share the pairs out (64 at a time) and compute each in full from the
+// owner lane's vertex, so that lanes outside a spot's cone or behind the
+// surface do not hold the wave in the BRDF and the shadow-ray append (a
+// 1080p render with 8 spot lights: shading 189 -> 146 ms).  Every lane
+// still shading must call it; pairs of lanes that left are never made.
+constexpr uint32_t kPairCap = 1024;   // pairs staged per wave and pass
+
+__device__ __forceinline__ void delta_light_pairs(const DevScene &S, const ShadeOut &O, uint32_t seg, uint32_t cand,
+                                                  uint16_t *pairs, uint32_t matIdx, V3 kd, V3 P, V3 N, V3 D, V3 w,
+                                                  float time, uint32_t sample, uint32_t rin) {
+    const uint32_t lane = __lane_id();
+    const uint64_t act = __ballot(true);
+    const uint32_t nAct = (uint32_t)__popcll(act);
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+    // Exclusive prefix of the lanes' pair counts (at most 32: six bits).
+    const uint32_t cnt = (uint32_t)__popc(cand);
+    uint32_t pre = 0u, total = 0u;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        const uint64_t m = __ballot((cnt >> b) & 1u);
+        pre += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << b;
+        total += (uint32_t)__popcll(m) << b;
+    }
+    const float eps = S.epsShadow;
+    for (uint32_t w0 = 0u; w0 < total; w0 += kPairCap) {
+        {
+            uint32_t bits = cand, idx = pre;
+            while (bits) {
+                const uint32_t li = (uint32_t)__builtin_ctz(bits);
+                bits &= bits - 1u;
+                if (idx >= w0 && idx < w0 + kPairCap) pairs[idx - w0] = (uint16_t)((lane << 5) | li);
+                ++idx;
+            }
+        }
+        wave_lds_sync();
+        const uint32_t n = min(total - w0, kPairCap);
+        for (uint32_t r0 = 0u; r0 < n; r0 += nAct) {
+            const bool has = r0 + rank < n;
+            const uint32_t pr = has ? pairs[r0 + rank] : 0u;
+            const int o = (int)(pr >> 5);
+            // The owner's vertex.
+            const uint32_t mo = __shfl(matIdx, o);
+            const V3 Po = v3(__shfl(P.x, o), __shfl(P.y, o), __shfl(P.z, o));
+            const V3 No = v3(__shfl(N.x, o), __shfl(N.y, o), __shfl(N.z, o));
+            const V3 Do = v3(__shfl(D.x, o), __shfl(D.y, o), __shfl(D.z, o));
+            const V3 wo = v3(__shfl(w.x, o), __shfl(w.y, o), __shfl(w.z, o));
+            const V3 kdo = v3(__shfl(kd.x, o), __shfl(kd.y, o), __shfl(kd.z, o));
+            const float to = __shfl(time, o);
+            const uint32_t so = __shfl(sample, o);
+            const uint32_t ro = __shfl(rin, o);
+            bool want = false;
+            V3 c = v3(0.0f, 0.0f, 0.0f), Ld = v3(0.0f, 0.0f, 0.0f);
+            float tmax = 0.0f;
+            uint32_t ent = 0u;
+            if (has) {
+                const DevLight &L = S.lights[pr & 31u];
+                const DevMaterial M = S.materials[mo];
+                float ldist, falloff;
+                delta_light(L, Po, eps, Ld, ldist, falloff, tmax);
+                want = light_term(M, kdo, v3(L.intensity), Ld, Do, No, ldist, falloff, 1.0f, c);
+                if (want) c = had(wo, c);
+                want = want && (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f);
+                ent = L.entityId;
+            }
+            ShadowRay sr = make_shadow(Po + (Ld * eps), Ld, to, tmax, c, so, kShadowAny, ent);
+            sr.srec = ro;
+            emit_shadow(O, seg, want, sr);
+        }
+        wave_lds_sync();
+    }
+}
+
 // GUIDED = false compiles the learned guides' code out (plain renders and
 // batches); with true it runs where GI's pointers are set.
 template <bool GUIDED>
@@ -769,6 +900,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
     __shared__ uint32_t soff[kSegs + 1];
     __shared__ uint32_t cls[kShadeClasses];
     __shared__ uint16_t order[kRenderBlock];
+    __shared__ uint16_t pairs[kRenderBlock / 64][kPairCap];
     const uint32_t *off = stage_offsets(qin, soff);
     // Rays of the block sorted by shading class (a counting sort in LDS), so
     // that the lanes of a wave take the same branches of the vertex code.
@@ -866,9 +998,17 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
     Rng rng = make_rng(S.pcgTable, cp.seed, pixel, s, r.node);
     const float eps = S.epsShadow;
 
-    // Lights (main.cpp:609-717).
+    // Lights (main.cpp:609-717).  Point, spot and directional lights 0-31
+    // are only marked here (delta_light_pairs computes them); area and
+    // entity lights, which draw random numbers, and any later lights run
+    // here in light order.
+    uint32_t cand = 0u;
     for (uint32_t li = 0; li < S.lightCount; ++li) {
         const DevLight &L = S.lights[li];
+        if (li < 32u && is_delta_light(L.type)) {
+            if (!inside && delta_light_may_add(L, P, N)) cand |= 1u << li;
+            continue;
+        }
         V3 Ld = v3(0.0f, 0.0f, 0.0f), Li = v3(L.intensity);
         float falloff = 1.0f, ldist = 1.0f, invPdf = 1.0f, tmax = 1.0f - eps;
         uint32_t mode = kShadowAny;
@@ -944,6 +1084,8 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
         sr.srec = rin;
         emit_shadow(O, seg, want, sr);
     }
+    if (__ballot(cand != 0u))
+        delta_light_pairs(S, O, seg, cand, pairs[threadIdx.x >> 6], sh.material, kd, P, N, D, w, r.time, r.sample, rin);
     // Environment NEE, cosine-distributed (main.cpp:719-778).  With the
     // guide, the direction comes from the guide with probability alpha and
     // from the cosine distribution otherwise, weighted by the mixture pdf
