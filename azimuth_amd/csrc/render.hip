@@ -2878,7 +2878,11 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
                     GI.scatLobes = (const uint2 *)B.scatLobes.ptr;
                 }
             }
-            if (guided)
+            // Levels of a guided render that use no guide (past GuideMaxDepth,
+            // outside training) take the plain kernel: with GI's pointers null
+            // both compute the same.
+            const bool guideCode = GI.lobes || GI.recs || GI.scatLobes || GI.srecs || GI.rayRec || O.nextRec;
+            if (guideCode)
                 hipLaunchKernelGGL(k_shade<true>, grid, dim3(kRenderBlock), 0, st, S, C->dg, cam, cp, qa, qin,
                                    (const PathHit *)B.hits.ptr, count, acc, O, GI, nullptr, nullptr);
             else
