@@ -890,9 +890,14 @@ __device__ __forceinline__ void delta_light_pairs(const DevScene &S, const Shade
     }
 }
 
-// GUIDED = false compiles the learned guides' code out (plain renders and
-// batches); with true it runs where GI's pointers are set.
-template <bool GUIDED>
+// Which learned-guide code a shading kernel holds: none (plain renders and
+// batches), the environment guide's (its lobes and training records), or
+// every guide's (with the scatter guide, its records and their links).
+// Guide code runs where GI's pointers are set; fewer live values leave the
+// register allocation of the lighter kernels less spilling.
+enum ShadeGuide { kShadeNoGuide = 0, kShadeEnvGuide = 1, kShadeAllGuides = 2 };
+
+template <int GUIDE>
 __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeometry &G, const CameraSetup &cam,
                                   const ChunkParams &cp, const PathRay *__restrict__ q, SegIn qin,
                                   const PathHit *__restrict__ hits, uint32_t n, AccWord *__restrict__ acc,
@@ -930,7 +935,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
     const PathRay r = q[slot];
     const PathHit h = hits[i];
     const uint32_t depth = r.flags & 0xFFu;
-    const uint32_t rin = GUIDED && GI.rayRec && depth > 0u ? GI.rayRec[slot] : kEmpty;
+    const uint32_t rin = GUIDE == kShadeAllGuides && GI.rayRec && depth > 0u ? GI.rayRec[slot] : kEmpty;
     const bool inside = (r.flags & 0x100u) != 0u;
     const uint32_t pixLocal = r.sample / cp.spp;
     const uint32_t pixel = cp.row0 * cam.width + pixLocal;
@@ -1093,7 +1098,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
     // estimator unbiased: the cosine term covers every direction with
     // non-zero contribution.
     if (S.useEnv) {
-        const uint2 *lb = GUIDED && GI.lobes ? GI.lobes + (size_t)i * 8 : nullptr;
+        const uint2 *lb = GUIDE != kShadeNoGuide && GI.lobes ? GI.lobes + (size_t)i * 8 : nullptr;
         const bool lum = cp.envLuminance != 0u;
         V3 L;
         bool fromGuide = false;
@@ -1138,7 +1143,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
         want = want && (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f);
         ShadowRay sr = make_shadow(P + (L * eps), L, r.time, INFINITY, c, r.sample, kShadowAny, 0u);
         sr.srec = rin;
-        if (GUIDED && GI.recs) {
+        if (GUIDE != kShadeNoGuide && GI.recs) {
             // Training sample: direction, weight = luminance(L_env) cos / pdf;
             // connect zeroes the weight when the direction is blocked.
             const uint32_t rseg = block % kSegs;
@@ -1172,7 +1177,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
     // keeps the estimator unbiased: the base distribution covers every
     // direction with non-zero BRDF.
     if (S.pathTrace) {
-        const uint2 *sl = GUIDED && GI.scatLobes ? GI.scatLobes + (size_t)i * 8 : nullptr;
+        const uint2 *sl = GUIDE == kShadeAllGuides && GI.scatLobes ? GI.scatLobes + (size_t)i * 8 : nullptr;
         bool fromGuide = false;
         if (sl) fromGuide = rng.uniform() < GI.alpha;
         V3 Sd;
@@ -1216,7 +1221,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
         bool want = childOk && invPdf > 0.0f && scatter_weight(M, kd, Sd, D, N, invPdf, cw);
         if (want) cw = had(w, cw);
         uint32_t crec = rin;
-        if (GUIDED && GI.srecs) {
+        if (GUIDE == kShadeAllGuides && GI.srecs) {
             // Training sample of the scatter guide: direction and
             // cos / (pdf lum(child throughput)); the luminance that the
             // child's subtree delivers to the sample, divided by the child
@@ -1325,7 +1330,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
 // nDev and cpd, when set, hold the level's ray count and the chunk's
 // parameters in device memory (levels replayed from a graph); the grid is
 // then sized for the largest count.
-template <bool GUIDED>
+template <int GUIDE>
 __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeometry G, CameraSetup cam, ChunkParams cp,
                                                         const PathRay *__restrict__ q, SegIn qin,
                                                         const PathHit *__restrict__ hits, uint32_t n,
@@ -1336,7 +1341,7 @@ __global__ __launch_bounds__(kRenderBlock, 4) void k_shade(DevScene S, DeviceGeo
         n = min(*nDev, n);
         if (blockIdx.x * kRenderBlock >= n) return;
     }
-    shade_body<GUIDED>(S, G, cam, cpd ? *cpd : cp, q, qin, hits, n, acc, O, GI, blockIdx.x);
+    shade_body<GUIDE>(S, G, cam, cpd ? *cpd : cp, q, qin, hits, n, acc, O, GI, blockIdx.x);
 }
 
 // The chunk's parameters and its camera-ray count (the count of level 0),
@@ -1832,7 +1837,7 @@ __global__ __launch_bounds__(BLOCK, WAVES) void kb_extend(const ExtArgs *__restr
 __global__ __launch_bounds__(kRenderBlock, 4) void kb_shade(const ShadeArgs *__restrict__ A, BatchMap M) {
     const uint32_t s = batch_scene(M, blockIdx.x);
     const ShadeArgs &a = A[s];
-    shade_body<false>(a.S, a.G, a.cam, a.cp, a.q, a.qin, a.hits, a.n, a.acc, a.O, a.GI, blockIdx.x - M.start[s]);
+    shade_body<kShadeNoGuide>(a.S, a.G, a.cam, a.cp, a.q, a.qin, a.hits, a.n, a.acc, a.O, a.GI, blockIdx.x - M.start[s]);
 }
 
 __global__ __launch_bounds__(kSegs) void kb_seg_scan(const SegScan *__restrict__ P) {
@@ -2637,7 +2642,7 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             if (e != hipSuccess) return e;
             GuideIO GI{nullptr, nullptr, recCnt, recSegCap, alpha};
             ShadeOut O{qb, (ShadowRay *)B.shadow.ptr, childCnt, shadowCnt, L.nseg, L.childSegCap, L.shadowSegCap};
-            hipLaunchKernelGGL(k_shade<false>, dim3(L.blocks), dim3(kRenderBlock), 0, st, S, C->dg, cam, unused, qa,
+            hipLaunchKernelGGL(k_shade<kShadeNoGuide>, dim3(L.blocks), dim3(kRenderBlock), 0, st, S, C->dg, cam, unused, qa,
                                qin, (const PathHit *)B.hits.ptr, M, acc, O, GI, cnt, cpd);
             SegScan sc;
             memset(&sc, 0, sizeof(sc));
@@ -2878,15 +2883,19 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
                     GI.scatLobes = (const uint2 *)B.scatLobes.ptr;
                 }
             }
-            // Levels of a guided render that use no guide (past GuideMaxDepth,
-            // outside training) take the plain kernel: with GI's pointers null
-            // both compute the same.
-            const bool guideCode = GI.lobes || GI.recs || GI.scatLobes || GI.srecs || GI.rayRec || O.nextRec;
-            if (guideCode)
-                hipLaunchKernelGGL(k_shade<true>, grid, dim3(kRenderBlock), 0, st, S, C->dg, cam, cp, qa, qin,
+            // Each level takes the lightest shading kernel that holds the guide
+            // code it uses (none past GuideMaxDepth and outside training); with
+            // GI's pointers null the kernels compute the same.
+            const bool scatCode = GI.scatLobes || GI.srecs || GI.rayRec || O.nextRec;
+            const bool envCode = GI.lobes || GI.recs;
+            if (scatCode)
+                hipLaunchKernelGGL(k_shade<kShadeAllGuides>, grid, dim3(kRenderBlock), 0, st, S, C->dg, cam, cp, qa, qin,
+                                   (const PathHit *)B.hits.ptr, count, acc, O, GI, nullptr, nullptr);
+            else if (envCode)
+                hipLaunchKernelGGL(k_shade<kShadeEnvGuide>, grid, dim3(kRenderBlock), 0, st, S, C->dg, cam, cp, qa, qin,
                                    (const PathHit *)B.hits.ptr, count, acc, O, GI, nullptr, nullptr);
             else
-                hipLaunchKernelGGL(k_shade<false>, grid, dim3(kRenderBlock), 0, st, S, C->dg, cam, cp, qa, qin,
+                hipLaunchKernelGGL(k_shade<kShadeNoGuide>, grid, dim3(kRenderBlock), 0, st, S, C->dg, cam, cp, qa, qin,
                                    (const PathHit *)B.hits.ptr, count, acc, O, GI, nullptr, nullptr);
             AZ_HIPR(C, hipGetLastError());
             SegScan sc;
