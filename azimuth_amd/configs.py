@@ -222,11 +222,13 @@ def procedural_envmap(width=256, height=128, sun_peak=5000.0, sun_sigma_deg=0.5,
 
 
 def feature_scene(width=48, height=48, spp=4, max_depth=5, seed=3, env=True, importance=True, aperture=0.0,
-                  focus=0.0, background_texture=False, image_bump=False):
+                  focus=0.0, background_texture=False, image_bump=False, extra_lights=0):
     """Every light type and BRDF model, textures, Perlin bump maps, glossy
     mirrors, dielectrics, motion blur and (env) an environment map; with
     background_texture (and env False) camera misses read a gradient
-    texture, with image_bump a back wall carries a 2D image bump map."""
+    texture, with image_bump a back wall carries a 2D image bump map;
+    extra_lights more point, spot, directional and area lights (seeded)
+    after the four of the scene."""
     from .scene import directional_light, point_light
     g = np.random.default_rng(seed)
     b = SceneBuilder(max_depth=max_depth, path_trace=True, importance_sampling=importance,
@@ -272,6 +274,23 @@ def feature_scene(width=48, height=48, spp=4, max_depth=5, seed=3, env=True, imp
     b.add_light(area_light((-1, 3.5, -1), (4, 4, 4), (0.5, 0, 0), (0, 0, 0.5), (0, -1, 0)))
     b.add_light(directional_light((-0.3, -1, -0.2), (0.4, 0.4, 0.4)))
     b.add_light(spot_light((0, 4, 0), (20, 20, 20), (0, -1, 0), 60.0, 40.0))
+    for k in range(extra_lights):
+        pos = (float(g.uniform(-3, 3)), float(g.uniform(1.5, 4.5)), float(g.uniform(-3, 3)))
+        col = tuple(float(x) for x in g.uniform(0.2, 1.5, 3))
+        kind = k % 5
+        if kind in (0, 3):
+            b.add_light(point_light(pos, col))
+        elif kind == 1:
+            tgt = np.array([g.uniform(-2, 2), 0.0, g.uniform(-2, 2)])
+            d = tgt - np.array(pos)
+            b.add_light(spot_light(pos, tuple(4 * c for c in col), tuple(d / np.linalg.norm(d)),
+                                   float(g.uniform(30, 90)), float(g.uniform(10, 30))))
+        elif kind == 2:
+            d = g.normal(size=3)
+            d[1] = -abs(d[1]) - 0.2
+            b.add_light(directional_light(tuple(d / np.linalg.norm(d)), tuple(0.1 * c for c in col)))
+        else:
+            b.add_light(area_light(pos, col, (0.3, 0, 0), (0, 0, 0.3), (0, -1, 0)))
     b.add_sphere((-2.5, 2.5, -2.5), 0.25, material=lamp, emitter=True, radiance=(8, 7, 6))
     lv = np.array([(2.5, 3.0, -2.5), (3.0, 3.0, -2.5), (3.0, 3.0, -2.0), (2.5, 3.0, -2.0)], np.float32)
     b.add_mesh(lv, np.array([[0, 1, 2], [0, 2, 3]], np.uint32), material=lamp, emitter=True, radiance=(6, 6, 8))

@@ -79,6 +79,19 @@ def test_feature_scene_gaussian_rows(gpu):
     compare(got, want, "features rows 10-27 gaussian")
 
 
+def test_many_lights(gpu):
+    """44 point, spot, directional and area lights and two emitters.  Point,
+    spot and directional lights 0-31 go through the shading kernel's pass
+    over marked (vertex, light) pairs, on floor-heavy waves more pairs than
+    one LDS window holds; the rest through the per-lane loop, area and entity
+    lights drawing their random numbers in light order between them."""
+    sc, cam = configs.feature_scene(40, 40, spp=4, max_depth=3, extra_lights=40)
+    gpu.upload_scene(sc)
+    got = gpu.render(cam, seed=13)
+    want = oracle.render(sc, cam, seed=13)
+    compare(got, want, "features with 44 lights 40x40 4spp")
+
+
 def test_backface_culling_render(gpu):
     sc, cam = configs.cornell_scene(32, 32, spp=4, max_depth=4)
     gpu.upload_scene(sc)
