@@ -428,8 +428,10 @@ def equal_time_mse(ctx, ref_spp=4096, spp=64, seeds=(1, 12)):
     against the reference's cosine sampling, trained on the first quarter of
     its spp) at the largest square spp whose render time stays within the
     plain render's; also the environment luminance distribution (the strongest
-    sampler that does not learn) at equal time, and the guide over it (up to
-    path depth 3).  Besides the equal-time ratios, MSE x seconds ratios.  Wall-
+    sampler that does not learn) at equal time, and the guide over it, used up
+    to path depth 1, 2 or 3 (deeper vertices take the luminance distribution
+    alone; each depth at its own equal-time spp, the best reported with all
+    three).  Besides the equal-time ratios, MSE x seconds ratios.  Wall-
     clock times around each render call, guide training included; each MSE is
     the mean over the given seeds.
 
@@ -488,24 +490,32 @@ def equal_time_mse(ctx, ref_spp=4096, spp=64, seeds=(1, 12)):
                          "seconds": round(t_ref, 1), "noise_mse": ref_noise},
            "seeds": list(seeds), "guide_target": "environment NEE", "plain": mse(spp)}
     t_plain = res["plain"]["seconds"]
-    # The guide over the luminance distribution is used up to path depth 3
-    # (deeper vertices take the luminance distribution alone), which pays on
-    # this scene; over the cosine distribution it is used at every depth.
-    for name, guide, env, md in (("guided", True, N.AZ_ENV_COSINE, 0), ("luminance", False, N.AZ_ENV_LUMINANCE, 0),
-                                 ("guided_over_luminance", True, N.AZ_ENV_LUMINANCE, 3)):
+    # Over the cosine distribution the guide is used at every depth (limited
+    # to depth 2 its MSE is 4.6x higher); over the luminance distribution it
+    # pays only at the first vertices, so depths 1-3 are each measured.
+    for name, guide, env, md in (("guided", True, N.AZ_ENV_COSINE, 0), ("luminance", False, N.AZ_ENV_LUMINANCE, 0)):
         res[name] = mse(equal_spp(guide, env, t_plain, md), guide, env, md)
         if guide:
             res[name]["guide_max_depth"] = md
         log(f"{name}: {res[name]}")
+    by_depth = {}
+    for md in (1, 2, 3):
+        by_depth[md] = mse(equal_spp(True, N.AZ_ENV_LUMINANCE, t_plain, md), True, N.AZ_ENV_LUMINANCE, md)
+        by_depth[md]["guide_max_depth"] = md
+        log(f"guided_over_luminance to depth {md}: {by_depth[md]}")
+    res["guided_over_luminance_by_depth"] = {str(md): r for md, r in by_depth.items()}
+    res["guided_over_luminance"] = min(by_depth.values(), key=lambda r: r["mse_net"])
     ratio = lambda a, b, k="mse_net": round(res[a][k] / max(res[b][k], 1e-30), 3)
     res["mse_ratio_plain_over_guided"] = round(res["plain"]["mse"] / res["guided"]["mse"], 3)
     res["net_mse_ratio_plain_over_guided"] = ratio("plain", "guided")
     res["net_mse_ratio_luminance_over_guided_over_luminance"] = ratio("luminance", "guided_over_luminance")
     # MSE x seconds (unbiased estimators: MSE falls as 1 / time), free of the
-    # square-spp steps of the equal-time renders.
+    # square-spp steps of the equal-time renders; the best depth's.
     res["time_normalized_ratio_plain_over_guided"] = ratio("plain", "guided", "mse_net_x_seconds")
-    res["time_normalized_ratio_luminance_over_guided_over_luminance"] = ratio(
-        "luminance", "guided_over_luminance", "mse_net_x_seconds")
+    best_t = min(by_depth.values(), key=lambda r: r["mse_net_x_seconds"])
+    res["time_normalized_ratio_luminance_over_guided_over_luminance"] = round(
+        res["luminance"]["mse_net_x_seconds"] / max(best_t["mse_net_x_seconds"], 1e-30), 3)
+    res["time_normalized_guided_over_luminance_depth"] = best_t["guide_max_depth"]
     return res
 
 
