@@ -63,7 +63,7 @@ struct DevPrimAttr {                 // per triangle, in primitive-id order
 struct DevItemShade {                // per top-level item
     uint32_t material, texture, isEmitter, kind;
     float radiance[3];
-    float pad;
+    uint32_t shadeClass;             // sort class of its hits in shading (emitter, material flags, BRDF model)
 };
 
 struct DevScene {

@@ -686,16 +686,18 @@ __device__ inline V3 bump(const DevScene &S, const DevTexture &T, V3 N, V3 P, fl
 
 // Shading class of a hit for the in-block sort of k_shade: misses, emitters,
 // then material flags (mirror, dielectric) and BRDF model, which select the
-// branches the vertex code takes.
+// branches the vertex code takes; an item's class is set at upload
+// (item_shade_class), so the sort key costs one load after the hit's.
 constexpr uint32_t kShadeClasses = 32;
 
-__device__ inline uint32_t shade_class(const DevScene &S, const PathHit &h) {
-    if (h.prim == kEmpty) return 0u;
-    const DevItemShade &sh = S.itemShade[h.item];
-    if (sh.isEmitter) return 1u;
-    const DevMaterial &M = S.materials[sh.material];
-    const uint32_t bt = (uint32_t)min(max(M.brdfType, 0), 6);
+inline uint32_t item_shade_class(bool emitter, const DevMaterial &M) {
+    if (emitter) return 1u;
+    const uint32_t bt = (uint32_t)std::min(std::max(M.brdfType, 0), 6);
     return 2u + (M.flags & 3u) + 4u * bt;
+}
+
+__device__ inline uint32_t shade_class(const DevScene &S, const PathHit &h) {
+    return h.prim == kEmpty ? 0u : S.itemShade[h.item].shadeClass;
 }
 
 // Hit point, shading normal and uv of a closest hit, as RayEntityIntersection
@@ -2288,6 +2290,7 @@ az_status az_upload_shading(az_gpu_ctx *C, const scene *S, const UploadInfo &inf
         d.material = e->MaterialIndex < S->MaterialCount ? e->MaterialIndex : 0u;
         d.texture = e->TextureIndex <= S->TextureCount ? e->TextureIndex : 0u;
         d.isEmitter = e->IsEmitter ? 1u : 0u;
+        d.shadeClass = item_shade_class(d.isEmitter != 0u, mats[d.material]);
         d.kind = e->Type == EntityType_Sphere ? kItemSphere : kItemInstance;
         d.radiance[0] = e->Radiance.x;
         d.radiance[1] = e->Radiance.y;
