@@ -455,6 +455,7 @@ struct GuideIO {
     // the waves appended them in.
     uint64_t *recKeys;
     uint64_t *srecKeys;
+    uint32_t keyNodeBits;            // the node's bits in a key: sample << keyNodeBits | node
 };
 
 // Guide query of a hit: position in the scene box, normal, roughness.
@@ -1164,7 +1165,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
                 g.wgt = guide_lum(I) * fmaxf(ca, 0.0f) * invPdf;
                 g.pad = 0.0f;
                 GI.recs[idx] = g;
-                GI.recKeys[idx] = ((uint64_t)r.sample << 32) | r.node;
+                GI.recKeys[idx] = ((uint64_t)r.sample << GI.keyNodeBits) | r.node;
                 sr.rec = idx;
             }
         }
@@ -1246,7 +1247,7 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
                 g.wgt = lw > 0.0f ? fmaxf(dot(Sd, N), 0.0f) * invPdf / lw : 0.0f;
                 g.pad = 0.0f;
                 GI.srecs[idx] = g;
-                GI.srecKeys[idx] = ((uint64_t)r.sample << 32) | r.node;
+                GI.srecKeys[idx] = ((uint64_t)r.sample << GI.keyNodeBits) | r.node;
                 GI.links[idx] = ScatterLink{0ull, rin, depth};
                 crec = idx;
             }
@@ -2440,6 +2441,23 @@ struct LevelCaps {
     size_t cap, shadowCap;
 };
 
+// Bits of v (0 for 0).
+static u32 bit_width(u32 v) {
+    u32 b = 0;
+    while (b < 32u && (v >> b) != 0u) ++b;
+    return b;
+}
+
+// Bits of the largest random-stream node of a vertex at depth <= maxDepth:
+// a camera ray's node is 1 (child_node(0, 0)) and a child's 5n + k + 1
+// (k < 4), so at most 2 * 5^d - 1 at depth d; 32 once that passes the
+// 32-bit ids (which then wrap).
+static u32 node_key_bits(u32 maxDepth) {
+    uint64_t m = 1;
+    for (u32 d = 0; d < maxDepth && m <= 0xFFFFFFFFull; ++d) m = 5u * m + 4u;
+    return m > 0xFFFFFFFFull ? 32u : std::max<u32>(1u, bit_width((u32)m));
+}
+
 static LevelCaps level_caps(u32 count, u32 maxChildren, u32 shadowPerRay) {
     LevelCaps L;
     L.blocks = (count + kRenderBlock - 1) / kRenderBlock;
@@ -2521,6 +2539,9 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
     // takes about kGuideSteps steps whatever the image size.
     const u32 chunkSamples = std::max<u32>(1u, C->renderTuning.chunkSamples);
     const u32 chunkSpp0 = std::max<u32>(1u, std::min<u32>(sppTotal, chunkSamples / std::max<u32>(1u, nPix)));
+    // Training records are drawn from in the order of their path vertex,
+    // key sample << keyNodeBits | node.
+    const u32 keyNodeBits = node_key_bits(S.maxDepth);
     const u32 trainWant = o.GuideChunkSpp ? o.GuideChunkSpp : std::max<u32>(1u, (2u << 20) / std::max<u32>(1u, nPix));
     const u32 trainChunkSpp = std::max<u32>(1u, std::min<u32>(std::min(chunkSpp0, trainWant), trainSpp / 8u));
     const u32 trainChunks = trainSpp ? (std::min(trainSpp, sppTotal) + trainChunkSpp - 1) / trainChunkSpp : 1u;
@@ -2838,6 +2859,7 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             GuideIO GI{nullptr, nullptr, recCnt, recSegCap, alpha};
             ShadeOut O{qb, (ShadowRay *)B.shadow.ptr, childCnt, shadowCnt, nseg, childSegCap, shadowSegCap};
             const bool guideLevel = o.GuideMaxDepth == 0 || depth <= o.GuideMaxDepth;
+            GI.keyNodeBits = keyNodeBits;
             if (envGuide && training) {
                 GI.recs = (GuideSample *)B.recs.ptr;
                 GI.recKeys = (uint64_t *)B.recKeys.ptr;
@@ -3018,9 +3040,12 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
                                    (const uint64_t *)(g == 0 ? B.recKeys.ptr : B.srecKeys.ptr), rin, R[g], kin, vin);
                 AZ_HIPR(C, hipGetLastError());
                 size_t tmpBytes = 0;
-                AZ_HIPR(C, rocprim::radix_sort_pairs(nullptr, tmpBytes, kin, kout, vin, order, R[g], 0, 64, st));
+                // Only the key's bits in use are sorted on.
+                const u32 keyBits = std::min<u32>(64u, keyNodeBits + bit_width(n - 1u));
+                AZ_HIPR(C, rocprim::radix_sort_pairs(nullptr, tmpBytes, kin, kout, vin, order, R[g], 0, keyBits, st));
                 AZ_HIPR(C, B.recSortTmp.alloc(std::max<size_t>(tmpBytes, 16)));
-                AZ_HIPR(C, rocprim::radix_sort_pairs(B.recSortTmp.ptr, tmpBytes, kin, kout, vin, order, R[g], 0, 64, st));
+                AZ_HIPR(C, rocprim::radix_sort_pairs(B.recSortTmp.ptr, tmpBytes, kin, kout, vin, order, R[g], 0, keyBits,
+                                                     st));
                 hipLaunchKernelGGL(k_rec_prefix_blocks, dim3(pb), dim3(kRenderBlock), 0, st, recs,
                                    (const uint32_t *)order, R[g], cum, sums);
                 hipLaunchKernelGGL(k_rec_prefix_sums, dim3(1), dim3(kRenderBlock), 0, st, sums, pb);
