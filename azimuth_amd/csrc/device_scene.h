@@ -27,11 +27,12 @@ constexpr uint32_t kMaxLeafItems = 16;
 constexpr uint32_t kMaxLeafFirst = (1u << 27) - 1;
 // Traversal stack entries per lane kept in scratch below the LDS ring.
 constexpr int kSpill = 80;
-// Rays a traversal lane walks per launch on average: a wave owns 64 x this
-// many consecutive rays and refills its lanes from them (launches too small
-// to fill the device twice take fewer).  8 against 4: fewer waves, each
-// staging its queue's segment offsets once for twice the rays; the 1080p
-// interior render 498 -> 488 ms, 16 gave 490.
+// Rays a lane of the renderer's traversal walks per launch on average: a
+// wave owns 64 x this many consecutive rays of a level's queue and refills
+// its lanes from them (launches too small to fill the device twice take
+// fewer).  8 against 4: fewer waves, each staging its queue's segment
+// offsets once for twice the rays; the 1080p interior render 498 -> 488 ms,
+// 16 gave 490.  Batch queries keep 4 (kTraceRaysPerLane, kernels.h).
 constexpr uint32_t kRaysPerLane = 8;
 
 inline AZ_HD bool ref_is_leaf(uint32_t r) { return (r & kLeafBit) != 0u; }

@@ -37,14 +37,18 @@ struct RenderTuning {
     int traceLoop = 1;   // while-while
 };
 
-// Rays per wave of a traversal launch of n rays: 64 x kRaysPerLane (each
-// lane walks several, refilled from the wave's range) when n fills at least
-// two rounds of the device's resident waves, else halved down to 64 (one per
+// Rays per wave of a traversal launch of n rays: 64 x perLane (each lane
+// walks several, refilled from the wave's range) when n fills at least two
+// rounds of the device's resident waves, else halved down to 64 (one per
 // lane), so that small launches still reach every SIMD.  `waves` is the
 // register target in waves per SIMD (0: the compiler's choice, taken as 5).
-__host__ __device__ inline uint32_t rays_per_wave(uint32_t n, int cuCount, int waves) {
+// The renderer's levels take kRaysPerLane, batch queries kTraceRaysPerLane.
+constexpr uint32_t kTraceRaysPerLane = 4;
+
+__host__ __device__ inline uint32_t rays_per_wave(uint32_t n, int cuCount, int waves,
+                                                  uint32_t perLane = kRaysPerLane) {
     const uint64_t resident = (uint64_t)cuCount * 4u * (uint64_t)(waves > 0 ? waves : 5);
-    uint32_t per = 64u * kRaysPerLane;
+    uint32_t per = 64u * perLane;
     while (per > 64u && (uint64_t)n < 2u * resident * per) per >>= 1;
     return per;
 }

@@ -73,7 +73,7 @@ __global__ __launch_bounds__(BLOCK, (WAVES > 0 ? WAVES : 1)) void k_trace(Device
 template <bool ANY, bool COUNT, int RING, int BLOCK, int WAVES, bool WW>
 static hipError_t launch_variant(const DeviceGeometry &G, const az_ray *rays, uint32_t n, az_hit *hits, float eps,
                                  int cull, unsigned long long *counters, int cuCount, hipStream_t s) {
-    const uint32_t per = rays_per_wave(n, cuCount, WAVES);
+    const uint32_t per = rays_per_wave(n, cuCount, WAVES, kTraceRaysPerLane);
     const uint32_t perBlock = (BLOCK / 64) * per;
     dim3 grid((n + perBlock - 1) / perBlock);
     hipLaunchKernelGGL((k_trace<ANY, COUNT, RING, BLOCK, WAVES, WW>), grid, dim3(BLOCK), 0, s, G, rays, n, hits, eps,
