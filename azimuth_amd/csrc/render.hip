@@ -675,11 +675,29 @@ __device__ inline V3 bump(const DevScene &S, const DevTexture &T, V3 N, V3 P, fl
         V3 dPv = Pv + Dv * N;
         return normalize(cross(dPv, dPu));
     }
+    // sample_perlin at P and at P + e along each axis; the offset points
+    // mostly lie in P's lattice cell, whose corner gradients (a chain of
+    // dependent permutation loads) are then fetched once.
     const float e = 0.001f;
-    float D = sample_perlin(S, T, P);
-    float Dx = sample_perlin(S, T, v3(P.x + e, P.y, P.z));
-    float Dy = sample_perlin(S, T, v3(P.x, P.y + e, P.z));
-    float Dz = sample_perlin(S, T, v3(P.x, P.y, P.z + e));
+    const V3 Q = P * T.scaling;
+    const int32_t I = (int32_t)floorf(Q.x), J = (int32_t)floorf(Q.y), K = (int32_t)floorf(Q.z);
+    V3 gc[8];
+    perlin_cell(S, T, I, J, K, gc);
+    float D = perlin_value(T, gc, I, J, K, Q);
+    float Dd[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const V3 Qa = v3(a == 0 ? P.x + e : P.x, a == 1 ? P.y + e : P.y, a == 2 ? P.z + e : P.z) * T.scaling;
+        const int32_t Ia = (int32_t)floorf(Qa.x), Ja = (int32_t)floorf(Qa.y), Ka = (int32_t)floorf(Qa.z);
+        if (Ia == I && Ja == J && Ka == K) {
+            Dd[a] = perlin_value(T, gc, I, J, K, Qa);
+        } else {
+            V3 ga[8];
+            perlin_cell(S, T, Ia, Ja, Ka, ga);
+            Dd[a] = perlin_value(T, ga, Ia, Ja, Ka, Qa);
+        }
+    }
+    const float Dx = Dd[0], Dy = Dd[1], Dz = Dd[2];
     V3 g = v3((Dx - D) / e, (Dy - D) / e, (Dz - D) / e);
     N = N - g * T.bumpMul;
     return normalize(N);

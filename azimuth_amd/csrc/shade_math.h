@@ -114,22 +114,40 @@ __device__ inline float perlin_weight(float v) {
     return 1.0f - ((((6.0f * v) - 15.0f) * v + 10.0f) * v * v * v);
 }
 
-__device__ inline float sample_perlin(const DevScene &S, const DevTexture &T, V3 P) {
-    float r = 0.0f;
-    P = P * T.scaling;
-    int32_t I = (int32_t)floorf(P.x), J = (int32_t)floorf(P.y), K = (int32_t)floorf(P.z);
+// The lattice gradients of the 8 corners of cell (I, J, K), corner
+// (x, y, z) at 4 z + 2 y + x: PerlinHash's permutation chain (texture.h:134-
+// 160, with a proper mod 256).
+__device__ inline void perlin_cell(const DevScene &S, const DevTexture &T, int32_t I, int32_t J, int32_t K, V3 g[8]) {
     const uint32_t *perm = S.perms + T.permOffset;
     for (int32_t z = 0; z < 2; ++z)
         for (int32_t y = 0; y < 2; ++y)
             for (int32_t x = 0; x < 2; ++x) {
                 uint32_t h = perm[mod256(I + x + (int32_t)perm[mod256(J + y + (int32_t)perm[mod256(K + z)])])] & 255u;
-                V3 e = v3(S.perlinGrad + 3 * h);
+                g[4 * z + 2 * y + x] = v3(S.perlinGrad + 3 * h);
+            }
+}
+
+// The noise at the scaled point P of cell (I, J, K) from its corners'
+// gradients, then the texture's appearance (texture.h:161-200).
+__device__ inline float perlin_value(const DevTexture &T, const V3 g[8], int32_t I, int32_t J, int32_t K, V3 P) {
+    float r = 0.0f;
+    for (int32_t z = 0; z < 2; ++z)
+        for (int32_t y = 0; y < 2; ++y)
+            for (int32_t x = 0; x < 2; ++x) {
                 V3 v = v3(P.x - (float)(I + x), P.y - (float)(J + y), P.z - (float)(K + z));
-                r += dot(e, v) * perlin_weight(v.x) * perlin_weight(v.y) * perlin_weight(v.z);
+                r += dot(g[4 * z + 2 * y + x], v) * perlin_weight(v.x) * perlin_weight(v.y) * perlin_weight(v.z);
             }
     if (T.perlinAppearance == 0) r = (r + 1.0f) / 2.0f;
     else r = fabsf(r);
     return r;
+}
+
+__device__ inline float sample_perlin(const DevScene &S, const DevTexture &T, V3 P) {
+    P = P * T.scaling;
+    int32_t I = (int32_t)floorf(P.x), J = (int32_t)floorf(P.y), K = (int32_t)floorf(P.z);
+    V3 g[8];
+    perlin_cell(S, T, I, J, K, g);
+    return perlin_value(T, g, I, J, K, P);
 }
 
 __device__ inline V3 fetch2d(const DevScene &S, const DevTexture &T, uint32_t x, uint32_t y) {
