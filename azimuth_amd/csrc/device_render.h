@@ -76,7 +76,7 @@ struct DevScene {
     const DevPrimAttr *prims;
     const DevItemShade *itemShade;
     const float *perlinGrad;         // 256 x 3
-    const uint64_t *pcgTable;        // 64 x (multiplier, increment factor)
+    const uint64_t *pcgTable;        // 8 bytes x 256 values x (multiplier, increment factor)
     DevTexture env;                  // environment map (type 2D) when useEnv
     DevTexture background;
     uint32_t lightCount;

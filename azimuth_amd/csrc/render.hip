@@ -1924,15 +1924,31 @@ namespace {
     } while (0)
 
 // PCG32 jump table: entry k holds mult^(2^k) and sum_{i<2^k} mult^i.
+// PCG32 jump-ahead table: entry 256 k + v advances a stream by v << 8k steps
+// as (multiplier, increment factor): state' = mult state + inc factor.  Jumps
+// of one generator commute, so make_rng composes one entry per non-zero byte
+// of its distance (the same state as stepping one by one).
 std::vector<uint64_t> pcgTable() {
-    std::vector<uint64_t> t(128);
+    uint64_t bm[64], bg[64];                 // jumps by 2^b steps
     uint64_t m = kPcgMult, g = 1u;
-    for (int k = 0; k < 64; ++k) {
-        t[2 * k] = m;
-        t[2 * k + 1] = g;
+    for (int b = 0; b < 64; ++b) {
+        bm[b] = m;
+        bg[b] = g;
         g = (m + 1u) * g;
         m = m * m;
     }
+    std::vector<uint64_t> t(2 * 8 * 256);
+    for (int k = 0; k < 8; ++k)
+        for (int v = 0; v < 256; ++v) {
+            uint64_t M = 1u, G = 0u;
+            for (int b = 0; b < 8; ++b)
+                if ((v >> b) & 1) {
+                    G = G * bm[8 * k + b] + bg[8 * k + b];
+                    M = M * bm[8 * k + b];
+                }
+            t[2 * (256 * k + v)] = M;
+            t[2 * (256 * k + v) + 1] = G;
+        }
     return t;
 }
 

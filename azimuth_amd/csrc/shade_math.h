@@ -91,14 +91,17 @@ __device__ inline Rng make_rng(const uint64_t *__restrict__ table, uint64_t seed
     r.next();
     r.state += seed;
     r.next();
-    uint64_t delta = ((uint64_t)sample << 38) + ((uint64_t)node << 6);
+    const uint64_t delta = ((uint64_t)sample << 38) + ((uint64_t)node << 6);
     uint64_t accMult = 1u, accPlus = 0u;
-    while (delta) {
-        int k = __builtin_ctzll(delta);
-        uint64_t m = table[2 * k], g = table[2 * k + 1];
+    // One table entry per non-zero byte of the distance (entry 256 k + v
+    // advances by v << 8k steps).
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t v = (uint32_t)(delta >> (8 * k)) & 255u;
+        if (v == 0u) continue;
+        const uint64_t m = table[2 * (256 * k + v)], g = table[2 * (256 * k + v) + 1];
         accMult *= m;
         accPlus = accPlus * m + r.inc * g;
-        delta &= delta - 1u;
     }
     r.state = accMult * r.state + accPlus;
     return r;
