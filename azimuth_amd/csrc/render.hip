@@ -96,7 +96,8 @@ typedef unsigned long long AccWord;
 constexpr double kAccUnit = 4294967296.0;   // units per 1.0
 
 __device__ inline AccWord acc_fixed(float v) {
-    const double x = fmin(fmax((double)v * kAccUnit, -0x1p62), 0x1p62);   // NaN: fmin/fmax return the other operand
+    if (!(v == v)) return 0ull;                                           // NaN adds nothing
+    const double x = fmin(fmax((double)v * kAccUnit, -0x1p62), 0x1p62);   // +-inf saturate
     return (AccWord)(long long)rint(x);
 }
 
