@@ -2493,6 +2493,10 @@ static u32 node_key_bits(u32 maxDepth) {
     return m > 0xFFFFFFFFull ? 32u : std::max<u32>(1u, bit_width((u32)m));
 }
 
+// Samples per guide training batch: the size of the render's batch buffer
+// and the most a weighted draw gathers into it.
+constexpr u32 kGuideBatch = 65536;
+
 static LevelCaps level_caps(u32 count, u32 maxChildren, u32 shadowPerRay) {
     LevelCaps L;
     L.blocks = (count + kRenderBlock - 1) / kRenderBlock;
@@ -2612,7 +2616,7 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
         AZ_HIPR(C, B.srecKeys.alloc((size_t)recCap * sizeof(uint64_t)));
         AZ_HIPR(C, B.links.alloc((size_t)recCap * sizeof(ScatterLink)));
     }
-    if (guided) AZ_HIPR(C, B.batch.alloc((size_t)65536 * sizeof(GuideSample)));
+    if (guided) AZ_HIPR(C, B.batch.alloc((size_t)kGuideBatch * sizeof(GuideSample)));
     // Samples per pixel per chunk: the tuned chunk size (16M camera samples
     // by default); while the guide trains, at least 8 chunks so that it is
     // updated often.
@@ -3063,7 +3067,7 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
                                        (const ScatterLink *)B.links.ptr, rin, R[g]);
                     AZ_HIPR(C, hipGetLastError());
                 }
-                u32 bn = std::min<u32>(65536u, R[g]) & ~15u;
+                u32 bn = std::min<u32>(kGuideBatch, R[g]) & ~15u;   // within the batch buffer
                 const u32 pb = (R[g] + kRenderBlock - 1) / kRenderBlock;
                 AZ_HIPR(C, B.recCum.alloc(((size_t)R[g] + pb) * sizeof(float)));
                 float *cum = (float *)B.recCum.ptr, *sums = cum + R[g];
