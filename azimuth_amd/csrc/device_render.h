@@ -87,6 +87,7 @@ struct DevScene {
     float epsIntersection, epsShadow;
     uint32_t triTotal;
     uint32_t maxChildren;            // most path rays one shading point can emit (1..4)
+    uint32_t entityLights;           // entity lights (their shadow rays need a closest hit)
     float boxMin[3], boxInv[3];      // scene box: guide positions are (P - min) * inv
     // Environment luminance distribution over the (u, v) cells of the map:
     // row CDF (envH + 1 floats) then one column CDF per row (envW + 1 each).

@@ -318,7 +318,9 @@ struct TraceCounters {
 // unclaimed one at the end of the same step (one ballot, no atomics), so a
 // wave keeps its lanes busy until the range runs out instead of waiting for
 // its longest ray.  MODE 0: closest hit with eps < t (tMax ignored); MODE 1:
-// any hit with eps < t < tMax (world t); MODE 2: per ray, as the pool says.
+// any hit with eps < t < tMax (world t); MODE 2: per ray, as the pool says;
+// MODE 3: any hit, children in MODE 2's order for occlusion queries (a
+// renderer's shadow rays when none needs a closest hit).
 // Pool provides
 //   bool fetch(uint32_t id, Ray3 &r, float &tMax)   -> true: any-hit ray
 //   void finish(uint32_t id, bool found, const HitRec &h)
@@ -329,13 +331,14 @@ template <int MODE, bool COUNT, int RING, int BLOCK, bool WW, class Pool>
 __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t begin, uint32_t end, float eps,
                                   bool cull, uint32_t *ldsLane, TraceCounters &cnt) {
     constexpr int kOrder = MODE == 0 ? kNearFirst : MODE == 1 ? kOverlapFirst : kFarExitFirst;
+    constexpr bool kAnyOnly = MODE == 1 || MODE == 3;
     RingStack<RING, BLOCK> stack;
     stack.base = ldsLane;
     uint32_t next = begin;                          // wave-uniform: first unclaimed ray
     uint32_t id = 0;
     bool have = false;
     Ray3 r{};
-    bool any = MODE == 1;
+    bool any = kAnyOnly;
     // An instance whose hit the reference's own box test rejected: the walk
     // of the ray is redone once without it.  Kept in the lane's LDS word
     // after its ring (read only at top-level instance leaves), not in a
@@ -368,7 +371,7 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
                 have = true;
                 float tMax = kFltMax;
                 const bool a = pool.fetch(id, r, tMax);
-                any = MODE == 2 ? a : MODE == 1;
+                any = MODE == 2 ? a : kAnyOnly;
                 dirLen = len3(r.dx, r.dy, r.dz);
                 *excl = kEmpty;
                 bestW = any ? tMax : kFltMax;

@@ -315,7 +315,8 @@ struct ConnectPool {
     }
 };
 
-template <int BLOCK, bool WW, bool COUNT = false>
+// ANYONLY: every shadow ray is an occlusion query (no entity lights).
+template <int BLOCK, bool WW, bool COUNT = false, bool ANYONLY = false>
 __device__ __forceinline__ void connect_body(const DeviceGeometry &G, const ShadowRay *__restrict__ sq, SegIn sin, uint32_t n,
                                     AccWord *__restrict__ acc, GuideSample *__restrict__ recs,
                                     ScatterLink *__restrict__ links, uint32_t perWave, uint32_t block,
@@ -327,11 +328,11 @@ __device__ __forceinline__ void connect_body(const DeviceGeometry &G, const Shad
     const uint32_t end = min(begin + perWave, n);
     ConnectPool pool{sq, off, sin, &G, acc, recs, links};
     TraceCounters c{0u, 0u, 0u};
-    trace_pool<2, COUNT, kRenderRing, BLOCK, WW>(G, pool, begin, end, 0.0f, false, smem + threadIdx.x, c);
+    trace_pool<ANYONLY ? 3 : 2, COUNT, kRenderRing, BLOCK, WW>(G, pool, begin, end, 0.0f, false, smem + threadIdx.x, c);
     if (COUNT) add_render_counters(counters, c);
 }
 
-template <int BLOCK, int WAVES, bool WW>
+template <int BLOCK, int WAVES, bool WW, bool ANYONLY>
 __global__ __launch_bounds__(BLOCK, WAVES) void k_connect(DeviceGeometry G, const ShadowRay *__restrict__ sq, SegIn sin,
                                                          uint32_t n, AccWord *__restrict__ acc,
                                                          GuideSample *__restrict__ recs,
@@ -342,7 +343,7 @@ __global__ __launch_bounds__(BLOCK, WAVES) void k_connect(DeviceGeometry G, cons
         perWave = device_rays_per_wave<BLOCK, WAVES>(n, cuCount);
         if ((uint64_t)blockIdx.x * (BLOCK / 64) * perWave >= n) return;
     }
-    connect_body<BLOCK, WW>(G, sq, sin, n, acc, recs, links, perWave, blockIdx.x);
+    connect_body<BLOCK, WW, false, ANYONLY>(G, sq, sin, n, acc, recs, links, perWave, blockIdx.x);
 }
 
 template <int BLOCK, int WAVES, bool WW>
@@ -384,7 +385,11 @@ static hipError_t launch_extend(const RenderTuning &T, int cuCount, const Device
     return hipErrorInvalidValue;
 }
 
-static hipError_t launch_connect(const RenderTuning &T, int cuCount, const DeviceGeometry &G, const ShadowRay *sq,
+// anyOnly: the scene has no entity lights, so every shadow ray is an
+// occlusion query (k_connect's any-hit-only walk; the interior render 486 ->
+// 483 ms).
+static hipError_t launch_connect(bool anyOnly, const RenderTuning &T, int cuCount, const DeviceGeometry &G,
+                                 const ShadowRay *sq,
                                  SegIn sin, uint32_t n, AccWord *acc, GuideSample *recs, ScatterLink *links,
                                  hipStream_t st, const uint32_t *nDev = nullptr, unsigned long long *counters = nullptr) {
     const uint32_t per = nDev ? 64u : rays_per_wave(n, cuCount, T.traceWaves);
@@ -401,8 +406,12 @@ static hipError_t launch_connect(const RenderTuning &T, int cuCount, const Devic
 #define AZ_CASE(B, W, F)                                                                                  \
     if (T.traceBlock == B && T.traceWaves == W && T.traceLoop == F) {                                   \
         const uint32_t perBlock = (B / 64) * per;                                                        \
-        hipLaunchKernelGGL((k_connect<B, W, F != 0>), dim3((n + perBlock - 1) / perBlock), dim3(B), 0, st, G, sq,   \
-                           sin, n, acc, recs, links, per, nDev, cuCount);                                \
+        if (anyOnly)                                                                                     \
+            hipLaunchKernelGGL((k_connect<B, W, F != 0, true>), dim3((n + perBlock - 1) / perBlock), dim3(B), 0, st, \
+                               G, sq, sin, n, acc, recs, links, per, nDev, cuCount);                     \
+        else                                                                                             \
+            hipLaunchKernelGGL((k_connect<B, W, F != 0, false>), dim3((n + perBlock - 1) / perBlock), dim3(B), 0,    \
+                               st, G, sq, sin, n, acc, recs, links, per, nDev, cuCount);                 \
         return hipGetLastError();                                                                        \
     }
     AZ_RENDER_TRACE_VARIANTS(AZ_CASE)
@@ -2211,6 +2220,16 @@ az_status az_upload_shading(az_gpu_ctx *C, const scene *S, const UploadInfo &inf
         matChildren = std::max<u32>(matChildren, ((d.flags & kMatMirror) ? 1u : 0u) +
                                                      ((d.flags & kMatTransparent) ? 2u : 0u));
     D.maxChildren = std::max<u32>(1u, (S->PathTrace ? 1u : 0u) + matChildren);
+    // Lights that shading samples as entity lights (every type but point,
+    // area, directional and spot): their shadow rays take a closest hit.
+    D.entityLights = 0u;
+    for (u32 i = 0; i < S->LightCount; ++i) {
+        const int32_t t = (int32_t)S->Lights[i].Type;
+        D.entityLights += t != LightType_PointLight && t != LightType_AreaLight && t != LightType_DirectionalLight &&
+                                  t != LightType_SpotLight
+                              ? 1u
+                              : 0u;
+    }
     // Textures (index 0 unused: entities use 1-based indices).
     std::vector<uint8_t> texels;
     std::vector<uint32_t> perms;
@@ -2718,7 +2737,7 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             sc.nseg[0] = sc.nseg[1] = L.nseg;
             sc.total = LT + 2 * depth + 2;
             hipLaunchKernelGGL(k_seg_scan, dim3(2), dim3(kSegs), 0, st, sc);
-            e = launch_connect(C->renderTuning, C->cuCount, C->dg, (const ShadowRay *)B.shadow.ptr,
+            e = launch_connect(S.entityLights == 0u, C->renderTuning, C->cuCount, C->dg, (const ShadowRay *)B.shadow.ptr,
                                SegIn{offShadow, L.nseg, L.shadowSegCap}, plan->shadowBound[depth], acc,
                                (GuideSample *)B.recs.ptr, (ScatterLink *)B.links.ptr, st, LT + 2 * depth + 3);
             if (e != hipSuccess) return e;
@@ -2980,7 +2999,8 @@ static az_status renderImpl(az_gpu_ctx *C, const camera *Cam, filter_type Filter
             seenShadow[depth] = hc[1];
             if (hc[1]) {
                 SegIn sin{offShadow, nseg, shadowSegCap};
-                AZ_HIPR(C, launch_connect(C->renderTuning, C->cuCount, C->dg, (const ShadowRay *)B.shadow.ptr, sin,
+                AZ_HIPR(C, launch_connect(S.entityLights == 0u, C->renderTuning, C->cuCount, C->dg,
+                                          (const ShadowRay *)B.shadow.ptr, sin,
                                           hc[1], acc, (GuideSample *)B.recs.ptr, (ScatterLink *)B.links.ptr, st,
                                           nullptr, counting ? workCnt + 3 : nullptr));
                 if (counting) C->renderStats.ShadowRays += hc[1];
