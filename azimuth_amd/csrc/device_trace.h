@@ -325,8 +325,9 @@ struct TraceCounters {
 //   bool fetch(uint32_t id, Ray3 &r, float &tMax)   -> true: any-hit ray
 //   void finish(uint32_t id, bool found, const HitRec &h)
 // Every lane of the wave must call it (it ends when the whole range is done).
-// ldsLane is the lane's column of a [RING + 1][BLOCK] LDS array: RING stack
-// slots and one word for an excluded instance.
+// ldsLane is the lane's column of a [RING + 4][BLOCK] LDS array: RING stack
+// slots, one word for an excluded instance and three for the world ray's
+// reciprocal direction.
 template <int MODE, bool COUNT, int RING, int BLOCK, bool WW, class Pool>
 __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t begin, uint32_t end, float eps,
                                   bool cull, uint32_t *ldsLane, TraceCounters &cnt) {
@@ -344,6 +345,9 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
     // after its ring (read only at top-level instance leaves), not in a
     // register, so that the walk's register budget is unchanged.
     uint32_t *const excl = ldsLane + RING * BLOCK;
+    // The world ray's reciprocal direction, kept in the three LDS words
+    // after excl for the returns from instances (three divisions each).
+    float *const winv = reinterpret_cast<float *>(ldsLane + (RING + 1) * BLOCK);
     float dirLen = 1.0f, bestW = kFltMax;
     uint32_t bestRank = 0;
     bool found = false;
@@ -392,6 +396,9 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
                 ix = 1.0f / dx;
                 iy = 1.0f / dy;
                 iz = 1.0f / dz;
+                winv[0] = ix;
+                winv[BLOCK] = iy;
+                winv[2 * BLOCK] = iz;
                 np = near_planes(ix, iy, iz);
                 inBlas = false;
                 localBest = kFltMax;
@@ -472,9 +479,9 @@ __device__ inline void trace_pool(const DeviceGeometry &G, Pool &pool, uint32_t 
             dx = r.dx;
             dy = r.dy;
             dz = r.dz;
-            ix = 1.0f / dx;
-            iy = 1.0f / dy;
-            iz = 1.0f / dz;
+            ix = winv[0];
+            iy = winv[BLOCK];
+            iz = winv[2 * BLOCK];
             np = near_planes(ix, iy, iz);
         } else if (!WW && !ref_is_leaf(cur)) {
             if (COUNT) cnt.nodes++;

@@ -232,7 +232,7 @@ template <int BLOCK, bool WW, bool COUNT = false>
 __device__ __forceinline__ void extend_body(const DeviceGeometry &G, const PathRay *__restrict__ q, SegIn qin, uint32_t n,
                                    PathHit *__restrict__ hits, float eps, int cull, uint32_t perWave, uint32_t block,
                                    unsigned long long *counters = nullptr) {
-    __shared__ uint32_t smem[(kRenderRing + 1) * BLOCK];
+    __shared__ uint32_t smem[(kRenderRing + 4) * BLOCK];
     __shared__ uint32_t soff[kSegs + 1];
     const uint32_t *off = stage_offsets(qin, soff);
     const uint32_t begin = (block * (BLOCK / 64) + (threadIdx.x >> 6)) * perWave;
@@ -321,7 +321,7 @@ __device__ __forceinline__ void connect_body(const DeviceGeometry &G, const Shad
                                     AccWord *__restrict__ acc, GuideSample *__restrict__ recs,
                                     ScatterLink *__restrict__ links, uint32_t perWave, uint32_t block,
                                     unsigned long long *counters = nullptr) {
-    __shared__ uint32_t smem[(kRenderRing + 1) * BLOCK];
+    __shared__ uint32_t smem[(kRenderRing + 4) * BLOCK];
     __shared__ uint32_t soff[kSegs + 1];
     const uint32_t *off = stage_offsets(sin, soff);
     const uint32_t begin = (block * (BLOCK / 64) + (threadIdx.x >> 6)) * perWave;

@@ -58,7 +58,7 @@ template <bool ANY, bool COUNT, int RING, int BLOCK, int WAVES, bool WW>
 __global__ __launch_bounds__(BLOCK, (WAVES > 0 ? WAVES : 1)) void k_trace(DeviceGeometry G, const az_ray *__restrict__ rays, uint32_t n,
                                                 az_hit *__restrict__ hits, float eps, int cull,
                                                 unsigned long long *__restrict__ counters, uint32_t perWave) {
-    __shared__ uint32_t smem[(RING + 1) * BLOCK];
+    __shared__ uint32_t smem[(RING + 4) * BLOCK];
     const uint32_t wave = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
     const uint32_t begin = wave * perWave;
     if (begin >= n) return;                          // whole waves only; no barriers follow
