@@ -937,12 +937,14 @@ __device__ __forceinline__ void shade_body(const DevScene &S, const DeviceGeomet
     __shared__ uint32_t cls[kShadeClasses];
     __shared__ uint16_t order[kRenderBlock];
     __shared__ uint16_t pairs[kRenderBlock / 64][kPairCap];
-    const uint32_t *off = stage_offsets(qin, soff);
     // Rays of the block sorted by shading class (a counting sort in LDS), so
     // that the lanes of a wave take the same branches of the vertex code.
+    // The key's loads are issued before the queue offsets are staged, so
+    // that the two wait together.
+    const uint32_t i0 = block * kRenderBlock + threadIdx.x;
+    const uint32_t key = i0 < n ? shade_class(S, hits[i0]) : kShadeClasses - 1u;
+    const uint32_t *off = stage_offsets(qin, soff);
     {
-        const uint32_t i0 = block * kRenderBlock + threadIdx.x;
-        const uint32_t key = i0 < n ? shade_class(S, hits[i0]) : kShadeClasses - 1u;
         if (threadIdx.x < kShadeClasses) cls[threadIdx.x] = 0u;
         __syncthreads();
         atomicAdd(&cls[key], 1u);
